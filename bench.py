@@ -1,0 +1,197 @@
+"""Benchmark: screen-mode NL-PF at 1920x1080 x 16 spp, depth 4 (5 vertex levels), 15x15
+cross-bilateral window, 16-stop colour ramp + hue shift 0.1 fused into every level.
+
+One step = one whole NL-PF pass (levels 4..0 + film) over 165,888,000 synthetic path
+vertices already resident in HBM.  Prints one JSON line (rank 0).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fp16] [--no-fp64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "practical-stylized_amd"))
+
+METRIC = "filtered path-vertices/s @1080p×16spp, plain vs FP16+MLP; HBM % of peak; L2 err"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOPS_PER_PAIR = 28.0  # guide distances 8, normal term 9, luminance 3, exp 1, accumulate 7
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def style(N):
+    rng = np.random.default_rng(2024)
+    pos = np.sort(rng.random(16)).astype(np.float32)
+    val = np.sort(rng.random(16)).astype(np.float32)
+    return N.style_table([N.ramp(pos, val), N.hue_shift(0.1, inner=0)]), 1
+
+
+def level_bytes(S, plane, L, rec_b):
+    """Algorithmic HBM bytes of each level launch: records once, child output once, output once."""
+    out = []
+    for d in range(L):
+        child = 12 if d + 1 < L else 0
+        out.append(S * plane * (rec_b + child + 12))
+    return out
+
+
+def timed_runs(h, desc, rec, anc, film, steps, warmup):
+    for _ in range(warmup):
+        h.screen_run(desc, rec, anc, film, None)
+    h.sync()
+    lv_ms = np.zeros(desc.levels)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.screen_run(desc, rec, anc, film, None)
+        st = h.stats()
+        lv_ms += np.array(st["level_ms"][: desc.levels])
+    h.sync()
+    wall = time.perf_counter() - t0
+    return wall / steps * 1e3, lv_ms / steps
+
+
+def cpu_baseline(h, rec, W, H, S, L, radius, sig, table, root):
+    """FP32 OpenMP restatement (oracle/) on one spp slice of the same records."""
+    import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    plane = W * H
+    host = rec.download(np.float32, L * 15 * plane).reshape(1, L, 15, H, W)  # slice 0
+    t0 = time.perf_counter()
+    O.screen_run(host, W, H, 1, L, radius, sig, nodes=table, root=root, precision="f32", threads=threads)
+    dt = time.perf_counter() - t0
+    verts = plane * L
+    return {"value": verts / dt, "unit": "vertices/s", "cores": threads, "kind": "port",
+            "sample": f"1 of {S} spp slices ({W}x{H}x{L} levels = {verts} vertices), FP32, "
+                      f"{threads} OpenMP threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--max-depth", type=int, default=4)
+    ap.add_argument("--radius", type=int, default=7)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "screen-mode NL-PF shards by spp slice with one film all-reduce, which is not built here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import nlpf as N
+
+    W, H, S, L, R = args.width, args.height, args.spp, args.max_depth + 1, args.radius
+    plane = W * H
+    sig = (0.05, 0.9, 0.5)
+    table, root = style(N)
+    h = N.Handle({"type": "pf", "mode": "screen", "max_depth": L - 1, "pixel_radius": R})
+    h.set_style(table, [root])
+    h.set_sigmas(*sig)
+    verts = plane * S * L
+
+    log(f"generating {verts} FP32 records ({verts * 60 / 1e9:.1f} GB)")
+    d32 = h.screen_desc(W, H, S, L, N.FP32, R)
+    rec32 = h.alloc(verts * 60)
+    h.generate_screen_records(d32, rec32, seed=1)
+    film32 = h.alloc(plane * 12)
+
+    log("timing FP32")
+    ms32, lv32 = timed_runs(h, d32, rec32, None, film32, args.steps, args.warmup)
+    log(f"FP32: {ms32:.2f} ms/step, levels {np.round(lv32, 2).tolist()} ms")
+    lb32 = level_bytes(S, plane, L, 60)
+    achieved_gbs = sum(lb32) / 1e9 / (lv32.sum() / 1e3)
+    pairs = verts * (2 * R + 1) ** 2
+    tflops = pairs * FLOPS_PER_PAIR / (lv32.sum() / 1e3) / 1e12
+
+    result = {
+        "metric": METRIC,
+        "value": verts / (ms32 / 1e3),
+        "unit": "vertices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"screen NL-PF {W}x{H}x{S}spp depth {L - 1} ({L} levels), "
+                               f"{2 * R + 1}x{2 * R + 1} window, ramp16+hue0.1, FP32 records",
+                   "vertices_per_step": verts, "sigmas": list(sig)},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "screen_level_kernel<float,float,7,4,4>",
+                     "algorithmic_bytes_per_launch": float(np.mean(lb32)),
+                     "avg_launch_ms": float(lv32.mean()),
+                     "valu": {"achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS,
+                              "frac": tflops / FP32_PEAK_TFLOPS, "flops_per_pair": FLOPS_PER_PAIR}},
+    }
+
+    film_ref = None
+    if not args.no_fp64:
+        log("FP64 GPU reference pass (error oracle)")
+        rec64 = h.alloc(verts * 120)
+        # widen FP32 -> FP64 slice by slice through the host
+        slice_n = L * 15 * plane
+        for s in range(S):
+            part = rec32.download(np.float32, slice_n, offset=s * slice_n * 4)
+            rec64.upload(part.astype(np.float64), offset=s * slice_n * 8)
+        d64 = h.screen_desc(W, H, S, L, N.FP64, R)
+        film64 = h.alloc(plane * 24)
+        t0 = time.perf_counter()
+        h.screen_run(d64, rec64, None, film64, None)
+        log(f"FP64 pass {time.perf_counter() - t0:.2f} s")
+        film_ref = film64.download(np.float64)
+        rec64.free()
+        film32_host = film32.download(np.float32).astype(np.float64)
+        result["l2_err_fp32_vs_fp64"] = float(np.linalg.norm(film32_host - film_ref) / np.linalg.norm(film_ref))
+
+    if not args.no_fp16:
+        log("FP16 records")
+        d16 = h.screen_desc(W, H, S, L, N.FP16, R)
+        rec16 = h.alloc(verts * 30)
+        anc = h.alloc(S * L * ((W + 7) // 8) * ((H + 7) // 8) * 12)
+        h.convert_records_fp16(d32, rec32, rec16, anc)
+        film16 = h.alloc(plane * 12)
+        ms16, lv16 = timed_runs(h, d16, rec16, anc, film16, args.steps, args.warmup)
+        lb16 = level_bytes(S, plane, L, 30)
+        fp16 = {"value": verts / (ms16 / 1e3), "unit": "vertices/s", "ms_per_step": ms16,
+                "hbm_gbs": sum(lb16) / 1e9 / (lv16.sum() / 1e3)}
+        if film_ref is not None:
+            f16 = film16.download(np.float32).astype(np.float64)
+            fp16["l2_err_vs_fp64"] = float(np.linalg.norm(f16 - film_ref) / np.linalg.norm(film_ref))
+        result["fp16"] = fp16
+        rec16.free()
+        anc.free()
+    result["fp16_mlp"] = None
+
+    if not args.no_cpu:
+        log("CPU baseline (oracle FP32 restatement)")
+        result["cpu_baseline"] = cpu_baseline(h, rec32, W, H, S, L, R, sig, table, root)
+        result["cpu_baseline"]["gpu_speedup"] = result["value"] / result["cpu_baseline"]["value"]
+    else:
+        result["cpu_baseline"] = None
+    h.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+/*
+ * nlpf.h — C ABI of the MI355X-native nonlinear path-filtering (NL-PF) library.
+ *
+ * The library replaces the `"type":"pf"` integrator of the reference
+ * (crates/akari_integrator/src/lm_path_filtering.rs:1802-1813, dispatched from
+ * crates/akari_integrator/src/lib.rs:169-177) and the stylization hooks it calls
+ * (crates/akari_render/src/svm/surface/mod.rs:262-286, 1824-1842).
+ *
+ * Conventions
+ *  - every entry point returns an int status (NLPF_OK == 0); on failure
+ *    nlpf_last_error(h) (or nlpf_last_error(NULL) for nlpf_create) holds a message;
+ *  - buffers are device pointers owned by the caller, or allocated through
+ *    nlpf_alloc (hipMalloc on the handle's device);
+ *  - one HIP stream per handle; calls on one handle are ordered on that stream;
+ *    a handle is thread-compatible but not thread-safe;
+ *  - no torch types cross this boundary: plain pointers and sizes only.
+ */
+#ifndef NLPF_H
+#define NLPF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+/* ---- status codes (the reference panics instead: lm_path_filtering.rs:829, 927, 938) ---- */
+#define NLPF_OK 0
+#define NLPF_E_INVALID_CONFIG 1
+#define NLPF_E_INVALID_ARG 2
+#define NLPF_E_OOM 3
+#define NLPF_E_HIP 4
+#define NLPF_E_UNSUPPORTED 5
+#define NLPF_E_NUMERIC 6
+
+/* ---- storage precision of record planes ---- */
+#define NLPF_FP32 0
+#define NLPF_FP16 1
+#define NLPF_FP64 2
+
+/* ---- stylization node types ----
+ * Each node wraps `inner` (its input surface) exactly as the reference's wrapper
+ * surfaces do: apply_stylization first runs inner, then its own transform.
+ *   BASE       plain BSDF, identity        (svm/surface/mod.rs:275-285 default)
+ *   TOON       ToonSurface                 (svm/surface/mod.rs:572-610)
+ *   PALETTE    ColorPaletteSurface         (svm/surface/mod.rs:696-735)
+ *   SATURATION SaturationAdjustSurface     (svm/surface/mod.rs:820-851)
+ *   REX        SaturationAdjustRexSurface  (svm/surface/mod.rs:1059-1089)
+ *   EIGHT_BIT  EightBitSurface             (svm/surface/mod.rs:1315-1345)
+ *   HALFTONE   HalfToneSurface             (svm/surface/mod.rs:1180-1233)
+ *   HUE_SHIFT  sty_hue_shift: parsed by the reference (akari_scenegraph/src/shader.rs:218-223)
+ *              but never compiled (svm/compiler.rs:478 todo!()); defined here as an HSV hue
+ *              rotation h <- fract(h + shift) using color.rs:551-589.
+ *   RAMP       colour ramp: no reference node; piecewise-linear luminance remap, hue preserved.
+ *   ENABLE_IF  EnableOrSetStylizationSurface, mode_set=false (svm/surface/mod.rs:959-976)
+ *   SET_IDX    EnableOrSetStylizationSurface, mode_set=true  (svm/surface/mod.rs:941-953)
+ *
+ * Parameter layout p[] per type:
+ *   TOON       p[0]=color_band_width
+ *   PALETTE    p[0]=indirect_scale p[1]=gamma p[2+3k..4+3k]=entry k rgb (count<=16)
+ *   SATURATION p[0]=power p[1]=indirect_scale; max_depth field
+ *   REX        p[0]=blend
+ *   EIGHT_BIT  p[0]=indirect_scale
+ *   HALFTONE   p[0]=indirect_scale p[1]=radius_scale p[2]=radius p[3]=grid_distance p[4]=blend
+ *              p[5..7]=dark_tone p[8..10]=light_tone p[11..22]=affine 3x4 row-major; max_depth field
+ *   HUE_SHIFT  p[0]=shift p[1]=indirect_scale (hue rotation is scale-equivariant, so unused)
+ *   RAMP       p[0..15]=stop positions (ascending) p[16..31]=stop values (count<=16)
+ * depth_mask: bit d set => the node's own transform runs at path depth d (bits >= 31 map to
+ * bit 31). 0xFFFFFFFF reproduces the reference (no depth gating on toon/palette/...). */
+#define NLPF_STY_BASE 0
+#define NLPF_STY_TOON 1
+#define NLPF_STY_PALETTE 2
+#define NLPF_STY_SATURATION 3
+#define NLPF_STY_REX 4
+#define NLPF_STY_EIGHT_BIT 5
+#define NLPF_STY_HALFTONE 6
+#define NLPF_STY_HUE_SHIFT 7
+#define NLPF_STY_RAMP 8
+#define NLPF_STY_ENABLE_IF 9
+#define NLPF_STY_SET_IDX 10
+
+#define NLPF_STY_MAX_PARAMS 56
+#define NLPF_STY_MAX_NODES 64
+#define NLPF_STY_MAX_MATERIALS 64
+
+typedef struct nlpf_style_node {
+    int32_t type;
+    int32_t inner;       /* wrapped node index, -1 = base BSDF (identity) */
+    int32_t else_;       /* ENABLE_IF else branch, -1 = null surface (identity) */
+    uint32_t idx;        /* ENABLE_IF / SET_IDX style index 0..3 */
+    uint32_t depth_mask; /* see above */
+    uint32_t max_depth;  /* SATURATION / HALFTONE max_depth */
+    uint32_t count;      /* PALETTE entries / RAMP stops */
+    uint32_t reserved;
+    float p[NLPF_STY_MAX_PARAMS];
+} nlpf_style_node; /* 256 bytes */
+
+/* StylizationFlags (svm/surface/mod.rs:58-101) packed into one byte:
+ * bit0 stylization_disabled, bit1 use_style_2, bit2 use_style_3, bit3 use_style_4. */
+#define NLPF_FLAG_DISABLED 1u
+#define NLPF_FLAG_STYLE2 2u
+#define NLPF_FLAG_STYLE3 4u
+#define NLPF_FLAG_STYLE4 8u
+
+typedef struct nlpf_handle nlpf_handle;
+
+/* ---- lifecycle ----
+ * config_json: the reference's "pf" method schema (lm_path_filtering.rs:264-298, all fields
+ * optional with the reference defaults: max_depth 7, repeats 1, branch 1, radius_scale
+ * [0.1,0.05], reuse [128,64], vram 2048, smis true, no_first_hit_stylization false,
+ * disable_flag_check false, cluster true, scene_bounds null) plus the screen-mode fields
+ * mode ("world"|"screen", default "world"), pixel_radius (7), sigma_pos (0.05),
+ * sigma_normal (0.9), sigma_lum (0.5), precision ("fp32"|"fp16"|"fp64"), seed (0).
+ * Unknown keys are ignored, as serde does for the reference (no deny_unknown_fields). */
+int nlpf_create(const char* config_json, nlpf_handle** out);
+int nlpf_destroy(nlpf_handle* h);
+/* Message of the last failing call on h; h == NULL gives the last nlpf_create failure. */
+const char* nlpf_last_error(const nlpf_handle* h);
+/* Parsed configuration echoed back as JSON (for host-side mirrors and tests). */
+int nlpf_get_config_json(const nlpf_handle* h, char* buf, size_t cap);
+/* Overwrite the screen-mode bandwidths (sigma_pos, sigma_normal, sigma_lum). */
+int nlpf_set_sigmas(nlpf_handle* h, float sigma_pos, float sigma_normal, float sigma_lum);
+
+/* ---- device memory (hipMalloc / hipMemcpy on the handle's device and stream) ---- */
+int nlpf_alloc(nlpf_handle* h, size_t bytes, void** dptr);
+int nlpf_free(nlpf_handle* h, void* dptr);
+int nlpf_upload(nlpf_handle* h, void* dst_device, const void* src_host, size_t bytes);
+int nlpf_download(nlpf_handle* h, void* dst_host, const void* src_device, size_t bytes);
+int nlpf_memset(nlpf_handle* h, void* dst_device, int value, size_t bytes);
+int nlpf_sync(nlpf_handle* h);
+
+/* ---- stylization ----
+ * Replaces the SVM compile of sty_* nodes (svm/compiler.rs:337-477) with a flat node table.
+ * material_roots[m] = root node of material m's stylization tree (-1 = unstylized). */
+int nlpf_set_style(nlpf_handle* h, const nlpf_style_node* nodes, uint32_t n_nodes,
+                   const int32_t* material_roots, uint32_t n_materials);
+
+/* Apply SurfaceClosure::apply_stylization (svm/surface/mod.rs:1824-1842) of material
+ * `material[i]` (or material 0 when material == NULL) at path depth depth[i] (or `depth0`
+ * when depth == NULL) with flags[i] (0 when NULL) and world position p[i] (halftone only;
+ * may be NULL otherwise) to rgb_in[i]; all arrays are device pointers, n elements,
+ * rgb as 3 interleaved floats. */
+int nlpf_stylize(nlpf_handle* h, const float* rgb_in, float* rgb_out, uint64_t n,
+                 const uint32_t* depth, uint32_t depth0, const uint8_t* flags,
+                 const uint16_t* material, const float* p);
+
+/* FP16 records store radiance divided by this power of two so fireflies stay finite. */
+#define NLPF_FP16_RADIANCE_SCALE 64.0f
+
+/* ---- screen mode: per-(spp slice, depth) cross-bilateral NL-PF ----
+ * Record planes, element type by precision (float / half / double), layout
+ *   records[((s*levels + d)*NLPF_SCREEN_PLANES + c)*H*W + y*W + x]
+ * planes c: 0-2 position, 3-5 normal, 6-8 albedo, 9-11 radiance, 12-14 throughput.
+ * FP16 positions are stored as offsets from a per-8x8-block FP32 anchor
+ *   anchors[((s*levels + d)*ceil(H/8) + by)*ceil(W/8) + bx][3]  (NULL for FP32/FP64).
+ * Filter per level (depth D..0):
+ *   in_i  = radiance_i + throughput_i * out_{d+1,i}        (out_{D+1} = 0)
+ *   w_ij  = exp(-|p_i-p_j|^2/2sp^2 - (1-n_i.n_j)^2/2sn^2 - (l_i-l_j)^2/2sl^2), l = lum(albedo)
+ *   out_i = S_d( sum_j w_ij in_j / sum_j w_ij ),  j over the (2r+1)^2 window clipped to the image
+ * S_d = material 0's stylization tree at depth d (flags 0). film[y][x][c] = mean_s out_{0}.
+ * Outputs are double for NLPF_FP64 and float otherwise:
+ *   film_rgb  [H][W][3];
+ *   level_out (optional, NULL to skip) every level's output [spp][levels][3][H][W]. */
+#define NLPF_SCREEN_PLANES 15
+typedef struct nlpf_screen_desc {
+    uint32_t width, height, spp, levels;
+    int32_t precision;
+    int32_t radius;
+} nlpf_screen_desc;
+
+int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                    const float* anchors, void* film_rgb, void* level_out);
+
+/* Kernel time (ms, HIP events on the handle's stream) of the last nlpf_screen_run, total
+ * and per level (levels entries, deepest level first in execution order, indexed by depth). */
+typedef struct nlpf_stats {
+    double total_ms;
+    double level_ms[32];
+    double film_ms;
+    uint64_t vertices;
+    uint64_t bytes_algorithmic;
+    double flops_algorithmic;
+} nlpf_stats;
+int nlpf_get_stats(const nlpf_handle* h, nlpf_stats* out);
+
+/* ---- synthetic records (procedural unit box + 2 spheres + quad light) ----
+ * Traces one path per (pixel, spp slice) through the analytic scene and writes screen-mode
+ * records: p, n from the hits; albedo U[0.1,0.9]; radiance lognormal(mu, sigma) with a
+ * `firefly_prob` fraction multiplied by `firefly_scale`; throughput U[0,1]; all from PCG32
+ * streams keyed by (seed, record index). FP16 output also writes the anchors. */
+typedef struct nlpf_gen_desc {
+    uint64_t seed;
+    float mu, sigma;
+    float firefly_prob, firefly_scale;
+    float albedo_lo, albedo_hi;
+} nlpf_gen_desc;
+int nlpf_generate_screen_records(nlpf_handle* h, const nlpf_screen_desc* desc,
+                                 const nlpf_gen_desc* gen, void* records, float* anchors);
+/* Convert FP32 records to FP16 records + anchors (same layout rules as above). */
+int nlpf_convert_records_fp16(nlpf_handle* h, const nlpf_screen_desc* desc_fp32,
+                              const float* records_fp32, void* records_fp16, float* anchors);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+#ifdef __cplusplus
+}
+#endif
+#endif /* NLPF_H */

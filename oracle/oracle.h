@@ -1,0 +1,51 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — CPU oracle of the NL-PF hot path.  Loaded by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker; never linked into
+ * or called by libnlpf.
+ *
+ * The node table uses the same 256-byte layout as the library's stylization nodes (the
+ * exchange format of the C ABI), so one table feeds both sides.
+ */
+#ifndef NLPF_ORACLE_H
+#define NLPF_ORACLE_H
+#include <stdint.h>
+
+enum {
+    OR_BASE = 0,
+    OR_TOON = 1,
+    OR_PALETTE = 2,
+    OR_SATURATION = 3,
+    OR_REX = 4,
+    OR_EIGHT_BIT = 5,
+    OR_HALFTONE = 6,
+    OR_HUE_SHIFT = 7,
+    OR_RAMP = 8,
+    OR_ENABLE_IF = 9,
+    OR_SET_IDX = 10
+};
+
+typedef struct or_node {
+    int32_t type, inner, else_;
+    uint32_t idx, depth_mask, max_depth, count, reserved;
+    float p[56];
+} or_node;
+
+void or_rgb_to_hsv(const double c[3], double hsv[3]);
+void or_hsv_to_rgb(const double hsv[3], double c[3]);
+void or_apply_style(const or_node* nodes, int root, uint32_t depth, uint32_t flags, const double in[3],
+                    const double p[3], double out[3]);
+uint32_t or_update_flags(const or_node* nodes, int root, uint32_t flags);
+int or_stylize_batch(const or_node* nodes, const int32_t* roots, uint32_t n_materials, const double* in,
+                     double* out, uint64_t n, const uint32_t* depth, uint32_t depth0, const uint8_t* flags,
+                     const uint16_t* material, const double* p);
+
+/* Screen-mode NL-PF over FP32 record planes [S][L][15][H][W] (positions already absolute).
+ * film [H][W][3], level_out optional [S][L][3][H][W]. */
+int or_screen_run_f64(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, double sigma_pos,
+                      double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
+                      double* film, double* level_out, int threads);
+int or_screen_run_f32(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, double sigma_pos,
+                      double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
+                      float* film, float* level_out, int threads);
+
+#endif

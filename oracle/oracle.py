@@ -1,0 +1,99 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes loader of the CPU oracle (oracle/liboracle.so).
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker.  Built by oracle/Makefile (gcc).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp, u32, u64, i32, dbl = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_double
+    lib.or_apply_style.argtypes = [vp, i32, u32, u32, vp, vp, vp]
+    lib.or_update_flags.argtypes = [vp, i32, u32]
+    lib.or_update_flags.restype = u32
+    lib.or_rgb_to_hsv.argtypes = [vp, vp]
+    lib.or_hsv_to_rgb.argtypes = [vp, vp]
+    lib.or_stylize_batch.argtypes = [vp, vp, u32, vp, vp, u64, vp, u32, vp, vp, vp]
+    for name in ("or_screen_run_f64", "or_screen_run_f32"):
+        fn = getattr(lib, name)
+        fn.argtypes = [u32, u32, u32, u32, i32, dbl, dbl, dbl, vp, vp, i32, vp, vp, i32]
+        fn.restype = i32
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def stylize(nodes, roots, rgb, depth=0, flags=None, material=None, pos=None):
+    lib = load()
+    rgb = np.ascontiguousarray(rgb, np.float64).reshape(-1, 3)
+    n = len(rgb)
+    out = np.empty_like(rgb)
+    dd, d0 = (None, int(depth)) if np.ndim(depth) == 0 else (np.ascontiguousarray(depth, np.uint32), 0)
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    mt = None if material is None else np.ascontiguousarray(material, np.uint16)
+    ps = None if pos is None else np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    nodes = np.ascontiguousarray(nodes)
+    roots = np.ascontiguousarray(roots, np.int32)
+    lib.or_stylize_batch(_p(nodes), _p(roots), len(roots), _p(rgb), _p(out), n, _p(dd), d0, _p(fl),
+                         _p(mt), _p(ps))
+    return out
+
+
+def update_flags(nodes, root, flags):
+    return int(load().or_update_flags(_p(np.ascontiguousarray(nodes)), int(root), int(flags)))
+
+
+def rgb_to_hsv(c):
+    c = np.ascontiguousarray(c, np.float64)
+    o = np.empty(3)
+    load().or_rgb_to_hsv(_p(c), _p(o))
+    return o
+
+
+def hsv_to_rgb(c):
+    c = np.ascontiguousarray(c, np.float64)
+    o = np.empty(3)
+    load().or_hsv_to_rgb(_p(c), _p(o))
+    return o
+
+
+def screen_run(records, W, H, S, L, radius, sigmas, nodes=None, root=-1, precision="f64",
+               want_levels=False, threads=0):
+    """records: float32 planes [S][L][15][H][W] with absolute positions."""
+    lib = load()
+    records = np.ascontiguousarray(records, np.float32)
+    assert records.size == S * L * 15 * H * W
+    dt = np.float64 if precision == "f64" else np.float32
+    film = np.empty((H, W, 3), dt)
+    lv = np.empty((S, L, 3, H, W), dt) if want_levels else None
+    if nodes is None or len(nodes) == 0:
+        nodes_arr, root = np.zeros(1, dtype=[("b", "V256")]), -1
+    else:
+        nodes_arr = np.ascontiguousarray(nodes)
+    fn = lib.or_screen_run_f64 if precision == "f64" else lib.or_screen_run_f32
+    rc = fn(W, H, S, L, radius, sigmas[0], sigmas[1], sigmas[2], _p(records), _p(nodes_arr), int(root),
+            _p(film), _p(lv), int(threads))
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return (film, lv) if want_levels else film

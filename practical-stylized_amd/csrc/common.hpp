@@ -1,0 +1,94 @@
+// Handle state shared by the C-ABI entry points and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/nlpf.h"
+
+namespace nlpf {
+
+// `"type":"pf"` method config (crates/akari_integrator/src/lm_path_filtering.rs:264-298) with
+// the reference defaults (:280-298), plus the screen-mode fields of this library.
+struct Config {
+    uint32_t max_depth = 7;
+    uint32_t repeats = 1;
+    uint32_t branch = 1;
+    std::vector<float> radius_scale{0.1f, 0.05f};
+    std::vector<uint32_t> reuse{128u, 64u};
+    uint64_t vram = 2048;  // MB
+    bool smis = true;
+    bool no_first_hit_stylization = false;
+    bool disable_flag_check = false;
+    bool cluster = true;
+    bool has_scene_bounds = false;
+    float scene_bounds[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    // screen mode
+    std::string mode = "world";
+    int pixel_radius = 7;
+    float sigma_pos = 0.05f, sigma_normal = 0.9f, sigma_lum = 0.5f;
+    int precision = NLPF_FP32;
+    uint64_t seed = 0;
+
+    // per-depth vectors repeat their last element (lm_path_filtering.rs:1337-1347, 1584-1586)
+    float radius_scale_at(uint32_t d) const {
+        return d < radius_scale.size() ? radius_scale[d] : radius_scale.back();
+    }
+    uint32_t reuse_at(uint32_t d) const { return d < reuse.size() ? reuse[d] : reuse.back(); }
+};
+
+bool parse_config(const char* json, Config& cfg, std::string& err);
+std::string config_to_json(const Config& cfg);
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace nlpf
+
+struct nlpf_handle {
+    nlpf::Config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // stylization table
+    nlpf_style_node* d_nodes = nullptr;
+    int32_t* d_roots = nullptr;
+    std::vector<nlpf_style_node> h_nodes;
+    std::vector<int32_t> h_roots;
+    // timing
+    std::vector<hipEvent_t> events;
+    nlpf_stats stats{};
+    // reusable device scratch
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+#define NLPF_FAIL(h, code, msg)          \
+    do {                                 \
+        (h)->err = (msg);                \
+        return (code);                   \
+    } while (0)
+
+#define NLPF_HIP(h, expr)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+            return e_ == hipErrorOutOfMemory ? NLPF_E_OOM : NLPF_E_HIP;                       \
+        }                                                                                     \
+    } while (0)
+
+namespace nlpf {
+// grow-only scratch owned by the handle
+int ensure_scratch(nlpf_handle* h, size_t bytes);
+// record the last kernel failure (launch errors are sticky per stream)
+int check_launch(nlpf_handle* h, const char* what);
+int record_event(nlpf_handle* h, size_t i);
+double event_ms(nlpf_handle* h, size_t i, size_t j);
+
+inline size_t elem_size(int precision) {
+    return precision == NLPF_FP16 ? 2 : (precision == NLPF_FP64 ? 8 : 4);
+}
+}  // namespace nlpf

@@ -1,0 +1,181 @@
+// Synthetic screen-mode records, FP16 packing and the standalone stylization kernel.
+#include "common.hpp"
+#include "scene.hpp"
+
+namespace nlpf {
+
+struct GenArgs {
+    float* out;  // fp32 planes for slices [s_begin, s_begin + s_count)
+    uint32_t W, H, L, s_begin, s_count;
+    uint64_t plane;
+    uint64_t seed;
+    float mu, sigma, fprob, fscale, alo, ahi;
+};
+
+// One thread per (pixel, slice).  Geometry: one camera path per (pixel, slice); the subpixel
+// jitter and every bounce's (u1,u2) are shared by all pixels of a (slice, depth), so each
+// level image is a coherent picture of the scene (screen-space neighbours stay neighbours in
+// world space on smooth surfaces).  Shading fields are i.i.d. per record from a PCG32 stream
+// keyed by the record's global index.
+__global__ void gen_screen_kernel(GenArgs g) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t npix = g.plane;
+    if (i >= npix * g.s_count) return;
+    const uint32_t sl = (uint32_t)(i / npix);
+    const uint64_t px = i - (uint64_t)sl * npix;
+    const uint32_t s = g.s_begin + sl;
+    const uint32_t x = (uint32_t)(px % g.W), y = (uint32_t)(px / g.W);
+    const uint64_t key = splitmix64(g.seed);
+
+    Pcg32 sg = pcg_seq_offset(0x5EED000000000000ull + s, key);
+    const float jx = pcg_uniform(sg), jy = pcg_uniform(sg);
+    v3<float> o, d;
+    camera_ray(g.W, g.H, (float)x + jx, (float)y + jy, o, d);
+    bool escaped = false;
+    v3<float> sky_p{0, 0, 0}, sky_n{0, 0, 1};
+    for (uint32_t lv = 0; lv < g.L; ++lv) {
+        const float bu1 = pcg_uniform(sg), bu2 = pcg_uniform(sg);
+        v3<float> p, n;
+        if (!escaped) {
+            const Hit h = scene_intersect(o, d);
+            p = h.p;
+            n = h.n;
+            if (dot(n, d) > 0.0f) n = n * -1.0f;
+            if (h.mat == NLPF_MAT_SKY) {
+                escaped = true;
+                sky_p = p;
+                sky_n = n;
+            }
+        } else {
+            p = sky_p;
+            n = sky_n;
+        }
+        const uint64_t rid = (((uint64_t)s * g.L + lv) * g.H + y) * g.W + x;
+        Pcg32 rg = pcg_seq_offset(rid, key);
+        const float a0 = g.alo + (g.ahi - g.alo) * pcg_uniform(rg);
+        const float a1 = g.alo + (g.ahi - g.alo) * pcg_uniform(rg);
+        const float a2 = g.alo + (g.ahi - g.alo) * pcg_uniform(rg);
+        // Box-Muller pairs for the three log-normal channels
+        const float r0 = m_sqrt(-2.0f * logf(pcg_uniform_open0(rg)));
+        const float t0 = 6.2831853f * pcg_uniform(rg);
+        const float r1 = m_sqrt(-2.0f * logf(pcg_uniform_open0(rg)));
+        const float t1 = 6.2831853f * pcg_uniform(rg);
+        const float z0 = r0 * cosf(t0), z1 = r0 * sinf(t0), z2 = r1 * cosf(t1);
+        const float ff = pcg_uniform(rg) < g.fprob ? g.fscale : 1.0f;
+        float rad[3] = {expf(g.mu + g.sigma * z0) * ff, expf(g.mu + g.sigma * z1) * ff,
+                        expf(g.mu + g.sigma * z2) * ff};
+        float thr[3] = {pcg_uniform(rg), pcg_uniform(rg), pcg_uniform(rg)};
+        if (escaped) {
+            rad[0] = rad[1] = rad[2] = 0.0f;
+            thr[0] = thr[1] = thr[2] = 0.0f;
+        }
+        float* base = g.out + ((uint64_t)sl * g.L + lv) * NLPF_SCREEN_PLANES * npix + px;
+        const float vals[NLPF_SCREEN_PLANES] = {p.x, p.y, p.z, n.x, n.y, n.z, a0, a1, a2,
+                                                 rad[0], rad[1], rad[2], thr[0], thr[1], thr[2]};
+#pragma unroll
+        for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) base[(uint64_t)c * npix] = vals[c];
+        if (!escaped) {
+            o = p + n * 1e-4f;
+            d = cosine_sample(n, bu1, bu2);
+        }
+    }
+}
+
+// FP32 -> FP16 records.  One wavefront per 8x8 pixel block of a (slice, depth) image: the
+// block's anchor is the mean position of its non-sky records, positions are stored as FP16
+// offsets from it, radiance is stored divided by NLPF_FP16_RADIANCE_SCALE (a power of two)
+// so fireflies stay finite.
+__global__ void pack_fp16_kernel(const float* in, __half* out, float* anchors, uint32_t W, uint32_t H,
+                                 uint32_t nslices, uint32_t abx, uint32_t aby, uint64_t plane) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t blk = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint64_t nblk = (uint64_t)abx * aby * nslices;
+    if (blk >= nblk) return;
+    const uint64_t sl = blk / ((uint64_t)abx * aby);
+    const uint32_t b = (uint32_t)(blk - sl * abx * aby);
+    const uint32_t bx = b % abx, by = b / abx;
+    const uint32_t x = bx * 8 + (lane & 7), y = by * 8 + (lane >> 3);
+    const bool inside = x < W && y < H;
+    const float* src = in + sl * NLPF_SCREEN_PLANES * plane;
+    __half* dst = out + sl * NLPF_SCREEN_PLANES * plane;
+    const uint64_t o = (uint64_t)y * W + x;
+    float p[3] = {0, 0, 0};
+    if (inside)
+        for (int c = 0; c < 3; ++c) p[c] = src[c * plane + o];
+    const bool use = inside && fabsf(p[0]) < 50.0f && fabsf(p[1]) < 50.0f && fabsf(p[2]) < 50.0f;
+    float acc[4] = {use ? p[0] : 0.0f, use ? p[1] : 0.0f, use ? p[2] : 0.0f, use ? 1.0f : 0.0f};
+    for (int off = 32; off > 0; off >>= 1)
+        for (int c = 0; c < 4; ++c) acc[c] += __shfl_xor(acc[c], off);
+    float anc[3] = {0, 0, 0};
+    if (acc[3] > 0.0f)
+        for (int c = 0; c < 3; ++c) anc[c] = acc[c] / acc[3];
+    if (lane == 0)
+        for (int c = 0; c < 3; ++c) anchors[blk * 3 + c] = anc[c];
+    if (!inside) return;
+    for (int c = 0; c < 3; ++c) dst[c * plane + o] = __float2half(p[c] - anc[c]);
+    for (int c = 3; c < NLPF_SCREEN_PLANES; ++c) {
+        float v = src[c * plane + o];
+        if (c >= 9 && c < 12) v *= 1.0f / NLPF_FP16_RADIANCE_SCALE;
+        dst[c * plane + o] = __float2half(v);
+    }
+}
+
+__global__ void stylize_kernel(const float* in, float* out, uint64_t n, const uint32_t* depth, uint32_t depth0,
+                               const uint8_t* flags, const uint16_t* material, const float* pos,
+                               const nlpf_style_node* nodes, const int32_t* roots, uint32_t n_materials) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t m = material ? material[i] : 0u;
+    const int root = m < n_materials ? roots[m] : -1;
+    v3<float> li{in[3 * i], in[3 * i + 1], in[3 * i + 2]};
+    v3<float> p{0, 0, 0};
+    if (pos) p = v3<float>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    if (root >= 0) li = apply_style<float>(nodes, root, depth ? depth[i] : depth0, flags ? flags[i] : 0u, li, p);
+    out[3 * i] = li.x;
+    out[3 * i + 1] = li.y;
+    out[3 * i + 2] = li.z;
+}
+
+hipError_t launch_gen_screen(float* out, uint32_t W, uint32_t H, uint32_t L, uint32_t s_begin, uint32_t s_count,
+                             const nlpf_gen_desc& gd, hipStream_t st) {
+    GenArgs g;
+    g.out = out;
+    g.W = W;
+    g.H = H;
+    g.L = L;
+    g.s_begin = s_begin;
+    g.s_count = s_count;
+    g.plane = (uint64_t)W * H;
+    g.seed = gd.seed;
+    g.mu = gd.mu;
+    g.sigma = gd.sigma;
+    g.fprob = gd.firefly_prob;
+    g.fscale = gd.firefly_scale;
+    g.alo = gd.albedo_lo;
+    g.ahi = gd.albedo_hi;
+    const uint64_t n = g.plane * s_count;
+    const uint32_t bs = 256;
+    hipLaunchKernelGGL(gen_screen_kernel, dim3((uint32_t)((n + bs - 1) / bs)), dim3(bs), 0, st, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_fp16(const float* in, __half* out, float* anchors, uint32_t W, uint32_t H, uint32_t nslices,
+                            hipStream_t st) {
+    const uint32_t abx = (W + 7) / 8, aby = (H + 7) / 8;
+    const uint64_t nblk = (uint64_t)abx * aby * nslices;
+    const uint32_t wpb = 4;
+    hipLaunchKernelGGL(pack_fp16_kernel, dim3((uint32_t)((nblk + wpb - 1) / wpb)), dim3(64 * wpb), 0, st, in, out,
+                       anchors, W, H, nslices, abx, aby, (uint64_t)W * H);
+    return hipGetLastError();
+}
+
+hipError_t launch_stylize(const float* in, float* out, uint64_t n, const uint32_t* depth, uint32_t depth0,
+                          const uint8_t* flags, const uint16_t* material, const float* pos,
+                          const nlpf_style_node* nodes, const int32_t* roots, uint32_t n_materials, hipStream_t st) {
+    const uint32_t bs = 256;
+    hipLaunchKernelGGL(stylize_kernel, dim3((uint32_t)((n + bs - 1) / bs)), dim3(bs), 0, st, in, out, n, depth,
+                       depth0, flags, material, pos, nodes, roots, n_materials);
+    return hipGetLastError();
+}
+
+}  // namespace nlpf

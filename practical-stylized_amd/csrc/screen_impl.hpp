@@ -1,0 +1,243 @@
+// Screen-mode NL-PF level kernel: LDS-tiled (2r+1)^2 cross-bilateral gather with the
+// stylization epilogue fused.
+//
+// One launch filters depth d of every spp slice.  A workgroup owns a 64 x (NW*R) output tile
+// of one (slice, depth) image: lanes map to the 64 columns, wave w owns rows [w*R, w*R+R),
+// and each lane keeps R vertically adjacent outputs in registers so that every neighbour it
+// reads from LDS is reused by up to R outputs.  The tile plus a RAD-pixel halo is staged in
+// LDS once, as pre-scaled guide features (p*kp, n, lum(albedo)*kl) and the level input
+// in = radiance + throughput * out_{d+1}.  Weights are exp2 of a negated quadratic form whose
+// log2(e)/(2 sigma^2) factors are folded into kp, kn, kl, so one v_exp_f32 per pair remains.
+// Row partial sums (2r+1 terms) are folded into Kahan-compensated totals.
+#pragma once
+#include "common.hpp"
+#include "style.hpp"
+
+namespace nlpf {
+
+template <typename A>
+struct ScreenArgs {
+    const void* rec;
+    const float* anchors;
+    A* out;  // [S][out_levels][3][H][W]
+    const nlpf_style_node* nodes;
+    int root;
+    int need_pos;
+    uint32_t W, H, S, L, d;
+    uint32_t out_levels, out_slot, child_slot;
+    int has_child;
+    uint64_t plane;
+    uint32_t tiles_x, tiles_y, nblocks;
+    uint32_t abx, aby;  // anchor blocks per row / column (fp16)
+    A kp, kn, kl;
+    A rscale;  // radiance decode factor (FP16 records store radiance / NLPF_FP16_RADIANCE_SCALE)
+};
+
+__device__ __forceinline__ float ld_rec(const float* p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ld_rec(const __half* p, size_t i) { return __half2float(p[i]); }
+__device__ __forceinline__ double ld_rec(const double* p, size_t i) { return p[i]; }
+
+__device__ __forceinline__ float exp2_neg(float e) { return __builtin_amdgcn_exp2f(-e); }
+__device__ __forceinline__ double exp2_neg(double e) { return exp2(-e); }
+
+template <typename T>
+__device__ __forceinline__ void kahan_add(T& sum, T& comp, T v) {
+    T y = v - comp;
+    T t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// block -> tile map: the dispatcher hands block b to XCD (b mod 8); give every XCD a
+// contiguous run of tiles so neighbouring tiles (which share halos) meet in one L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+    const uint32_t per = nb >> 3;
+    return b < (per << 3) ? (b & 7u) * per + (b >> 3) : b;
+}
+
+template <typename S, typename A, int RAD, int NW, int R>
+__global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) {
+    constexpr int TW = 64, TH = NW * R, WT = TW + 2 * RAD, HT = TH + 2 * RAD, NT = 64 * NW;
+    constexpr int K = 2 * RAD + 1;
+    __shared__ A sh[10][HT][WT];
+
+    const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
+    const uint32_t tiles = a.tiles_x * a.tiles_y;
+    const uint32_t s = t / tiles, tile = t - s * tiles;
+    const int tx0 = (int)(tile % a.tiles_x) * TW, ty0 = (int)(tile / a.tiles_x) * TH;
+    const int W = (int)a.W, H = (int)a.H;
+    const size_t plane = a.plane;
+    const S* rec = (const S*)a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
+    const A* child = a.has_child ? a.out + ((size_t)s * a.out_levels + a.child_slot) * 3 * plane : nullptr;
+    const float* anc = a.anchors ? a.anchors + (size_t)((size_t)s * a.L + a.d) * a.abx * a.aby * 3 : nullptr;
+
+    // ---- stage tile + halo in LDS ----
+    for (int i = threadIdx.x; i < WT * HT; i += NT) {
+        const int r = i / WT, c = i - r * WT;
+        const int gx = tx0 - RAD + c, gy = ty0 - RAD + r;
+        A f[10];
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t o = (size_t)gy * W + gx;
+            A px = (A)ld_rec(rec, o), py = (A)ld_rec(rec, plane + o), pz = (A)ld_rec(rec, 2 * plane + o);
+            if (anc) {
+                const float* q = anc + ((size_t)(gy >> 3) * a.abx + (gx >> 3)) * 3;
+                px += (A)q[0];
+                py += (A)q[1];
+                pz += (A)q[2];
+            }
+            const A nx = (A)ld_rec(rec, 3 * plane + o), ny = (A)ld_rec(rec, 4 * plane + o),
+                    nz = (A)ld_rec(rec, 5 * plane + o);
+            const A ax = (A)ld_rec(rec, 6 * plane + o), ay = (A)ld_rec(rec, 7 * plane + o),
+                    az = (A)ld_rec(rec, 8 * plane + o);
+            A ir = (A)ld_rec(rec, 9 * plane + o) * a.rscale, ig = (A)ld_rec(rec, 10 * plane + o) * a.rscale,
+              ib = (A)ld_rec(rec, 11 * plane + o) * a.rscale;
+            if (child) {
+                const A tr = (A)ld_rec(rec, 12 * plane + o), tg = (A)ld_rec(rec, 13 * plane + o),
+                        tb = (A)ld_rec(rec, 14 * plane + o);
+                ir = fma(tr, child[o], ir);
+                ig = fma(tg, child[plane + o], ig);
+                ib = fma(tb, child[2 * plane + o], ib);
+            }
+            const A l = lum709(v3<A>{ax, ay, az});
+            f[0] = px * a.kp;
+            f[1] = py * a.kp;
+            f[2] = pz * a.kp;
+            f[3] = nx;
+            f[4] = ny;
+            f[5] = nz;
+            f[6] = l * a.kl;
+            f[7] = ir;
+            f[8] = ig;
+            f[9] = ib;
+        } else {
+            // outside the image: a far-away guide position drives the weight to exactly 0
+            f[0] = f[1] = f[2] = (A)1e18;
+            f[3] = f[4] = f[5] = f[6] = f[7] = f[8] = f[9] = (A)0;
+        }
+#pragma unroll
+        for (int q = 0; q < 10; ++q) sh[q][r][c] = f[q];
+    }
+    __syncthreads();
+
+    // ---- gather ----
+    const int lc = threadIdx.x & 63;
+    const int r0 = (threadIdx.x >> 6) * R;
+    A cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R];
+    A Wt[R], Sx[R], Sy[R], Sz[R], cW[R], cX[R], cY[R], cZ[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int rr = r0 + k + RAD, cc = lc + RAD;
+        cpx[k] = sh[0][rr][cc];
+        cpy[k] = sh[1][rr][cc];
+        cpz[k] = sh[2][rr][cc];
+        cnx[k] = sh[3][rr][cc];
+        cny[k] = sh[4][rr][cc];
+        cnz[k] = sh[5][rr][cc];
+        cl[k] = sh[6][rr][cc];
+        Wt[k] = Sx[k] = Sy[k] = Sz[k] = cW[k] = cX[k] = cY[k] = cZ[k] = (A)0;
+    }
+    const A kn = a.kn;
+    for (int r = 0; r < R + 2 * RAD; ++r) {
+        A rw[R], rx[R], ry[R], rz[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) rw[k] = rx[k] = ry[k] = rz[k] = (A)0;
+        const int lr = r0 + r;
+#pragma unroll
+        for (int dx = 0; dx < K; ++dx) {
+            const int lcol = lc + dx;
+            const A npx = sh[0][lr][lcol], npy = sh[1][lr][lcol], npz = sh[2][lr][lcol];
+            const A nnx = sh[3][lr][lcol], nny = sh[4][lr][lcol], nnz = sh[5][lr][lcol];
+            const A nl = sh[6][lr][lcol];
+            const A nir = sh[7][lr][lcol], nig = sh[8][lr][lcol], nib = sh[9][lr][lcol];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (r - k >= 0 && r - k < K) {
+                    const A ex = cpx[k] - npx, ey = cpy[k] - npy, ez = cpz[k] - npz;
+                    A e = ex * ex;
+                    e = fma(ey, ey, e);
+                    e = fma(ez, ez, e);
+                    A nn = cnx[k] * nnx;
+                    nn = fma(cny[k], nny, nn);
+                    nn = fma(cnz[k], nnz, nn);
+                    const A tt = fma(-kn, nn, kn);
+                    e = fma(tt, tt, e);
+                    const A dl = cl[k] - nl;
+                    e = fma(dl, dl, e);
+                    const A w = exp2_neg(e);
+                    rw[k] += w;
+                    rx[k] = fma(w, nir, rx[k]);
+                    ry[k] = fma(w, nig, ry[k]);
+                    rz[k] = fma(w, nib, rz[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (r - k >= 0 && r - k < K) {
+                kahan_add(Wt[k], cW[k], rw[k]);
+                kahan_add(Sx[k], cX[k], rx[k]);
+                kahan_add(Sy[k], cY[k], ry[k]);
+                kahan_add(Sz[k], cZ[k], rz[k]);
+            }
+        }
+    }
+
+    // ---- epilogue: normalise, stylize, store ----
+    A* outp = a.out + ((size_t)s * a.out_levels + a.out_slot) * 3 * plane;
+    const S* recp = rec;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int ox = tx0 + lc, oy = ty0 + r0 + k;
+        if (ox < W && oy < H) {
+            const A inv = (A)1 / Wt[k];
+            v3<A> F{Sx[k] * inv, Sy[k] * inv, Sz[k] * inv};
+            const size_t o = (size_t)oy * W + ox;
+            v3<A> p{(A)0, (A)0, (A)0};
+            if (a.need_pos) {
+                p = v3<A>{(A)ld_rec(recp, o), (A)ld_rec(recp, plane + o), (A)ld_rec(recp, 2 * plane + o)};
+                if (anc) {
+                    const float* q = anc + ((size_t)(oy >> 3) * a.abx + (ox >> 3)) * 3;
+                    p.x += (A)q[0];
+                    p.y += (A)q[1];
+                    p.z += (A)q[2];
+                }
+            }
+            if (a.root >= 0) F = apply_style<A>(a.nodes, a.root, a.d, 0u, F, p);
+            outp[o] = F.x;
+            outp[plane + o] = F.y;
+            outp[2 * plane + o] = F.z;
+        }
+    }
+}
+
+// film[y][x][c] = mean over slices of out_0 (deterministic slice order)
+template <typename A>
+__global__ void screen_film_kernel(const A* out, uint32_t S, uint32_t out_levels, uint32_t slot0,
+                                   uint64_t plane, A* film) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane) return;
+    A acc[3] = {(A)0, (A)0, (A)0};
+    for (uint32_t s = 0; s < S; ++s) {
+        const A* o = out + ((size_t)s * out_levels + slot0) * 3 * plane;
+        acc[0] += o[i];
+        acc[1] += o[plane + i];
+        acc[2] += o[2 * plane + i];
+    }
+    const A inv = (A)1 / (A)S;
+    film[3 * i] = acc[0] * inv;
+    film[3 * i + 1] = acc[1] * inv;
+    film[3 * i + 2] = acc[2] * inv;
+}
+
+template <typename S, typename A, int RAD, int NW, int R>
+hipError_t launch_screen_level(const ScreenArgs<A>& a0, hipStream_t st) {
+    constexpr int TW = 64, TH = NW * R;
+    ScreenArgs<A> a = a0;
+    a.tiles_x = (a.W + TW - 1) / TW;
+    a.tiles_y = (a.H + TH - 1) / TH;
+    a.nblocks = a.tiles_x * a.tiles_y * a.S;
+    hipLaunchKernelGGL((screen_level_kernel<S, A, RAD, NW, R>), dim3(a.nblocks), dim3(64 * NW), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace nlpf

@@ -1,0 +1,358 @@
+"""Python host mirror of libnlpf's C ABI (ctypes).
+
+This is the binding a Python caller (tests, bench.py) uses; a Rust caller binds the same
+symbols through the `extern "C"` block shown in INTEGRATION.md.  It keeps the reference
+integrator's configuration surface: `Config` carries the fields and defaults of
+`lm_path_filtering::Config` (crates/akari_integrator/src/lm_path_filtering.rs:264-298).
+
+The library must be built (practical-stylized_amd/build.py); loading fails loudly otherwise —
+there is no CPU fallback.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnlpf.so")
+
+NLPF_OK = 0
+FP32, FP16, FP64 = 0, 1, 2
+PRECISIONS = {"fp32": FP32, "fp16": FP16, "fp64": FP64}
+SCREEN_PLANES = 15
+FP16_RADIANCE_SCALE = 64.0
+
+STY = dict(BASE=0, TOON=1, PALETTE=2, SATURATION=3, REX=4, EIGHT_BIT=5, HALFTONE=6,
+           HUE_SHIFT=7, RAMP=8, ENABLE_IF=9, SET_IDX=10)
+FLAG_DISABLED, FLAG_STYLE2, FLAG_STYLE3, FLAG_STYLE4 = 1, 2, 4, 8
+ALL_DEPTHS = 0xFFFFFFFF
+
+# nlpf_style_node, 256 bytes (include/nlpf.h)
+STYLE_NODE_DTYPE = np.dtype([
+    ("type", "<i4"), ("inner", "<i4"), ("else_", "<i4"), ("idx", "<u4"),
+    ("depth_mask", "<u4"), ("max_depth", "<u4"), ("count", "<u4"), ("reserved", "<u4"),
+    ("p", "<f4", (56,)),
+])
+assert STYLE_NODE_DTYPE.itemsize == 256
+
+# defaults of lm_path_filtering::Config (lm_path_filtering.rs:280-298) + screen-mode fields
+DEFAULT_CONFIG = {
+    "max_depth": 7, "repeats": 1, "branch": 1, "radius_scale": [0.1, 0.05],
+    "reuse": [128, 64], "vram": 2048, "smis": True, "no_first_hit_stylization": False,
+    "disable_flag_check": False, "cluster": True, "scene_bounds": None,
+    "mode": "world", "pixel_radius": 7, "sigma_pos": 0.05, "sigma_normal": 0.9,
+    "sigma_lum": 0.5, "precision": "fp32", "seed": 0,
+}
+
+
+class NlpfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nlpf error {code}: {msg}")
+        self.code = code
+
+
+class _ScreenDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("levels", C.c_uint32), ("precision", C.c_int32), ("radius", C.c_int32)]
+
+
+class _GenDesc(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("mu", C.c_float), ("sigma", C.c_float),
+                ("firefly_prob", C.c_float), ("firefly_scale", C.c_float),
+                ("albedo_lo", C.c_float), ("albedo_hi", C.c_float)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("level_ms", C.c_double * 32), ("film_ms", C.c_double),
+                ("vertices", C.c_uint64), ("bytes_algorithmic", C.c_uint64),
+                ("flops_algorithmic", C.c_double)]
+
+
+_lib = None
+
+# (name, restype, argtypes)
+_SIGS = [
+    ("nlpf_create", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("nlpf_destroy", C.c_int, [C.c_void_p]),
+    ("nlpf_last_error", C.c_char_p, [C.c_void_p]),
+    ("nlpf_get_config_json", C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    ("nlpf_set_sigmas", C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+    ("nlpf_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("nlpf_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("nlpf_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("nlpf_download", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("nlpf_memset", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]),
+    ("nlpf_sync", C.c_int, [C.c_void_p]),
+    ("nlpf_set_style", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+    ("nlpf_stylize", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                               C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("nlpf_screen_run", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    ("nlpf_get_stats", C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
+    ("nlpf_generate_screen_records", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc),
+                                               C.POINTER(_GenDesc), C.c_void_p, C.c_void_p]),
+    ("nlpf_convert_records_fp16", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p,
+                                            C.c_void_p, C.c_void_p]),
+]
+
+
+def exported_symbols():
+    return [s[0] for s in _SIGS]
+
+
+def load(path=LIB_PATH):
+    """Load libnlpf.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"libnlpf.so not built at {path}: run practical-stylized_amd/build.py")
+    lib = C.CDLL(path)
+    for name, res, args in _SIGS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceBuffer):
+        return C.c_void_p(x.ptr)
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    raise TypeError(type(x))
+
+
+class DeviceBuffer:
+    """A hipMalloc'd block owned by a Handle."""
+
+    def __init__(self, handle, nbytes):
+        self.handle = handle
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        handle._check(handle.lib.nlpf_alloc(handle.h, C.c_size_t(max(1, self.nbytes)), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr, offset=0):
+        arr = np.ascontiguousarray(arr)
+        assert offset + arr.nbytes <= self.nbytes, (offset, arr.nbytes, self.nbytes)
+        self.handle._check(self.handle.lib.nlpf_upload(
+            self.handle.h, C.c_void_p(self.ptr + offset), arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+        return self
+
+    def download(self, dtype, count=None, offset=0):
+        dtype = np.dtype(dtype)
+        count = (self.nbytes - offset) // dtype.itemsize if count is None else int(count)
+        assert offset + count * dtype.itemsize <= self.nbytes
+        out = np.empty(count, dtype)
+        self.handle._check(self.handle.lib.nlpf_download(
+            self.handle.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset), out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.handle.lib.nlpf_free(self.handle.h, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+
+class Handle:
+    """One NL-PF context (one HIP stream), configured by the reference's "pf" JSON."""
+
+    def __init__(self, config=None):
+        self.lib = load()
+        if config is None:
+            config = {}
+        js = config if isinstance(config, str) else json.dumps(config)
+        h = C.c_void_p()
+        rc = self.lib.nlpf_create(js.encode(), C.byref(h))
+        if rc != NLPF_OK:
+            raise NlpfError(rc, self.lib.nlpf_last_error(None).decode())
+        self.h = h
+        self._bufs = []
+
+    def _check(self, rc):
+        if rc != NLPF_OK:
+            raise NlpfError(rc, self.lib.nlpf_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            for b in self._bufs:
+                b.free()
+            self.lib.nlpf_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def config(self):
+        buf = C.create_string_buffer(4096)
+        self._check(self.lib.nlpf_get_config_json(self.h, buf, 4096))
+        return json.loads(buf.value.decode())
+
+    def set_sigmas(self, sp, sn, sl):
+        self._check(self.lib.nlpf_set_sigmas(self.h, sp, sn, sl))
+
+    def alloc(self, nbytes):
+        b = DeviceBuffer(self, nbytes)
+        self._bufs.append(b)
+        return b
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        return self.alloc(arr.nbytes).upload(arr)
+
+    def sync(self):
+        self._check(self.lib.nlpf_sync(self.h))
+
+    def set_style(self, nodes, roots):
+        nodes = np.ascontiguousarray(nodes, dtype=STYLE_NODE_DTYPE)
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        self._check(self.lib.nlpf_set_style(self.h, nodes.ctypes.data_as(C.c_void_p), len(nodes),
+                                            roots.ctypes.data_as(C.c_void_p), len(roots)))
+
+    def stylize(self, rgb, depth=0, flags=None, material=None, pos=None):
+        """Stylize host rgb[n,3] on the GPU; depth is an int or an array of n depths."""
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        n = len(rgb)
+        din = self.to_device(rgb)
+        dout = self.alloc(rgb.nbytes)
+        dd = None
+        d0 = 0
+        if np.ndim(depth) == 0:
+            d0 = int(depth)
+        else:
+            dd = self.to_device(np.asarray(depth, np.uint32))
+        df = None if flags is None else self.to_device(np.asarray(flags, np.uint8))
+        dm = None if material is None else self.to_device(np.asarray(material, np.uint16))
+        dp = None if pos is None else self.to_device(np.asarray(pos, np.float32).reshape(-1, 3))
+        self._check(self.lib.nlpf_stylize(self.h, _ptr(din), _ptr(dout), n, _ptr(dd), d0, _ptr(df),
+                                          _ptr(dm), _ptr(dp)))
+        out = dout.download(np.float32, 3 * n).reshape(n, 3)
+        for b in (din, dout, dd, df, dm, dp):
+            if b is not None:
+                b.free()
+        return out
+
+    # ---- screen mode ----
+    @staticmethod
+    def screen_desc(W, H, S, L, precision=FP32, radius=7):
+        return _ScreenDesc(W, H, S, L, precision, radius)
+
+    def screen_run(self, desc, records, anchors=None, film=None, level_out=None):
+        self._check(self.lib.nlpf_screen_run(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
+                                             _ptr(film), _ptr(level_out)))
+
+    def generate_screen_records(self, desc, records, anchors=None, seed=1, mu=-1.0, sigma=1.5,
+                                firefly_prob=1e-3, firefly_scale=1e3, albedo_lo=0.1, albedo_hi=0.9):
+        g = _GenDesc(seed, mu, sigma, firefly_prob, firefly_scale, albedo_lo, albedo_hi)
+        self._check(self.lib.nlpf_generate_screen_records(self.h, C.byref(desc), C.byref(g),
+                                                          _ptr(records), _ptr(anchors)))
+
+    def convert_records_fp16(self, desc, records32, records16, anchors):
+        self._check(self.lib.nlpf_convert_records_fp16(self.h, C.byref(desc), _ptr(records32),
+                                                       _ptr(records16), _ptr(anchors)))
+
+    def stats(self):
+        s = _Stats()
+        self._check(self.lib.nlpf_get_stats(self.h, C.byref(s)))
+        return {"total_ms": s.total_ms, "level_ms": list(s.level_ms), "film_ms": s.film_ms,
+                "vertices": s.vertices, "bytes_algorithmic": s.bytes_algorithmic,
+                "flops_algorithmic": s.flops_algorithmic}
+
+
+# ---- stylization node builders ----
+def node(kind, inner=-1, else_=-1, idx=0, depth_mask=ALL_DEPTHS, max_depth=0, count=0, params=()):
+    n = np.zeros((), STYLE_NODE_DTYPE)
+    n["type"] = STY[kind]
+    n["inner"] = inner
+    n["else_"] = else_
+    n["idx"] = idx
+    n["depth_mask"] = depth_mask
+    n["max_depth"] = max_depth
+    n["count"] = count
+    p = np.zeros(56, np.float32)
+    p[: len(params)] = params
+    n["p"] = p
+    return n
+
+
+def toon(width, inner=-1, depth_mask=ALL_DEPTHS):
+    return node("TOON", inner=inner, depth_mask=depth_mask, params=[width])
+
+
+def ramp(positions, values, inner=-1, depth_mask=ALL_DEPTHS):
+    positions = np.asarray(positions, np.float32)
+    values = np.asarray(values, np.float32)
+    assert len(positions) == len(values) <= 16
+    p = np.zeros(32, np.float32)
+    p[: len(positions)] = positions
+    p[16: 16 + len(values)] = values
+    return node("RAMP", inner=inner, depth_mask=depth_mask, count=len(positions), params=p)
+
+
+def hue_shift(shift, inner=-1, depth_mask=ALL_DEPTHS, indirect_scale=1.0):
+    return node("HUE_SHIFT", inner=inner, depth_mask=depth_mask, params=[shift, indirect_scale])
+
+
+def saturation(power, indirect_scale=1.0, max_depth=0, inner=-1, depth_mask=ALL_DEPTHS):
+    return node("SATURATION", inner=inner, depth_mask=depth_mask, max_depth=max_depth,
+                params=[power, indirect_scale])
+
+
+def rex(blend, inner=-1, depth_mask=ALL_DEPTHS):
+    return node("REX", inner=inner, depth_mask=depth_mask, params=[blend])
+
+
+def eight_bit(indirect_scale=1.0, inner=-1, depth_mask=ALL_DEPTHS):
+    return node("EIGHT_BIT", inner=inner, depth_mask=depth_mask, params=[indirect_scale])
+
+
+def palette_hex(entries):
+    """Palette hex strings -> rgb/255, as the SVM compiler does (svm/compiler.rs:420-453)."""
+    out = []
+    for e in entries:
+        x = int(e.replace("0x", ""), 16)
+        out.append([((x >> 16) & 0xFF) / 255.0, ((x >> 8) & 0xFF) / 255.0, (x & 0xFF) / 255.0])
+    return np.asarray(out, np.float32)
+
+
+def palette(colors, gamma=1.0, indirect_scale=1.0, inner=-1, depth_mask=ALL_DEPTHS):
+    colors = np.asarray(colors, np.float32).reshape(-1, 3)
+    assert len(colors) <= 16
+    return node("PALETTE", inner=inner, depth_mask=depth_mask, count=len(colors),
+                params=np.concatenate([[indirect_scale, gamma], colors.ravel()]))
+
+
+def halftone(indirect_scale, radius_scale, radius, grid_distance, blend, dark, light, affine,
+             max_depth=0, inner=-1, depth_mask=ALL_DEPTHS):
+    affine = np.asarray(affine, np.float32).reshape(3, 4)
+    params = np.concatenate([[indirect_scale, radius_scale, radius, grid_distance, blend],
+                             np.asarray(dark, np.float32), np.asarray(light, np.float32),
+                             affine.ravel()])
+    return node("HALFTONE", inner=inner, depth_mask=depth_mask, max_depth=max_depth, params=params)
+
+
+def enable_if(idx, inner=-1, else_=-1):
+    return node("ENABLE_IF", inner=inner, else_=else_, idx=idx)
+
+
+def set_idx(idx, inner=-1):
+    return node("SET_IDX", inner=inner, idx=idx)
+
+
+def depth_mask(*depths):
+    m = 0
+    for d in depths:
+        m |= 1 << min(d, 31)
+    return m
+
+
+def style_table(nodes):
+    return np.array(list(nodes), dtype=STYLE_NODE_DTYPE)

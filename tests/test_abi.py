@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every entry point include/nlpf.h declares; struct
+layouts of the Python mirror match the header; config errors are reported before any GPU
+work (no GPU needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import nlpf as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "nlpf.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nlpf_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = N.load()
+    names = header_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(N.exported_symbols()) == set(names)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    src = tmp_path / "sz.c"
+    src.write_text('#include "nlpf.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){'
+                   'printf("%zu %zu %zu %zu %zu\\n", sizeof(nlpf_style_node), sizeof(nlpf_screen_desc),'
+                   ' sizeof(nlpf_gen_desc), sizeof(nlpf_stats), offsetof(nlpf_style_node, p));return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    sizes = [int(v) for v in out]
+    assert sizes[0] == N.STYLE_NODE_DTYPE.itemsize == 256
+    assert sizes[1] == C.sizeof(N._ScreenDesc)
+    assert sizes[2] == C.sizeof(N._GenDesc)
+    assert sizes[3] == C.sizeof(N._Stats)
+    assert sizes[4] == N.STYLE_NODE_DTYPE.fields["p"][1]
+
+
+@pytest.mark.parametrize("cfg,msg", [
+    ("{", "config JSON"),
+    ('{"type":"pt"}', "type"),
+    ('{"max_depth":-1}', "max_depth"),
+    ('{"reuse":[]}', "reuse"),
+    ('{"radius_scale":[0]}', "radius_scale"),
+    ('{"branch":0}', "branch"),
+    ('{"smis":1}', "smis"),
+    ('{"scene_bounds":[[0,0,0],[1,1]]}', "scene_bounds"),
+    ('{"mode":"tiles"}', "mode"),
+    ('{"pixel_radius":9}', "pixel_radius"),
+    ('{"sigma_pos":0}', "sigma_pos"),
+    ('{"precision":"bf16"}', "precision"),
+])
+def test_invalid_config_rejected_before_gpu(cfg, msg):
+    lib = N.load()
+    h = C.c_void_p()
+    rc = lib.nlpf_create(cfg.encode(), C.byref(h))
+    assert rc == 1  # NLPF_E_INVALID_CONFIG
+    assert msg in lib.nlpf_last_error(None).decode()
+    assert not h.value
+
+
+def test_null_arguments_rejected():
+    lib = N.load()
+    assert lib.nlpf_create(b"{}", None) == 2
+    assert lib.nlpf_destroy(None) == 2
+    assert lib.nlpf_sync(None) == 2

@@ -1,0 +1,167 @@
+"""GPU parity of the screen-mode NL-PF (HIP kernels through the C ABI) against the FP64 CPU
+oracle on the same GPU-generated records.
+
+Tolerance: 1e-5 relative per pixel for FP32 and FP16 storage (FP16 records are compared on
+their decoded FP32 values, which is what the kernel computes with); 1e-10 for the FP64
+kernel.  Toon band edges: a 1-ulp difference flips a band, so a toon pixel may differ only
+where the oracle's pre-quantisation value lies within 1e-5 of a band edge (and, one level
+up, only inside the filter window of such a pixel)."""
+import numpy as np
+import pytest
+
+import nlpf as N
+import oracle as O
+from gpu_util import decode_fp16, gen_screen, host_records_fp32, pixel_rel_err, run_screen
+
+pytestmark = pytest.mark.gpu
+SIG = (0.05, 0.9, 0.5)
+
+
+@pytest.fixture(scope="module")
+def h():
+    with N.Handle({"type": "pf", "mode": "screen"}) as hh:
+        yield hh
+
+
+def ramp_hue_table(seed=7):
+    rng = np.random.default_rng(seed)
+    pos = np.sort(rng.random(16)).astype(np.float32)
+    val = np.sort(rng.random(16)).astype(np.float32)
+    return N.style_table([N.ramp(pos, val), N.hue_shift(0.1, inner=0)]), 1
+
+
+def test_config_echo(h):
+    cfg = h.config()
+    for k, v in N.DEFAULT_CONFIG.items():
+        if k == "mode":
+            assert cfg[k] == "screen"
+        elif isinstance(v, float):
+            assert abs(cfg[k] - v) < 1e-6, k
+        else:
+            assert cfg[k] == v, k
+
+
+def check_levels(gpu_lv, or_lv, tol):
+    err = pixel_rel_err(np.moveaxis(gpu_lv, 2, -1), np.moveaxis(or_lv, 2, -1))
+    assert err.max() <= tol, (err.max(), np.unravel_index(err.argmax(), err.shape))
+
+
+@pytest.mark.parametrize("radius,W,H,S,L", [(7, 150, 70, 2, 4), (3, 97, 61, 3, 3), (1, 64, 16, 1, 2)])
+def test_fp32_parity_ramp_hue(h, radius, W, H, S, L):
+    table, root = ramp_hue_table()
+    h.set_style(table, [root])
+    h.set_sigmas(*SIG)
+    desc, rec, _ = gen_screen(h, W, H, S, L, N.FP32, seed=radius, radius=radius)
+    film, lv = run_screen(h, desc, rec)
+    host = host_records_fp32(rec, W, H, S, L)
+    of, olv = O.screen_run(host, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+    check_levels(lv, olv, 1e-5)
+    assert pixel_rel_err(film, of).max() <= 1e-5
+    assert np.isfinite(film).all()
+    rec.free()
+
+
+def test_fp32_parity_unstylized_and_determinism(h):
+    h.set_style(N.style_table([N.toon(0.25)]), [-1])
+    W, H, S, L = 130, 50, 2, 3
+    desc, rec, _ = gen_screen(h, W, H, S, L, N.FP32, seed=11)
+    film, lv = run_screen(h, desc, rec)
+    film2, lv2 = run_screen(h, desc, rec)
+    assert np.array_equal(film, film2) and np.array_equal(lv, lv2)  # bitwise deterministic
+    host = host_records_fp32(rec, W, H, S, L)
+    of, olv = O.screen_run(host, W, H, S, L, 7, SIG, want_levels=True)
+    check_levels(lv, olv, 1e-5)
+    rec.free()
+
+
+def test_fp16_parity_on_decoded_records(h):
+    table, root = ramp_hue_table(3)
+    h.set_style(table, [root])
+    W, H, S, L = 110, 45, 2, 3
+    desc, rec, anc = gen_screen(h, W, H, S, L, N.FP16, seed=5)
+    film, lv = run_screen(h, desc, rec, anc)
+    dec = decode_fp16(rec, anc, W, H, S, L)
+    of, olv = O.screen_run(dec, W, H, S, L, 7, SIG, nodes=table, root=root, want_levels=True)
+    check_levels(lv, olv, 1e-5)
+    assert pixel_rel_err(film, of).max() <= 1e-5
+    rec.free()
+    anc.free()
+
+
+def test_fp64_kernel_parity(h):
+    table, root = ramp_hue_table(4)
+    h.set_style(table, [root])
+    W, H, S, L = 80, 40, 2, 3
+    desc32, rec32, _ = gen_screen(h, W, H, S, L, N.FP32, seed=9)
+    host = host_records_fp32(rec32, W, H, S, L)
+    rec64 = h.to_device(host.astype(np.float64))
+    desc64 = h.screen_desc(W, H, S, L, N.FP64, 7)
+    film, lv = run_screen(h, desc64, rec64)
+    of, olv = O.screen_run(host, W, H, S, L, 7, SIG, nodes=table, root=root, want_levels=True)
+    check_levels(lv, olv, 1e-10)
+    assert pixel_rel_err(film, of).max() <= 1e-10
+    rec32.free()
+    rec64.free()
+
+
+def test_toon_at_depth1_band_edges(h):
+    """Config-1 style: 4-level toon at depth 1 only, 7x7 window."""
+    table = N.style_table([N.toon(0.25, depth_mask=N.depth_mask(1))])
+    h.set_style(table, [0])
+    W, H, S, L, R = 96, 64, 2, 4, 3
+    desc, rec, _ = gen_screen(h, W, H, S, L, N.FP32, seed=21, radius=R)
+    film, lv = run_screen(h, desc, rec)
+    host = host_records_fp32(rec, W, H, S, L)
+    of, olv = O.screen_run(host, W, H, S, L, R, SIG, nodes=table, root=0, want_levels=True)
+    # pre-toon values at depth 1: same run with the toon switched off at depth 1 only gives
+    # identical deeper levels, and its depth-1 output is the quantiser's input
+    off = N.style_table([N.toon(0.25, depth_mask=0)])
+    _, plv = O.screen_run(host, W, H, S, L, R, SIG, nodes=off, root=0, want_levels=True)
+    check_levels(lv[:, 2:], olv[:, 2:], 1e-5)
+    pre = np.moveaxis(plv[:, 1], 1, -1)  # (S, H, W, 3)
+    lum = pre @ np.array([0.2126, 0.7152, 0.0722])
+    enc = np.where(lum > 1, lum / 0.25, np.where(lum <= 0.0031308, lum * 12.92,
+                                                   1.055 * np.power(np.maximum(lum, 0), 1 / 2.4) - 0.055) / 0.25)
+    near_edge = np.abs(enc - np.round(enc)) < 1e-5 * np.maximum(1, np.abs(enc))
+    e1 = pixel_rel_err(np.moveaxis(lv[:, 1], 1, -1), np.moveaxis(olv[:, 1], 1, -1))
+    bad1 = e1 > 1e-5
+    assert not (bad1 & ~near_edge).any()
+    # depth 0 may differ only within R pixels of a flipped depth-1 pixel of the same slice
+    from scipy.ndimage import binary_dilation
+    allowed = np.stack([binary_dilation(bad1[s], iterations=R, structure=np.ones((3, 3))) for s in range(S)])
+    e0 = pixel_rel_err(np.moveaxis(lv[:, 0], 1, -1), np.moveaxis(olv[:, 0], 1, -1))
+    assert not ((e0 > 1e-5) & ~allowed).any()
+    # and only a tiny fraction flips at all
+    assert bad1.mean() < 1e-3
+    rec.free()
+
+
+def test_fp16_error_against_fp64_oracle(h):
+    """FP16 storage / FP32 accumulation vs the FP64 oracle on the original FP32 records."""
+    table, root = ramp_hue_table(5)
+    h.set_style(table, [root])
+    W, H, S, L = 128, 72, 2, 3
+    desc32, rec32, _ = gen_screen(h, W, H, S, L, N.FP32, seed=33)
+    host = host_records_fp32(rec32, W, H, S, L)
+    d16 = h.screen_desc(W, H, S, L, N.FP16, 7)
+    rec16 = h.alloc(rec32.nbytes // 2)
+    anc = h.alloc(S * L * ((W + 7) // 8) * ((H + 7) // 8) * 3 * 4)
+    h.convert_records_fp16(desc32, rec32, rec16, anc)
+    film16, _ = run_screen(h, d16, rec16, anc, want_levels=False)
+    of = O.screen_run(host, W, H, S, L, 7, SIG, nodes=table, root=root)
+    rel_l2 = np.linalg.norm(film16 - of) / np.linalg.norm(of)
+    assert rel_l2 < 1e-3, rel_l2
+    for b in (rec32, rec16, anc):
+        b.free()
+
+
+def test_invalid_screen_arguments(h):
+    desc = h.screen_desc(0, 4, 1, 1)
+    with pytest.raises(N.NlpfError):
+        h.screen_run(desc, 1234, None, 5678, None)
+    desc = h.screen_desc(8, 8, 1, 1, N.FP32, 9)
+    with pytest.raises(N.NlpfError):
+        h.screen_run(desc, 1234, None, 5678, None)
+    desc = h.screen_desc(8, 8, 1, 1, N.FP16, 3)
+    with pytest.raises(N.NlpfError):  # FP16 without anchors
+        h.screen_run(desc, 1234, None, 5678, None)
