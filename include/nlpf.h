@@ -184,20 +184,60 @@ typedef struct nlpf_stats {
     uint64_t vertices;
     uint64_t bytes_algorithmic;
     double flops_algorithmic;
+    /* world mode: points gathered and neighbours visited per depth (the reference logs the
+     * average, lm_path_filtering.rs:1767-1773) */
+    uint64_t points[32];
+    uint64_t neighbors[32];
 } nlpf_stats;
 int nlpf_get_stats(const nlpf_handle* h, nlpf_stats* out);
 
-/* ---- synthetic records (procedural unit box + 2 spheres + quad light) ----
- * Traces one path per (pixel, spp slice) through the analytic scene and writes screen-mode
- * records: p, n from the hits; albedo U[0.1,0.9]; radiance lognormal(mu, sigma) with a
- * `firefly_prob` fraction multiplied by `firefly_scale`; throughput U[0,1]; all from PCG32
- * streams keyed by (seed, record index). FP16 output also writes the anchors. */
+/* Parameters of the synthetic record generators below. */
 typedef struct nlpf_gen_desc {
     uint64_t seed;
     float mu, sigma;
     float firefly_prob, firefly_scale;
     float albedo_lo, albedo_hi;
 } nlpf_gen_desc;
+
+/* ---- world mode: the reference's hash-grid NL-PF for Lambertian records ----
+ * Replaces PathFiltering::render's filter loop (lm_path_filtering.rs:1673-1788) for records
+ * traced elsewhere (kernel_mk_trace :1062-1218 or nlpf_generate_world_records).
+ * N = width*height*branch paths, path t = b*width*height + pixel.  Float planes per level d,
+ * layout records[(d*NLPF_WORLD_PLANES + c)*N + t]:
+ *   0-2 p, 3-5 ns (shading normal), 6-8 ng (geometric normal), 9-11 wo, 12-14 albedo R
+ *   (diffuse reflectance, f = R/pi |cos|), 15-17 le, 18 mis (emission MIS weight),
+ *   19-21 direct.wi, 22-24 direct.irradiance, 25 direct.pdf, 26 direct.mis, 27 bsdf_pdf.
+ * flags / new_flags: StylizationFlags bytes [L][N]; material: stylization material [L][N];
+ * path_depth[t]: last depth with a vertex, -1 for a path whose camera ray missed.
+ * The indirect record of depth d is produced by the filter of depth d+1 (kernel_propagate,
+ * :973-998).  Output film_rgb [H][W][3] = film after kernel_splat + kernel_update_film
+ * (:428-506, depth-0 re-stylization of the spp mean included); radiance_out (optional)
+ * [L][N][3] = every vertex's filtered, stylized radiance (v.radiance). Config fields used:
+ * max_depth, branch, radius_scale, reuse, vram (batching), smis, cluster,
+ * no_first_hit_stylization, disable_flag_check, scene_bounds, seed, hash_table_size. */
+#define NLPF_WORLD_PLANES 28
+typedef struct nlpf_world_desc {
+    uint32_t width, height, branch, levels;
+    float aabb_min[3], aabb_max[3]; /* scene AABB, used unless the config sets scene_bounds */
+} nlpf_world_desc;
+int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const float* records,
+                   const uint8_t* flags, const uint8_t* new_flags, const uint16_t* material,
+                   const int32_t* path_depth, float* film_rgb, float* radiance_out);
+
+/* Trace the procedural scene (below) into world-mode records, one path per (pixel, branch),
+ * as kernel_mk_trace does: camera ray, NEE to the quad light with its radiance stylized at
+ * depth+1 by material `light material`, cosine BSDF sampling, StylizationFlags updated by
+ * each hit's material root. Materials: 0 floor 1 ceiling 2 back 3 left 4 right 5 sphere0
+ * 6 sphere1 7 light; wall/sphere albedos U[albedo_lo, albedo_hi] per material from `seed`. */
+int nlpf_generate_world_records(nlpf_handle* h, const nlpf_world_desc* desc,
+                                const nlpf_gen_desc* gen, float* records, uint8_t* flags,
+                                uint8_t* new_flags, uint16_t* material, int32_t* path_depth);
+
+/* ---- synthetic records (procedural unit box + 2 spheres + quad light) ----
+ * Traces one path per (pixel, spp slice) through the analytic scene and writes screen-mode
+ * records: p, n from the hits; albedo U[0.1,0.9]; radiance lognormal(mu, sigma) with a
+ * `firefly_prob` fraction multiplied by `firefly_scale`; throughput U[0,1]; all from PCG32
+ * streams keyed by (seed, record index). FP16 output also writes the anchors. */
 int nlpf_generate_screen_records(nlpf_handle* h, const nlpf_screen_desc* desc,
                                  const nlpf_gen_desc* gen, void* records, float* anchors);
 /* Convert FP32 records to FP16 records + anchors (same layout rules as above). */

@@ -48,4 +48,14 @@ int or_screen_run_f32(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius
                       double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
                       float* film, float* level_out, int threads);
 
+/* World-mode NL-PF (cluster = false), FP64; see oracle_world.c. */
+int or_world_neighbor_counts(const float* pos, uint32_t n, const float pmin[3], const float pmax[3], float radius,
+                             uint64_t table, uint32_t* out_count);
+int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const float aabb_min[3],
+                 const float aabb_max[3], const float* rec, const uint8_t* flags, const uint8_t* new_flags,
+                 const uint16_t* material, const int32_t* path_depth, const float* radius_scale, uint32_t n_rs,
+                 const uint32_t* reuse, uint32_t n_reuse, uint64_t vram_mb, int smis, int no_first_hit_stylization,
+                 int disable_flag_check, uint64_t seed, uint64_t table, const or_node* nodes, const int32_t* roots,
+                 uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out);
+
 #endif

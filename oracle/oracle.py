@@ -36,6 +36,11 @@ def load():
         fn = getattr(lib, name)
         fn.argtypes = [u32, u32, u32, u32, i32, dbl, dbl, dbl, vp, vp, i32, vp, vp, i32]
         fn.restype = i32
+    lib.or_world_neighbor_counts.argtypes = [vp, u32, vp, vp, C.c_float, u64, vp]
+    lib.or_world_neighbor_counts.restype = i32
+    lib.or_world_run.argtypes = [u32, u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, u32, u64, i32, i32,
+                                 i32, u64, u64, vp, vp, u32, vp, vp, vp]
+    lib.or_world_run.restype = i32
     _lib = lib
     return lib
 
@@ -97,3 +102,46 @@ def screen_run(records, W, H, S, L, radius, sigmas, nodes=None, root=-1, precisi
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return (film, lv) if want_levels else film
+
+
+def world_neighbor_counts(pos, pmin, pmax, radius, table):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    out = np.empty(len(pos), np.uint32)
+    pmin = np.ascontiguousarray(pmin, np.float32)
+    pmax = np.ascontiguousarray(pmax, np.float32)
+    rc = load().or_world_neighbor_counts(_p(pos), len(pos), _p(pmin), _p(pmax), float(radius), int(table), _p(out))
+    assert rc == 0
+    return out
+
+
+def world_run(rec, flags, new_flags, material, path_depth, W, H, branch, L, aabb_min, aabb_max, cfg,
+              nodes=None, roots=(), want_radiance=False):
+    """cfg: dict with the "pf" fields (radius_scale, reuse, vram, smis, no_first_hit_stylization,
+    disable_flag_check, seed, hash_table_size)."""
+    lib = load()
+    N = W * H * branch
+    rec = np.ascontiguousarray(rec, np.float32)
+    assert rec.size == L * 28 * N
+    film = np.empty((H, W, 3), np.float64)
+    rad = np.empty((L, N, 3), np.float64) if want_radiance else None
+    nbr = np.zeros(32, np.uint64)
+    rs = np.ascontiguousarray(cfg.get("radius_scale", [0.1, 0.05]), np.float32)
+    ru = np.ascontiguousarray(cfg.get("reuse", [128, 64]), np.uint32)
+    if nodes is None or len(nodes) == 0:
+        nodes_arr = np.zeros(1, dtype=[("b", "V256")])
+        roots = np.full(1, -1, np.int32)
+    else:
+        nodes_arr = np.ascontiguousarray(nodes)
+        roots = np.ascontiguousarray(roots, np.int32)
+    rc = lib.or_world_run(W, H, branch, L, _p(np.ascontiguousarray(aabb_min, np.float32)),
+                          _p(np.ascontiguousarray(aabb_max, np.float32)), _p(rec),
+                          _p(np.ascontiguousarray(flags, np.uint8)), _p(np.ascontiguousarray(new_flags, np.uint8)),
+                          _p(np.ascontiguousarray(material, np.uint16)),
+                          _p(np.ascontiguousarray(path_depth, np.int32)), _p(rs), len(rs), _p(ru), len(ru),
+                          int(cfg.get("vram", 2048)), int(cfg.get("smis", True)),
+                          int(cfg.get("no_first_hit_stylization", False)), int(cfg.get("disable_flag_check", False)),
+                          int(cfg.get("seed", 0)), int(cfg.get("hash_table_size", 64 * 1024 * 1024)),
+                          _p(nodes_arr), _p(roots), len(roots), _p(film), _p(rad), _p(nbr))
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return (film, rad, nbr) if want_radiance else film

@@ -1,0 +1,336 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — world-mode NL-PF oracle (CPU, FP64).  Loaded only by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker.
+ *
+ * Restated from crates/akari_integrator/src/lm_path_filtering.rs:
+ *   hash / get_cell / for_each_neighbor      :20-123 (27 cells, dz-dy-dx order, every bucket
+ *                                                     walked, bucket collisions revisit entries)
+ *   mis_weight (balance heuristic)            :305-316
+ *   evaluate_filtered_radiance                :757-972 (reservoir :813-821, direct :839-863,
+ *                                                     indirect + SMIS :864-945, stylize :948-971)
+ *   kernel_filter / kernel_propagate          :999-1060, :973-998
+ *   depth loop, radius / resolution           :1563-1598, :1673-1774
+ *   kernel_splat / kernel_update_film         :479-506, :428-477 (+ film.rs:196-229)
+ * with DiffuseBsdf (svm/surface/diffuse.rs:22-41) behind SurfaceClosure::evaluate
+ * (svm/surface/mod.rs:1701-1744) and Pcg32 (sampler/mod.rs:79-137).
+ *
+ * Determinism choices shared with the library (the reference's order is scheduling-dependent):
+ * grid entries of one bucket are visited in ascending path id; the reservoir RNG of the
+ * thread handling the t-th point of a batch is Pcg32 new_seq_offset(t, splitmix64(seed)),
+ * carried across depths.  Clustering is not restated here (cluster = false only).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+#define NP 28
+#define INV_PI 0.318309886183790671538
+
+typedef struct {
+    uint64_t state, inc;
+} pcg_t;
+static uint32_t pcg_next(pcg_t* g) {
+    uint64_t old = g->state;
+    g->state = old * 0x5851f42d4c957f2dull + g->inc;
+    uint32_t xs = (uint32_t)(((old >> 18) ^ old) >> 27);
+    uint32_t rot = (uint32_t)(old >> 59);
+    return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+}
+static pcg_t pcg_seq(uint64_t seq, uint64_t off) {
+    pcg_t g = {0, (seq << 1) | 1};
+    pcg_next(&g);
+    g.state += off;
+    pcg_next(&g);
+    return g;
+}
+static uint64_t splitmix(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+typedef struct {
+    const float* rec;
+    uint64_t N;
+} lvl_t;
+static double F(lvl_t l, int c, uint64_t i) { return l.rec[(uint64_t)c * l.N + i]; }
+
+static int valid_dir(const double ns[3], const double ng[3], const double wo[3], const double wi[3]) {
+#define SG(x) ((x) > 0 ? 1.0 : -1.0)
+#define D3(a, b) ((a)[0] * (b)[0] + (a)[1] * (b)[1] + (a)[2] * (b)[2])
+    double fl = SG(D3(ng, ns));
+    return SG(fl * D3(wo, ns)) * SG(D3(wo, ng)) > 0 && SG(fl * D3(wi, ns)) * SG(D3(wi, ng)) > 0;
+}
+/* returns pdf; *fcos = |cos_i| / pi when the pair is valid and same-hemisphere */
+static double diffuse(const double ns[3], const double ng[3], const double wo[3], const double wi[3], double* fcos) {
+    *fcos = 0;
+    if (!valid_dir(ns, ng, wo, wi)) return 0;
+    double co = D3(wo, ns), ci = D3(wi, ns);
+    if (!(co * ci >= 0)) return 0;
+    *fcos = fabs(ci) * INV_PI;
+    return *fcos;
+}
+
+static uint32_t bucket_of(const float pmin[3], const float pmax[3], const uint32_t res[3], uint64_t table,
+                          const int32_t cell[3]) {
+    (void)pmin;
+    (void)pmax;
+    (void)res;
+    uint64_t h = ((uint64_t)(uint32_t)cell[0] * 73856093ull) ^ ((uint64_t)(uint32_t)cell[1] * 19349663ull) ^
+                 ((uint64_t)(uint32_t)cell[2] * 83492791ull);
+    return (uint32_t)(h % table);
+}
+static void cell_of(const float pmin[3], const float pmax[3], const uint32_t res[3], float px, float py, float pz,
+                    int32_t c[3]) {
+    float p[3] = {px, py, pz};
+    for (int i = 0; i < 3; ++i) {
+        float t = (p[i] - pmin[i]) / (pmax[i] - pmin[i]);
+        t = (t < 0 ? 0 : (t > 1 ? 1 : t)) * (float)res[i];
+        uint32_t u = (uint32_t)t;
+        c[i] = (int32_t)(u < res[i] - 1 ? u : res[i] - 1);
+    }
+}
+
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* Neighbour multiset of every point of one level: for tests of the grid itself.
+ * out_count[t] = number of entries visited within the radius (duplicates included). */
+int or_world_neighbor_counts(const float* pos, uint32_t n, const float pmin[3], const float pmax[3], float radius,
+                             uint64_t table, uint32_t* out_count) {
+    uint32_t res[3];
+    for (int i = 0; i < 3; ++i) res[i] = (uint32_t)floorf((pmax[i] - pmin[i]) / radius);
+    uint64_t* kv = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1));
+    if (!kv) return -1;
+    for (uint32_t t = 0; t < n; ++t) {
+        int32_t c[3];
+        cell_of(pmin, pmax, res, pos[3 * t], pos[3 * t + 1], pos[3 * t + 2], c);
+        kv[t] = ((uint64_t)bucket_of(pmin, pmax, res, table, c) << 32) | t;
+    }
+    qsort(kv, n, sizeof(uint64_t), cmp_u64);
+    for (uint32_t t = 0; t < n; ++t) {
+        int32_t c[3];
+        cell_of(pmin, pmax, res, pos[3 * t], pos[3 * t + 1], pos[3 * t + 2], c);
+        uint32_t cnt = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int32_t nc[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+                    if (nc[0] < 0 || nc[1] < 0 || nc[2] < 0 || nc[0] >= (int)res[0] || nc[1] >= (int)res[1] ||
+                        nc[2] >= (int)res[2])
+                        continue;
+                    uint64_t b = bucket_of(pmin, pmax, res, table, nc);
+                    /* lower bound of bucket b */
+                    uint32_t lo = 0, hi = n;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) / 2;
+                        if ((kv[mid] >> 32) < b) lo = mid + 1; else hi = mid;
+                    }
+                    for (uint32_t e = lo; e < n && (kv[e] >> 32) == b; ++e) {
+                        uint32_t j = (uint32_t)kv[e];
+                        float ddx = pos[3 * j] - pos[3 * t], ddy = pos[3 * j + 1] - pos[3 * t + 1],
+                              ddz = pos[3 * j + 2] - pos[3 * t + 2];
+                        if (sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) <= radius) ++cnt;
+                    }
+                }
+        out_count[t] = cnt;
+    }
+    free(kv);
+    return 0;
+}
+
+/*
+ * Whole world-mode pass.  Arguments mirror nlpf_world_run; cfg_* carry the "pf" config.
+ * film [H][W][3] (double), radiance_out optional [L][N][3] (double).
+ */
+int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const float aabb_min[3],
+                 const float aabb_max[3], const float* rec, const uint8_t* flags, const uint8_t* new_flags,
+                 const uint16_t* material, const int32_t* path_depth, const float* radius_scale, uint32_t n_rs,
+                 const uint32_t* reuse, uint32_t n_reuse, uint64_t vram_mb, int smis, int no_first_hit_stylization,
+                 int disable_flag_check, uint64_t seed, uint64_t table, const or_node* nodes, const int32_t* roots,
+                 uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out) {
+    const uint64_t npix = (uint64_t)W * H, N = npix * branch;
+    double* ind = (double*)calloc(8 * N, sizeof(double));
+    double* rad = (double*)calloc(3 * N * L, sizeof(double));
+    uint32_t* pts = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    uint64_t* kv = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    if (!ind || !rad || !pts || !kv) return -1;
+    uint32_t kmax = 1;
+    for (uint32_t d = 0; d < L; ++d) {
+        uint32_t K = d < n_reuse ? reuse[d] : reuse[n_reuse - 1];
+        if (K > kmax) kmax = K;
+    }
+    uint64_t batch_max = 1;
+    for (uint32_t d = 0; d < L; ++d) {
+        uint64_t K = d < n_reuse ? reuse[d] : reuse[n_reuse - 1];
+        uint64_t b = vram_mb * 1024 * 1024 / (K * 4);
+        if (b < 1) b = 1;
+        if (b > batch_max) batch_max = b;
+    }
+    if (batch_max > N) batch_max = N;
+    pcg_t* rng = (pcg_t*)malloc(sizeof(pcg_t) * batch_max);
+    uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * kmax);
+    if (!rng || !buf) return -1;
+    const uint64_t key = splitmix(seed);
+    for (uint64_t t = 0; t < batch_max; ++t) rng[t] = pcg_seq(t, key);
+    float ext[3];
+    for (int i = 0; i < 3; ++i) ext[i] = aabb_max[i] - aabb_min[i];
+    const float avg = (ext[0] + ext[1] + ext[2]) / 3.0f;
+    for (int d = (int)L - 1; d >= 0; --d) {
+        lvl_t lv = {rec + (uint64_t)d * NP * N, N};
+        uint32_t n = 0;
+        for (uint64_t t = 0; t < N; ++t)
+            if (path_depth[t] >= d) pts[n++] = (uint32_t)t;
+        const float radius = avg * 0.01f * (d < (int)n_rs ? radius_scale[d] : radius_scale[n_rs - 1]);
+        uint32_t res[3];
+        for (int i = 0; i < 3; ++i) res[i] = (uint32_t)floorf(ext[i] / radius);
+        for (uint32_t e = 0; e < n; ++e) {
+            int32_t c[3];
+            uint32_t i = pts[e];
+            cell_of(aabb_min, aabb_max, res, (float)F(lv, 0, i), (float)F(lv, 1, i), (float)F(lv, 2, i), c);
+            kv[e] = ((uint64_t)bucket_of(aabb_min, aabb_max, res, table, c) << 32) | i;
+        }
+        qsort(kv, n, sizeof(uint64_t), cmp_u64);
+        const uint32_t K = d < (int)n_reuse ? reuse[d] : reuse[n_reuse - 1];
+        uint64_t batch = vram_mb * 1024 * 1024 / ((uint64_t)K * 4);
+        if (batch < 1) batch = 1;
+        uint64_t nbr_total = 0;
+        double* radd = rad + (uint64_t)d * 3 * N;
+        for (uint32_t e = 0; e < n; ++e) {
+            const uint32_t c = pts[e];
+            const uint32_t tl = (uint32_t)(e % batch);
+            const float qx = (float)F(lv, 0, c), qy = (float)F(lv, 1, c), qz = (float)F(lv, 2, c);
+            const double ns[3] = {F(lv, 3, c), F(lv, 4, c), F(lv, 5, c)};
+            const double ng[3] = {F(lv, 6, c), F(lv, 7, c), F(lv, 8, c)};
+            const double wo[3] = {F(lv, 9, c), F(lv, 10, c), F(lv, 11, c)};
+            const double R[3] = {F(lv, 12, c), F(lv, 13, c), F(lv, 14, c)};
+            const uint8_t myf = new_flags[(uint64_t)d * N + c];
+            int32_t cell[3];
+            cell_of(aabb_min, aabb_max, res, qx, qy, qz, cell);
+            uint32_t cnt = 0;
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int32_t nc[3] = {cell[0] + dx, cell[1] + dy, cell[2] + dz};
+                        if (nc[0] < 0 || nc[1] < 0 || nc[2] < 0 || nc[0] >= (int)res[0] || nc[1] >= (int)res[1] ||
+                            nc[2] >= (int)res[2])
+                            continue;
+                        uint64_t b = bucket_of(aabb_min, aabb_max, res, table, nc);
+                        uint32_t lo = 0, hi = n;
+                        while (lo < hi) {
+                            uint32_t mid = (lo + hi) / 2;
+                            if ((kv[mid] >> 32) < b) lo = mid + 1; else hi = mid;
+                        }
+                        for (uint32_t k = lo; k < n && (kv[k] >> 32) == b; ++k) {
+                            uint32_t j = (uint32_t)kv[k];
+                            float ddx = (float)F(lv, 0, j) - qx, ddy = (float)F(lv, 1, j) - qy,
+                                  ddz = (float)F(lv, 2, j) - qz;
+                            if (!(sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) <= radius)) continue;
+                            if (!disable_flag_check && new_flags[(uint64_t)d * N + j] != myf) continue;
+                            if (cnt < K) {
+                                buf[cnt] = j;
+                            } else {
+                                uint32_t r = pcg_next(&rng[tl]) % (cnt + 1);
+                                if (r < K) buf[r] = j;
+                            }
+                            ++cnt;
+                        }
+                    }
+            nbr_total += cnt;
+            const uint32_t used = cnt < K ? cnt : K;
+            double direct[3] = {0, 0, 0}, indirect[3] = {0, 0, 0};
+            for (uint32_t k = 0; k < used; ++k) {
+                const uint32_t j = buf[k];
+                const double dpdf = F(lv, 25, j);
+                if (dpdf > 0) {
+                    double wi[3] = {F(lv, 19, j), F(lv, 20, j), F(lv, 21, j)}, fc;
+                    double bp = diffuse(ns, ng, wo, wi, &fc);
+                    double w = dpdf / (dpdf + bp);
+                    for (int ch = 0; ch < 3; ++ch)
+                        direct[ch] += R[ch] * fc * F(lv, 22 + ch, j) * F(lv, 26, j) * w / dpdf / (double)used;
+                }
+                double wi[3] = {ind[j], ind[N + j], ind[2 * N + j]};
+                if (sqrt(wi[0] * wi[0] + wi[1] * wi[1] + wi[2] * wi[2]) == 0) continue;
+                double fc;
+                double bp = diffuse(ns, ng, wo, wi, &fc);
+                double reused[3];
+                for (int ch = 0; ch < 3; ++ch) reused[ch] = R[ch] * fc * ind[(3 + ch) * N + j] * ind[6 * N + j];
+                if (smis) {
+                    double sum = 0;
+                    for (uint32_t m = 0; m < used; ++m) {
+                        uint32_t o = buf[m];
+                        double ons[3] = {F(lv, 3, o), F(lv, 4, o), F(lv, 5, o)};
+                        double ong[3] = {F(lv, 6, o), F(lv, 7, o), F(lv, 8, o)};
+                        double owo[3] = {F(lv, 9, o), F(lv, 10, o), F(lv, 11, o)}, f2;
+                        sum += diffuse(ons, ong, owo, wi, &f2);
+                    }
+                    for (int ch = 0; ch < 3; ++ch) indirect[ch] += reused[ch] / sum;
+                } else if (bp > 0) {
+                    for (int ch = 0; ch < 3; ++ch) indirect[ch] += reused[ch] / bp / (double)used;
+                }
+            }
+            double Lr[3];
+            for (int ch = 0; ch < 3; ++ch) Lr[ch] = F(lv, 15 + ch, c) + direct[ch] + indirect[ch];
+            if (d > 0 || !no_first_hit_stylization) {
+                uint16_t m = material[(uint64_t)d * N + c];
+                int root = m < n_materials ? roots[m] : -1;
+                double p[3] = {F(lv, 0, c), F(lv, 1, c), F(lv, 2, c)};
+                if (root >= 0) or_apply_style(nodes, root, (uint32_t)d, myf, Lr, p, Lr);
+            }
+            for (int ch = 0; ch < 3; ++ch) radd[3 * (uint64_t)c + ch] = Lr[ch];
+        }
+        if (nbr_out) nbr_out[d] = nbr_total;
+        /* propagate to depth d-1 */
+        memset(ind, 0, sizeof(double) * 8 * N);
+        if (d > 0) {
+            lvl_t pv = {rec + (uint64_t)(d - 1) * NP * N, N};
+            for (uint32_t e = 0; e < n; ++e) {
+                uint32_t i = pts[e];
+                for (int ch = 0; ch < 3; ++ch) {
+                    ind[ch * N + i] = -F(lv, 9 + ch, i);
+                    ind[(3 + ch) * N + i] = radd[3 * (uint64_t)i + ch];
+                }
+                ind[6 * N + i] = F(lv, 18, i);
+                ind[7 * N + i] = F(pv, 27, i);
+            }
+        }
+    }
+    /* splat + update_film */
+    lvl_t l0 = {rec, N};
+    for (uint64_t px = 0; px < npix; ++px) {
+        double s[3] = {0, 0, 0}, acc[3] = {0, 0, 0}, wsum = 0;
+        for (uint32_t b = 0; b < branch; ++b) {
+            uint64_t t = (uint64_t)b * npix + px;
+            if (path_depth[t] < 0) continue;
+            for (int ch = 0; ch < 3; ++ch) s[ch] += rad[3 * t + ch];
+        }
+        for (uint32_t b = 0; b < branch; ++b) {
+            uint64_t t = (uint64_t)b * npix + px;
+            if (path_depth[t] < 0) continue;
+            double Lr[3] = {s[0] / branch, s[1] / branch, s[2] / branch};
+            if (!no_first_hit_stylization) {
+                uint16_t m = material[t];
+                int root = m < n_materials ? roots[m] : -1;
+                double p[3] = {F(l0, 0, t), F(l0, 1, t), F(l0, 2, t)};
+                if (root >= 0) or_apply_style(nodes, root, 0u, flags[t], Lr, p, Lr);
+            }
+            if (isnan(Lr[0]) || isnan(Lr[1]) || isnan(Lr[2])) Lr[0] = Lr[1] = Lr[2] = 0;
+            for (int ch = 0; ch < 3; ++ch) acc[ch] += Lr[ch];
+            wsum += 1;
+        }
+        for (int ch = 0; ch < 3; ++ch) film[3 * px + ch] = wsum > 0 ? acc[ch] / wsum : 0;
+    }
+    if (radiance_out) memcpy(radiance_out, rad, sizeof(double) * 3 * N * L);
+    free(ind);
+    free(rad);
+    free(pts);
+    free(kv);
+    free(rng);
+    free(buf);
+    return 0;
+}

@@ -30,6 +30,8 @@ struct Config {
     float sigma_pos = 0.05f, sigma_normal = 0.9f, sigma_lum = 0.5f;
     int precision = NLPF_FP32;
     uint64_t seed = 0;
+    // hash-grid bucket count; the reference fixes 64 Mi heads (lm_path_filtering.rs:1667)
+    uint64_t hash_table_size = 64ull * 1024 * 1024;
 
     // per-depth vectors repeat their last element (lm_path_filtering.rs:1337-1347, 1584-1586)
     float radius_scale_at(uint32_t d) const {

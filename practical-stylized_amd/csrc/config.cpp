@@ -329,6 +329,13 @@ bool parse_config(const char* json, Config& cfg, std::string& err) {
         } else if (k == "seed") {
             if (!get_uint(v, "seed", u, err)) return false;
             c.seed = u;
+        } else if (k == "hash_table_size") {
+            if (!get_uint(v, "hash_table_size", u, err)) return false;
+            if (u < 1 || u > (1ull << 30)) {
+                err = "hash_table_size must be in [1, 2^30]";
+                return false;
+            }
+            c.hash_table_size = u;
         }
         // other keys (e.g. "final_gather" in configs/teaser/nl-pf.json) are ignored like serde does
     }
@@ -357,7 +364,7 @@ std::string config_to_json(const Config& c) {
     o << ",\"mode\":\"" << c.mode << "\",\"pixel_radius\":" << c.pixel_radius << ",\"sigma_pos\":" << c.sigma_pos
       << ",\"sigma_normal\":" << c.sigma_normal << ",\"sigma_lum\":" << c.sigma_lum << ",\"precision\":\""
       << (c.precision == NLPF_FP16 ? "fp16" : c.precision == NLPF_FP64 ? "fp64" : "fp32")
-      << "\",\"seed\":" << c.seed << "}";
+      << "\",\"seed\":" << c.seed << ",\"hash_table_size\":" << c.hash_table_size << "}";
     return o.str();
 }
 

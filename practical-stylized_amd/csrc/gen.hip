@@ -179,3 +179,187 @@ hipError_t launch_stylize(const float* in, float* out, uint64_t n, const uint32_
 }
 
 }  // namespace nlpf
+
+// ---------------------------------------------------------------------------------------
+// world-mode records: kernel_mk_trace (lm_path_filtering.rs:1062-1218) on the procedural scene
+// ---------------------------------------------------------------------------------------
+namespace nlpf {
+
+struct WGenArgs {
+    float* rec;
+    uint8_t* flags;
+    uint8_t* new_flags;
+    uint16_t* material;
+    int32_t* path_depth;
+    uint32_t W, H, L;
+    uint64_t N, npix;
+    uint64_t seed;
+    float alo, ahi;
+    const nlpf_style_node* nodes;
+    const int32_t* roots;
+    uint32_t n_materials;
+};
+
+__device__ __forceinline__ int mat_root(const WGenArgs& g, int m) {
+    return (g.roots && (uint32_t)m < g.n_materials) ? g.roots[m] : -1;
+}
+
+__device__ v3<float> material_albedo(int m, uint64_t key, float lo, float hi) {
+    Pcg32 r = pcg_seq_offset(0xA1BED0ull + (uint64_t)m, key);
+    const float a = lo + (hi - lo) * pcg_uniform(r);
+    const float b = lo + (hi - lo) * pcg_uniform(r);
+    const float c = lo + (hi - lo) * pcg_uniform(r);
+    return v3<float>{a, b, c};
+}
+
+#define LIGHT_EMISSION 15.0f
+#define LIGHT_AREA (4.0f * SceneConst::lh * SceneConst::lh)
+
+__global__ void gen_world_kernel(WGenArgs g) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.N) return;
+    const uint64_t px = t % g.npix;
+    const uint32_t x = (uint32_t)(px % g.W), y = (uint32_t)(px / g.W);
+    const uint64_t key = splitmix64(g.seed);
+    Pcg32 rng = pcg_seq_offset(t, key);
+    v3<float> o, dir;
+    const float jx = pcg_uniform(rng), jy = pcg_uniform(rng);
+    camera_ray(g.W, g.H, (float)x + jx, (float)y + jy, o, dir);
+    uint32_t flags = 0;
+    float prev_bsdf_pdf = 0.0f;
+    int32_t last = -1;
+    const uint64_t N = g.N;
+    for (uint32_t d = 0; d < g.L; ++d) {
+        const Hit hit = scene_intersect(o, dir);
+        if (hit.mat == NLPF_MAT_SKY) break;
+        last = (int32_t)d;
+        const v3<float> p = hit.p, ng = hit.n, ns = hit.n;
+        const v3<float> wo{-dir.x, -dir.y, -dir.z};
+        const int mat = hit.mat;
+        const v3<float> R = material_albedo(mat, key, g.alo, g.ahi);
+        // AreaLight::le, one-sided (light/area.rs:36-49); MIS against light sampling for d > 0
+        float le = 0.0f, mis = 1.0f;
+        if (mat == NLPF_MAT_LIGHT) {
+            le = dot(ng, dir) < 0.0f ? LIGHT_EMISSION : 0.0f;
+            if (d > 0) {
+                const float dist2 = hit.t * hit.t;
+                const float lpdf = dist2 / (LIGHT_AREA * m_max(fabsf(dot(ng, dir)), 1e-6f));
+                mis = prev_bsdf_pdf / (prev_bsdf_pdf + lpdf);
+            }
+        }
+        const uint32_t new_flags = update_flags(g.nodes, mat_root(g, mat), flags);
+        float* r = g.rec + (uint64_t)d * NLPF_WORLD_PLANES * N + t;
+        const float base[19] = {p.x, p.y, p.z, ns.x, ns.y, ns.z, ng.x, ng.y, ng.z, wo.x, wo.y, wo.z,
+                                R.x, R.y, R.z, le, le, le, mis};
+        for (int c = 0; c < 19; ++c) r[(uint64_t)c * N] = base[c];
+        g.flags[(uint64_t)d * N + t] = (uint8_t)flags;
+        g.new_flags[(uint64_t)d * N + t] = (uint8_t)new_flags;
+        g.material[(uint64_t)d * N + t] = (uint16_t)mat;
+        flags = new_flags;
+        float direct[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // wi, irradiance, pdf, mis
+        float bsdf_pdf = 0.0f;
+        const float u0 = pcg_uniform(rng), u1 = pcg_uniform(rng), u2 = pcg_uniform(rng);
+        (void)u0;
+        const float b1 = pcg_uniform(rng), b2 = pcg_uniform(rng);
+        bool cont = false;
+        if (d + 1 < g.L) {
+            // next-event estimation to the quad light (sample_light :1260-1316, area.rs:51-110)
+            const v3<float> lp{-SceneConst::lh + 2.0f * SceneConst::lh * u1, SceneConst::ly,
+                               -SceneConst::lh + 2.0f * SceneConst::lh * u2};
+            v3<float> wi = lp - p;
+            const float dist2 = dot(wi, wi);
+            const v3<float> ln{0.0f, -1.0f, 0.0f};
+            if (dist2 > 0.0f) {
+                const float dist = m_sqrt(dist2);
+                wi = wi / dist;
+                const float cosl = fabsf(dot(ln, wi));
+                if (cosl > 0.0f) {
+                    const float e = dot(wi, ln) < 0.0f ? LIGHT_EMISSION : 0.0f;
+                    v3<float> li{e, e, e};
+                    li = apply_style<float>(g.nodes, mat_root(g, NLPF_MAT_LIGHT), d + 1, new_flags, li, lp);
+                    const float lpdf = dist2 / (LIGHT_AREA * cosl);
+                    const v3<float> ff = dot(ng, wi) >= 0.0f ? ng : ng * -1.0f;
+                    const Hit sh = scene_intersect(p + ff * 1e-4f, wi);
+                    const bool occluded = sh.mat != NLPF_MAT_SKY && sh.t < dist * (1.0f - 1e-3f);
+                    if (!occluded) {
+                        direct[0] = wi.x;
+                        direct[1] = wi.y;
+                        direct[2] = wi.z;
+                        direct[3] = li.x;
+                        direct[4] = li.y;
+                        direct[5] = li.z;
+                        direct[6] = lpdf;
+                        direct[7] = 1.0f;
+                    }
+                }
+            }
+            // cosine BSDF sample on wo's side (diffuse.rs:42-55) and its pdf (SurfaceClosure::sample)
+            const float sr = m_sqrt(b1), phi = 6.2831853f * b2;
+            const float lx = sr * cosf(phi), ly = sr * sinf(phi);
+            float lz = m_sqrt(m_max(0.0f, 1.0f - lx * lx - ly * ly));
+            v3<float> tt, bb;
+            onb(ns, tt, bb);
+            v3<float> wi2 = tt * lx + bb * ly + ns * lz;
+            if (dot(wo, ns) * lz < 0.0f) wi2 = wi2 * -1.0f;
+            const float co = dot(wo, ns), ci = dot(wi2, ns);
+            auto sgn = [](float v) { return v > 0.0f ? 1.0f : -1.0f; };
+            const float fl = sgn(dot(ng, ns));
+            const bool valid = (sgn(fl * co) * sgn(dot(wo, ng)) > 0.0f) && (sgn(fl * ci) * sgn(dot(wi2, ng)) > 0.0f);
+            if (valid && co * ci >= 0.0f) bsdf_pdf = fabsf(ci) * 0.318309886f;
+            if (bsdf_pdf > 0.0f) {
+                const v3<float> ff = dot(ng, wi2) >= 0.0f ? ng : ng * -1.0f;
+                o = p + ff * 1e-4f;
+                dir = wi2;
+                prev_bsdf_pdf = bsdf_pdf;
+                cont = true;
+            }
+        }
+        for (int c = 0; c < 8; ++c) r[(uint64_t)(19 + c) * N] = direct[c];
+        r[(uint64_t)27 * N] = bsdf_pdf;
+        if (!cont) break;
+    }
+    g.path_depth[t] = last;
+}
+
+hipError_t launch_gen_world(const WGenArgs& g, hipStream_t st) {
+    const uint32_t bs = 256;
+    hipLaunchKernelGGL(gen_world_kernel, dim3((uint32_t)((g.N + bs - 1) / bs)), dim3(bs), 0, st, g);
+    return hipGetLastError();
+}
+
+}  // namespace nlpf
+
+extern "C" int nlpf_generate_world_records(nlpf_handle* h, const nlpf_world_desc* desc, const nlpf_gen_desc* gen,
+                                           float* records, uint8_t* flags, uint8_t* new_flags, uint16_t* material,
+                                           int32_t* path_depth) {
+    using namespace nlpf;
+    if (!h) return NLPF_E_INVALID_ARG;
+    if (!desc || !gen || !records || !flags || !new_flags || !material || !path_depth)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_generate_world_records: NULL argument");
+    if (desc->width == 0 || desc->height == 0 || desc->branch == 0 || desc->levels == 0 || desc->levels > 31)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: bad sizes");
+    WGenArgs g{};
+    g.rec = records;
+    g.flags = flags;
+    g.new_flags = new_flags;
+    g.material = material;
+    g.path_depth = path_depth;
+    g.W = desc->width;
+    g.H = desc->height;
+    g.L = desc->levels;
+    g.npix = (uint64_t)desc->width * desc->height;
+    g.N = g.npix * desc->branch;
+    g.seed = gen->seed;
+    g.alo = gen->albedo_lo;
+    g.ahi = gen->albedo_hi;
+    g.nodes = h->d_nodes;
+    g.roots = h->d_roots;
+    g.n_materials = (uint32_t)h->h_roots.size();
+    NLPF_HIP(h, hipMemsetAsync(records, 0, sizeof(float) * NLPF_WORLD_PLANES * g.N * g.L, h->stream));
+    NLPF_HIP(h, hipMemsetAsync(flags, 0, g.N * g.L, h->stream));
+    NLPF_HIP(h, hipMemsetAsync(new_flags, 0, g.N * g.L, h->stream));
+    NLPF_HIP(h, hipMemsetAsync(material, 0, 2 * g.N * g.L, h->stream));
+    NLPF_HIP(h, launch_gen_world(g, h->stream));
+    NLPF_HIP(h, hipStreamSynchronize(h->stream));
+    return NLPF_OK;
+}

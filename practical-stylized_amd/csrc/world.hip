@@ -1,0 +1,729 @@
+// World mode: the reference's NL-PF (world-space hash-grid neighbour reuse with MIS/SMIS and
+// stylization) for Lambertian path vertices, made deterministic.
+//
+// Per depth d = D..0 (PathFiltering::render, lm_path_filtering.rs:1673-1774):
+//   gather   paths with a depth-d vertex, compacted in path order      (kernel_gather_points :507-531)
+//   grid     Teschner-hashed cell buckets, built by a stable radix sort of (bucket, path) pairs
+//            instead of the CAS push-front lists (HashGrid::build_kernel :230-255); a bucket's
+//            entries are visited in ascending path order, every bucket of the 27-cell
+//            neighbourhood is walked (duplicates from colliding cells included, :60-123)
+//   cluster  (optional) deterministic seeding / joining                 (:532-623, 1675-1710)
+//   filter   evaluate_filtered_radiance + stylization                   (:757-972, 999-1060)
+//   propagate the stylized radiance to the parent's indirect record     (kernel_propagate :973-998)
+// then kernel_splat + kernel_update_film (:428-506) as one per-pixel deterministic kernel.
+#include <hipcub/hipcub.hpp>
+#include "common.hpp"
+#include "scene.hpp"
+
+namespace nlpf {
+
+#define WP NLPF_WORLD_PLANES
+constexpr float kInvPi = 0.318309886183790671538f;
+
+// per-path reuse payload, gathered once per level so neighbour reads are 16-byte loads
+struct __align__(16) NbrRec {
+    float4 dwi_pdf;   // direct.wi, direct.pdf
+    float4 dirr_mis;  // direct.irradiance, direct.mis
+    float4 iwi_mis;   // indirect.wi, indirect.mis
+    float4 iirr;      // indirect.irradiance, flags-equal key in w (as bits)
+    float4 ns_wox;    // ns, wo.x
+    float4 ng_woy;    // ng, wo.y
+    float4 woz;       // wo.z, -, -, -
+};
+
+struct Grid {
+    float pmin[3], pmax[3];
+    uint32_t res[3];
+    float radius;
+    uint64_t table;
+};
+
+// hash (lm_path_filtering.rs:20-23), u64 arithmetic
+__device__ __forceinline__ uint32_t cell_bucket(uint32_t x, uint32_t y, uint32_t z, uint64_t table) {
+    const uint64_t hsh = ((uint64_t)x * 73856093ull) ^ ((uint64_t)y * 19349663ull) ^ ((uint64_t)z * 83492791ull);
+    return (uint32_t)(hsh % table);
+}
+// HashGridVar::get_cell (:46-53)
+__device__ __forceinline__ void get_cell(const Grid& g, float px, float py, float pz, uint32_t c[3]) {
+    const float p[3] = {px, py, pz};
+    for (int i = 0; i < 3; ++i) {
+        float t = (p[i] - g.pmin[i]) / (g.pmax[i] - g.pmin[i]);
+        t = fminf(fmaxf(t, 0.0f), 1.0f) * (float)g.res[i];
+        uint32_t u = (uint32_t)t;
+        c[i] = u < g.res[i] - 1 ? u : g.res[i] - 1;
+    }
+}
+
+// SurfaceClosure::check_wo_wi_valid (svm/surface/mod.rs:1701-1716)
+__device__ __forceinline__ bool wo_wi_valid(float3 ns, float3 ng, float3 wo, float3 wi) {
+    auto sgn = [](float x) { return x > 0.0f ? 1.0f : -1.0f; };
+    auto d3 = [](float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    const float flipped = sgn(d3(ng, ns));
+    return (sgn(flipped * d3(wo, ns)) * sgn(d3(wo, ng)) > 0.0f) && (sgn(flipped * d3(wi, ns)) * sgn(d3(wi, ng)) > 0.0f);
+}
+// DiffuseBsdf::evaluate_impl (svm/surface/diffuse.rs:22-41) through SurfaceClosure::evaluate
+// (mod.rs:1725-1744): returns the pdf; fcos = |cos_i| / pi (multiply by R for f*cos)
+__device__ __forceinline__ float diffuse_eval(float3 ns, float3 ng, float3 wo, float3 wi, float& fcos) {
+    fcos = 0.0f;
+    if (!wo_wi_valid(ns, ng, wo, wi)) return 0.0f;
+    const float co = wo.x * ns.x + wo.y * ns.y + wo.z * ns.z;
+    const float ci = wi.x * ns.x + wi.y * ns.y + wi.z * ns.z;
+    if (!(co * ci >= 0.0f)) return 0.0f;
+    const float pdf = fabsf(ci) * kInvPi;
+    fcos = pdf;
+    return pdf;
+}
+
+// |d| without fused multiply-adds, so the radius test rounds exactly like plain IEEE code
+__device__ __forceinline__ float dist3(float x, float y, float z) {
+    return sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
+}
+
+__device__ __forceinline__ float pl(const float* rec, uint64_t N, int c, uint64_t i) { return rec[(uint64_t)c * N + i]; }
+
+__global__ void w_flag_kernel(const int32_t* path_depth, uint64_t N, int d, uint8_t* flag) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) flag[i] = path_depth[i] >= d ? 1 : 0;
+}
+
+__global__ void w_keys_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g, uint32_t* keys) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    uint32_t c[3];
+    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
+    keys[t] = cell_bucket(c[0], c[1], c[2], g.table);
+}
+
+__global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint32_t n, const float* rec, uint64_t N,
+                                uint32_t* bstart, uint32_t* bend, float4* pos_sorted) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t k = keys[e];
+    if (e == 0 || keys[e - 1] != k) bstart[k] = e;
+    if (e == n - 1 || keys[e + 1] != k) bend[k] = e + 1;
+    const uint32_t i = ids[e];
+    pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
+}
+
+__global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint32_t* bstart, uint32_t* bend) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    bstart[keys[e]] = 0;
+    bend[keys[e]] = 0;
+}
+
+// per-path payload of level d (indirect record from the previous propagation)
+__global__ void w_pack_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, const float* ind,
+                              const uint8_t* new_flags, NbrRec* pk) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    NbrRec r;
+    r.dwi_pdf = make_float4(pl(rec, N, 19, i), pl(rec, N, 20, i), pl(rec, N, 21, i), pl(rec, N, 25, i));
+    r.dirr_mis = make_float4(pl(rec, N, 22, i), pl(rec, N, 23, i), pl(rec, N, 24, i), pl(rec, N, 26, i));
+    r.iwi_mis = make_float4(ind[i], ind[N + i], ind[2 * N + i], ind[6 * N + i]);
+    r.iirr = make_float4(ind[3 * N + i], ind[4 * N + i], ind[5 * N + i], __uint_as_float((uint32_t)new_flags[i]));
+    r.ns_wox = make_float4(pl(rec, N, 3, i), pl(rec, N, 4, i), pl(rec, N, 5, i), pl(rec, N, 9, i));
+    r.ng_woy = make_float4(pl(rec, N, 6, i), pl(rec, N, 7, i), pl(rec, N, 8, i), pl(rec, N, 10, i));
+    r.woz = make_float4(pl(rec, N, 11, i), 0.0f, 0.0f, 0.0f);
+    pk[i] = r;
+}
+
+struct FilterArgs {
+    const float* rec;
+    uint64_t N;
+    const uint32_t* pts;
+    uint32_t off, count;
+    const uint8_t* new_flags;
+    const uint16_t* material;
+    const uint32_t* cluster_id;  // nullptr when clustering is off
+    const NbrRec* pk;
+    const float4* pos_sorted;
+    const uint32_t* bstart;
+    const uint32_t* bend;
+    Grid g;
+    uint32_t K;
+    uint32_t* nbuf;
+    Pcg32* rng;
+    const nlpf_style_node* nodes;
+    const int32_t* roots;
+    uint32_t n_materials;
+    int depth;
+    int smis, flag_check, stylize;
+    float* rad_out;     // [N][3] level radiance (v.radiance)
+    unsigned long long* nbr_count;
+};
+
+__global__ void __launch_bounds__(256) w_filter_kernel(FilterArgs a) {
+    const uint32_t tl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tl >= a.count) return;
+    const uint64_t N = a.N;
+    const uint32_t c = a.pts[a.off + tl];
+    const float3 p = make_float3(pl(a.rec, N, 0, c), pl(a.rec, N, 1, c), pl(a.rec, N, 2, c));
+    const float3 ns = make_float3(pl(a.rec, N, 3, c), pl(a.rec, N, 4, c), pl(a.rec, N, 5, c));
+    const float3 ng = make_float3(pl(a.rec, N, 6, c), pl(a.rec, N, 7, c), pl(a.rec, N, 8, c));
+    const float3 wo = make_float3(pl(a.rec, N, 9, c), pl(a.rec, N, 10, c), pl(a.rec, N, 11, c));
+    const float3 R = make_float3(pl(a.rec, N, 12, c), pl(a.rec, N, 13, c), pl(a.rec, N, 14, c));
+    const float3 le = make_float3(pl(a.rec, N, 15, c), pl(a.rec, N, 16, c), pl(a.rec, N, 17, c));
+    const uint32_t myflags = a.new_flags[c];
+    const uint32_t mycluster = a.cluster_id ? a.cluster_id[c] : 0u;
+    Pcg32 rng = a.rng[tl];
+    uint32_t* buf = a.nbuf + (uint64_t)tl * a.K;
+
+    float3 q = p;
+    if (a.cluster_id) q = make_float3(pl(a.rec, N, 0, mycluster), pl(a.rec, N, 1, mycluster), pl(a.rec, N, 2, mycluster));
+    uint32_t cell[3];
+    get_cell(a.g, q.x, q.y, q.z, cell);
+    uint32_t cnt = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)a.g.res[0] || cy >= (int)a.g.res[1] || cz >= (int)a.g.res[2])
+                    continue;
+                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
+                const uint32_t e1 = a.bend[b];
+                for (uint32_t e = a.bstart[b]; e < e1; ++e) {
+                    const float4 qe = a.pos_sorted[e];
+                    if (!(dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius)) continue;
+                    const uint32_t j = __float_as_uint(qe.w);
+                    if (a.cluster_id && a.cluster_id[j] != mycluster) continue;
+                    if (a.flag_check && (uint32_t)a.new_flags[j] != myflags) continue;
+                    // reservoir over the reuse cap (:813-821)
+                    if (cnt < a.K) {
+                        buf[cnt] = j;
+                    } else {
+                        const uint32_t r = pcg_next(rng) % (cnt + 1);
+                        if (r < a.K) buf[r] = j;
+                    }
+                    ++cnt;
+                }
+            }
+    atomicAdd(a.nbr_count, (unsigned long long)cnt);
+    const uint32_t used = cnt < a.K ? cnt : a.K;
+    const float inv_used = 1.0f / (float)used;
+    float3 direct = make_float3(0, 0, 0), indirect = make_float3(0, 0, 0);
+    for (uint32_t k = 0; k < used; ++k) {
+        const uint32_t j = buf[k];
+        const NbrRec& r = a.pk[j];
+        const float4 dw = r.dwi_pdf;
+        if (dw.w > 0.0f) {
+            float fcos;
+            const float bpdf = diffuse_eval(ns, ng, wo, make_float3(dw.x, dw.y, dw.z), fcos);
+            const float4 di = r.dirr_mis;
+            const float w = dw.w / (dw.w + bpdf);  // mis_weight, balance heuristic (:305-316)
+            const float s = fcos * di.w * w / dw.w * inv_used;
+            direct.x += R.x * s * di.x;
+            direct.y += R.y * s * di.y;
+            direct.z += R.z * s * di.z;
+        }
+        const float4 iw = r.iwi_mis;
+        const float3 wi = make_float3(iw.x, iw.y, iw.z);
+        if (sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z) == 0.0f) continue;
+        float fcos;
+        const float bpdf = diffuse_eval(ns, ng, wo, wi, fcos);
+        const float4 ir = r.iirr;
+        const float3 reused = make_float3(R.x * fcos * ir.x * iw.w, R.y * fcos * ir.y * iw.w, R.z * fcos * ir.z * iw.w);
+        if (a.smis) {
+            float sum_pdf = 0.0f;
+            for (uint32_t m = 0; m < used; ++m) {
+                const NbrRec& o = a.pk[buf[m]];
+                const float4 a0 = o.ns_wox, a1 = o.ng_woy, a2 = o.woz;
+                float fc;
+                sum_pdf += diffuse_eval(make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
+                                        make_float3(a0.w, a1.w, a2.x), wi, fc);
+            }
+            indirect.x += reused.x / sum_pdf;
+            indirect.y += reused.y / sum_pdf;
+            indirect.z += reused.z / sum_pdf;
+        } else if (bpdf > 0.0f) {
+            indirect.x += reused.x / bpdf * inv_used;
+            indirect.y += reused.y / bpdf * inv_used;
+            indirect.z += reused.z / bpdf * inv_used;
+        }
+    }
+    v3<float> L{le.x + direct.x + indirect.x, le.y + direct.y + indirect.y, le.z + direct.z + indirect.z};
+    if (a.stylize) {
+        const uint32_t m = a.material[c];
+        const int root = m < a.n_materials ? a.roots[m] : -1;
+        if (root >= 0) L = apply_style<float>(a.nodes, root, (uint32_t)a.depth, myflags, L, v3<float>{p.x, p.y, p.z});
+    }
+    a.rad_out[3 * (uint64_t)c] = L.x;
+    a.rad_out[3 * (uint64_t)c + 1] = L.y;
+    a.rad_out[3 * (uint64_t)c + 2] = L.z;
+    a.rng[tl] = rng;
+}
+
+// parent (depth d-1) indirect record <- child's filtered radiance (kernel_propagate :973-998,
+// record fields from kernel_filter :1043-1058)
+__global__ void w_propagate_kernel(const float* rec_child, const float* rec_parent, uint64_t N, const uint32_t* pts,
+                                   uint32_t n, const float* rad, float* ind) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    ind[i] = -pl(rec_child, N, 9, i);
+    ind[N + i] = -pl(rec_child, N, 10, i);
+    ind[2 * N + i] = -pl(rec_child, N, 11, i);
+    ind[3 * N + i] = rad[3 * (uint64_t)i];
+    ind[4 * N + i] = rad[3 * (uint64_t)i + 1];
+    ind[5 * N + i] = rad[3 * (uint64_t)i + 2];
+    ind[6 * N + i] = pl(rec_child, N, 18, i);
+    ind[7 * N + i] = pl(rec_parent, N, 27, i);
+}
+
+__global__ void w_fill_kernel(float* p, uint64_t n, float v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// kernel_splat + kernel_update_film + Film::add_sample (:428-506, film.rs:196-229) per pixel,
+// branches summed in order
+__global__ void w_film_kernel(const float* rad0, const float* rec0, const int32_t* path_depth, const uint8_t* flags0,
+                              const uint16_t* material0, uint64_t npix, uint32_t branch, uint64_t N,
+                              const nlpf_style_node* nodes, const int32_t* roots, uint32_t n_materials,
+                              int stylize, float* film) {
+    const uint64_t px = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= npix) return;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (uint32_t b = 0; b < branch; ++b) {
+        const uint64_t t = (uint64_t)b * npix + px;
+        if (path_depth[t] < 0) continue;
+        sx += rad0[3 * t];
+        sy += rad0[3 * t + 1];
+        sz += rad0[3 * t + 2];
+    }
+    const float scale = 1.0f / (float)branch;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
+    for (uint32_t b = 0; b < branch; ++b) {
+        const uint64_t t = (uint64_t)b * npix + px;
+        if (path_depth[t] < 0) continue;
+        v3<float> L{sx * scale, sy * scale, sz * scale};
+        if (stylize) {
+            const uint32_t m = material0[t];
+            const int root = m < n_materials ? roots[m] : -1;
+            if (root >= 0)
+                L = apply_style<float>(nodes, root, 0u, flags0[t], L,
+                                       v3<float>{pl(rec0, N, 0, t), pl(rec0, N, 1, t), pl(rec0, N, 2, t)});
+        }
+        if (isnan(L.x) || isnan(L.y) || isnan(L.z)) L = v3<float>{0.0f, 0.0f, 0.0f};
+        ax += L.x;
+        ay += L.y;
+        az += L.z;
+        wsum += 1.0f;
+    }
+    film[3 * px] = wsum > 0.0f ? ax / wsum : 0.0f;
+    film[3 * px + 1] = wsum > 0.0f ? ay / wsum : 0.0f;
+    film[3 * px + 2] = wsum > 0.0f ? az / wsum : 0.0f;
+}
+
+__global__ void w_rng_init_kernel(Pcg32* rng, uint32_t n, uint64_t key) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) rng[t] = pcg_seq_offset(t, key);
+}
+
+// ---- deterministic clustering (replaces the CAS seeding / joining of :532-623) ----
+// seed per bucket = smallest unclustered path id in it
+__global__ void w_cluster_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+                                      const uint32_t* cluster_id, uint32_t* bucket_seed) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    if (cluster_id[i] != 0xFFFFFFFFu) return;
+    uint32_t c[3];
+    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
+    atomicMin(&bucket_seed[cell_bucket(c[0], c[1], c[2], g.table)], i);
+}
+__global__ void w_cluster_mark_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+                                           uint32_t* cluster_id, const uint32_t* bucket_seed) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    if (cluster_id[i] != 0xFFFFFFFFu) return;
+    uint32_t c[3];
+    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
+    if (bucket_seed[cell_bucket(c[0], c[1], c[2], g.table)] == i) cluster_id[i] = i;
+}
+__global__ void w_cluster_reset_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+                                            uint32_t* bucket_seed) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    uint32_t c[3];
+    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
+    bucket_seed[cell_bucket(c[0], c[1], c[2], g.table)] = 0xFFFFFFFFu;
+}
+// every unclustered vertex proposes to the first seed (visit order of :60-123) within the
+// radius that still has room; key = seed << 32 | vertex so a sort groups proposals by seed
+// with the smallest vertex ids first
+__global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+                                         const float4* pos_sorted, const uint32_t* bstart, const uint32_t* bend,
+                                         const uint32_t* cluster_id, const uint32_t* cluster_size, uint32_t cap,
+                                         unsigned long long* prop, uint32_t* n_prop, uint32_t* n_unclustered) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    if (cluster_id[i] != 0xFFFFFFFFu) return;
+    atomicAdd(n_unclustered, 1u);
+    const float px = pl(rec, N, 0, i), py = pl(rec, N, 1, i), pz = pl(rec, N, 2, i);
+    uint32_t cell[3];
+    get_cell(g, px, py, pz, cell);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] || cz >= (int)g.res[2])
+                    continue;
+                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
+                for (uint32_t e = bstart[b]; e < bend[b]; ++e) {
+                    const float4 qe = pos_sorted[e];
+                    if (!(dist3(qe.x - px, qe.y - py, qe.z - pz) <= g.radius)) continue;
+                    const uint32_t j = __float_as_uint(qe.w);
+                    if (cluster_id[j] == j && cluster_size[j] < cap) {
+                        const uint32_t slot = atomicAdd(n_prop, 1u);
+                        prop[slot] = ((unsigned long long)j << 32) | i;
+                        return;
+                    }
+                }
+            }
+}
+// accept the first (cap - size) proposals of each seed, in ascending vertex order
+__global__ void w_cluster_accept_kernel(const unsigned long long* prop_sorted, uint32_t n, uint32_t* cluster_id,
+                                        uint32_t* cluster_size, uint32_t cap, uint32_t* n_accepted) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t seed = (uint32_t)(prop_sorted[e] >> 32);
+    uint32_t first = e;
+    while (first > 0 && (uint32_t)(prop_sorted[first - 1] >> 32) == seed) --first;
+    const uint32_t rank = e - first;
+    const uint32_t size0 = cluster_size[seed];
+    if (size0 + rank < cap) {
+        cluster_id[(uint32_t)prop_sorted[e]] = seed;
+        atomicAdd(n_accepted, 1u);
+    }
+}
+__global__ void w_cluster_size_kernel(const unsigned long long* prop_sorted, uint32_t n, uint32_t* cluster_size,
+                                      uint32_t cap) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t seed = (uint32_t)(prop_sorted[e] >> 32);
+    if (e + 1 < n && (uint32_t)(prop_sorted[e + 1] >> 32) == seed) return;  // last of its run
+    uint32_t first = e;
+    while (first > 0 && (uint32_t)(prop_sorted[first - 1] >> 32) == seed) --first;
+    const uint32_t proposals = e - first + 1;
+    const uint32_t size0 = cluster_size[seed];
+    const uint32_t room = cap > size0 ? cap - size0 : 0u;
+    cluster_size[seed] = size0 + (proposals < room ? proposals : room);
+}
+
+}  // namespace nlpf
+
+// ---------------------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------------------
+namespace nlpf {
+
+struct WorldWork {
+    uint64_t N = 0, table = 0, nbuf_elems = 0, rng_n = 0, cub_bytes = 0;
+    uint8_t* flag = nullptr;
+    uint32_t *pts = nullptr, *keys = nullptr, *keys_sorted = nullptr, *ids_sorted = nullptr;
+    uint32_t *bstart = nullptr, *bend = nullptr, *bucket_seed = nullptr;
+    uint32_t *cluster_id = nullptr, *cluster_size = nullptr, *counters = nullptr, *nbuf = nullptr;
+    unsigned long long *prop = nullptr, *prop_sorted = nullptr, *nbr = nullptr;
+    float4* pos_sorted = nullptr;
+    NbrRec* pk = nullptr;
+    float *ind = nullptr, *rad = nullptr;
+    Pcg32* rng = nullptr;
+    void* cub = nullptr;
+};
+
+static void world_free(WorldWork& w) {
+    void* ps[] = {w.flag, w.pts, w.keys, w.keys_sorted, w.ids_sorted, w.bstart, w.bend, w.bucket_seed,
+                  w.cluster_id, w.cluster_size, w.counters, w.nbuf, w.prop, w.prop_sorted, w.nbr,
+                  w.pos_sorted, w.pk, w.ind, w.rad, w.rng, w.cub};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    w = WorldWork{};
+}
+
+static inline dim3 grid1(uint64_t n, uint32_t bs = 256) { return dim3((uint32_t)((n + bs - 1) / bs)); }
+
+}  // namespace nlpf
+
+struct nlpf_world_state {
+    nlpf::WorldWork w;
+};
+
+namespace nlpf {
+// workspace owned by the handle (allocated on first use, grown as needed)
+static std::vector<std::pair<nlpf_handle*, nlpf_world_state*>> g_world;
+static nlpf_world_state* world_state(nlpf_handle* h) {
+    for (auto& kv : g_world)
+        if (kv.first == h) return kv.second;
+    g_world.emplace_back(h, new nlpf_world_state());
+    return g_world.back().second;
+}
+void world_release(nlpf_handle* h) {
+    for (size_t i = 0; i < g_world.size(); ++i)
+        if (g_world[i].first == h) {
+            world_free(g_world[i].second->w);
+            delete g_world[i].second;
+            g_world.erase(g_world.begin() + i);
+            return;
+        }
+}
+
+template <typename T>
+static int dalloc(nlpf_handle* h, T** p, uint64_t n) {
+    NLPF_HIP(h, hipMalloc((void**)p, sizeof(T) * (n ? n : 1)));
+    return NLPF_OK;
+}
+
+static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t table, uint64_t nbuf_elems,
+                         uint64_t rng_n) {
+    if (w.N >= N && w.table == table && w.nbuf_elems >= nbuf_elems && w.rng_n >= rng_n) return NLPF_OK;
+    NLPF_HIP(h, hipStreamSynchronize(h->stream));
+    world_free(w);
+    int rc = 0;
+    rc |= dalloc(h, &w.flag, N);
+    rc |= dalloc(h, &w.pts, N);
+    rc |= dalloc(h, &w.keys, N);
+    rc |= dalloc(h, &w.keys_sorted, N);
+    rc |= dalloc(h, &w.ids_sorted, N);
+    rc |= dalloc(h, &w.bstart, table);
+    rc |= dalloc(h, &w.bend, table);
+    rc |= dalloc(h, &w.bucket_seed, table);
+    rc |= dalloc(h, &w.cluster_id, N);
+    rc |= dalloc(h, &w.cluster_size, N);
+    rc |= dalloc(h, &w.counters, 8);
+    rc |= dalloc(h, &w.nbuf, nbuf_elems);
+    rc |= dalloc(h, &w.prop, N);
+    rc |= dalloc(h, &w.prop_sorted, N);
+    rc |= dalloc(h, &w.nbr, 32);
+    rc |= dalloc(h, &w.pos_sorted, N);
+    rc |= dalloc(h, &w.pk, N);
+    rc |= dalloc(h, &w.ind, 8 * N);
+    rc |= dalloc(h, &w.rad, 3 * N);
+    rc |= dalloc(h, &w.rng, rng_n);
+    if (rc) return rc;
+    // temp storage for the hipcub primitives at the largest size
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    hipcub::CountingInputIterator<uint32_t> it(0);
+    NLPF_HIP(h, hipcub::DeviceSelect::Flagged(nullptr, b1, it, w.flag, w.pts, w.counters, (int)N, h->stream));
+    NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(nullptr, b2, w.keys, w.keys_sorted, w.pts, w.ids_sorted, (int)N, 0,
+                                                   32, h->stream));
+    NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(nullptr, b3, w.prop, w.prop_sorted, (int)N, 0, 64, h->stream));
+    w.cub_bytes = std::max(b1, std::max(b2, b3));
+    NLPF_HIP(h, hipMalloc(&w.cub, w.cub_bytes));
+    // empty bucket ranges; the filter of every depth restores the entries it set
+    NLPF_HIP(h, hipMemsetAsync(w.bstart, 0, sizeof(uint32_t) * table, h->stream));
+    NLPF_HIP(h, hipMemsetAsync(w.bend, 0, sizeof(uint32_t) * table, h->stream));
+    NLPF_HIP(h, hipMemsetAsync(w.bucket_seed, 0xFF, sizeof(uint32_t) * table, h->stream));
+    w.N = N;
+    w.table = table;
+    w.nbuf_elems = nbuf_elems;
+    w.rng_n = rng_n;
+    return NLPF_OK;
+}
+
+static uint32_t read_u32(nlpf_handle* h, const uint32_t* d, int& rc) {
+    uint32_t v = 0;
+    if (hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        h->err = "device readback failed";
+        rc = NLPF_E_HIP;
+    }
+    return v;
+}
+
+}  // namespace nlpf
+
+using namespace nlpf;
+
+extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const float* rec, const uint8_t* flags,
+                              const uint8_t* new_flags, const uint16_t* material, const int32_t* path_depth,
+                              float* film, float* radiance_out) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    if (!desc || !rec || !flags || !new_flags || !material || !path_depth || !film)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_world_run: NULL argument");
+    const Config& cfg = h->cfg;
+    if (desc->width == 0 || desc->height == 0 || desc->branch == 0 || desc->levels == 0)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: width, height, branch and levels must be >= 1");
+    if (desc->levels != cfg.max_depth + 1)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: levels must equal config max_depth + 1");
+    if (desc->branch != cfg.branch) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: branch must equal config branch");
+    const uint64_t npix = (uint64_t)desc->width * desc->height;
+    const uint64_t N = npix * desc->branch;
+    if (N >= (1ull << 31)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: too many paths (< 2^31)");
+    const uint32_t L = desc->levels;
+    float bmin[3], bmax[3];
+    for (int i = 0; i < 3; ++i) {
+        bmin[i] = cfg.has_scene_bounds ? cfg.scene_bounds[0][i] : desc->aabb_min[i];
+        bmax[i] = cfg.has_scene_bounds ? cfg.scene_bounds[1][i] : desc->aabb_max[i];
+        if (!(bmax[i] > bmin[i])) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: empty scene AABB");
+    }
+    uint32_t kmax = 0;
+    for (uint32_t d = 0; d < L; ++d) kmax = std::max(kmax, cfg.reuse_at(d));
+    // batch = vram * 2^20 / (4 K) vertices (lm_path_filtering.rs:1714-1721)
+    uint64_t batch_max = 0;
+    for (uint32_t d = 0; d < L; ++d) {
+        const uint64_t K = cfg.reuse_at(d);
+        batch_max = std::max(batch_max, std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / (K * 4)));
+    }
+    batch_max = std::min<uint64_t>(batch_max, N);
+    const uint64_t nbuf_elems = std::max<uint64_t>(1, (uint64_t)cfg.vram * 1024 * 1024 / 4 + 1);
+    nlpf_world_state* ws = world_state(h);
+    WorldWork& w = ws->w;
+    int rc = world_reserve(h, w, N, cfg.hash_table_size, std::max<uint64_t>(nbuf_elems, kmax), batch_max);
+    if (rc) return rc;
+    const uint32_t table_bits = std::max(1, (int)std::ceil(std::log2((double)cfg.hash_table_size)));
+    hipStream_t st = h->stream;
+
+    NLPF_HIP(h, hipMemsetAsync(w.nbr, 0, sizeof(unsigned long long) * 32, st));
+    hipLaunchKernelGGL(w_rng_init_kernel, grid1(batch_max), dim3(256), 0, st, w.rng, (uint32_t)batch_max,
+                       splitmix64(cfg.seed));
+    hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
+    if (radiance_out)
+        hipLaunchKernelGGL(w_fill_kernel, grid1(3 * N * L), dim3(256), 0, st, radiance_out, 3 * N * L, 0.0f);
+    if ((rc = check_launch(h, "world setup"))) return rc;
+    rc = record_event(h, 0);
+    if (rc) return rc;
+
+    nlpf_stats stt{};
+    const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
+    const float avg_ext = (ext[0] + ext[1] + ext[2]) / 3.0f;
+    for (int d = (int)L - 1; d >= 0; --d) {
+        const float* recd = rec + (uint64_t)d * WP * N;
+        float* rad = radiance_out ? radiance_out + (uint64_t)d * 3 * N : w.rad;
+        hipLaunchKernelGGL(w_flag_kernel, grid1(N), dim3(256), 0, st, path_depth, N, d, w.flag);
+        hipcub::CountingInputIterator<uint32_t> it(0);
+        size_t tb = w.cub_bytes;
+        NLPF_HIP(h, hipcub::DeviceSelect::Flagged(w.cub, tb, it, w.flag, w.pts, w.counters, (int)N, st));
+        const uint32_t n = read_u32(h, w.counters, rc);
+        if (rc) return rc;
+        stt.points[d] = n;
+        if (n > 0) {
+            Grid g;
+            const float radius = avg_ext * 0.01f * cfg.radius_scale_at((uint32_t)d);
+            for (int i = 0; i < 3; ++i) {
+                g.pmin[i] = bmin[i];
+                g.pmax[i] = bmax[i];
+                const float r = std::floor(ext[i] / radius);
+                if (!(r >= 1.0f) || r > 4.0e9f)
+                    NLPF_FAIL(h, NLPF_E_INVALID_CONFIG, "hash grid resolution out of range for this radius_scale");
+                g.res[i] = (uint32_t)r;
+            }
+            g.radius = radius;
+            g.table = cfg.hash_table_size;
+            hipLaunchKernelGGL(w_keys_kernel, grid1(n), dim3(256), 0, st, rec + (uint64_t)d * WP * N, N, w.pts, n, g,
+                               w.keys);
+            tb = w.cub_bytes;
+            NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.keys_sorted, w.pts, w.ids_sorted,
+                                                           (int)n, 0, (int)table_bits, st));
+            hipLaunchKernelGGL(w_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, w.ids_sorted, n, recd, N,
+                               w.bstart, w.bend, w.pos_sorted);
+            hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind,
+                               new_flags + (uint64_t)d * N, w.pk);
+            if ((rc = check_launch(h, "world grid"))) return rc;
+            const uint32_t K = cfg.reuse_at((uint32_t)d);
+            if (cfg.cluster) {
+                NLPF_HIP(h, hipMemsetAsync(w.cluster_id, 0xFF, sizeof(uint32_t) * N, st));
+                NLPF_HIP(h, hipMemsetAsync(w.cluster_size, 0, sizeof(uint32_t) * N, st));
+                for (int outer = 0;; ++outer) {
+                    if (outer > 4096) NLPF_FAIL(h, NLPF_E_NUMERIC, "clustering did not converge");
+                    hipLaunchKernelGGL(w_cluster_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                                       w.cluster_id, w.bucket_seed);
+                    hipLaunchKernelGGL(w_cluster_mark_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                                       w.cluster_id, w.bucket_seed);
+                    hipLaunchKernelGGL(w_cluster_reset_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                                       w.bucket_seed);
+                    uint32_t n_uncl = 0;
+                    for (int inner = 0;; ++inner) {
+                        NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * 4, st));
+                        hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                                           w.pos_sorted, w.bstart, w.bend, w.cluster_id, w.cluster_size, K, w.prop,
+                                           w.counters + 1, w.counters + 2);
+                        if ((rc = check_launch(h, "world cluster"))) return rc;
+                        uint32_t hc[4];
+                        NLPF_HIP(h, hipMemcpyAsync(hc, w.counters, sizeof(hc), hipMemcpyDeviceToHost, st));
+                        NLPF_HIP(h, hipStreamSynchronize(st));
+                        const uint32_t n_prop = hc[1];
+                        n_uncl = hc[2];
+                        if (n_prop == 0) break;
+                        tb = w.cub_bytes;
+                        NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(w.cub, tb, w.prop, w.prop_sorted, (int)n_prop, 0,
+                                                                      64, st));
+                        hipLaunchKernelGGL(w_cluster_accept_kernel, grid1(n_prop), dim3(256), 0, st, w.prop_sorted,
+                                           n_prop, w.cluster_id, w.cluster_size, K, w.counters + 3);
+                        hipLaunchKernelGGL(w_cluster_size_kernel, grid1(n_prop), dim3(256), 0, st, w.prop_sorted, n_prop,
+                                           w.cluster_size, K);
+                        if (inner > 1 << 20) NLPF_FAIL(h, NLPF_E_NUMERIC, "cluster joining did not converge");
+                    }
+                    if (n_uncl == 0) break;
+                }
+            }
+            const uint64_t batch = std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
+            for (uint64_t off = 0; off < n; off += batch) {
+                const uint32_t cnt = (uint32_t)std::min<uint64_t>(batch, n - off);
+                FilterArgs fa;
+                fa.rec = recd;
+                fa.N = N;
+                fa.pts = w.pts;
+                fa.off = (uint32_t)off;
+                fa.count = cnt;
+                fa.new_flags = new_flags + (uint64_t)d * N;
+                fa.material = material + (uint64_t)d * N;
+                fa.cluster_id = cfg.cluster ? w.cluster_id : nullptr;
+                fa.pk = w.pk;
+                fa.pos_sorted = w.pos_sorted;
+                fa.bstart = w.bstart;
+                fa.bend = w.bend;
+                fa.g = g;
+                fa.K = K;
+                fa.nbuf = w.nbuf;
+                fa.rng = w.rng;
+                fa.nodes = h->d_nodes;
+                fa.roots = h->d_roots;
+                fa.n_materials = (uint32_t)h->h_roots.size();
+                fa.depth = d;
+                fa.smis = cfg.smis;
+                fa.flag_check = !cfg.disable_flag_check;
+                fa.stylize = (d > 0 || !cfg.no_first_hit_stylization) && !h->h_roots.empty();
+                fa.rad_out = rad;
+                fa.nbr_count = w.nbr + d;
+                hipLaunchKernelGGL(w_filter_kernel, grid1(cnt), dim3(256), 0, st, fa);
+            }
+            if ((rc = check_launch(h, "world filter"))) return rc;
+            hipLaunchKernelGGL(w_clear_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bstart, w.bend);
+        }
+        if (d > 0) {
+            hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
+            if (n > 0)
+                hipLaunchKernelGGL(w_propagate_kernel, grid1(n), dim3(256), 0, st, recd,
+                                   rec + (uint64_t)(d - 1) * WP * N, N, w.pts, n, rad, w.ind);
+        }
+        if ((rc = check_launch(h, "world propagate"))) return rc;
+        rc = record_event(h, 1 + (L - 1 - d));
+        if (rc) return rc;
+    }
+    const float* rad0 = radiance_out ? radiance_out : w.rad;
+    // level 0 radiance: when radiance_out is NULL, w.rad still holds depth 0 (last level filtered)
+    hipLaunchKernelGGL(w_film_kernel, grid1(npix), dim3(256), 0, st, rad0, rec, path_depth, flags, material, npix,
+                       desc->branch, N, h->d_nodes, h->d_roots, (uint32_t)h->h_roots.size(),
+                       (int)(!cfg.no_first_hit_stylization && !h->h_roots.empty()), film);
+    if ((rc = check_launch(h, "world film"))) return rc;
+    rc = record_event(h, 1 + L);
+    if (rc) return rc;
+    unsigned long long nb[32];
+    NLPF_HIP(h, hipMemcpyAsync(nb, w.nbr, sizeof(nb), hipMemcpyDeviceToHost, st));
+    NLPF_HIP(h, hipStreamSynchronize(st));
+    stt.total_ms = event_ms(h, 0, 1 + L);
+    for (uint32_t i = 0; i < L; ++i) stt.level_ms[L - 1 - i] = event_ms(h, i, i + 1);
+    stt.film_ms = event_ms(h, L, L + 1);
+    for (uint32_t d = 0; d < L && d < 32; ++d) {
+        stt.neighbors[d] = nb[d];
+        stt.vertices += stt.points[d];
+    }
+    h->stats = stt;
+    return NLPF_OK;
+}

@@ -42,7 +42,7 @@ DEFAULT_CONFIG = {
     "reuse": [128, 64], "vram": 2048, "smis": True, "no_first_hit_stylization": False,
     "disable_flag_check": False, "cluster": True, "scene_bounds": None,
     "mode": "world", "pixel_radius": 7, "sigma_pos": 0.05, "sigma_normal": 0.9,
-    "sigma_lum": 0.5, "precision": "fp32", "seed": 0,
+    "sigma_lum": 0.5, "precision": "fp32", "seed": 0, "hash_table_size": 64 * 1024 * 1024,
 }
 
 
@@ -66,7 +66,16 @@ class _GenDesc(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("level_ms", C.c_double * 32), ("film_ms", C.c_double),
                 ("vertices", C.c_uint64), ("bytes_algorithmic", C.c_uint64),
-                ("flops_algorithmic", C.c_double)]
+                ("flops_algorithmic", C.c_double), ("points", C.c_uint64 * 32),
+                ("neighbors", C.c_uint64 * 32)]
+
+
+class _WorldDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("branch", C.c_uint32),
+                ("levels", C.c_uint32), ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3)]
+
+
+WORLD_PLANES = 28
 
 
 _lib = None
@@ -94,6 +103,10 @@ _SIGS = [
                                                C.POINTER(_GenDesc), C.c_void_p, C.c_void_p]),
     ("nlpf_convert_records_fp16", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p,
                                             C.c_void_p, C.c_void_p]),
+    ("nlpf_world_run", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("nlpf_generate_world_records", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.POINTER(_GenDesc),
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 
@@ -264,7 +277,56 @@ class Handle:
         self._check(self.lib.nlpf_get_stats(self.h, C.byref(s)))
         return {"total_ms": s.total_ms, "level_ms": list(s.level_ms), "film_ms": s.film_ms,
                 "vertices": s.vertices, "bytes_algorithmic": s.bytes_algorithmic,
-                "flops_algorithmic": s.flops_algorithmic}
+                "flops_algorithmic": s.flops_algorithmic, "points": list(s.points),
+                "neighbors": list(s.neighbors)}
+
+    # ---- world mode ----
+    @staticmethod
+    def world_desc(W, H, branch, L, aabb_min=(-0.5, -0.5, -0.5), aabb_max=(0.5, 0.5, 0.5)):
+        return _WorldDesc(W, H, branch, L, (C.c_float * 3)(*aabb_min), (C.c_float * 3)(*aabb_max))
+
+    def alloc_world(self, desc):
+        n = desc.width * desc.height * desc.branch
+        L = desc.levels
+        return {"records": self.alloc(L * WORLD_PLANES * n * 4), "flags": self.alloc(L * n),
+                "new_flags": self.alloc(L * n), "material": self.alloc(L * n * 2),
+                "path_depth": self.alloc(n * 4)}
+
+    def generate_world_records(self, desc, bufs, seed=1, albedo_lo=0.2, albedo_hi=0.8):
+        g = _GenDesc(seed, 0.0, 0.0, 0.0, 0.0, albedo_lo, albedo_hi)
+        self._check(self.lib.nlpf_generate_world_records(
+            self.h, C.byref(desc), C.byref(g), _ptr(bufs["records"]), _ptr(bufs["flags"]),
+            _ptr(bufs["new_flags"]), _ptr(bufs["material"]), _ptr(bufs["path_depth"])))
+
+    def upload_world(self, desc, host):
+        bufs = self.alloc_world(desc)
+        for k in bufs:
+            bufs[k].upload(np.ascontiguousarray(host[k]))
+        return bufs
+
+    def download_world(self, desc, bufs):
+        n = desc.width * desc.height * desc.branch
+        L = desc.levels
+        return {"records": bufs["records"].download(np.float32, L * WORLD_PLANES * n),
+                "flags": bufs["flags"].download(np.uint8, L * n),
+                "new_flags": bufs["new_flags"].download(np.uint8, L * n),
+                "material": bufs["material"].download(np.uint16, L * n),
+                "path_depth": bufs["path_depth"].download(np.int32, n)}
+
+    def world_run(self, desc, bufs, want_radiance=False):
+        n = desc.width * desc.height * desc.branch
+        L = desc.levels
+        film = self.alloc(desc.width * desc.height * 12)
+        rad = self.alloc(L * n * 12) if want_radiance else None
+        self._check(self.lib.nlpf_world_run(
+            self.h, C.byref(desc), _ptr(bufs["records"]), _ptr(bufs["flags"]), _ptr(bufs["new_flags"]),
+            _ptr(bufs["material"]), _ptr(bufs["path_depth"]), _ptr(film), _ptr(rad)))
+        f = film.download(np.float32).reshape(desc.height, desc.width, 3)
+        r = rad.download(np.float32).reshape(L, n, 3) if want_radiance else None
+        film.free()
+        if rad is not None:
+            rad.free()
+        return f, r
 
 
 # ---- stylization node builders ----

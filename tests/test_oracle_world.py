@@ -1,0 +1,251 @@
+"""Pins the world-mode oracle (the reference's hash-grid NL-PF) against an independent numpy
+restatement on tiny inputs, the brute-force radius search (with and without bucket
+collisions), and closed forms of the Lambertian reuse estimator (no GPU)."""
+import numpy as np
+import pytest
+
+import nlpf as N
+
+NP = 28
+AMIN = np.array([-0.5, -0.5, -0.5], np.float32)
+AMAX = np.array([0.5, 0.5, 0.5], np.float32)
+
+
+def teschner(c, table):
+    c = np.asarray(c, np.uint64)
+    return ((c[..., 0] * np.uint64(73856093)) ^ (c[..., 1] * np.uint64(19349663)) ^ (c[..., 2] * np.uint64(83492791))) \
+        % np.uint64(table)
+
+
+def cells(pos, res):
+    t = (pos - AMIN) / (AMAX - AMIN)
+    t = np.clip(t, 0, 1).astype(np.float32) * res.astype(np.float32)
+    return np.minimum(t.astype(np.uint32), res - 1)
+
+
+@pytest.mark.parametrize("table", [64 * 1024 * 1024, 7, 1])
+def test_grid_neighbour_multiset(oracle, table):
+    rng = np.random.default_rng(table)
+    pos = (rng.random((600, 3)) - 0.5).astype(np.float32)
+    radius = np.float32(0.12)
+    got = oracle.world_neighbor_counts(pos, AMIN, AMAX, radius, table)
+    res = np.floor((AMAX - AMIN) / radius).astype(np.uint32)
+    c = cells(pos, res).astype(np.int64)
+    b = teschner(c, table)
+    dist = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1, dtype=np.float32))
+    within = dist <= radius
+    expect = np.zeros(len(pos), np.int64)
+    for t in range(len(pos)):
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    nc = c[t] + np.array([dx, dy, dz])
+                    if (nc < 0).any() or (nc >= res).any():
+                        continue
+                    bb = teschner(nc, table)
+                    expect[t] += int((within[t] & (b == bb)).sum())
+    assert np.array_equal(got, expect)
+    if table == 64 * 1024 * 1024:  # no collisions: exactly the brute-force radius search
+        assert np.array_equal(got, within.sum(1))
+    else:  # collisions only add duplicates
+        assert (got >= within.sum(1)).all() and (got > within.sum(1)).any()
+
+
+def make_records(rng, W, H, branch, L, same_normal=False, tight=False):
+    Np = W * H * branch
+    rec = np.zeros((L, NP, Np), np.float32)
+    for d in range(L):
+        if tight:
+            rec[d, 0:3] = (rng.random((3, Np)) * 0.02 - 0.01).astype(np.float32)
+        else:
+            rec[d, 0:3] = (rng.random((3, Np)) - 0.5).astype(np.float32) * 0.3
+        if same_normal:
+            n = np.tile(np.array([[0.0], [0.0], [1.0]]), (1, Np))
+        else:
+            n = rng.normal(0, 0.3, (3, Np)) + np.array([[0.0], [0.0], [1.0]])
+            n /= np.linalg.norm(n, axis=0)
+        rec[d, 3:6] = n
+        if d == 0:
+            wo = rng.normal(0, 0.3, (3, Np)) + np.array([[0.0], [0.0], [1.0]])
+            rec[d, 9:12] = wo / np.linalg.norm(wo, axis=0)
+        else:
+            # the child is reached along the parent's sampled direction: wo_child = -wi_parent,
+            # with wi_parent on the parent's side of its normal (as BSDF sampling produces)
+            pn = rec[d - 1, 3:6].astype(np.float64)
+            wi = rng.normal(0, 0.4, (3, Np)) + pn
+            wi /= np.linalg.norm(wi, axis=0)
+            c = (wi * pn).sum(0)
+            wi = np.where(c < 0, wi - 2 * c * pn, wi)  # reflect into the parent's hemisphere
+            rec[d, 9:12] = -wi
+        wo_d = rec[d, 9:12].astype(np.float64)
+        flip = (wo_d * rec[d, 3:6]).sum(0) < 0
+        rec[d, 3:6, flip] *= -1  # keep wo strictly on the normal's side
+        rec[d, 6:9] = rec[d, 3:6]
+        rec[d, 12:15] = rng.uniform(0.2, 0.8, (3, Np))
+        rec[d, 15:18] = rng.uniform(0, 0.5, (3, Np)) * (rng.random(Np) < 0.2)
+        rec[d, 18] = rng.uniform(0.5, 1.0, Np)
+        wi = rng.normal(0, 0.5, (3, Np)) + np.array([[0.0], [0.0], [1.0]])
+        rec[d, 19:22] = wi / np.linalg.norm(wi, axis=0)
+        rec[d, 22:25] = rng.uniform(0, 3, (3, Np))
+        rec[d, 25] = rng.uniform(0.1, 2, Np) * (rng.random(Np) < 0.8)
+        rec[d, 26] = 1.0
+        rec[d, 27] = rng.uniform(0.1, 0.3, Np)
+    depth = rng.integers(-1, L, Np).astype(np.int32)
+    return rec, depth
+
+
+def numpy_world(rec, depth, W, H, branch, L, radius_scale, smis, nodes=None, roots=(), flags=None, new_flags=None,
+                material=None, no_first_hit_stylization=True):
+    """Independent restatement for cnt <= K (no reservoir) and no bucket collisions: the
+    neighbour set of a vertex is every depth-d vertex within the radius with equal flags."""
+    Np = W * H * branch
+    ext = (AMAX - AMIN).astype(np.float32)
+    avg = np.float32((ext[0] + ext[1] + ext[2]) / np.float32(3))
+    ind = np.zeros((8, Np))
+    rad = np.zeros((L, Np, 3))
+    nf = np.zeros((L, Np), np.uint8) if new_flags is None else new_flags
+    mat = np.zeros((L, Np), np.uint16) if material is None else material
+    for d in range(L - 1, -1, -1):
+        r = rec[d].astype(np.float64)
+        pts = np.nonzero(depth >= d)[0]
+        radius = np.float32(avg * np.float32(0.01) * np.float32(radius_scale))
+        P = rec[d, 0:3, pts].astype(np.float32)  # (n, 3)
+        dist = np.sqrt(((P[:, None, :] - P[None, :, :]) ** 2).sum(-1, dtype=np.float32))
+        for a, c in enumerate(pts):
+            nb = pts[(dist[a] <= radius) & (nf[d, pts] == nf[d, c])]
+            ns, ng, wo, R = r[3:6, c], r[6:9, c], r[9:12, c], r[12:15, c]
+
+            def ev(ns_, ng_, wo_, wi_):
+                sg = lambda x: 1.0 if x > 0 else -1.0
+                fl = sg(ng_ @ ns_)
+                ok = sg(fl * (wo_ @ ns_)) * sg(wo_ @ ng_) > 0 and sg(fl * (wi_ @ ns_)) * sg(wi_ @ ng_) > 0
+                if not ok or not ((wo_ @ ns_) * (wi_ @ ns_) >= 0):
+                    return 0.0
+                return abs(wi_ @ ns_) / np.pi
+
+            dsum = np.zeros(3)
+            isum = np.zeros(3)
+            for j in nb:
+                pd = r[25, j]
+                if pd > 0:
+                    b = ev(ns, ng, wo, r[19:22, j])
+                    dsum += R * b * r[22:25, j] * r[26, j] * (pd / (pd + b)) / pd / len(nb)
+                wi = ind[0:3, j]
+                if np.linalg.norm(wi) == 0:
+                    continue
+                b = ev(ns, ng, wo, wi)
+                reused = R * b * ind[3:6, j] * ind[6, j]
+                if smis:
+                    s = sum(ev(r[3:6, k], r[6:9, k], r[9:12, k], wi) for k in nb)
+                    isum += reused / s
+                elif b > 0:
+                    isum += reused / b / len(nb)
+            Lr = r[15:18, c] + dsum + isum
+            if nodes is not None and roots[mat[d, c]] >= 0 and (d > 0 or not no_first_hit_stylization):
+                import oracle as O
+                Lr = O.stylize(nodes, roots, Lr[None], depth=d, flags=[nf[d, c]], material=[mat[d, c]],
+                               pos=r[0:3, c][None])[0]
+            rad[d, c] = Lr
+        ind[:] = 0
+        if d > 0:
+            for c in pts:
+                ind[0:3, c] = -r[9:12, c]
+                ind[3:6, c] = rad[d, c]
+                ind[6, c] = r[18, c]
+                ind[7, c] = rec[d - 1, 27, c]
+    return rad
+
+
+@pytest.mark.parametrize("smis", [True, False])
+def test_matches_numpy_restatement(oracle, smis):
+    rng = np.random.default_rng(7 + smis)
+    W, H, branch, L = 6, 5, 2, 3
+    rec, depth = make_records(rng, W, H, branch, L)
+    cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "smis": smis, "no_first_hit_stylization": True}
+    Np = W * H * branch
+    z8 = np.zeros((L, Np), np.uint8)
+    z16 = np.zeros((L, Np), np.uint16)
+    film, rad, nbr = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg,
+                                      want_radiance=True)
+    ref = numpy_world(rec, depth, W, H, branch, L, 8.0, smis)
+    for d in range(L):
+        m = depth >= d
+        assert np.allclose(rad[d][m], ref[d][m], rtol=1e-12, atol=1e-14), d
+    # film: mean of valid branches (no first-hit stylization)
+    rr = rad[0].reshape(branch, H * W, 3)
+    valid = (depth >= 0).reshape(branch, H * W)
+    expect = np.where(valid.any(0)[:, None], (rr * valid[..., None]).sum(0) / branch, 0)
+    assert np.allclose(film.reshape(-1, 3), expect, rtol=1e-12)
+
+
+def test_flags_gate_and_stylization(oracle):
+    rng = np.random.default_rng(11)
+    W, H, branch, L = 5, 4, 2, 2
+    rec, depth = make_records(rng, W, H, branch, L)
+    Np = W * H * branch
+    nf = rng.choice([0, N.FLAG_STYLE2, N.FLAG_STYLE3], (L, Np)).astype(np.uint8)
+    mat = rng.integers(0, 2, (L, Np)).astype(np.uint16)
+    nodes = N.style_table([N.rex(0.4), N.hue_shift(0.2), N.enable_if(1, inner=1, else_=0)])
+    roots = [2, 0]
+    for nfhs in (True, False):
+        cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "smis": True, "no_first_hit_stylization": nfhs}
+        film, rad, _ = oracle.world_run(rec, nf, nf, mat, depth, W, H, branch, L, AMIN, AMAX, cfg, nodes=nodes,
+                                        roots=roots, want_radiance=True)
+        ref = numpy_world(rec, depth, W, H, branch, L, 8.0, True, nodes=nodes, roots=roots, new_flags=nf,
+                          material=mat, no_first_hit_stylization=nfhs)
+        for d in range(L):
+            m = depth >= d
+            assert np.allclose(rad[d][m], ref[d][m], rtol=1e-12, atol=1e-14)
+
+
+def test_lambertian_closed_forms(oracle):
+    """Same normals, wo and wi in the upper hemisphere: SMIS and 1/(pdf|N|) both reduce the
+    indirect term to R * mean_j(L_j mis_j); the direct term is the balance-heuristic sum."""
+    rng = np.random.default_rng(3)
+    W, H, branch, L = 4, 4, 1, 2
+    rec, _ = make_records(rng, W, H, branch, L, same_normal=True, tight=True)
+    depth = np.full(W * H, 1, np.int32)
+    rec[1, 25] = 0.0  # no direct light at the top level: its output is le
+    cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "no_first_hit_stylization": True}
+    Np = W * H
+    z8 = np.zeros((L, Np), np.uint8)
+    z16 = np.zeros((L, Np), np.uint16)
+    for smis in (True, False):
+        cfg["smis"] = smis
+        _, rad, _ = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg,
+                                     want_radiance=True)
+        r = rec.astype(np.float64)
+        assert np.allclose(rad[1], r[1, 15:18].T, rtol=1e-12)
+        ind = (r[1, 15:18] * r[1, 18]).mean(1)  # mean_j L_j mis_j over the whole (tight) cluster
+        cosd = r[0, 21]  # wi.z with n = +z
+        pd = r[0, 25]
+        dterm = ((cosd / np.pi) * r[0, 22:25] * r[0, 26] / (pd + cosd / np.pi) * (pd > 0)).mean(1)
+        expect = r[0, 15:18].T + r[0, 12:15].T * (ind + dterm)[None, :]
+        assert np.allclose(rad[0], expect, rtol=1e-10)
+
+
+def test_reservoir_cap_and_film_restylization(oracle):
+    rng = np.random.default_rng(5)
+    W, H, branch, L = 3, 3, 3, 1
+    rec, _ = make_records(rng, W, H, branch, L, same_normal=True, tight=True)
+    depth = np.zeros(W * H * branch, np.int32)
+    Np = W * H * branch
+    z8 = np.zeros((L, Np), np.uint8)
+    z16 = np.zeros((L, Np), np.uint16)
+    # K = 1: every vertex reuses exactly one of the 27 candidates
+    cfg = {"radius_scale": [8.0], "reuse": [1], "smis": False, "no_first_hit_stylization": True}
+    _, rad, nbr = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg, want_radiance=True)
+    assert nbr[0] == Np * Np  # every vertex sees all of them
+    r = rec[0].astype(np.float64)
+    single = r[15:18].T[:, None, :] + r[12:15].T[:, None, :] * (
+        (r[21] / np.pi) * r[22:25] * r[26] / (r[25] + r[21] / np.pi) * (r[25] > 0))[None].transpose(0, 2, 1)
+    for c in range(Np):
+        assert np.isclose(single[c], rad[0, c], rtol=1e-10).all(1).any()
+    # film re-applies depth-0 stylization to the branch mean (kernel_update_film :451-470)
+    nodes = N.style_table([N.rex(0.5)])
+    cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "smis": True, "no_first_hit_stylization": False}
+    film, rad, _ = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg, nodes=nodes,
+                                    roots=[0], want_radiance=True)
+    mean = rad[0].reshape(branch, W * H, 3).mean(0)
+    again = oracle.stylize(nodes, [0], mean, depth=0)
+    assert np.allclose(film.reshape(-1, 3), again, rtol=1e-12)
