@@ -175,6 +175,17 @@ typedef struct nlpf_screen_desc {
 int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
                     const float* anchors, void* film_rgb, void* level_out);
 
+/* Adjoint of nlpf_screen_run (FP32 records) with respect to the bandwidths and every colour-ramp
+ * node's stop values, for a loss whose gradient with respect to the film is dL_dfilm
+ * ([H][W][3], device).  level_out must hold the forward pass's level outputs (same records,
+ * sigmas and style table).  Weights are recomputed per tile, never stored.  grad_params
+ * (device, float[3 + 16 * ramps]) receives dL/dsigma_pos, dL/dsigma_normal, dL/dsigma_lum,
+ * then dL/dvalue[k] of the r-th RAMP node of the style table at 3 + 16 r + k.  At most 2 ramp
+ * nodes and a chain of at most 8 stylization nodes.  No reference counterpart (the reference
+ * has no differentiable NL-PF). */
+int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                         const void* level_out, const float* dL_dfilm, float* grad_params);
+
 /* Kernel time (ms, HIP events on the handle's stream) of the last nlpf_screen_run, total
  * and per level (levels entries, deepest level first in execution order, indexed by depth). */
 typedef struct nlpf_stats {

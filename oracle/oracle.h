@@ -58,4 +58,10 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                  int disable_flag_check, uint64_t seed, uint64_t table, const or_node* nodes, const int32_t* roots,
                  uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out);
 
+/* FP64 reference gradient of the screen-mode NL-PF; returns the number of parameters
+ * (3 + 16 * ramp nodes) written to grad, or -1. */
+int or_screen_grad_f64(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, const double sig[3],
+                       const float* rec, const or_node* nodes, uint32_t n_nodes, int root, const double* dfilm,
+                       double* grad, double* loss_unused);
+
 #endif

@@ -41,6 +41,8 @@ def load():
     lib.or_world_run.argtypes = [u32, u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, u32, u64, i32, i32,
                                  i32, u64, u64, vp, vp, u32, vp, vp, vp]
     lib.or_world_run.restype = i32
+    lib.or_screen_grad_f64.argtypes = [u32, u32, u32, u32, i32, vp, vp, vp, u32, i32, vp, vp, vp]
+    lib.or_screen_grad_f64.restype = i32
     _lib = lib
     return lib
 
@@ -145,3 +147,19 @@ def world_run(rec, flags, new_flags, material, path_depth, W, H, branch, L, aabb
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return (film, rad, nbr) if want_radiance else film
+
+
+def screen_grad(records, W, H, S, L, radius, sigmas, nodes, root, dfilm):
+    """FP64 reference gradient: [dL/dsigma_pos, dL/dsigma_normal, dL/dsigma_lum, ramp stops...]."""
+    lib = load()
+    records = np.ascontiguousarray(records, np.float32)
+    nodes = np.ascontiguousarray(nodes)
+    nr = int((nodes["type"] == 8).sum()) if len(nodes) else 0
+    grad = np.zeros(3 + 16 * nr)
+    sig = np.ascontiguousarray(sigmas, np.float64)
+    dfilm = np.ascontiguousarray(dfilm, np.float64)
+    n = lib.or_screen_grad_f64(W, H, S, L, radius, _p(sig), _p(records), _p(nodes), len(nodes), int(root),
+                               _p(dfilm), _p(grad), None)
+    if n < 0:
+        raise MemoryError("oracle allocation failed")
+    return grad

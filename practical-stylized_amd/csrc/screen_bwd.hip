@@ -1,0 +1,537 @@
+// Adjoint of the screen-mode NL-PF with respect to the filter bandwidths (sigma_pos,
+// sigma_normal, sigma_lum) and the stop values of every colour-ramp node, given dLoss/dfilm.
+//
+// Forward, per level d and vertex i:  in_j = rad_j + thr_j * out_{d+1,j},
+//   w_ij = 2^-(e_p + e_n + e_l),  e_k = log2(e) * term_k / (2 sigma_k^2),
+//   F_i = sum_j w_ij in_j / W_i,  out_i = S_d(F_i).
+// Reverse, level 0 -> D:
+//   (A) per i: recompute F_i and the sigma moments A_k = sum_j w e_k, B_k = sum_j w e_k in_j
+//       (weights are recomputed per tile, never stored); h_i = J_S(F_i)^T g_i;
+//       dL/dsigma_k += (2 ln2 / sigma_k) h_i . (B_k - F_i A_k) / W_i;  ramp stops from J_S;
+//       u_i = h_i / W_i.
+//   (B) per j: g_{d+1,j} = thr_j * sum_i w_ij u_i   (w symmetric, windows symmetric).
+// Parameter gradients are reduced per workgroup into fixed slots and summed in a fixed order,
+// so the result is bitwise reproducible.
+#include "common.hpp"
+#include "dual.hpp"
+#include "screen_impl.hpp"
+
+namespace nlpf {
+
+#define ADJ_MAX_RAMPS 2
+#define ADJ_NP (3 + 16 * ADJ_MAX_RAMPS)
+#define ADJ_MAX_CHAIN 8
+
+struct BwdArgs {
+    const float* rec;       // FP32 record planes [S][L][15][H][W]
+    const float* level_out; // forward outputs [S][L][3][H][W]
+    const float* g_in;      // dL/dout_d   [S][3][H][W]
+    float* g_out;           // dL/dout_{d+1} [S][3][H][W] (pass B)
+    float* u;               // h / W        [S][3][H][W]
+    float* partial;         // [nblocks][ADJ_NP] accumulated over levels
+    const nlpf_style_node* nodes;
+    const int32_t* ramp_slot;  // per node: ramp ordinal or -1
+    int root;
+    uint32_t W, H, S, L, d;
+    uint64_t plane;
+    uint32_t tiles_x, tiles_y, nblocks;
+    float kp, kn, kl;
+    float sig[3];
+    int has_child;
+};
+
+// chain of nodes applied to F at depth d (innermost first), as apply_style walks it with flags 0
+__device__ __forceinline__ int style_chain(const nlpf_style_node* nodes, int root, int* seq) {
+    int tmp[NLPF_STYLE_STACK];
+    int n = 0, cur = root;
+    while (cur >= 0 && n < NLPF_STYLE_STACK) {
+        const nlpf_style_node& nd = nodes[cur];
+        if (nd.type == NLPF_STY_ENABLE_IF) {
+            cur = nd.idx == 0 ? nd.inner : nd.else_;
+        } else if (nd.type == NLPF_STY_SET_IDX || nd.type == NLPF_STY_BASE) {
+            cur = nd.inner;
+        } else {
+            tmp[n++] = cur;
+            cur = nd.inner;
+        }
+    }
+    const int m = n < ADJ_MAX_CHAIN ? n : ADJ_MAX_CHAIN;
+    for (int k = 0; k < m; ++k) seq[k] = tmp[n - 1 - k];
+    return m;
+}
+
+// h = J_S(F)^T g; ramp stop gradients accumulated into pg[3 + 16 * slot + k]
+__device__ __forceinline__ void style_vjp(const BwdArgs& a, v3<float> F, v3<float> g, float* pg, v3<float>& h) {
+    int seq[ADJ_MAX_CHAIN];
+    const int n = a.root >= 0 ? style_chain(a.nodes, a.root, seq) : 0;
+    v3<float> xs[ADJ_MAX_CHAIN];
+    v3<float> x = F;
+    const v3<float> p0{0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < ADJ_MAX_CHAIN; ++k) {
+        if (k < n) {
+            xs[k] = x;
+            x = apply_node<float>(a.nodes[seq[k]], a.d, x, p0);
+        }
+    }
+#pragma unroll
+    for (int k = ADJ_MAX_CHAIN - 1; k >= 0; --k) {
+        if (k < n) {
+            const nlpf_style_node& nd = a.nodes[seq[k]];
+            const v3<float> xi = xs[k];
+            if (nd.type == NLPF_STY_RAMP && depth_active(nd.depth_mask, a.d)) {
+                const int slot = a.ramp_slot[seq[k]];
+                const v3<float> c = clamp3(xi, 0.0f, 100000.0f);
+                const float l = lum709(c);
+                if (slot >= 0 && l > 0.0f) {
+                    const float q = (g.x * c.x + g.y * c.y + g.z * c.z) / l;
+                    uint32_t kk = 0;
+                    while (kk < nd.count && nd.p[kk] <= l) ++kk;
+                    float w0 = 0.0f, w1 = 0.0f;
+                    int i0 = 0, i1 = 0;
+                    if (kk == 0) {
+                        w0 = 1.0f;
+                    } else if (kk == nd.count) {
+                        i0 = (int)nd.count - 1;
+                        w0 = 1.0f;
+                    } else {
+                        const float t = (l - nd.p[kk - 1]) / (nd.p[kk] - nd.p[kk - 1]);
+                        i0 = (int)kk - 1;
+                        i1 = (int)kk;
+                        w0 = 1.0f - t;
+                        w1 = t;
+                    }
+#pragma unroll
+                    for (int r = 0; r < ADJ_MAX_RAMPS; ++r) {
+                        if (r == slot) {
+#pragma unroll
+                            for (int m = 0; m < 16; ++m)
+                                pg[3 + 16 * r + m] += (m == i0 ? w0 * q : 0.0f) + (m == i1 ? w1 * q : 0.0f);
+                        }
+                    }
+                }
+            }
+            const v3<Dual3> xd{Dual3::var(xi.x, 0), Dual3::var(xi.y, 1), Dual3::var(xi.z, 2)};
+            const v3<Dual3> pd{Dual3(0.0f), Dual3(0.0f), Dual3(0.0f)};
+            const v3<Dual3> yd = apply_node<Dual3>(nd, a.d, xd, pd);
+            v3<float> gn;
+            gn.x = g.x * yd.x.d[0] + g.y * yd.y.d[0] + g.z * yd.z.d[0];
+            gn.y = g.x * yd.x.d[1] + g.y * yd.y.d[1] + g.z * yd.z.d[1];
+            gn.z = g.x * yd.x.d[2] + g.y * yd.y.d[2] + g.z * yd.z.d[2];
+            g = gn;
+        }
+    }
+    h = g;
+}
+
+template <int RAD, int NW, int R>
+__global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
+    constexpr int TW = 64, TH = NW * R, WT = TW + 2 * RAD, HT = TH + 2 * RAD, NT = 64 * NW;
+    constexpr int K = 2 * RAD + 1;
+    __shared__ float sh[10][HT][WT];
+    __shared__ float red[NW][ADJ_NP];
+
+    const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
+    const uint32_t tiles = a.tiles_x * a.tiles_y;
+    const uint32_t s = t / tiles, tile = t - s * tiles;
+    const int tx0 = (int)(tile % a.tiles_x) * TW, ty0 = (int)(tile / a.tiles_x) * TH;
+    const int W = (int)a.W, H = (int)a.H;
+    const size_t plane = a.plane;
+    const float* rec = a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
+    const float* child = a.has_child ? a.level_out + ((size_t)s * a.L + a.d + 1) * 3 * plane : nullptr;
+
+    for (int i = threadIdx.x; i < WT * HT; i += NT) {
+        const int r = i / WT, c = i - r * WT;
+        const int gx = tx0 - RAD + c, gy = ty0 - RAD + r;
+        float f[10];
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t o = (size_t)gy * W + gx;
+            float ir = rec[9 * plane + o], ig = rec[10 * plane + o], ib = rec[11 * plane + o];
+            if (child) {
+                ir = fmaf(rec[12 * plane + o], child[o], ir);
+                ig = fmaf(rec[13 * plane + o], child[plane + o], ig);
+                ib = fmaf(rec[14 * plane + o], child[2 * plane + o], ib);
+            }
+            f[0] = rec[o] * a.kp;
+            f[1] = rec[plane + o] * a.kp;
+            f[2] = rec[2 * plane + o] * a.kp;
+            f[3] = rec[3 * plane + o];
+            f[4] = rec[4 * plane + o];
+            f[5] = rec[5 * plane + o];
+            f[6] = lum709(v3<float>{rec[6 * plane + o], rec[7 * plane + o], rec[8 * plane + o]}) * a.kl;
+            f[7] = ir;
+            f[8] = ig;
+            f[9] = ib;
+        } else {
+            f[0] = f[1] = f[2] = 1e18f;
+            f[3] = f[4] = f[5] = f[6] = f[7] = f[8] = f[9] = 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 10; ++q) sh[q][r][c] = f[q];
+    }
+    __syncthreads();
+
+    const int lc = threadIdx.x & 63;
+    const int r0 = (threadIdx.x >> 6) * R;
+    float cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R];
+    float Wt[R], Sx[R], Sy[R], Sz[R], cW[R], cX[R], cY[R], cZ[R];
+    float Am[3][R], Bm[3][3][R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int rr = r0 + k + RAD, cc = lc + RAD;
+        cpx[k] = sh[0][rr][cc];
+        cpy[k] = sh[1][rr][cc];
+        cpz[k] = sh[2][rr][cc];
+        cnx[k] = sh[3][rr][cc];
+        cny[k] = sh[4][rr][cc];
+        cnz[k] = sh[5][rr][cc];
+        cl[k] = sh[6][rr][cc];
+        Wt[k] = Sx[k] = Sy[k] = Sz[k] = cW[k] = cX[k] = cY[k] = cZ[k] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            Am[q][k] = 0.0f;
+            Bm[q][0][k] = Bm[q][1][k] = Bm[q][2][k] = 0.0f;
+        }
+    }
+    const float kn = a.kn;
+    for (int r = 0; r < R + 2 * RAD; ++r) {
+        float rw[R], rx[R], ry[R], rz[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) rw[k] = rx[k] = ry[k] = rz[k] = 0.0f;
+        const int lr = r0 + r;
+#pragma unroll
+        for (int dx = 0; dx < K; ++dx) {
+            const int lcol = lc + dx;
+            const float npx = sh[0][lr][lcol], npy = sh[1][lr][lcol], npz = sh[2][lr][lcol];
+            const float nnx = sh[3][lr][lcol], nny = sh[4][lr][lcol], nnz = sh[5][lr][lcol];
+            const float nl = sh[6][lr][lcol];
+            const float nir = sh[7][lr][lcol], nig = sh[8][lr][lcol], nib = sh[9][lr][lcol];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (r - k >= 0 && r - k < K) {
+                    const float ex = cpx[k] - npx, ey = cpy[k] - npy, ez = cpz[k] - npz;
+                    float ep = ex * ex;
+                    ep = fmaf(ey, ey, ep);
+                    ep = fmaf(ez, ez, ep);
+                    float nn = cnx[k] * nnx;
+                    nn = fmaf(cny[k], nny, nn);
+                    nn = fmaf(cnz[k], nnz, nn);
+                    const float tt = fmaf(-kn, nn, kn);
+                    const float en = tt * tt;
+                    const float dl = cl[k] - nl;
+                    const float el = dl * dl;
+                    const float w = exp2_neg(ep + en + el);
+                    rw[k] += w;
+                    rx[k] = fmaf(w, nir, rx[k]);
+                    ry[k] = fmaf(w, nig, ry[k]);
+                    rz[k] = fmaf(w, nib, rz[k]);
+                    const float wp = w * ep, wn = w * en, wl = w * el;
+                    Am[0][k] += wp;
+                    Am[1][k] += wn;
+                    Am[2][k] += wl;
+                    Bm[0][0][k] = fmaf(wp, nir, Bm[0][0][k]);
+                    Bm[0][1][k] = fmaf(wp, nig, Bm[0][1][k]);
+                    Bm[0][2][k] = fmaf(wp, nib, Bm[0][2][k]);
+                    Bm[1][0][k] = fmaf(wn, nir, Bm[1][0][k]);
+                    Bm[1][1][k] = fmaf(wn, nig, Bm[1][1][k]);
+                    Bm[1][2][k] = fmaf(wn, nib, Bm[1][2][k]);
+                    Bm[2][0][k] = fmaf(wl, nir, Bm[2][0][k]);
+                    Bm[2][1][k] = fmaf(wl, nig, Bm[2][1][k]);
+                    Bm[2][2][k] = fmaf(wl, nib, Bm[2][2][k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (r - k >= 0 && r - k < K) {
+                kahan_add(Wt[k], cW[k], rw[k]);
+                kahan_add(Sx[k], cX[k], rx[k]);
+                kahan_add(Sy[k], cY[k], ry[k]);
+                kahan_add(Sz[k], cZ[k], rz[k]);
+            }
+        }
+    }
+
+    float pg[ADJ_NP];
+#pragma unroll
+    for (int q = 0; q < ADJ_NP; ++q) pg[q] = 0.0f;
+    const float* gin = a.g_in + (size_t)s * 3 * plane;
+    float* up = a.u + (size_t)s * 3 * plane;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int ox = tx0 + lc, oy = ty0 + r0 + k;
+        if (ox < W && oy < H) {
+            const size_t o = (size_t)oy * W + ox;
+            const float inv = 1.0f / Wt[k];
+            const v3<float> F{Sx[k] * inv, Sy[k] * inv, Sz[k] * inv};
+            const v3<float> g{gin[o], gin[plane + o], gin[2 * plane + o]};
+            v3<float> hh;
+            style_vjp(a, F, g, pg, hh);
+            const float Fv[3] = {F.x, F.y, F.z};
+            const float hv[3] = {hh.x, hh.y, hh.z};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc = fmaf(hv[c], Bm[q][c][k] - Fv[c] * Am[q][k], acc);
+                pg[q] += acc * inv * (2.0f * 0.69314718056f / a.sig[q]);
+            }
+            up[o] = hh.x * inv;
+            up[plane + o] = hh.y * inv;
+            up[2 * plane + o] = hh.z * inv;
+        }
+    }
+    // deterministic block reduction: wave butterflies, then waves in order
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < ADJ_NP; ++q) {
+        float v = pg[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red[wv][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < ADJ_NP) {
+        float v = 0.0f;
+        for (int w2 = 0; w2 < NW; ++w2) v += red[w2][threadIdx.x];
+        a.partial[(size_t)blockIdx.x * ADJ_NP + threadIdx.x] += v;
+    }
+}
+
+// g_{d+1, j} = thr_j * sum_i w_ij u_i
+template <int RAD, int NW, int R>
+__global__ void __launch_bounds__(64 * NW) screen_bwd_b_kernel(BwdArgs a) {
+    constexpr int TW = 64, TH = NW * R, WT = TW + 2 * RAD, HT = TH + 2 * RAD, NT = 64 * NW;
+    constexpr int K = 2 * RAD + 1;
+    __shared__ float sh[10][HT][WT];
+    const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
+    const uint32_t tiles = a.tiles_x * a.tiles_y;
+    const uint32_t s = t / tiles, tile = t - s * tiles;
+    const int tx0 = (int)(tile % a.tiles_x) * TW, ty0 = (int)(tile / a.tiles_x) * TH;
+    const int W = (int)a.W, H = (int)a.H;
+    const size_t plane = a.plane;
+    const float* rec = a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
+    const float* up = a.u + (size_t)s * 3 * plane;
+    for (int i = threadIdx.x; i < WT * HT; i += NT) {
+        const int r = i / WT, c = i - r * WT;
+        const int gx = tx0 - RAD + c, gy = ty0 - RAD + r;
+        float f[10];
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t o = (size_t)gy * W + gx;
+            f[0] = rec[o] * a.kp;
+            f[1] = rec[plane + o] * a.kp;
+            f[2] = rec[2 * plane + o] * a.kp;
+            f[3] = rec[3 * plane + o];
+            f[4] = rec[4 * plane + o];
+            f[5] = rec[5 * plane + o];
+            f[6] = lum709(v3<float>{rec[6 * plane + o], rec[7 * plane + o], rec[8 * plane + o]}) * a.kl;
+            f[7] = up[o];
+            f[8] = up[plane + o];
+            f[9] = up[2 * plane + o];
+        } else {
+            f[0] = f[1] = f[2] = 1e18f;
+            f[3] = f[4] = f[5] = f[6] = f[7] = f[8] = f[9] = 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 10; ++q) sh[q][r][c] = f[q];
+    }
+    __syncthreads();
+    const int lc = threadIdx.x & 63;
+    const int r0 = (threadIdx.x >> 6) * R;
+    float cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R], Sx[R], Sy[R], Sz[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int rr = r0 + k + RAD, cc = lc + RAD;
+        cpx[k] = sh[0][rr][cc];
+        cpy[k] = sh[1][rr][cc];
+        cpz[k] = sh[2][rr][cc];
+        cnx[k] = sh[3][rr][cc];
+        cny[k] = sh[4][rr][cc];
+        cnz[k] = sh[5][rr][cc];
+        cl[k] = sh[6][rr][cc];
+        Sx[k] = Sy[k] = Sz[k] = 0.0f;
+    }
+    const float kn = a.kn;
+    for (int r = 0; r < R + 2 * RAD; ++r) {
+        const int lr = r0 + r;
+#pragma unroll
+        for (int dx = 0; dx < K; ++dx) {
+            const int lcol = lc + dx;
+            const float npx = sh[0][lr][lcol], npy = sh[1][lr][lcol], npz = sh[2][lr][lcol];
+            const float nnx = sh[3][lr][lcol], nny = sh[4][lr][lcol], nnz = sh[5][lr][lcol];
+            const float nl = sh[6][lr][lcol];
+            const float ux = sh[7][lr][lcol], uy = sh[8][lr][lcol], uz = sh[9][lr][lcol];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (r - k >= 0 && r - k < K) {
+                    const float ex = cpx[k] - npx, ey = cpy[k] - npy, ez = cpz[k] - npz;
+                    float e = ex * ex;
+                    e = fmaf(ey, ey, e);
+                    e = fmaf(ez, ez, e);
+                    float nn = cnx[k] * nnx;
+                    nn = fmaf(cny[k], nny, nn);
+                    nn = fmaf(cnz[k], nnz, nn);
+                    const float tt = fmaf(-kn, nn, kn);
+                    e = fmaf(tt, tt, e);
+                    const float dl = cl[k] - nl;
+                    e = fmaf(dl, dl, e);
+                    const float w = exp2_neg(e);
+                    Sx[k] = fmaf(w, ux, Sx[k]);
+                    Sy[k] = fmaf(w, uy, Sy[k]);
+                    Sz[k] = fmaf(w, uz, Sz[k]);
+                }
+            }
+        }
+    }
+    float* gout = a.g_out + (size_t)s * 3 * plane;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int ox = tx0 + lc, oy = ty0 + r0 + k;
+        if (ox < W && oy < H) {
+            const size_t o = (size_t)oy * W + ox;
+            gout[o] = rec[12 * plane + o] * Sx[k];
+            gout[plane + o] = rec[13 * plane + o] * Sy[k];
+            gout[2 * plane + o] = rec[14 * plane + o] * Sz[k];
+        }
+    }
+}
+
+__global__ void bwd_seed_kernel(const float* dfilm, float* g0, uint32_t S, uint64_t plane) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane * S) return;
+    const uint64_t s = i / plane, px = i - s * plane;
+    const float inv = 1.0f / (float)S;
+    for (int c = 0; c < 3; ++c) g0[(s * 3 + c) * plane + px] = dfilm[3 * px + c] * inv;
+}
+
+// fixed-order Kahan sum of the per-workgroup partials
+__global__ void bwd_reduce_kernel(const float* partial, uint32_t nblocks, float* out) {
+    const int q = threadIdx.x;
+    if (q >= ADJ_NP) return;
+    float s = 0.0f, c = 0.0f;
+    for (uint32_t b = 0; b < nblocks; ++b) kahan_add(s, c, partial[(size_t)b * ADJ_NP + q]);
+    out[q] = s;
+}
+
+template <int RAD>
+hipError_t launch_bwd(BwdArgs a, bool pass_b, hipStream_t st) {
+    constexpr int NW = 4, R = 2, TW = 64, TH = NW * R;
+    a.tiles_x = (a.W + TW - 1) / TW;
+    a.tiles_y = (a.H + TH - 1) / TH;
+    a.nblocks = a.tiles_x * a.tiles_y * a.S;
+    if (pass_b)
+        hipLaunchKernelGGL((screen_bwd_b_kernel<RAD, NW, R>), dim3(a.nblocks), dim3(64 * NW), 0, st, a);
+    else
+        hipLaunchKernelGGL((screen_bwd_a_kernel<RAD, NW, R>), dim3(a.nblocks), dim3(64 * NW), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t screen_bwd_level(const BwdArgs& a, int radius, bool pass_b, hipStream_t st) {
+    switch (radius) {
+        case 1: return launch_bwd<1>(a, pass_b, st);
+        case 2: return launch_bwd<2>(a, pass_b, st);
+        case 3: return launch_bwd<3>(a, pass_b, st);
+        case 4: return launch_bwd<4>(a, pass_b, st);
+        case 5: return launch_bwd<5>(a, pass_b, st);
+        case 6: return launch_bwd<6>(a, pass_b, st);
+        case 7: return launch_bwd<7>(a, pass_b, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+uint32_t screen_bwd_blocks(uint32_t W, uint32_t H, uint32_t S) {
+    return ((W + 63) / 64) * ((H + 7) / 8) * S;
+}
+
+}  // namespace nlpf
+
+using namespace nlpf;
+
+extern "C" int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                                    const void* level_out, const float* dL_dfilm, float* grad_params) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    if (!desc || !records || !level_out || !dL_dfilm || !grad_params)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_backward: NULL argument");
+    if (desc->precision != NLPF_FP32) NLPF_FAIL(h, NLPF_E_UNSUPPORTED, "adjoint runs on FP32 records");
+    if (desc->width == 0 || desc->height == 0 || desc->spp == 0 || desc->levels == 0 || desc->levels > 31 ||
+        desc->radius < 1 || desc->radius > 7)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_backward: bad screen desc");
+    // ramp ordinals, chain length limits
+    std::vector<int32_t> slots(h->h_nodes.size(), -1);
+    int nr = 0;
+    for (size_t i = 0; i < h->h_nodes.size(); ++i)
+        if (h->h_nodes[i].type == NLPF_STY_RAMP) slots[i] = nr++;
+    if (nr > ADJ_MAX_RAMPS) NLPF_FAIL(h, NLPF_E_UNSUPPORTED, "adjoint supports at most 2 ramp nodes");
+    const int root = h->h_roots.empty() ? -1 : h->h_roots[0];
+    {
+        int n = 0, cur = root;
+        while (cur >= 0 && n <= NLPF_STYLE_STACK) {
+            const nlpf_style_node& nd = h->h_nodes[cur];
+            if (nd.type == NLPF_STY_ENABLE_IF) cur = nd.idx == 0 ? nd.inner : nd.else_;
+            else {
+                if (nd.type != NLPF_STY_SET_IDX && nd.type != NLPF_STY_BASE) ++n;
+                cur = nd.inner;
+            }
+        }
+        if (n > ADJ_MAX_CHAIN) NLPF_FAIL(h, NLPF_E_UNSUPPORTED, "adjoint supports stylization chains of <= 8 nodes");
+    }
+    const uint64_t plane = (uint64_t)desc->width * desc->height;
+    const uint32_t S = desc->spp, L = desc->levels;
+    const uint32_t nb = screen_bwd_blocks(desc->width, desc->height, S);
+    // scratch: g (2 slots), u, partials, ramp slots
+    const size_t gbytes = (size_t)S * 3 * plane * sizeof(float);
+    const size_t need = 3 * gbytes + (size_t)nb * ADJ_NP * sizeof(float) + 64 * sizeof(int32_t) + 256;
+    int rc = ensure_scratch(h, need);
+    if (rc) return rc;
+    char* base = (char*)h->scratch;
+    float* g0 = (float*)base;
+    float* g1 = (float*)(base + gbytes);
+    float* u = (float*)(base + 2 * gbytes);
+    float* partial = (float*)(base + 3 * gbytes);
+    int32_t* d_slots = (int32_t*)(base + 3 * gbytes + (size_t)nb * ADJ_NP * sizeof(float));
+    hipStream_t st = h->stream;
+    if (!slots.empty())
+        NLPF_HIP(h, hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    NLPF_HIP(h, hipMemsetAsync(partial, 0, (size_t)nb * ADJ_NP * sizeof(float), st));
+    hipLaunchKernelGGL(bwd_seed_kernel, dim3((uint32_t)((plane * S + 255) / 256)), dim3(256), 0, st, dL_dfilm, g0, S,
+                       plane);
+    const double log2e = 1.4426950408889634;
+    BwdArgs a{};
+    a.rec = (const float*)records;
+    a.level_out = (const float*)level_out;
+    a.partial = partial;
+    a.u = u;
+    a.nodes = h->d_nodes;
+    a.ramp_slot = d_slots;
+    a.root = root;
+    a.W = desc->width;
+    a.H = desc->height;
+    a.S = S;
+    a.L = L;
+    a.plane = plane;
+    a.kp = (float)sqrt(log2e / (2.0 * (double)h->cfg.sigma_pos * h->cfg.sigma_pos));
+    a.kn = (float)sqrt(log2e / (2.0 * (double)h->cfg.sigma_normal * h->cfg.sigma_normal));
+    a.kl = (float)sqrt(log2e / (2.0 * (double)h->cfg.sigma_lum * h->cfg.sigma_lum));
+    a.sig[0] = h->cfg.sigma_pos;
+    a.sig[1] = h->cfg.sigma_normal;
+    a.sig[2] = h->cfg.sigma_lum;
+    rc = record_event(h, 0);
+    if (rc) return rc;
+    float* gin = g0;
+    float* gout = g1;
+    for (uint32_t d = 0; d < L; ++d) {
+        a.d = d;
+        a.has_child = d + 1 < L;
+        a.g_in = gin;
+        a.g_out = gout;
+        NLPF_HIP(h, screen_bwd_level(a, desc->radius, false, st));
+        if (d + 1 < L) NLPF_HIP(h, screen_bwd_level(a, desc->radius, true, st));
+        std::swap(gin, gout);
+    }
+    hipLaunchKernelGGL(bwd_reduce_kernel, dim3(1), dim3(64), 0, st, partial, nb, grad_params);
+    if ((rc = check_launch(h, "backward reduce"))) return rc;
+    rc = record_event(h, 1);
+    if (rc) return rc;
+    NLPF_HIP(h, hipStreamSynchronize(st));
+    h->stats.total_ms = event_ms(h, 0, 1);
+    return NLPF_OK;
+}

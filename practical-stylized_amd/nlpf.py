@@ -103,6 +103,8 @@ _SIGS = [
                                                C.POINTER(_GenDesc), C.c_void_p, C.c_void_p]),
     ("nlpf_convert_records_fp16", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p,
                                             C.c_void_p, C.c_void_p]),
+    ("nlpf_screen_backward", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
     ("nlpf_world_run", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_generate_world_records", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.POINTER(_GenDesc),
@@ -261,6 +263,11 @@ class Handle:
     def screen_run(self, desc, records, anchors=None, film=None, level_out=None):
         self._check(self.lib.nlpf_screen_run(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
                                              _ptr(film), _ptr(level_out)))
+
+    def screen_backward(self, desc, records, level_out, dfilm, grad):
+        """Adjoint pass; grad receives [dsigma_pos, dsigma_normal, dsigma_lum, ramp stops...]."""
+        self._check(self.lib.nlpf_screen_backward(self.h, C.byref(desc), _ptr(records), _ptr(level_out),
+                                                  _ptr(dfilm), _ptr(grad)))
 
     def generate_screen_records(self, desc, records, anchors=None, seed=1, mu=-1.0, sigma=1.5,
                                 firefly_prob=1e-3, firefly_scale=1e3, albedo_lo=0.1, albedo_hi=0.9):

@@ -1,0 +1,109 @@
+"""GPU adjoint of the screen-mode NL-PF (nlpf_screen_backward through the C ABI): checked
+against the FP64 reference gradient of the CPU oracle on the same records, and against
+central finite differences of the GPU's FP64 forward pass (kink-aware: at a kink of the
+piecewise-linear ramp / HSV chain inside the step the derivative equals a one-sided
+difference).  Parameters: sigma_pos, sigma_normal, sigma_lum and the 16 ramp stop values;
+loss = sum (film - target)^2 with the target rendered at perturbed parameters."""
+import numpy as np
+import pytest
+
+import nlpf as N
+import oracle as O
+from gpu_util import gen_screen, host_records_fp32
+
+pytestmark = pytest.mark.gpu
+SIG = np.array([0.05, 0.9, 0.5])
+
+
+def problem(h, W, H, S, L, R, seed):
+    rng = np.random.default_rng(seed)
+    pos = np.sort(rng.random(16)).astype(np.float32)
+    val = np.sort(rng.random(16)).astype(np.float32)
+    table = N.style_table([N.ramp(pos, val), N.hue_shift(0.1, inner=0)])
+    desc, rec, _ = gen_screen(h, W, H, S, L, N.FP32, seed=seed, radius=R)
+    # target: parameters perturbed by N(0, 0.05) (stop values) and exp(N(0, 0.05)) (sigmas)
+    tt = table.copy()
+    tt[0]["p"][16:32] = (tt[0]["p"][16:32] + rng.normal(0, 0.05, 16)).astype(np.float32)
+    tsig = SIG * np.exp(rng.normal(0, 0.05, 3))
+    h.set_style(tt, [1])
+    h.set_sigmas(*tsig)
+    film = h.alloc(W * H * 12)
+    h.screen_run(desc, rec, None, film, None)
+    target = film.download(np.float32).reshape(H, W, 3).astype(np.float64)
+    h.set_style(table, [1])
+    h.set_sigmas(*SIG)
+    return desc, rec, table, target, film
+
+
+def gpu_grad(h, desc, rec, film, target):
+    W, H, S, L = desc.width, desc.height, desc.spp, desc.levels
+    lv = h.alloc(S * L * 3 * W * H * 4)
+    h.screen_run(desc, rec, None, film, lv)
+    f = film.download(np.float32).reshape(H, W, 3).astype(np.float64)
+    dfilm = h.to_device((2 * (f - target)).astype(np.float32))
+    grad = h.alloc(19 * 4)
+    h.screen_backward(desc, rec, lv, dfilm, grad)
+    g = grad.download(np.float32, 19).astype(np.float64)
+    g2 = h.alloc(19 * 4)
+    h.screen_backward(desc, rec, lv, dfilm, g2)
+    assert np.array_equal(g, g2.download(np.float32, 19))  # bitwise reproducible
+    return g, f
+
+
+@pytest.mark.parametrize("R", [3, 7])
+def test_adjoint_matches_fp64_reference_gradient(R):
+    with N.Handle({"type": "pf", "mode": "screen"}) as h:
+        W, H, S, L = 72, 48, 2, 3
+        desc, rec, table, target, film = problem(h, W, H, S, L, R, seed=R)
+        g, f = gpu_grad(h, desc, rec, film, target)
+        host = host_records_fp32(rec, W, H, S, L)
+        ref = O.screen_grad(host, W, H, S, L, R, SIG, table, 1, 2 * (f - target))
+        scale = np.abs(ref).max()
+        assert np.abs(g - ref).max() <= 2e-3 * scale, (g, ref)
+        for k in range(3):
+            assert abs(g[k] - ref[k]) <= 2e-3 * max(abs(ref[k]), 1e-3 * scale), (k, g[k], ref[k])
+
+
+def test_adjoint_matches_fp64_gpu_finite_differences():
+    """C3 shape, scaled down: 4 levels, 15x15 window, ramp(16) + hue shift."""
+    with N.Handle({"type": "pf", "mode": "screen"}) as h:
+        W, H, S, L, R = 240, 136, 2, 4, 7
+        desc, rec, table, target, film = problem(h, W, H, S, L, R, seed=77)
+        g, _ = gpu_grad(h, desc, rec, film, target)
+        host = host_records_fp32(rec, W, H, S, L)
+        rec64 = h.to_device(host.astype(np.float64))
+        d64 = h.screen_desc(W, H, S, L, N.FP64, R)
+        f64 = h.alloc(W * H * 24)
+
+        def loss(sig, tab):
+            h.set_style(tab, [1])
+            h.set_sigmas(*sig)
+            h.screen_run(d64, rec64, None, f64, None)
+            f = f64.download(np.float64).reshape(H, W, 3)
+            return float(((f - target) ** 2).sum())
+
+        l0 = loss(SIG, table)
+        fd = []
+        for k in range(3):
+            hs = 1e-3 * SIG[k]
+            sp, sm = SIG.copy(), SIG.copy()
+            sp[k] += hs
+            sm[k] -= hs
+            lp, lm = loss(sp, table), loss(sm, table)
+            fd.append([(lp - lm) / (2 * hs), (lp - l0) / hs, (l0 - lm) / hs])
+        for m in range(16):
+            tp, tm = table.copy(), table.copy()
+            tp[0]["p"][16 + m] += 1e-3
+            tm[0]["p"][16 + m] -= 1e-3
+            v0 = float(table[0]["p"][16 + m])
+            up, um = float(tp[0]["p"][16 + m]) - v0, v0 - float(tm[0]["p"][16 + m])
+            lp, lm = loss(SIG, tp), loss(SIG, tm)
+            fd.append([(lp - lm) / (up + um), (lp - l0) / up, (l0 - lm) / um])
+        h.set_style(table, [1])
+        h.set_sigmas(*SIG)
+        scale = max(abs(c[0]) for c in fd)
+        for q in range(19):
+            err = min(abs(c - g[q]) for c in fd[q])
+            # h = 1e-3 (the step the north-star check names): many vertices straddle kinks at
+            # this size, so the agreement is at the 1% level of the largest component
+            assert err <= 1e-2 * max(abs(fd[q][0]), 0.05 * scale), (q, fd[q], g[q])
