@@ -16,6 +16,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "practical-stylized_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "filtered path-vertices/s @1080p×16spp, plain vs FP16+MLP; HBM % of peak; L2 err"
 HBM_PEAK_GBS = 8000.0
@@ -138,7 +139,7 @@ def main():
                    "vertices_per_step": verts, "sigmas": list(sig)},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "screen_level_kernel<float,float,7,4,4>",
+                     "kernel": "screen_level_kernel<float,float,7,4,2>",
                      "algorithmic_bytes_per_launch": float(np.mean(lb32)),
                      "avg_launch_ms": float(lv32.mean()),
                      "valu": {"achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS,

@@ -58,18 +58,25 @@ typedef struct {
 } lvl_t;
 static double F(lvl_t l, int c, uint64_t i) { return l.rec[(uint64_t)c * l.N + i]; }
 
+/* Hemisphere and leak tests are decided in FP32, as the reference computes them (its
+ * closures run in f32); the estimator arithmetic itself stays FP64. */
+static float dotf(const double a[3], const double b[3]) {
+    float p0 = (float)a[0] * (float)b[0], p1 = (float)a[1] * (float)b[1], p2 = (float)a[2] * (float)b[2];
+    float s = p0 + p1;
+    return s + p2;
+}
 static int valid_dir(const double ns[3], const double ng[3], const double wo[3], const double wi[3]) {
-#define SG(x) ((x) > 0 ? 1.0 : -1.0)
-#define D3(a, b) ((a)[0] * (b)[0] + (a)[1] * (b)[1] + (a)[2] * (b)[2])
-    double fl = SG(D3(ng, ns));
-    return SG(fl * D3(wo, ns)) * SG(D3(wo, ng)) > 0 && SG(fl * D3(wi, ns)) * SG(D3(wi, ng)) > 0;
+#define SG(x) ((x) > 0 ? 1.0f : -1.0f)
+    float fl = SG(dotf(ng, ns));
+    return SG(fl * dotf(wo, ns)) * SG(dotf(wo, ng)) > 0 && SG(fl * dotf(wi, ns)) * SG(dotf(wi, ng)) > 0;
 }
 /* returns pdf; *fcos = |cos_i| / pi when the pair is valid and same-hemisphere */
 static double diffuse(const double ns[3], const double ng[3], const double wo[3], const double wi[3], double* fcos) {
     *fcos = 0;
     if (!valid_dir(ns, ng, wo, wi)) return 0;
-    double co = D3(wo, ns), ci = D3(wi, ns);
-    if (!(co * ci >= 0)) return 0;
+    float cof = dotf(wo, ns), cif = dotf(wi, ns);
+    if (!(cof * cif >= 0.0f)) return 0;
+    double ci = ns[0] * wi[0] + ns[1] * wi[1] + ns[2] * wi[2];
     *fcos = fabs(ci) * INV_PI;
     return *fcos;
 }

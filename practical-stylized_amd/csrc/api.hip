@@ -1,14 +1,27 @@
 // C-ABI entry points of libnlpf (declared in include/nlpf.h).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include "common.hpp"
 #include "screen_impl.hpp"
 
 namespace nlpf {
-hipError_t screen_level_fp32(const ScreenArgs<float>& a, int radius, hipStream_t st);
-hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, hipStream_t st);
-hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, hipStream_t st);
+hipError_t screen_level_fp32(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st);
+hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st);
+hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, int variant, hipStream_t st);
+int screen_variants_fp32();
+
+// tile variant of the screen gather: NLPF_SCREEN_VARIANT overrides the default (for tuning)
+static int screen_variant() {
+    static int v = -2;
+    if (v == -2) {
+        const char* e = getenv("NLPF_SCREEN_VARIANT");
+        v = e ? atoi(e) : 0;
+        if (v < 0 || v >= screen_variants_fp32()) v = 0;
+    }
+    return v;
+}
 hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
                             float* film, hipStream_t st);
 hipError_t screen_film_fp64(const double* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
@@ -320,7 +333,7 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
             a.kn = kn;
             a.kl = kl;
             a.rscale = 1.0;
-            e = screen_level_fp64(a, desc->radius, h->stream);
+            e = screen_level_fp64(a, desc->radius, 0, h->stream);
         } else {
             ScreenArgs<float> a{};
             a.rec = records;
@@ -345,8 +358,8 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
             a.kn = (float)kn;
             a.kl = (float)kl;
             a.rscale = desc->precision == NLPF_FP16 ? NLPF_FP16_RADIANCE_SCALE : 1.0f;
-            e = desc->precision == NLPF_FP16 ? screen_level_fp16(a, desc->radius, h->stream)
-                                             : screen_level_fp32(a, desc->radius, h->stream);
+            e = desc->precision == NLPF_FP16 ? screen_level_fp16(a, desc->radius, screen_variant(), h->stream)
+                                             : screen_level_fp32(a, desc->radius, screen_variant(), h->stream);
         }
         if (e != hipSuccess) NLPF_FAIL(h, NLPF_E_HIP, std::string("screen level launch: ") + hipGetErrorString(e));
         rc = record_event(h, 1 + (L - 1 - d));

@@ -1,28 +1,34 @@
-// Screen-mode level kernels, fp16 record storage (float accumulation), radius 1..7.
+// Screen-mode level kernels, fp16 record storage (float accumulation), radius 1..7,
+// tile variants (waves per workgroup, rows per lane) selected at run time.
 #include "screen_impl.hpp"
 
 namespace nlpf {
 
-hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, hipStream_t st) {
-    switch (radius) {
-        case 1: return launch_screen_level<__half, float, 1, 4, 4>(a, st);
-        case 2: return launch_screen_level<__half, float, 2, 4, 4>(a, st);
-        case 3: return launch_screen_level<__half, float, 3, 4, 4>(a, st);
-        case 4: return launch_screen_level<__half, float, 4, 4, 4>(a, st);
-        case 5: return launch_screen_level<__half, float, 5, 4, 4>(a, st);
-        case 6: return launch_screen_level<__half, float, 6, 4, 4>(a, st);
-        case 7: return launch_screen_level<__half, float, 7, 4, 4>(a, st);
-        default: return hipErrorInvalidValue;
+template <int RAD>
+static hipError_t dispatch_fp16(const ScreenArgs<float>& a, int variant, hipStream_t st) {
+    constexpr int K = 2 * RAD + 1;
+    switch (variant) {
+        case 1: return launch_screen_level<__half, float, RAD, 4, (4 < K ? 4 : K)>(a, st);
+        case 0: return launch_screen_level<__half, float, RAD, 4, (2 < K ? 2 : K)>(a, st);
+        case 2: return launch_screen_level<__half, float, RAD, 8, (2 < K ? 2 : K)>(a, st);
+        case 3: return launch_screen_level<__half, float, RAD, 2, (4 < K ? 4 : K)>(a, st);
+        default: return launch_screen_level<__half, float, RAD, 4, (2 < K ? 2 : K)>(a, st);
     }
 }
 
-hipError_t screen_film_fp16(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
-                          float* film, hipStream_t st) {
-    const uint32_t bs = 256;
-    const uint64_t nb = (plane + bs - 1) / bs;
-    hipLaunchKernelGGL(screen_film_kernel<float>, dim3((uint32_t)nb), dim3(bs), 0, st, out, S, out_levels, slot0,
-                       plane, film);
-    return hipGetLastError();
+int screen_variants_fp16() { return 4; }
+
+hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st) {
+    switch (radius) {
+        case 1: return dispatch_fp16<1>(a, variant, st);
+        case 2: return dispatch_fp16<2>(a, variant, st);
+        case 3: return dispatch_fp16<3>(a, variant, st);
+        case 4: return dispatch_fp16<4>(a, variant, st);
+        case 5: return dispatch_fp16<5>(a, variant, st);
+        case 6: return dispatch_fp16<6>(a, variant, st);
+        case 7: return dispatch_fp16<7>(a, variant, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace nlpf

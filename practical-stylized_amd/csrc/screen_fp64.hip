@@ -1,17 +1,29 @@
-// Screen-mode level kernels, fp64 record storage (double accumulation), radius 1..7.
+// Screen-mode level kernels, fp64 record storage (double accumulation), radius 1..7,
+// tile variants (waves per workgroup, rows per lane) selected at run time.
 #include "screen_impl.hpp"
 
 namespace nlpf {
 
-hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, hipStream_t st) {
+template <int RAD>
+static hipError_t dispatch_fp64(const ScreenArgs<double>& a, int variant, hipStream_t st) {
+    constexpr int K = 2 * RAD + 1;
+    switch (variant) {
+        case 0: return launch_screen_level<double, double, RAD, 4, (2 < K ? 2 : K)>(a, st);
+        default: return launch_screen_level<double, double, RAD, 4, (2 < K ? 2 : K)>(a, st);
+    }
+}
+
+int screen_variants_fp64() { return 1; }
+
+hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, int variant, hipStream_t st) {
     switch (radius) {
-        case 1: return launch_screen_level<double, double, 1, 4, 2>(a, st);
-        case 2: return launch_screen_level<double, double, 2, 4, 2>(a, st);
-        case 3: return launch_screen_level<double, double, 3, 4, 2>(a, st);
-        case 4: return launch_screen_level<double, double, 4, 4, 2>(a, st);
-        case 5: return launch_screen_level<double, double, 5, 4, 2>(a, st);
-        case 6: return launch_screen_level<double, double, 6, 4, 2>(a, st);
-        case 7: return launch_screen_level<double, double, 7, 4, 2>(a, st);
+        case 1: return dispatch_fp64<1>(a, variant, st);
+        case 2: return dispatch_fp64<2>(a, variant, st);
+        case 3: return dispatch_fp64<3>(a, variant, st);
+        case 4: return dispatch_fp64<4>(a, variant, st);
+        case 5: return dispatch_fp64<5>(a, variant, st);
+        case 6: return dispatch_fp64<6>(a, variant, st);
+        case 7: return dispatch_fp64<7>(a, variant, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -55,11 +55,24 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
     return b < (per << 3) ? (b & 7u) * per + (b >> 3) : b;
 }
 
+template <typename A>
+struct alignas(4 * sizeof(A)) V4 {
+    A x, y, z, w;
+};
+template <typename A>
+struct alignas(2 * sizeof(A)) V2 {
+    A x, y;
+};
+
 template <typename S, typename A, int RAD, int NW, int R>
 __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) {
+    static_assert(R <= 2 * RAD + 1, "rows per lane must not exceed the window height");
     constexpr int TW = 64, TH = NW * R, WT = TW + 2 * RAD, HT = TH + 2 * RAD, NT = 64 * NW;
     constexpr int K = 2 * RAD + 1;
-    __shared__ A sh[10][HT][WT];
+    // guide features and level input of the tile + halo: (p*kp, lum*kl), (n, in.r), (in.g, in.b)
+    __shared__ V4<A> shA[HT * WT];
+    __shared__ V4<A> shB[HT * WT];
+    __shared__ V2<A> shC[HT * WT];
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
@@ -75,7 +88,8 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     for (int i = threadIdx.x; i < WT * HT; i += NT) {
         const int r = i / WT, c = i - r * WT;
         const int gx = tx0 - RAD + c, gy = ty0 - RAD + r;
-        A f[10];
+        V4<A> fa, fb;
+        V2<A> fc;
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const size_t o = (size_t)gy * W + gx;
             A px = (A)ld_rec(rec, o), py = (A)ld_rec(rec, plane + o), pz = (A)ld_rec(rec, 2 * plane + o);
@@ -85,106 +99,106 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
                 py += (A)q[1];
                 pz += (A)q[2];
             }
-            const A nx = (A)ld_rec(rec, 3 * plane + o), ny = (A)ld_rec(rec, 4 * plane + o),
-                    nz = (A)ld_rec(rec, 5 * plane + o);
             const A ax = (A)ld_rec(rec, 6 * plane + o), ay = (A)ld_rec(rec, 7 * plane + o),
                     az = (A)ld_rec(rec, 8 * plane + o);
             A ir = (A)ld_rec(rec, 9 * plane + o) * a.rscale, ig = (A)ld_rec(rec, 10 * plane + o) * a.rscale,
               ib = (A)ld_rec(rec, 11 * plane + o) * a.rscale;
             if (child) {
-                const A tr = (A)ld_rec(rec, 12 * plane + o), tg = (A)ld_rec(rec, 13 * plane + o),
-                        tb = (A)ld_rec(rec, 14 * plane + o);
-                ir = fma(tr, child[o], ir);
-                ig = fma(tg, child[plane + o], ig);
-                ib = fma(tb, child[2 * plane + o], ib);
+                ir = fma((A)ld_rec(rec, 12 * plane + o), child[o], ir);
+                ig = fma((A)ld_rec(rec, 13 * plane + o), child[plane + o], ig);
+                ib = fma((A)ld_rec(rec, 14 * plane + o), child[2 * plane + o], ib);
             }
-            const A l = lum709(v3<A>{ax, ay, az});
-            f[0] = px * a.kp;
-            f[1] = py * a.kp;
-            f[2] = pz * a.kp;
-            f[3] = nx;
-            f[4] = ny;
-            f[5] = nz;
-            f[6] = l * a.kl;
-            f[7] = ir;
-            f[8] = ig;
-            f[9] = ib;
+            fa = V4<A>{px * a.kp, py * a.kp, pz * a.kp, lum709(v3<A>{ax, ay, az}) * a.kl};
+            fb = V4<A>{(A)ld_rec(rec, 3 * plane + o), (A)ld_rec(rec, 4 * plane + o), (A)ld_rec(rec, 5 * plane + o),
+                       ir};
+            fc = V2<A>{ig, ib};
         } else {
             // outside the image: a far-away guide position drives the weight to exactly 0
-            f[0] = f[1] = f[2] = (A)1e18;
-            f[3] = f[4] = f[5] = f[6] = f[7] = f[8] = f[9] = (A)0;
+            fa = V4<A>{(A)1e18, (A)1e18, (A)1e18, (A)0};
+            fb = V4<A>{(A)0, (A)0, (A)0, (A)0};
+            fc = V2<A>{(A)0, (A)0};
         }
-#pragma unroll
-        for (int q = 0; q < 10; ++q) sh[q][r][c] = f[q];
+        shA[i] = fa;
+        shB[i] = fb;
+        shC[i] = fc;
     }
     __syncthreads();
 
     // ---- gather ----
+    // Each lane owns R vertically adjacent outputs; input row r of the lane's strip feeds the
+    // outputs k with 0 <= r - k <= 2 RAD.  The edge rows (r < R-1 or r > 2 RAD) run in fully
+    // unrolled loops where that test is a compile-time constant; the middle rows feed every
+    // output, so no branch sits between the R independent dependency chains.
     const int lc = threadIdx.x & 63;
     const int r0 = (threadIdx.x >> 6) * R;
     A cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R];
     A Wt[R], Sx[R], Sy[R], Sz[R], cW[R], cX[R], cY[R], cZ[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int rr = r0 + k + RAD, cc = lc + RAD;
-        cpx[k] = sh[0][rr][cc];
-        cpy[k] = sh[1][rr][cc];
-        cpz[k] = sh[2][rr][cc];
-        cnx[k] = sh[3][rr][cc];
-        cny[k] = sh[4][rr][cc];
-        cnz[k] = sh[5][rr][cc];
-        cl[k] = sh[6][rr][cc];
+        const V4<A> ca = shA[(r0 + k + RAD) * WT + lc + RAD];
+        const V4<A> cb = shB[(r0 + k + RAD) * WT + lc + RAD];
+        cpx[k] = ca.x;
+        cpy[k] = ca.y;
+        cpz[k] = ca.z;
+        cl[k] = ca.w;
+        cnx[k] = cb.x;
+        cny[k] = cb.y;
+        cnz[k] = cb.z;
         Wt[k] = Sx[k] = Sy[k] = Sz[k] = cW[k] = cX[k] = cY[k] = cZ[k] = (A)0;
     }
     const A kn = a.kn;
-    for (int r = 0; r < R + 2 * RAD; ++r) {
+    auto row = [&](int r, int klo, int khi) __attribute__((always_inline)) {
+        const V4<A>* pa = shA + (r0 + r) * WT + lc;
+        const V4<A>* pb = shB + (r0 + r) * WT + lc;
+        const V2<A>* pc = shC + (r0 + r) * WT + lc;
         A rw[R], rx[R], ry[R], rz[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) rw[k] = rx[k] = ry[k] = rz[k] = (A)0;
-        const int lr = r0 + r;
 #pragma unroll
         for (int dx = 0; dx < K; ++dx) {
-            const int lcol = lc + dx;
-            const A npx = sh[0][lr][lcol], npy = sh[1][lr][lcol], npz = sh[2][lr][lcol];
-            const A nnx = sh[3][lr][lcol], nny = sh[4][lr][lcol], nnz = sh[5][lr][lcol];
-            const A nl = sh[6][lr][lcol];
-            const A nir = sh[7][lr][lcol], nig = sh[8][lr][lcol], nib = sh[9][lr][lcol];
+            const V4<A> na = pa[dx];
+            const V4<A> nb = pb[dx];
+            const V2<A> nc = pc[dx];
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                if (r - k >= 0 && r - k < K) {
-                    const A ex = cpx[k] - npx, ey = cpy[k] - npy, ez = cpz[k] - npz;
+                if (k >= klo && k <= khi) {
+                    const A ex = cpx[k] - na.x, ey = cpy[k] - na.y, ez = cpz[k] - na.z;
                     A e = ex * ex;
                     e = fma(ey, ey, e);
                     e = fma(ez, ez, e);
-                    A nn = cnx[k] * nnx;
-                    nn = fma(cny[k], nny, nn);
-                    nn = fma(cnz[k], nnz, nn);
+                    A nn = cnx[k] * nb.x;
+                    nn = fma(cny[k], nb.y, nn);
+                    nn = fma(cnz[k], nb.z, nn);
                     const A tt = fma(-kn, nn, kn);
                     e = fma(tt, tt, e);
-                    const A dl = cl[k] - nl;
+                    const A dl = cl[k] - na.w;
                     e = fma(dl, dl, e);
                     const A w = exp2_neg(e);
                     rw[k] += w;
-                    rx[k] = fma(w, nir, rx[k]);
-                    ry[k] = fma(w, nig, ry[k]);
-                    rz[k] = fma(w, nib, rz[k]);
+                    rx[k] = fma(w, nb.w, rx[k]);
+                    ry[k] = fma(w, nc.x, ry[k]);
+                    rz[k] = fma(w, nc.y, rz[k]);
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            if (r - k >= 0 && r - k < K) {
+            if (k >= klo && k <= khi) {
                 kahan_add(Wt[k], cW[k], rw[k]);
                 kahan_add(Sx[k], cX[k], rx[k]);
                 kahan_add(Sy[k], cY[k], ry[k]);
                 kahan_add(Sz[k], cZ[k], rz[k]);
             }
         }
-    }
+    };
+#pragma unroll
+    for (int r = 0; r < R - 1; ++r) row(r, 0, r);
+    for (int r = R - 1; r <= 2 * RAD; ++r) row(r, 0, R - 1);
+#pragma unroll
+    for (int i = 0; i < R - 1; ++i) row(2 * RAD + 1 + i, i + 1, R - 1);
 
     // ---- epilogue: normalise, stylize, store ----
     A* outp = a.out + ((size_t)s * a.out_levels + a.out_slot) * 3 * plane;
-    const S* recp = rec;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int ox = tx0 + lc, oy = ty0 + r0 + k;
@@ -194,7 +208,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
             const size_t o = (size_t)oy * W + ox;
             v3<A> p{(A)0, (A)0, (A)0};
             if (a.need_pos) {
-                p = v3<A>{(A)ld_rec(recp, o), (A)ld_rec(recp, plane + o), (A)ld_rec(recp, 2 * plane + o)};
+                p = v3<A>{(A)ld_rec(rec, o), (A)ld_rec(rec, plane + o), (A)ld_rec(rec, 2 * plane + o)};
                 if (anc) {
                     const float* q = anc + ((size_t)(oy >> 3) * a.abx + (ox >> 3)) * 3;
                     p.x += (A)q[0];

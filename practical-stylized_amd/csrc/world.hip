@@ -54,21 +54,26 @@ __device__ __forceinline__ void get_cell(const Grid& g, float px, float py, floa
     }
 }
 
+// dot product without fused multiply-adds: the reference decides hemisphere and leak tests
+// in plain FP32, so every sign test below rounds exactly like that arithmetic
+__device__ __forceinline__ float dot_rn(float3 a, float3 b) {
+    return __fadd_rn(__fadd_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)), __fmul_rn(a.z, b.z));
+}
 // SurfaceClosure::check_wo_wi_valid (svm/surface/mod.rs:1701-1716)
 __device__ __forceinline__ bool wo_wi_valid(float3 ns, float3 ng, float3 wo, float3 wi) {
     auto sgn = [](float x) { return x > 0.0f ? 1.0f : -1.0f; };
-    auto d3 = [](float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
-    const float flipped = sgn(d3(ng, ns));
-    return (sgn(flipped * d3(wo, ns)) * sgn(d3(wo, ng)) > 0.0f) && (sgn(flipped * d3(wi, ns)) * sgn(d3(wi, ng)) > 0.0f);
+    const float flipped = sgn(dot_rn(ng, ns));
+    return (sgn(flipped * dot_rn(wo, ns)) * sgn(dot_rn(wo, ng)) > 0.0f) &&
+           (sgn(flipped * dot_rn(wi, ns)) * sgn(dot_rn(wi, ng)) > 0.0f);
 }
 // DiffuseBsdf::evaluate_impl (svm/surface/diffuse.rs:22-41) through SurfaceClosure::evaluate
 // (mod.rs:1725-1744): returns the pdf; fcos = |cos_i| / pi (multiply by R for f*cos)
 __device__ __forceinline__ float diffuse_eval(float3 ns, float3 ng, float3 wo, float3 wi, float& fcos) {
     fcos = 0.0f;
     if (!wo_wi_valid(ns, ng, wo, wi)) return 0.0f;
-    const float co = wo.x * ns.x + wo.y * ns.y + wo.z * ns.z;
-    const float ci = wi.x * ns.x + wi.y * ns.y + wi.z * ns.z;
-    if (!(co * ci >= 0.0f)) return 0.0f;
+    const float co = dot_rn(wo, ns);
+    const float ci = dot_rn(wi, ns);
+    if (!(__fmul_rn(co, ci) >= 0.0f)) return 0.0f;
     const float pdf = fabsf(ci) * kInvPi;
     fcos = pdf;
     return pdf;
@@ -77,6 +82,14 @@ __device__ __forceinline__ float diffuse_eval(float3 ns, float3 ng, float3 wo, f
 // |d| without fused multiply-adds, so the radius test rounds exactly like plain IEEE code
 __device__ __forceinline__ float dist3(float x, float y, float z) {
     return sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
+}
+
+template <typename T>
+__device__ __forceinline__ void kahan_add(T& sum, T& comp, T v) {
+    T y = v - comp;
+    T t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
 }
 
 __device__ __forceinline__ float pl(const float* rec, uint64_t N, int c, uint64_t i) { return rec[(uint64_t)c * N + i]; }
@@ -203,7 +216,9 @@ __global__ void __launch_bounds__(256) w_filter_kernel(FilterArgs a) {
     atomicAdd(a.nbr_count, (unsigned long long)cnt);
     const uint32_t used = cnt < a.K ? cnt : a.K;
     const float inv_used = 1.0f / (float)used;
-    float3 direct = make_float3(0, 0, 0), indirect = make_float3(0, 0, 0);
+    // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
+    // the exact sum (a plain running sum drifts by ~n ulp)
+    float dsum[3] = {0, 0, 0}, dc[3] = {0, 0, 0}, isum[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
     for (uint32_t k = 0; k < used; ++k) {
         const uint32_t j = buf[k];
         const NbrRec& r = a.pk[j];
@@ -213,10 +228,10 @@ __global__ void __launch_bounds__(256) w_filter_kernel(FilterArgs a) {
             const float bpdf = diffuse_eval(ns, ng, wo, make_float3(dw.x, dw.y, dw.z), fcos);
             const float4 di = r.dirr_mis;
             const float w = dw.w / (dw.w + bpdf);  // mis_weight, balance heuristic (:305-316)
-            const float s = fcos * di.w * w / dw.w * inv_used;
-            direct.x += R.x * s * di.x;
-            direct.y += R.y * s * di.y;
-            direct.z += R.z * s * di.z;
+            const float sc = fcos * di.w * w / dw.w * inv_used;
+            kahan_add(dsum[0], dc[0], R.x * sc * di.x);
+            kahan_add(dsum[1], dc[1], R.y * sc * di.y);
+            kahan_add(dsum[2], dc[2], R.z * sc * di.z);
         }
         const float4 iw = r.iwi_mis;
         const float3 wi = make_float3(iw.x, iw.y, iw.z);
@@ -226,23 +241,26 @@ __global__ void __launch_bounds__(256) w_filter_kernel(FilterArgs a) {
         const float4 ir = r.iirr;
         const float3 reused = make_float3(R.x * fcos * ir.x * iw.w, R.y * fcos * ir.y * iw.w, R.z * fcos * ir.z * iw.w);
         if (a.smis) {
-            float sum_pdf = 0.0f;
+            float sum_pdf = 0.0f, sc = 0.0f;
             for (uint32_t m = 0; m < used; ++m) {
                 const NbrRec& o = a.pk[buf[m]];
                 const float4 a0 = o.ns_wox, a1 = o.ng_woy, a2 = o.woz;
                 float fc;
-                sum_pdf += diffuse_eval(make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
-                                        make_float3(a0.w, a1.w, a2.x), wi, fc);
+                kahan_add(sum_pdf, sc,
+                          diffuse_eval(make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
+                                       make_float3(a0.w, a1.w, a2.x), wi, fc));
             }
-            indirect.x += reused.x / sum_pdf;
-            indirect.y += reused.y / sum_pdf;
-            indirect.z += reused.z / sum_pdf;
+            kahan_add(isum[0], ic[0], reused.x / sum_pdf);
+            kahan_add(isum[1], ic[1], reused.y / sum_pdf);
+            kahan_add(isum[2], ic[2], reused.z / sum_pdf);
         } else if (bpdf > 0.0f) {
-            indirect.x += reused.x / bpdf * inv_used;
-            indirect.y += reused.y / bpdf * inv_used;
-            indirect.z += reused.z / bpdf * inv_used;
+            kahan_add(isum[0], ic[0], reused.x / bpdf * inv_used);
+            kahan_add(isum[1], ic[1], reused.y / bpdf * inv_used);
+            kahan_add(isum[2], ic[2], reused.z / bpdf * inv_used);
         }
     }
+    const float3 direct = make_float3(dsum[0], dsum[1], dsum[2]);
+    const float3 indirect = make_float3(isum[0], isum[1], isum[2]);
     v3<float> L{le.x + direct.x + indirect.x, le.y + direct.y + indirect.y, le.z + direct.z + indirect.z};
     if (a.stylize) {
         const uint32_t m = a.material[c];

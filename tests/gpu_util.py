@@ -5,6 +5,15 @@ import numpy as np
 import nlpf as N
 
 
+def ramp_stops(rng, n=16, min_gap=1.0 / 64):
+    """Sorted U[0,1] stop positions pushed apart to at least min_gap: a near-vertical ramp
+    segment amplifies one FP32 ulp of luminance beyond any fixed per-pixel tolerance."""
+    pos = np.sort(rng.random(n))
+    pos = pos * (1 - min_gap * (n - 1)) + min_gap * np.arange(n)
+    val = np.sort(rng.random(n))
+    return pos.astype(np.float32), val.astype(np.float32)
+
+
 def gen_screen(h, W, H, S, L, precision=N.FP32, seed=1, radius=7):
     desc = h.screen_desc(W, H, S, L, precision, radius)
     plane = W * H

@@ -9,7 +9,7 @@ import pytest
 
 import nlpf as N
 import oracle as O
-from gpu_util import gen_screen, host_records_fp32
+from gpu_util import gen_screen, host_records_fp32, ramp_stops
 
 pytestmark = pytest.mark.gpu
 SIG = np.array([0.05, 0.9, 0.5])
@@ -17,8 +17,7 @@ SIG = np.array([0.05, 0.9, 0.5])
 
 def problem(h, W, H, S, L, R, seed):
     rng = np.random.default_rng(seed)
-    pos = np.sort(rng.random(16)).astype(np.float32)
-    val = np.sort(rng.random(16)).astype(np.float32)
+    pos, val = ramp_stops(rng)
     table = N.style_table([N.ramp(pos, val), N.hue_shift(0.1, inner=0)])
     desc, rec, _ = gen_screen(h, W, H, S, L, N.FP32, seed=seed, radius=R)
     # target: parameters perturbed by N(0, 0.05) (stop values) and exp(N(0, 0.05)) (sigmas)

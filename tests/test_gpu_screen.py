@@ -11,7 +11,7 @@ import pytest
 
 import nlpf as N
 import oracle as O
-from gpu_util import decode_fp16, gen_screen, host_records_fp32, pixel_rel_err, run_screen
+from gpu_util import decode_fp16, gen_screen, host_records_fp32, pixel_rel_err, ramp_stops, run_screen
 
 pytestmark = pytest.mark.gpu
 SIG = (0.05, 0.9, 0.5)
@@ -24,9 +24,7 @@ def h():
 
 
 def ramp_hue_table(seed=7):
-    rng = np.random.default_rng(seed)
-    pos = np.sort(rng.random(16)).astype(np.float32)
-    val = np.sort(rng.random(16)).astype(np.float32)
+    pos, val = ramp_stops(np.random.default_rng(seed))
     return N.style_table([N.ramp(pos, val), N.hue_shift(0.1, inner=0)]), 1
 
 
@@ -37,6 +35,8 @@ def test_config_echo(h):
             assert cfg[k] == "screen"
         elif isinstance(v, float):
             assert abs(cfg[k] - v) < 1e-6, k
+        elif isinstance(v, list):
+            assert np.allclose(cfg[k], v, rtol=1e-6), k
         else:
             assert cfg[k] == v, k
 
@@ -97,7 +97,8 @@ def test_fp64_kernel_parity(h):
     rec64 = h.to_device(host.astype(np.float64))
     desc64 = h.screen_desc(W, H, S, L, N.FP64, 7)
     film, lv = run_screen(h, desc64, rec64)
-    of, olv = O.screen_run(host, W, H, S, L, 7, SIG, nodes=table, root=root, want_levels=True)
+    sig32 = tuple(float(np.float32(v)) for v in SIG)  # the library keeps the bandwidths in FP32
+    of, olv = O.screen_run(host, W, H, S, L, 7, sig32, nodes=table, root=root, want_levels=True)
     check_levels(lv, olv, 1e-10)
     assert pixel_rel_err(film, of).max() <= 1e-10
     rec32.free()
