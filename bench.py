@@ -59,20 +59,32 @@ def timed_runs(h, desc, rec, anc, film, steps, warmup):
     return wall / steps * 1e3, lv_ms / steps
 
 
-def cpu_baseline(h, rec, W, H, S, L, radius, sig, table, root):
-    """FP32 OpenMP restatement (oracle/) on one spp slice of the same records."""
+def cpu_baseline(h, rec, W, H, S, L, radius, sig, table, root, slices):
+    """FP32 OpenMP restatement (oracle/) on the first `slices` spp slices of the same records."""
     import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     plane = W * H
-    host = rec.download(np.float32, L * 15 * plane).reshape(1, L, 15, H, W)  # slice 0
+    host = rec.download(np.float32, slices * L * 15 * plane).reshape(slices, L, 15, H, W)
     t0 = time.perf_counter()
-    O.screen_run(host, W, H, 1, L, radius, sig, nodes=table, root=root, precision="f32", threads=threads)
+    O.screen_run(host, W, H, slices, L, radius, sig, nodes=table, root=root, precision="f32", threads=threads)
     dt = time.perf_counter() - t0
-    verts = plane * L
+    verts = plane * L * slices
     return {"value": verts / dt, "unit": "vertices/s", "cores": threads, "kind": "port",
-            "sample": f"1 of {S} spp slices ({W}x{H}x{L} levels = {verts} vertices), FP32, "
-                      f"{threads} OpenMP threads, {dt:.1f} s"}
+            "sample": f"{slices} of {S} spp slices ({W}x{H}x{L} levels x {slices} = {verts} vertices), "
+                      f"FP32 OpenMP restatement of the same filter, {threads} threads, {dt:.1f} s"}
+
+
+def measured_traffic():
+    """HBM bytes per launch of the level kernel from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "screen_level_fp32_1080p16spp_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -86,8 +98,11 @@ def main():
     ap.add_argument("--max-depth", type=int, default=4)
     ap.add_argument("--radius", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-slices", type=int, default=16, help="spp slices timed by the CPU baseline")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one FP32 pass and one 1 GiB device copy, for rocprofv3 counter runs")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -112,6 +127,16 @@ def main():
     rec32 = h.alloc(verts * 60)
     h.generate_screen_records(d32, rec32, seed=1)
     film32 = h.alloc(plane * 12)
+
+    if args.pmc_probe:
+        h.screen_run(d32, rec32, None, film32, None)
+        src = h.alloc(1 << 30)
+        dst = h.alloc(1 << 30)
+        h.copy(dst, src, 1 << 30)
+        h.sync()
+        log("pmc probe: one FP32 pass (levels 4..0 + film) and one 1 GiB device-to-device copy")
+        h.close()
+        return
 
     log("timing FP32")
     ms32, lv32 = timed_runs(h, d32, rec32, None, film32, args.steps, args.warmup)
@@ -138,7 +163,8 @@ def main():
                                f"{2 * R + 1}x{2 * R + 1} window, ramp16+hue0.1, FP32 records",
                    "vertices_per_step": verts, "sigmas": list(sig)},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if (W, H, S, L, R) == (1920, 1080, 16, 5, 7) else None,
                      "kernel": "screen_level_kernel<float,float,7,4,2>",
                      "algorithmic_bytes_per_launch": float(np.mean(lb32)),
                      "avg_launch_ms": float(lv32.mean()),
@@ -186,7 +212,8 @@ def main():
 
     if not args.no_cpu:
         log("CPU baseline (oracle FP32 restatement)")
-        result["cpu_baseline"] = cpu_baseline(h, rec32, W, H, S, L, R, sig, table, root)
+        result["cpu_baseline"] = cpu_baseline(h, rec32, W, H, S, L, R, sig, table, root,
+                                              max(1, min(args.cpu_slices, S)))
         result["cpu_baseline"]["gpu_speedup"] = result["value"] / result["cpu_baseline"]["value"]
     else:
         result["cpu_baseline"] = None

@@ -131,6 +131,7 @@ int nlpf_free(nlpf_handle* h, void* dptr);
 int nlpf_upload(nlpf_handle* h, void* dst_device, const void* src_host, size_t bytes);
 int nlpf_download(nlpf_handle* h, void* dst_host, const void* src_device, size_t bytes);
 int nlpf_memset(nlpf_handle* h, void* dst_device, int value, size_t bytes);
+int nlpf_copy(nlpf_handle* h, void* dst_device, const void* src_device, size_t bytes);
 int nlpf_sync(nlpf_handle* h);
 
 /* ---- stylization ----

@@ -201,6 +201,11 @@ int nlpf_memset(nlpf_handle* h, void* dst, int value, size_t bytes) {
     NLPF_HIP(h, hipMemsetAsync(dst, value, bytes, h->stream));
     return NLPF_OK;
 }
+int nlpf_copy(nlpf_handle* h, void* dst, const void* src, size_t bytes) {
+    if (!h || (!dst && bytes) || (!src && bytes)) return NLPF_E_INVALID_ARG;
+    NLPF_HIP(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream));
+    return NLPF_OK;
+}
 int nlpf_sync(nlpf_handle* h) {
     if (!h) return NLPF_E_INVALID_ARG;
     NLPF_HIP(h, hipStreamSynchronize(h->stream));

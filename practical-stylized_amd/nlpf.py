@@ -92,6 +92,7 @@ _SIGS = [
     ("nlpf_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("nlpf_download", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("nlpf_memset", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]),
+    ("nlpf_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("nlpf_sync", C.c_int, [C.c_void_p]),
     ("nlpf_set_style", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("nlpf_stylize", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
@@ -222,6 +223,9 @@ class Handle:
     def to_device(self, arr):
         arr = np.ascontiguousarray(arr)
         return self.alloc(arr.nbytes).upload(arr)
+
+    def copy(self, dst, src, nbytes):
+        self._check(self.lib.nlpf_copy(self.h, _ptr(dst), _ptr(src), nbytes))
 
     def sync(self):
         self._check(self.lib.nlpf_sync(self.h))
