@@ -245,6 +245,39 @@ int nlpf_generate_world_records(nlpf_handle* h, const nlpf_world_desc* desc,
                                 const nlpf_gen_desc* gen, float* records, uint8_t* flags,
                                 uint8_t* new_flags, uint16_t* material, int32_t* path_depth);
 
+/* ---- learned inner-expectation cache (radiance-cache MLP) ----
+ * 84 inputs (clamped position, sin/cos of 2^k pi p for k = 0..11 per coordinate, normal,
+ * albedo, view direction towards the parent vertex) padded to 96, five hidden layers of 64
+ * with ReLU, 3 linear outputs (padded to 16); BF16 operands on MFMA with FP32 accumulation,
+ * FP32 master weights and Adam.  Parameters: W_l [out][in] then b_l [out] for l = 0..5
+ * (nlpf_mlp_param_count() floats).  Samples address screen records as slice*W*H + pixel
+ * at level `depth`; `feats` ([n][96] floats, device) replaces the record encoder when given.
+ * No reference counterpart: the reference's learned cache is tiny-cuda-nn's NRC
+ * (crates/akari_integrator/src/nrc_pt.rs:1159-1353), a different network. */
+uint64_t nlpf_mlp_param_count(void);
+int nlpf_mlp_init(nlpf_handle* h, uint64_t seed);
+int nlpf_mlp_get_params(nlpf_handle* h, float* host_params, uint64_t n);
+int nlpf_mlp_set_params(nlpf_handle* h, const float* host_params, uint64_t n);
+int nlpf_mlp_encode(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                    const float* anchors, uint32_t depth, const uint32_t* idx, uint64_t n,
+                    float* feats);
+int nlpf_mlp_infer(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                   const float* anchors, uint32_t depth, const uint32_t* idx, const float* feats,
+                   uint64_t n, float* out_rgb);
+/* One training step on `batch` samples (a multiple of 128) against target [batch][3] (device):
+ * L2 loss mean_i |y_i - t_i|^2, Adam (beta 0.9/0.999, eps 1e-8) with learning rate lr when
+ * apply_update != 0.  loss (host) receives the pre-update loss, grad (device, optional,
+ * nlpf_mlp_param_count() floats) the gradient. */
+int nlpf_mlp_train_step(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                        const float* anchors, uint32_t depth, const uint32_t* idx,
+                        const float* feats, const float* target, uint32_t batch, float lr,
+                        int apply_update, float* loss, float* grad);
+/* FP64 VALU evaluation of the same network (bf16 weights and activations as on MFMA, FP64
+ * sums): the error oracle of the learned path. */
+int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                       const float* anchors, uint32_t depth, const uint32_t* idx, uint64_t n,
+                       double* out_rgb);
+
 /* ---- synthetic records (procedural unit box + 2 spheres + quad light) ----
  * Traces one path per (pixel, spp slice) through the analytic scene and writes screen-mode
  * records: p, n from the hits; albedo U[0.1,0.9]; radiance lognormal(mu, sigma) with a

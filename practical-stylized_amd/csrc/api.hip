@@ -35,6 +35,7 @@ hipError_t launch_stylize(const float* in, float* out, uint64_t n, const uint32_
                           const nlpf_style_node* nodes, const int32_t* roots, uint32_t n_materials, hipStream_t st);
 
 void world_release(nlpf_handle* h);
+void mlp_release(nlpf_handle* h);
 
 int ensure_scratch(nlpf_handle* h, size_t bytes) {
     if (h->scratch_bytes >= bytes) return NLPF_OK;
@@ -136,6 +137,7 @@ int nlpf_destroy(nlpf_handle* h) {
     if (!h) return NLPF_E_INVALID_ARG;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     world_release(h);
+    mlp_release(h);
     if (h->d_nodes) (void)hipFree(h->d_nodes);
     if (h->d_roots) (void)hipFree(h->d_roots);
     if (h->scratch) (void)hipFree(h->scratch);

@@ -74,7 +74,10 @@ __device__ __forceinline__ float diffuse_eval(float3 ns, float3 ng, float3 wo, f
     const float co = dot_rn(wo, ns);
     const float ci = dot_rn(wi, ns);
     if (!(__fmul_rn(co, ci) >= 0.0f)) return 0.0f;
-    const float pdf = fabsf(ci) * kInvPi;
+    // |cos| near grazing is a cancellation-prone difference of products: take its magnitude
+    // from an FP64 dot (the FP32 value above only decides the hemisphere, as in the reference)
+    const double cd = (double)wi.x * ns.x + (double)wi.y * ns.y + (double)wi.z * ns.z;
+    const float pdf = (float)(fabs(cd) * 0.318309886183790671538);
     fcos = pdf;
     return pdf;
 }

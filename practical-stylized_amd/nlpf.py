@@ -106,6 +106,19 @@ _SIGS = [
                                             C.c_void_p, C.c_void_p]),
     ("nlpf_screen_backward", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p]),
+    ("nlpf_mlp_param_count", C.c_uint64, []),
+    ("nlpf_mlp_init", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("nlpf_mlp_get_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nlpf_mlp_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("nlpf_mlp_encode", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p, C.c_uint32,
+                                  C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("nlpf_mlp_infer", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p, C.c_uint32,
+                                 C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("nlpf_mlp_train_step", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_int,
+                                      C.POINTER(C.c_float), C.c_void_p]),
+    ("nlpf_mlp_reference", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.c_void_p, C.c_uint64, C.c_void_p]),
     ("nlpf_world_run", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_generate_world_records", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.POINTER(_GenDesc),
@@ -282,6 +295,41 @@ class Handle:
     def convert_records_fp16(self, desc, records32, records16, anchors):
         self._check(self.lib.nlpf_convert_records_fp16(self.h, C.byref(desc), _ptr(records32),
                                                        _ptr(records16), _ptr(anchors)))
+
+    # ---- learned cache ----
+    def mlp_param_count(self):
+        return int(self.lib.nlpf_mlp_param_count())
+
+    def mlp_init(self, seed=0):
+        self._check(self.lib.nlpf_mlp_init(self.h, seed))
+
+    def mlp_get_params(self):
+        out = np.empty(self.mlp_param_count(), np.float32)
+        self._check(self.lib.nlpf_mlp_get_params(self.h, out.ctypes.data_as(C.c_void_p), len(out)))
+        return out
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self.lib.nlpf_mlp_set_params(self.h, p.ctypes.data_as(C.c_void_p), len(p)))
+
+    def mlp_encode(self, desc, records, anchors, depth, idx, n, feats):
+        self._check(self.lib.nlpf_mlp_encode(self.h, C.byref(desc), _ptr(records), _ptr(anchors), depth,
+                                             _ptr(idx), n, _ptr(feats)))
+
+    def mlp_infer(self, desc, records, anchors, depth, idx, feats, n, out):
+        self._check(self.lib.nlpf_mlp_infer(self.h, None if desc is None else C.byref(desc), _ptr(records),
+                                            _ptr(anchors), depth, _ptr(idx), _ptr(feats), n, _ptr(out)))
+
+    def mlp_train_step(self, desc, records, anchors, depth, idx, feats, target, batch, lr, apply=True, grad=None):
+        loss = C.c_float()
+        self._check(self.lib.nlpf_mlp_train_step(self.h, None if desc is None else C.byref(desc), _ptr(records),
+                                                 _ptr(anchors), depth, _ptr(idx), _ptr(feats), _ptr(target), batch,
+                                                 lr, int(apply), C.byref(loss), _ptr(grad)))
+        return loss.value
+
+    def mlp_reference(self, desc, records, anchors, depth, idx, n, out):
+        self._check(self.lib.nlpf_mlp_reference(self.h, C.byref(desc), _ptr(records), _ptr(anchors), depth,
+                                                _ptr(idx), n, _ptr(out)))
 
     def stats(self):
         s = _Stats()
