@@ -278,6 +278,42 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
                        const float* anchors, uint32_t depth, const uint32_t* idx, uint64_t n,
                        double* out_rgb);
 
+/* Screen-mode NL-PF with the learned cache standing in for the filtered estimate at depth
+ * `depth` (records deeper than `depth` are read only for the judge slice and fallback tiles):
+ *   judge slice  spp slice `judge_slice` filtered exactly at every level: training targets
+ *                (its unstylized depth-`depth` values at `train_records` vertices, Adam steps
+ *                of `batch`, rate `lr`) and its exact level-0 image;
+ *   inference    the network's estimate at depth `depth` for every slice, stylized there;
+ *   judge        levels depth-1..0 of the judge slice rerun on the estimate; 64x8 tiles whose
+ *                relative L2 error against the exact level-0 image exceeds `tile_threshold`
+ *                are filtered exactly at depths levels-1..depth in every slice (with the
+ *                dependency halo: the level-0 tile mask dilated by one tile per level);
+ *   levels depth-1..0 and the film as nlpf_screen_run.
+ * The network must be initialised (nlpf_mlp_init); training continues across calls.
+ * film: float [H][W][3]; level_out (optional): float [spp][levels][3][H][W], where the levels
+ * deeper than `depth` of the non-judge slices are defined only inside fallback tiles. */
+typedef struct nlpf_mlp_run_desc {
+    uint32_t depth;
+    uint32_t judge_slice;
+    uint32_t train_records;
+    uint32_t batch;
+    float lr;
+    float tile_threshold;
+    uint64_t seed;
+} nlpf_mlp_run_desc;
+typedef struct nlpf_mlp_run_stats {
+    double total_ms, judge_ms, train_ms, infer_ms, fallback_ms, filter_ms;
+    uint64_t tiles;                   /* 64x8 tiles per image */
+    uint64_t fallback_tiles;          /* level-0 tiles over the threshold */
+    uint64_t fallback_tiles_filtered; /* tile-levels refiltered exactly per slice (halo included) */
+    float train_loss;                 /* last training batch's loss */
+    float cache_rel_l2;               /* estimate vs exact unstylized depth-`depth` values, judge slice */
+    float judge_rel_l2;               /* level-0 image from the estimate vs exact, judge slice */
+} nlpf_mlp_run_stats;
+int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                        const float* anchors, const nlpf_mlp_run_desc* run, float* film,
+                        float* level_out, nlpf_mlp_run_stats* stats);
+
 /* ---- synthetic records (procedural unit box + 2 spheres + quad light) ----
  * Traces one path per (pixel, spp slice) through the analytic scene and writes screen-mode
  * records: p, n from the hits; albedo U[0.1,0.9]; radiance lognormal(mu, sigma) with a

@@ -207,7 +207,9 @@ struct MlpInferArgs {
     const uint32_t* idx;  // sample list (slice * W*H + pixel) or nullptr for 0..n-1
     const float* feats;   // optional raw features [n][96] instead of records
     uint64_t n;
-    float* out;           // [n][3]
+    float* out;           // [n][3], or planar [slice][out_levels][3][W*H] (slot out_slot) when planar != 0
+    int planar;
+    uint32_t out_levels, out_slot;
 };
 
 __global__ void __launch_bounds__(256) mlp_infer_kernel(MlpInferArgs a) {
@@ -243,9 +245,19 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(MlpInferArgs a) {
             for (int q = 0; q < MLP_QT; ++q) {
                 const uint64_t i = base + 16 * q + lane;
                 if (i < a.n) {
-                    a.out[3 * i] = y[q][0];
-                    a.out[3 * i + 1] = y[q][1];
-                    a.out[3 * i + 2] = y[q][2];
+                    if (a.planar) {
+                        const uint64_t sm = a.idx ? a.idx[i] : i;
+                        const uint64_t plane = (uint64_t)a.W * a.H;
+                        const uint64_t sl = sm / plane, px = sm - sl * plane;
+                        float* o = a.out + ((uint64_t)sl * a.out_levels + a.out_slot) * 3 * plane + px;
+                        o[0] = y[q][0];
+                        o[plane] = y[q][1];
+                        o[2 * plane] = y[q][2];
+                    } else {
+                        a.out[3 * i] = y[q][0];
+                        a.out[3 * i + 1] = y[q][1];
+                        a.out[3 * i + 2] = y[q][2];
+                    }
                 }
             }
         }
@@ -933,3 +945,23 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
 }
 
 }  // extern "C"
+
+namespace nlpf {
+// internal entry points used by the learned-cache NL-PF pass (mlp_run.hip)
+// estimates for samples 0..n-1 written into slot `out_slot` of planar level data
+// [slice][out_levels][3][W*H]
+int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                     uint32_t depth, uint64_t n, float* planar_out, uint32_t out_levels, uint32_t out_slot) {
+    nlpf_mlp_state* s;
+    int rc = mlp_need(h, &s);
+    if (rc) return rc;
+    MlpInferArgs a = infer_args(s, desc, records, anchors, depth, nullptr, nullptr, n, planar_out);
+    a.planar = 1;
+    a.out_levels = out_levels;
+    a.out_slot = out_slot;
+    const uint64_t per_block = 4 * 16 * MLP_QT;
+    const uint64_t blocks = std::min<uint64_t>((n + per_block - 1) / per_block, 256 * 64);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3((uint32_t)blocks), dim3(256), 0, h->stream, a);
+    return check_launch(h, "mlp infer");
+}
+}  // namespace nlpf

@@ -1,0 +1,354 @@
+// Screen-mode NL-PF with the learned inner-expectation cache at depth k.
+//
+// Per frame (one call):
+//   1. judge slice  levels D..k of one spp slice filtered exactly (plain NL-PF); the unstylized
+//                   depth-k values F_k are the training targets; levels k-1..0 from the exact
+//                   depth-k output give the slice's exact level-0 image E;
+//   2. train        `train_records` judge vertices, Adam steps of `batch` on BF16 MFMA;
+//   3. infer        the network estimates F_k for every slice (levels > k are not read),
+//                   stylized at depth k like a filtered value;
+//   4. judge        levels k-1..0 of the judge slice rerun on the estimate give P; per 64x8
+//                   tile, the relative L2 of P against E is the error the cache puts into the
+//                   image;
+//   5. fallback     tiles above the threshold are filtered exactly at depths D..k in every
+//                   slice: a level-0 tile depends on depth-d tiles up to d tiles away (window
+//                   radius <= 7 < tile height 8), so the depth-d mask is the level-0 mask
+//                   dilated d times;
+//   6. levels k-1..0 plain NL-PF on the mixed depth-k output, then the film.
+#include "common.hpp"
+#include "screen_impl.hpp"
+#include "scene.hpp"
+
+namespace nlpf {
+hipError_t screen_level_fp32(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st);
+hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st);
+hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
+                            float* film, hipStream_t st);
+int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                     uint32_t depth, uint64_t n, float* planar_out, uint32_t out_levels, uint32_t out_slot);
+
+__global__ void gather_targets_kernel(const float* F, uint64_t plane, const uint32_t* idx, uint32_t n, float* t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t px = idx[i];
+    t[3 * i] = F[px];
+    t[3 * i + 1] = F[plane + px];
+    t[3 * i + 2] = F[2 * plane + px];
+}
+
+// out = S_d(in) on planar [S][3][plane] level data (flags 0, material 0)
+template <typename S>
+__global__ void style_planar_kernel(float* io, uint32_t Sn, uint64_t plane, uint32_t out_levels, uint32_t slot,
+                                    const nlpf_style_node* nodes, int root, uint32_t d, const S* rec,
+                                    const float* anchors, uint32_t W, uint32_t H, uint32_t L, int need_pos) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane * Sn) return;
+    const uint64_t s = i / plane, px = i - s * plane;
+    float* o = io + (s * out_levels + slot) * 3 * plane + px;
+    v3<float> p{0.0f, 0.0f, 0.0f};
+    if (need_pos) {
+        const S* r = rec + (s * L + d) * NLPF_SCREEN_PLANES * plane + px;
+        p = v3<float>{(float)r[0], (float)r[plane], (float)r[2 * plane]};
+        if (anchors) {
+            const uint32_t abx = (W + 7) / 8, aby = (H + 7) / 8, x = (uint32_t)(px % W), y = (uint32_t)(px / W);
+            const float* q = anchors + ((s * L + d) * abx * aby + (uint64_t)(y >> 3) * abx + (x >> 3)) * 3;
+            p.x += q[0];
+            p.y += q[1];
+            p.z += q[2];
+        }
+    }
+    v3<float> v{o[0], o[plane], o[2 * plane]};
+    v = apply_style<float>(nodes, root, d, 0u, v, p);
+    o[0] = v.x;
+    o[plane] = v.y;
+    o[2 * plane] = v.z;
+}
+
+// squared error and squared norm of the prediction per 64x8 tile of the judge slice
+// (one workgroup per tile, fixed-order reduction)
+__global__ void tile_error_kernel(const float* pred, const float* judge, uint32_t W, uint32_t H, uint32_t tiles_x,
+                                  double* num_den) {
+    __shared__ double sn[256], sd[256];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t x0 = (tile % tiles_x) * 64, y0 = (tile / tiles_x) * 8;
+    const uint64_t plane = (uint64_t)W * H;
+    double num = 0.0, den = 0.0;
+    for (uint32_t k = threadIdx.x; k < 512; k += blockDim.x) {
+        const uint32_t x = x0 + (k & 63), y = y0 + (k >> 6);
+        if (x < W && y < H) {
+            const uint64_t o = (uint64_t)y * W + x;
+            for (int c = 0; c < 3; ++c) {
+                const double a = pred[c * plane + o], b = judge[c * plane + o];
+                num += (a - b) * (a - b);
+                den += b * b;
+            }
+        }
+    }
+    sn[threadIdx.x] = num;
+    sd[threadIdx.x] = den;
+    __syncthreads();
+    for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sn[threadIdx.x] += sn[threadIdx.x + off];
+            sd[threadIdx.x] += sd[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        num_den[2 * tile] = sn[0];
+        num_den[2 * tile + 1] = sd[0];
+    }
+}
+
+static bool tree_has(const std::vector<nlpf_style_node>& nodes, int root, int type, int depth = 0) {
+    if (root < 0 || depth > NLPF_STYLE_STACK) return false;
+    const nlpf_style_node& n = nodes[root];
+    if (n.type == type) return true;
+    if (tree_has(nodes, n.inner, type, depth + 1)) return true;
+    return n.type == NLPF_STY_ENABLE_IF && tree_has(nodes, n.else_, type, depth + 1);
+}
+
+}  // namespace nlpf
+
+using namespace nlpf;
+
+static void dilate_tiles(std::vector<uint8_t>& m, uint32_t tx, uint32_t ty) {
+    std::vector<uint8_t> o(m.size(), 0);
+    for (uint32_t y = 0; y < ty; ++y)
+        for (uint32_t x = 0; x < tx; ++x) {
+            uint8_t v = 0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int xx = (int)x + dx, yy = (int)y + dy;
+                    if (xx >= 0 && yy >= 0 && xx < (int)tx && yy < (int)ty) v |= m[(size_t)yy * tx + xx];
+                }
+            o[(size_t)y * tx + x] = v;
+        }
+    m.swap(o);
+}
+
+extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                                   const float* anchors, const nlpf_mlp_run_desc* run, float* film, float* level_out,
+                                   nlpf_mlp_run_stats* stats) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    if (!desc || !records || !run || !film) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run_mlp: NULL argument");
+    if (desc->precision != NLPF_FP32 && desc->precision != NLPF_FP16)
+        NLPF_FAIL(h, NLPF_E_UNSUPPORTED, "the learned cache runs on FP32 or FP16 records");
+    if (desc->precision == NLPF_FP16 && !anchors) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "FP16 records need anchors");
+    const uint32_t W = desc->width, H = desc->height, S = desc->spp, L = desc->levels, k = run->depth;
+    if (W == 0 || H == 0 || S == 0 || L == 0 || L > 31 || desc->radius < 1 || desc->radius > 7)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run_mlp: bad screen desc");
+    if (k < 1 || k >= L) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "cache depth must be in [1, levels - 1]");
+    if (run->judge_slice >= S) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "judge slice out of range");
+    if (run->batch == 0 || run->batch % 128 != 0 || run->train_records % run->batch != 0)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "batch must be a multiple of 128 dividing train_records");
+    const uint64_t plane = (uint64_t)W * H;
+    if (run->train_records > plane) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "more training records than judge vertices");
+    if (plane * S >= (1ull << 32)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many vertices");
+    const int fp16 = desc->precision == NLPF_FP16;
+    const uint32_t tiles_x = (W + 63) / 64, tiles_y = (H + 7) / 8, ntiles = tiles_x * tiles_y;
+    const uint32_t sj = run->judge_slice;
+    const size_t esz = fp16 ? 2 : 4;
+    const uint32_t abx = (W + 7) / 8, aby = (H + 7) / 8;
+
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t lv_bytes = level_out ? 0 : (size_t)S * L * 3 * plane * sizeof(float);
+    const size_t f_bytes = 3 * plane * sizeof(float);
+    const size_t tgt_bytes = (size_t)run->train_records * 3 * sizeof(float);
+    const size_t idx_bytes = (size_t)run->train_records * sizeof(uint32_t);
+    const size_t mask_bytes = (size_t)L * ntiles;
+    const size_t nd_bytes = (size_t)ntiles * 2 * sizeof(double);
+    int rc = ensure_scratch(h, al(lv_bytes) + 3 * al(f_bytes) + al(tgt_bytes) + 2 * al(idx_bytes) + al(mask_bytes) +
+                                   2 * al(nd_bytes));
+    if (rc) return rc;
+    char* base = (char*)h->scratch;
+    auto take = [&](size_t b) {
+        char* p = base;
+        base += al(b);
+        return p;
+    };
+    float* lv = level_out ? level_out : (float*)take(lv_bytes);
+    float* Fj = (float*)take(f_bytes);  // judge slice: exact unstylized F_k (training targets)
+    float* Xj = (float*)take(f_bytes);  // judge slice: exact stylized depth-k output
+    float* Ej = (float*)take(f_bytes);  // judge slice: exact level-0 output
+    float* tgt = (float*)take(tgt_bytes);
+    uint32_t* lidx = (uint32_t*)take(idx_bytes);  // pixel ids within the judge slice
+    uint32_t* gidx = (uint32_t*)take(idx_bytes);  // sample ids s*W*H + pixel
+    uint8_t* masks = (uint8_t*)take(mask_bytes);
+    double* nd_img = (double*)take(nd_bytes);
+    double* nd_fit = (double*)take(nd_bytes);
+    hipStream_t st = h->stream;
+
+    const int root = h->h_roots.empty() ? -1 : h->h_roots[0];
+    const int need_pos = root >= 0 && tree_has(h->h_nodes, root, NLPF_STY_HALFTONE);
+    const double log2e = 1.4426950408889634;
+    ScreenArgs<float> a0{};
+    a0.rec = records;
+    a0.anchors = fp16 ? anchors : nullptr;
+    a0.out = lv;
+    a0.nodes = h->d_nodes;
+    a0.root = root;
+    a0.need_pos = need_pos;
+    a0.W = W;
+    a0.H = H;
+    a0.S = S;
+    a0.L = L;
+    a0.out_levels = L;
+    a0.plane = plane;
+    a0.abx = abx;
+    a0.aby = aby;
+    a0.kp = (float)sqrt(log2e / (2.0 * (double)h->cfg.sigma_pos * h->cfg.sigma_pos));
+    a0.kn = (float)sqrt(log2e / (2.0 * (double)h->cfg.sigma_normal * h->cfg.sigma_normal));
+    a0.kl = (float)sqrt(log2e / (2.0 * (double)h->cfg.sigma_lum * h->cfg.sigma_lum));
+    a0.rscale = fp16 ? NLPF_FP16_RADIANCE_SCALE : 1.0f;
+    a0.tile_mask = nullptr;
+    auto level = [&](ScreenArgs<float> a, uint32_t d) -> hipError_t {
+        a.d = d;
+        a.out_slot = d;
+        a.child_slot = d + 1;
+        a.has_child = d + 1 < L;
+        return fp16 ? screen_level_fp16(a, desc->radius, 0, st) : screen_level_fp32(a, desc->radius, 0, st);
+    };
+    auto slice_rec = [&](uint32_t s) {
+        return (const void*)((const char*)records + (size_t)s * L * NLPF_SCREEN_PLANES * plane * esz);
+    };
+    auto slice_anc = [&](uint32_t s) { return fp16 ? anchors + (size_t)s * L * abx * aby * 3 : (const float*)nullptr; };
+    auto slot = [&](uint32_t s, uint32_t d) { return lv + ((size_t)s * L + d) * 3 * plane; };
+    ScreenArgs<float> aj = a0;  // the judge slice alone
+    aj.S = 1;
+    aj.rec = slice_rec(sj);
+    aj.anchors = slice_anc(sj);
+    aj.out = slot(sj, 0);
+    nlpf_screen_desc d1 = *desc;
+    d1.spp = 1;
+    // stylize depth k of `n` slices starting at slice s0 (planar level data, records of those slices)
+    auto style_k = [&](uint32_t s0, uint32_t n) -> int {
+        if (root < 0) return NLPF_OK;
+        const uint64_t cnt = plane * n;
+        float* io = slot(s0, 0);
+        if (fp16)
+            hipLaunchKernelGGL(style_planar_kernel<__half>, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, io,
+                               n, plane, L, k, h->d_nodes, root, k, (const __half*)slice_rec(s0), slice_anc(s0), W, H,
+                               L, need_pos);
+        else
+            hipLaunchKernelGGL(style_planar_kernel<float>, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, io,
+                               n, plane, L, k, h->d_nodes, root, k, (const float*)slice_rec(s0), nullptr, W, H, L,
+                               need_pos);
+        return check_launch(h, "style depth k");
+    };
+
+    nlpf_mlp_run_stats rs{};
+    rc = record_event(h, 0);
+    if (rc) return rc;
+    // 1. judge slice, exact: levels D..k+1, unstylized F_k (targets), stylized depth k, levels k-1..0
+    for (int d = (int)L - 1; d > (int)k; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
+    {
+        ScreenArgs<float> af = aj;
+        af.root = -1;
+        NLPF_HIP(h, level(af, k));
+    }
+    NLPF_HIP(h, hipMemcpyAsync(Fj, slot(sj, k), f_bytes, hipMemcpyDeviceToDevice, st));
+    if ((rc = style_k(sj, 1))) return rc;
+    NLPF_HIP(h, hipMemcpyAsync(Xj, slot(sj, k), f_bytes, hipMemcpyDeviceToDevice, st));
+    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
+    NLPF_HIP(h, hipMemcpyAsync(Ej, slot(sj, 0), f_bytes, hipMemcpyDeviceToDevice, st));
+    rc = record_event(h, 1);
+    if (rc) return rc;
+    // 2. online training on `train_records` judge vertices (a PCG stream of `seed` picks them)
+    {
+        std::vector<uint32_t> perm(plane);
+        for (uint64_t i = 0; i < plane; ++i) perm[i] = (uint32_t)i;
+        Pcg32 g = pcg_seq_offset(0x7EA1ull, splitmix64(run->seed));
+        for (uint32_t i = 0; i < run->train_records; ++i) {
+            const uint32_t j = i + (uint32_t)(((uint64_t)pcg_next(g) * (plane - i)) >> 32);
+            std::swap(perm[i], perm[j]);
+        }
+        std::vector<uint32_t> ids(run->train_records);
+        for (uint32_t i = 0; i < run->train_records; ++i) ids[i] = sj * (uint32_t)plane + perm[i];
+        NLPF_HIP(h, hipMemcpyAsync(lidx, perm.data(), idx_bytes, hipMemcpyHostToDevice, st));
+        NLPF_HIP(h, hipMemcpyAsync(gidx, ids.data(), idx_bytes, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(gather_targets_kernel, dim3((run->train_records + 255) / 256), dim3(256), 0, st, Fj, plane,
+                           lidx, run->train_records, tgt);
+        if ((rc = check_launch(h, "gather targets"))) return rc;
+        float loss = 0.0f;
+        for (uint32_t b = 0; b < run->train_records / run->batch; ++b) {
+            rc = nlpf_mlp_train_step(h, desc, records, anchors, k, gidx + (size_t)b * run->batch, nullptr,
+                                     tgt + (size_t)b * run->batch * 3, run->batch, run->lr, 1, &loss, nullptr);
+            if (rc) return rc;
+        }
+        // the host vectors must outlive the asynchronous uploads
+        NLPF_HIP(h, hipStreamSynchronize(st));
+        rs.train_loss = loss;
+    }
+    rc = record_event(h, 2);
+    if (rc) return rc;
+    // 3. inference at depth k for every slice, then the depth-k stylization
+    rc = mlp_infer_planar(h, desc, records, anchors, k, plane * S, lv, L, k);
+    if (rc) return rc;
+    hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, slot(sj, k), Fj, W, H, tiles_x, nd_fit);
+    if ((rc = check_launch(h, "cache fit"))) return rc;
+    if ((rc = style_k(0, S))) return rc;
+    rc = record_event(h, 3);
+    if (rc) return rc;
+    // 4. judge: levels k-1..0 of the judge slice on the estimate, per-tile error against E
+    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
+    hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, slot(sj, 0), Ej, W, H, tiles_x, nd_img);
+    if ((rc = check_launch(h, "tile judge"))) return rc;
+    std::vector<double> hi(2 * (size_t)ntiles), hf(2 * (size_t)ntiles);
+    NLPF_HIP(h, hipMemcpyAsync(hi.data(), nd_img, nd_bytes, hipMemcpyDeviceToHost, st));
+    NLPF_HIP(h, hipMemcpyAsync(hf.data(), nd_fit, nd_bytes, hipMemcpyDeviceToHost, st));
+    NLPF_HIP(h, hipStreamSynchronize(st));
+    std::vector<uint8_t> m(ntiles, 0);
+    double in_ = 0.0, id_ = 0.0, fn_ = 0.0, fd_ = 0.0;
+    uint64_t nfall = 0;
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        in_ += hi[2 * t];
+        id_ += hi[2 * t + 1];
+        fn_ += hf[2 * t];
+        fd_ += hf[2 * t + 1];
+        const double e = hi[2 * t + 1] > 0.0 ? std::sqrt(hi[2 * t] / hi[2 * t + 1]) : (hi[2 * t] > 0.0 ? 1e30 : 0.0);
+        if (e > (double)run->tile_threshold) {
+            m[t] = 1;
+            ++nfall;
+        }
+    }
+    rs.judge_rel_l2 = id_ > 0.0 ? (float)std::sqrt(in_ / id_) : 0.0f;
+    rs.cache_rel_l2 = fd_ > 0.0 ? (float)std::sqrt(fn_ / fd_) : 0.0f;
+    rs.tiles = ntiles;
+    rs.fallback_tiles = nfall;
+    // 5. exact depth-k output for the judge slice, exact filtering of the fallback tiles
+    NLPF_HIP(h, hipMemcpyAsync(slot(sj, k), Xj, f_bytes, hipMemcpyDeviceToDevice, st));
+    if (nfall) {
+        std::vector<uint8_t> hm((size_t)L * ntiles, 0);
+        for (uint32_t d = 1; d < L; ++d) {
+            dilate_tiles(m, tiles_x, tiles_y);
+            if (d >= k) std::copy(m.begin(), m.end(), hm.begin() + (size_t)d * ntiles);
+        }
+        for (uint32_t d = k; d < L; ++d)
+            for (uint32_t t = 0; t < ntiles; ++t) rs.fallback_tiles_filtered += hm[(size_t)d * ntiles + t];
+        NLPF_HIP(h, hipMemcpyAsync(masks, hm.data(), mask_bytes, hipMemcpyHostToDevice, st));
+        for (int d = (int)L - 1; d >= (int)k; --d) {
+            ScreenArgs<float> am = a0;
+            am.tile_mask = masks + (size_t)d * ntiles;
+            NLPF_HIP(h, level(am, (uint32_t)d));
+        }
+        NLPF_HIP(h, hipStreamSynchronize(st));  // hm outlives its upload
+    }
+    rc = record_event(h, 4);
+    if (rc) return rc;
+    // 6. plain levels k-1..0 on the mixed depth-k output, then the film
+    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(a0, (uint32_t)d));
+    NLPF_HIP(h, screen_film_fp32(lv, S, L, 0, plane, film, st));
+    rc = record_event(h, 5);
+    if (rc) return rc;
+    NLPF_HIP(h, hipStreamSynchronize(st));
+    rs.judge_ms = event_ms(h, 0, 1);
+    rs.train_ms = event_ms(h, 1, 2);
+    rs.infer_ms = event_ms(h, 2, 3);
+    rs.fallback_ms = event_ms(h, 3, 4);
+    rs.filter_ms = event_ms(h, 4, 5);
+    rs.total_ms = event_ms(h, 0, 5);
+    if (stats) *stats = rs;
+    h->stats.total_ms = rs.total_ms;
+    return NLPF_OK;
+}

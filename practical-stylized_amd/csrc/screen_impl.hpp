@@ -31,6 +31,7 @@ struct ScreenArgs {
     uint32_t abx, aby;  // anchor blocks per row / column (fp16)
     A kp, kn, kl;
     A rscale;  // radiance decode factor (FP16 records store radiance / NLPF_FP16_RADIANCE_SCALE)
+    const uint8_t* tile_mask;  // optional [tiles_y][tiles_x]: 0 = skip the tile (all slices)
 };
 
 __device__ __forceinline__ float ld_rec(const float* p, size_t i) { return p[i]; }
@@ -77,6 +78,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
     const uint32_t s = t / tiles, tile = t - s * tiles;
+    if (a.tile_mask && !a.tile_mask[tile]) return;  // uniform per workgroup, before any barrier
     const int tx0 = (int)(tile % a.tiles_x) * TW, ty0 = (int)(tile / a.tiles_x) * TH;
     const int W = (int)a.W, H = (int)a.H;
     const size_t plane = a.plane;

@@ -75,6 +75,18 @@ class _WorldDesc(C.Structure):
                 ("levels", C.c_uint32), ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3)]
 
 
+class _MlpRunDesc(C.Structure):
+    _fields_ = [("depth", C.c_uint32), ("judge_slice", C.c_uint32), ("train_records", C.c_uint32),
+                ("batch", C.c_uint32), ("lr", C.c_float), ("tile_threshold", C.c_float), ("seed", C.c_uint64)]
+
+
+class _MlpRunStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("judge_ms", C.c_double), ("train_ms", C.c_double),
+                ("infer_ms", C.c_double), ("fallback_ms", C.c_double), ("filter_ms", C.c_double),
+                ("tiles", C.c_uint64), ("fallback_tiles", C.c_uint64), ("fallback_tiles_filtered", C.c_uint64),
+                ("train_loss", C.c_float), ("cache_rel_l2", C.c_float), ("judge_rel_l2", C.c_float)]
+
+
 WORLD_PLANES = 28
 
 
@@ -119,6 +131,8 @@ _SIGS = [
                                       C.POINTER(C.c_float), C.c_void_p]),
     ("nlpf_mlp_reference", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p, C.c_uint32,
                                      C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("nlpf_screen_run_mlp", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
+                                      C.POINTER(_MlpRunDesc), C.c_void_p, C.c_void_p, C.POINTER(_MlpRunStats)]),
     ("nlpf_world_run", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_generate_world_records", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.POINTER(_GenDesc),
@@ -330,6 +344,16 @@ class Handle:
     def mlp_reference(self, desc, records, anchors, depth, idx, n, out):
         self._check(self.lib.nlpf_mlp_reference(self.h, C.byref(desc), _ptr(records), _ptr(anchors), depth,
                                                 _ptr(idx), n, _ptr(out)))
+
+    def screen_run_mlp(self, desc, records, anchors, film, level_out=None, depth=2, judge_slice=0,
+                       train_records=1 << 16, batch=1 << 14, lr=1e-2, tile_threshold=0.05, seed=0):
+        """Screen NL-PF with the learned cache at `depth` (one frame; training continues across
+        calls).  Returns the run statistics as a dict."""
+        r = _MlpRunDesc(depth, judge_slice, train_records, batch, lr, tile_threshold, seed)
+        st = _MlpRunStats()
+        self._check(self.lib.nlpf_screen_run_mlp(self.h, C.byref(desc), _ptr(records), _ptr(anchors), C.byref(r),
+                                                 _ptr(film), _ptr(level_out), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _MlpRunStats._fields_}
 
     def stats(self):
         s = _Stats()
