@@ -87,6 +87,54 @@ def measured_traffic():
     return None
 
 
+def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
+    """FP16 records + learned cache at `--cache-depth` (2^16 training records per frame,
+    batch 2^14, Adam lr 1e-2).  The judge slice rotates over the spp slices from frame to
+    frame; `--mlp-frames` untimed frames train the cache first.  Reported twice: with the
+    per-tile fallback at `--tile-threshold`, and without fallback."""
+    log(f"FP16 + learned cache at depth {args.cache_depth}")
+    h.mlp_init(1)
+    kw = dict(depth=args.cache_depth, train_records=1 << 16, batch=1 << 14, lr=1e-2)
+    frame = 0
+
+    def frames(n, thr):
+        nonlocal frame
+        acc, st = {}, None
+        h.sync()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            st = h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame, tile_threshold=thr,
+                                  **kw)
+            frame += 1
+            for k in ("judge_ms", "train_ms", "infer_ms", "fallback_ms", "filter_ms", "fallback_tiles"):
+                acc[k] = acc.get(k, 0.0) + st[k]
+        h.sync()
+        ms = (time.perf_counter() - t0) / max(n, 1) * 1e3
+        return ms, {k: v / max(n, 1) for k, v in acc.items()}, st
+
+    frames(args.mlp_frames, 1e30)
+    out = {}
+    for name, thr in (("fallback", args.tile_threshold), ("no_fallback", 1e30)):
+        frames(args.warmup, thr)
+        ms, ph, st = frames(args.steps, thr)
+        leg = {"value": verts / (ms / 1e3), "unit": "vertices/s", "ms_per_step": ms,
+               "phases_ms": {k[:-3]: ph[k] for k in ph if k.endswith("_ms")},
+               "tile_threshold": thr if thr < 1e29 else None,
+               "fallback_tile_frac": ph["fallback_tiles"] / st["tiles"],
+               "judge_rel_l2": st["judge_rel_l2"], "cache_rel_l2": st["cache_rel_l2"], "train_loss": st["train_loss"]}
+        if film_ref is not None:
+            f = film.download(np.float32).astype(np.float64)
+            leg["l2_err_vs_fp64"] = float(np.linalg.norm(f - film_ref) / np.linalg.norm(film_ref))
+        log(f"  {name}: {ms:.2f} ms/step, phases {np.round(list(leg['phases_ms'].values()), 2).tolist()}, "
+            f"fallback {leg['fallback_tile_frac']:.3f}, L2 {leg.get('l2_err_vs_fp64')}")
+        out[name] = leg
+    res = dict(out["fallback"])
+    res["no_fallback"] = out["no_fallback"]
+    res.update({"cache_depth": args.cache_depth, "train_frames_before_timing": args.mlp_frames,
+                "train_records_per_frame": 1 << 16, "batch": 1 << 14, "lr": 1e-2})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +149,10 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=16, help="spp slices timed by the CPU baseline")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--cache-depth", type=int, default=2, help="depth the learned cache answers for")
+    ap.add_argument("--mlp-frames", type=int, default=8, help="untimed training frames before the timed ones")
+    ap.add_argument("--tile-threshold", type=float, default=0.05)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one FP32 pass and one 1 GiB device copy, for rocprofv3 counter runs")
     args = ap.parse_args()
@@ -206,9 +258,11 @@ def main():
             f16 = film16.download(np.float32).astype(np.float64)
             fp16["l2_err_vs_fp64"] = float(np.linalg.norm(f16 - film_ref) / np.linalg.norm(film_ref))
         result["fp16"] = fp16
+        result["fp16_mlp"] = None if args.no_mlp else mlp_leg(h, args, d16, rec16, anc, film16, film_ref, verts, S)
         rec16.free()
         anc.free()
-    result["fp16_mlp"] = None
+    else:
+        result["fp16_mlp"] = None
 
     if not args.no_cpu:
         log("CPU baseline (oracle FP32 restatement)")

@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE ONLY — FP64 numpy oracle of the radiance-cache MLP (the checker of the
 library's BF16-MFMA kernels).  Imported only by tests/ and bench.py's checks.
 
-Network: 96(84 used)-64-64-64-64-64-16(3 used), ReLU, parameters W_l [out][in] then b_l.
+Network: 96(84 used, column layout in column_layout())-64-64-64-64-64-16(3 used), ReLU,
+parameters W_l [out][in] then b_l.
 The MFMA kernels feed bf16 operands (weights, inputs, every layer's ReLU output and every
 back-propagated dZ) and accumulate in FP32; the oracle rounds the same tensors to bf16
 (round-to-nearest-even) and accumulates in FP64.
@@ -77,20 +78,46 @@ def loss_and_grad(params, feats, target):
     return loss, g
 
 
+def column_layout():
+    """What each of the 96 input columns (the columns of W_0) holds: ("sin"|"cos", k, c) for
+    sin/cos(2^k pi p_c), ("p"|"n"|"a"|"v", c) for the raw features, ("pad",) for zeros.
+    Column P belongs to MFMA lane group g = (P >> 2) & 3 as its slot
+    u = 8 (P // 32) + (4 if P % 32 >= 16 else 0) + P % 4; slots u < 18 hold octave
+    k = 3 g + u // 6 of coordinate c = (u % 6) // 2 (sin for even u, cos for odd u); slots
+    18..23 hold p, n (g = 0), albedo, view (g = 1) and padding (g = 2, 3)."""
+    cols = []
+    for P in range(IN):
+        g = (P >> 2) & 3
+        u = 8 * (P // 32) + (4 if P % 32 >= 16 else 0) + P % 4
+        if u < 18:
+            cols.append(("cos" if u & 1 else "sin", 3 * g + u // 6, (u % 6) // 2))
+        elif g < 2:
+            r = u - 18
+            cols.append(((("p", "n"), ("a", "v"))[g][r // 3], r % 3))
+        else:
+            cols.append(("pad",))
+    return cols
+
+
+COLUMNS = column_layout()
+PAD = np.array([c[0] == "pad" for c in COLUMNS])
+
+
 def encode(p, n, a, v):
-    """84 features (+12 zero pads) of vertices: p, n, a, v are [m][3] arrays."""
+    """96 input columns (84 features + 12 zero pads) of vertices: p, n, a, v are [m][3]."""
     p = np.asarray(p, np.float64)
-    m = p.shape[0]
-    f = np.zeros((m, IN))
-    f[:, 0:3] = np.clip(p, -2, 2)
-    for q in range(6 * FREQ):
-        c, k, cosine = q % 3, (q // 3) >> 1, (q // 3) & 1
-        t = np.ldexp(p[:, c].astype(np.float32), k - 1).astype(np.float64)
-        fr = t - np.floor(t)
-        f[:, 3 + q] = np.cos(2 * np.pi * fr) if cosine else np.sin(2 * np.pi * fr)
-    f[:, 75:78] = n
-    f[:, 78:81] = a
-    f[:, 81:84] = v
+    raw = {"p": np.clip(p, -2, 2), "n": np.asarray(n, np.float64), "a": np.asarray(a, np.float64),
+           "v": np.asarray(v, np.float64)}
+    f = np.zeros((p.shape[0], IN))
+    for P, col in enumerate(COLUMNS):
+        if col[0] in ("sin", "cos"):
+            _, k, c = col
+            # 2^k pi p through the exactly reduced fraction of 2^(k-1) p (p is float32 data)
+            t = np.ldexp(p[:, c].astype(np.float32), k - 1).astype(np.float64)
+            fr = t - np.floor(t)
+            f[:, P] = np.cos(2 * np.pi * fr) if col[0] == "cos" else np.sin(2 * np.pi * fr)
+        elif col[0] != "pad":
+            f[:, P] = raw[col[0]][:, col[1]]
     return f
 
 

@@ -33,24 +33,39 @@ __device__ __forceinline__ int phys_feature(int s, int g, int j) {
     return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
 }
 
-// 84 input features of a depth-d vertex: clamped position, 12-octave sin/cos of each
-// coordinate, normal, albedo, view direction (towards the parent vertex; the camera at d=0)
+// 84 input features of a depth-d vertex (+12 zero pads): clamped position, 12-octave sin/cos
+// of each coordinate, normal, albedo, view direction (towards the parent vertex; the camera
+// at d=0).  Column P of the encoded input (= column P of W_0) is computed by lane group
+// g = (P >> 2) & 3 of the MFMA B operand, as its slot u = 8 (P / 32) + j (j as in
+// phys_feature); the columns are assigned so every lane group runs the same code:
+//   slots u < 18    sin (u even) / cos (u odd) of 2^k pi p_c, k = 3 g + u / 6, c = (u % 6) / 2
+//   slots 18..23    g = 0: clamped p, n;  g = 1: albedo, view;  g = 2, 3: zero padding
 struct VertexIn {
     float p[3], n[3], a[3], v[3];
 };
-__device__ __forceinline__ float mlp_feature(const VertexIn& x, int f) {
-    if (f < 3) return fminf(fmaxf(x.p[f], -2.0f), 2.0f);
-    if (f < 3 + 6 * MLP_FREQ) {
-        const int q = f - 3, c = q % 3, k = (q / 3) >> 1, cosine = (q / 3) & 1;
-        // sin(2^k pi p) through the exactly reduced fraction of 2^(k-1) p
-        const float t = ldexpf(x.p[c], k - 1);
-        const float fr = t - floorf(t);
-        return cosine ? cospif(2.0f * fr) : sinpif(2.0f * fr);
+__host__ __device__ constexpr int col_group(int P) { return (P >> 2) & 3; }
+__host__ __device__ constexpr int col_slot(int P) { return 8 * (P >> 5) + ((P & 31) >= 16 ? 4 : 0) + (P & 3); }
+__host__ __device__ constexpr bool col_is_pad(int P) { return col_group(P) >= 2 && col_slot(P) >= 18; }
+
+// sin / cos of 2^k pi x through the exactly reduced fraction of 2^(k-1) x
+__device__ __forceinline__ void octave_sincos(float x, int k, float& sn, float& cs) {
+    const float t = ldexpf(x, k - 1);
+    sincospif(2.0f * (t - floorf(t)), &sn, &cs);
+}
+
+// feature of column P (reference path: the encode-only and FP64 reference kernels)
+__device__ inline float mlp_feature(const VertexIn& x, int P) {
+    const int g = col_group(P), u = col_slot(P);
+    if (u < 18) {
+        const int c = (u % 6) >> 1;
+        float sn, cs;
+        octave_sincos(c == 0 ? x.p[0] : c == 1 ? x.p[1] : x.p[2], 3 * g + u / 6, sn, cs);
+        return (u & 1) ? cs : sn;
     }
-    const int r = f - (3 + 6 * MLP_FREQ);
-    if (r < 3) return x.n[r];
-    if (r < 6) return x.a[r - 3];
-    if (r < 9) return x.v[r - 6];
+    const int r = u - 18;
+    if (g == 0) return r < 3 ? fminf(fmaxf(r == 0 ? x.p[0] : r == 1 ? x.p[1] : x.p[2], -2.0f), 2.0f)
+                             : (r == 3 ? x.n[0] : r == 4 ? x.n[1] : x.n[2]);
+    if (g == 1) return r < 3 ? (r == 0 ? x.a[0] : r == 1 ? x.a[1] : x.a[2]) : (r == 3 ? x.v[0] : r == 4 ? x.v[1] : x.v[2]);
     return 0.0f;
 }
 
@@ -121,30 +136,34 @@ struct MlpParamsDev {
 
 constexpr int MLP_QT = 4;  // sample tiles (of 16) per wave
 
-// Forward through all layers for 64 samples per wave.  Input fragments come from `xin`
-// (feature-major permuted bf16 fragments built by the caller).  When `act` is non-null the
-// post-ReLU activations of layers 0..4 (the inputs of layers 1..5) and the encoded inputs
-// are written feature-major: act[l][feature][sample] (bf16), for the backward kernels.
-__device__ __forceinline__ void mlp_forward_wave(const MlpParamsDev& P, bf16x8 (&xin)[MLP_QT][3], f32x4 (&y)[MLP_QT],
-                                                 int lane) {
+// Fragments of the weights staged in LDS: fragment f = (layer, 16-row tile t, K-step s),
+// 64 lanes x 16 B each, in the order layer 0 (t, s) 4x3, layers 1..4 4x2, layer 5 1x2.
+constexpr int MLP_NFRAG = 12 + 4 * 8 + 2;
+__host__ __device__ constexpr int frag_index(int l, int t, int s) {
+    return l == 0 ? 3 * t + s : (l < MLP_LAYERS - 1 ? 12 + 8 * (l - 1) + 2 * t + s : 44 + s);
+}
+constexpr int MLP_NBIAS = 5 * MLP_HID + MLP_OUT;
+__host__ __device__ constexpr int bias_index(int l) { return MLP_HID * l; }
+
+// Forward through all layers for 16*MLP_QT samples per wave.  Input fragments come from
+// `xin` (built by the caller); weights and biases from `wf(l, t, s)` / `bf(l, t)`.
+template <typename WF, typename BF>
+__device__ __forceinline__ void mlp_forward_wave(const WF& wf, const BF& bf, bf16x8 (&xin)[MLP_QT][3],
+                                                 f32x4 (&y)[MLP_QT]) {
     f32x4 acc[MLP_QT][4];
     bf16x8 xb[MLP_QT][2];
-    // layer 0: 96 -> 64
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int g = lane >> 4;
-        f32x4 bias;
-        for (int i = 0; i < 4; ++i) bias[i] = P.b[0][16 * t + 4 * g + i];
+        const f32x4 bias = bf(0, t);
 #pragma unroll
         for (int q = 0; q < MLP_QT; ++q) acc[q][t] = bias;
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            const bf16x8 a = load_wfrag(P.w[0], MLP_IN, 16 * t, s, lane);
+            const bf16x8 w = wf(0, t, s);
 #pragma unroll
-            for (int q = 0; q < MLP_QT; ++q) acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xin[q][s], acc[q][t], 0, 0, 0);
+            for (int q = 0; q < MLP_QT; ++q) acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xin[q][s], acc[q][t], 0, 0, 0);
         }
     }
-    // hidden layers 1..4: 64 -> 64
 #pragma unroll
     for (int l = 1; l < MLP_LAYERS - 1; ++l) {
 #pragma unroll
@@ -154,48 +173,52 @@ __device__ __forceinline__ void mlp_forward_wave(const MlpParamsDev& P, bf16x8 (
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int g = lane >> 4;
-            f32x4 bias;
-            for (int i = 0; i < 4; ++i) bias[i] = P.b[l][16 * t + 4 * g + i];
+            const f32x4 bias = bf(l, t);
 #pragma unroll
             for (int q = 0; q < MLP_QT; ++q) acc[q][t] = bias;
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                const bf16x8 a = load_wfrag(P.w[l], MLP_HID, 16 * t, s, lane);
+                const bf16x8 w = wf(l, t, s);
 #pragma unroll
                 for (int q = 0; q < MLP_QT; ++q)
-                    acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[q][s], acc[q][t], 0, 0, 0);
+                    acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[q][s], acc[q][t], 0, 0, 0);
             }
         }
     }
-    // output layer: 64 -> 16 (3 used)
 #pragma unroll
     for (int q = 0; q < MLP_QT; ++q) {
         xb[q][0] = relu_pack(acc[q][0], acc[q][1]);
         xb[q][1] = relu_pack(acc[q][2], acc[q][3]);
     }
-    {
-        const int g = lane >> 4;
-        f32x4 bias;
-        for (int i = 0; i < 4; ++i) bias[i] = P.b[MLP_LAYERS - 1][4 * g + i];
+    const f32x4 bias = bf(MLP_LAYERS - 1, 0);
 #pragma unroll
-        for (int q = 0; q < MLP_QT; ++q) y[q] = bias;
+    for (int q = 0; q < MLP_QT; ++q) y[q] = bias;
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const bf16x8 a = load_wfrag(P.w[MLP_LAYERS - 1], MLP_HID, 0, s, lane);
+    for (int s = 0; s < 2; ++s) {
+        const bf16x8 w = wf(MLP_LAYERS - 1, 0, s);
 #pragma unroll
-            for (int q = 0; q < MLP_QT; ++q) y[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[q][s], y[q], 0, 0, 0);
-        }
+        for (int q = 0; q < MLP_QT; ++q) y[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[q][s], y[q], 0, 0, 0);
     }
 }
 
-// encoded input fragments of the lane's sample for the 3 K-steps
+// encoded input fragments of the lane's sample for the 3 K-steps (slot u = 8 s + j)
 __device__ __forceinline__ void encode_frags(const VertexIn& v, int lane, bf16x8 (&x)[3]) {
     const int g = lane >> 4;
+    float u[24];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) octave_sincos(v.p[c], 3 * g + i, u[6 * i + 2 * c], u[6 * i + 2 * c + 1]);
+    const bool g0 = g == 0, g1 = g == 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        u[18 + c] = g0 ? fminf(fmaxf(v.p[c], -2.0f), 2.0f) : (g1 ? v.a[c] : 0.0f);
+        u[21 + c] = g0 ? v.n[c] : (g1 ? v.v[c] : 0.0f);
+    }
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) x[s][j] = (__bf16)mlp_feature(v, phys_feature(s, g, j));
+        for (int j = 0; j < 8; ++j) x[s][j] = (__bf16)u[8 * s + j];
 }
 
 struct MlpInferArgs {
@@ -212,19 +235,40 @@ struct MlpInferArgs {
     uint32_t out_levels, out_slot;
 };
 
-__global__ void __launch_bounds__(256) mlp_infer_kernel(MlpInferArgs a) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+// Persistent inference: the workgroup stages all weight fragments and biases in LDS once
+// (47 KB), then its 4 waves loop over tiles of 16*MLP_QT samples.
+__global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
+    __shared__ bf16x8 wl[MLP_NFRAG][64];
+    __shared__ f32x4 bl[MLP_NBIAS / 4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4;
+    for (int f = wv; f < MLP_NFRAG; f += 4) {
+        const int l = f < 12 ? 0 : (f < 44 ? 1 + (f - 12) / 8 : MLP_LAYERS - 1);
+        const int r = f < 12 ? f : (f < 44 ? (f - 12) % 8 : f - 44);
+        const int ks = l == 0 ? 3 : 2;
+        wl[f][lane] = load_wfrag(a.P.w[l], mlp_in(l), 16 * (r / ks), r % ks, lane);
+    }
+    for (int i = threadIdx.x; i < MLP_NBIAS; i += blockDim.x) {
+        const int l = i / MLP_HID;
+        ((float*)bl)[i] = a.P.b[l][i - MLP_HID * l];
+    }
+    __syncthreads();
+    const auto wf = [&](int l, int t, int s) { return wl[frag_index(l, t, s)][lane]; };
+    const auto bf = [&](int l, int t) { return bl[(bias_index(l) + 16 * t + 4 * g) >> 2]; };
+
+    const uint64_t wave = (uint64_t)blockIdx.x * 4 + wv;
+    const uint64_t nwaves = (uint64_t)gridDim.x * 4;
     const uint64_t per = 16 * MLP_QT;
+    const uint64_t plane = (uint64_t)a.W * a.H;
     for (uint64_t base = wave * per; base < a.n; base += nwaves * per) {
+        // keeps the (loop-invariant) LDS weight reads inside the loop: hoisted, the 46
+        // fragments would hold 184 VGPRs for the whole loop
+        asm volatile("" ::: "memory");
         bf16x8 xin[MLP_QT][3];
 #pragma unroll
         for (int q = 0; q < MLP_QT; ++q) {
             const uint64_t i = base + 16 * q + (lane & 15);
             if (i < a.n) {
                 if (a.feats) {
-                    const int g = lane >> 4;
                     for (int s = 0; s < 3; ++s)
                         for (int j = 0; j < 8; ++j) xin[q][s][j] = (__bf16)a.feats[i * MLP_IN + phys_feature(s, g, j)];
                 } else {
@@ -239,15 +283,14 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(MlpInferArgs a) {
             }
         }
         f32x4 y[MLP_QT];
-        mlp_forward_wave(a.P, xin, y, lane);
-        if ((lane >> 4) == 0) {
+        mlp_forward_wave(wf, bf, xin, y);
+        if (g == 0) {
 #pragma unroll
             for (int q = 0; q < MLP_QT; ++q) {
                 const uint64_t i = base + 16 * q + lane;
                 if (i < a.n) {
                     if (a.planar) {
                         const uint64_t sm = a.idx ? a.idx[i] : i;
-                        const uint64_t plane = (uint64_t)a.W * a.H;
                         const uint64_t sl = sm / plane, px = sm - sl * plane;
                         float* o = a.out + ((uint64_t)sl * a.out_levels + a.out_slot) * 3 * plane + px;
                         o[0] = y[q][0];
@@ -493,7 +536,7 @@ __global__ void mlp_loss_reduce_kernel(const float* part, uint32_t n, float* out
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the bf16 copies W and W^T.
-// Padding (output rows 3..15 of the last layer, input columns 84..95 of the first) stays 0.
+// Padding (output rows 3..15 of the last layer, the 12 pad input columns of the first) stays 0.
 struct MlpAdamArgs {
     float* param;
     float* m;
@@ -533,7 +576,7 @@ __global__ void mlp_adam_kernel(MlpAdamArgs a) {
     int l, o, i;
     bool bias;
     mlp_locate(p, l, o, i, bias);
-    const bool pad = (l == MLP_LAYERS - 1 && o >= 3) || (l == 0 && !bias && i >= 84);
+    const bool pad = (l == MLP_LAYERS - 1 && o >= 3) || (l == 0 && !bias && col_is_pad(i));
     float th = a.param[p];
     if (!pad && a.step > 0) {
         const float gr = a.grad[p];
@@ -759,7 +802,7 @@ int nlpf_mlp_init(nlpf_handle* h, uint64_t seed) {
         const float a = (l == MLP_LAYERS - 1 ? 0.5f : 1.0f) * std::sqrt(6.0f / (float)fan_in);
         for (int o = 0; o < mlp_out(l); ++o)
             for (int i = 0; i < mlp_in(l); ++i) {
-                const bool pad = (l == MLP_LAYERS - 1 && o >= 3) || (l == 0 && i >= 84);
+                const bool pad = (l == MLP_LAYERS - 1 && o >= 3) || (l == 0 && col_is_pad(i));
                 p[mlp_woff(l) + o * mlp_in(l) + i] = pad ? 0.0f : a * u(rng);
             }
     }
@@ -828,7 +871,7 @@ int nlpf_mlp_infer(nlpf_handle* h, const nlpf_screen_desc* desc, const void* rec
     if (n == 0) return NLPF_OK;
     MlpInferArgs a = infer_args(s, desc, records, anchors, depth, idx, feats, n, out_rgb);
     const uint64_t per_block = 4 * 16 * MLP_QT;
-    const uint64_t blocks = std::min<uint64_t>((n + per_block - 1) / per_block, 256 * 64);
+    const uint64_t blocks = std::min<uint64_t>((n + per_block - 1) / per_block, 768);  // 3 per CU (LDS), persistent
     rc = record_event(h, 0);
     if (rc) return rc;
     hipLaunchKernelGGL(mlp_infer_kernel, dim3((uint32_t)blocks), dim3(256), 0, h->stream, a);
@@ -960,7 +1003,7 @@ int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* r
     a.out_levels = out_levels;
     a.out_slot = out_slot;
     const uint64_t per_block = 4 * 16 * MLP_QT;
-    const uint64_t blocks = std::min<uint64_t>((n + per_block - 1) / per_block, 256 * 64);
+    const uint64_t blocks = std::min<uint64_t>((n + per_block - 1) / per_block, 768);  // 3 per CU (LDS), persistent
     hipLaunchKernelGGL(mlp_infer_kernel, dim3((uint32_t)blocks), dim3(256), 0, h->stream, a);
     return check_launch(h, "mlp infer");
 }

@@ -16,7 +16,7 @@ def random_params(rng):
     for l, ((i, j), (wo, bo)) in enumerate(zip(M.DIMS, M.OFFS)):
         w = rng.normal(0, np.sqrt(2.0 / i), (j, i))
         if l == 0:
-            w[:, 84:] = 0
+            w[:, M.PAD] = 0
         if l == M.LAYERS - 1:
             w[3:] = 0
         p[wo:wo + i * j] = w.ravel()
@@ -26,7 +26,7 @@ def random_params(rng):
 
 def random_feats(rng, n):
     f = rng.uniform(-1, 1, (n, M.IN)).astype(np.float32)
-    f[:, 84:] = 0
+    f[:, M.PAD] = 0
     return f
 
 
@@ -79,7 +79,7 @@ def test_gradient_and_adam_match_oracle(h):
     e1, _, _ = M.adam(params.astype(np.float64), g, np.zeros(M.NPARAM), np.zeros(M.NPARAM), 1, 1e-2)
     pad = np.zeros(M.NPARAM, bool)
     wo, bo = M.OFFS[0]
-    pad[wo:bo] = (np.arange(96 * 64) % 96) >= 84
+    pad[wo:bo] = M.PAD[np.arange(96 * 64) % 96]
     wo, bo = M.OFFS[-1]
     pad[wo + 3 * 64:bo] = True
     pad[bo + 3:bo + 16] = True

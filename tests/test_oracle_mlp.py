@@ -41,10 +41,31 @@ def test_gradient_matches_finite_differences(monkeypatch):
 
 def test_encoder_known_values():
     p = np.array([[0.25, -0.5, 0.125]])
-    f = M.encode(p, np.array([[0, 0, 1.0]]), np.array([[0.2, 0.3, 0.4]]), np.array([[0, 1.0, 0]]))
-    # octave k, coordinate c: sin at 3 + 6k + c, cos at 6 + 6k + c
-    for k in range(M.FREQ):
-        for c in range(3):
-            assert np.isclose(f[0, 3 + 6 * k + c], np.sin(2 ** k * np.pi * p[0, c]), atol=1e-12)
-            assert np.isclose(f[0, 6 + 6 * k + c], np.cos(2 ** k * np.pi * p[0, c]), atol=1e-12)
-    assert np.allclose(f[0, 75:84], [0, 0, 1, 0.2, 0.3, 0.4, 0, 1, 0]) and np.all(f[0, 84:] == 0)
+    n, a, v = np.array([[0, 0, 1.0]]), np.array([[0.2, 0.3, 0.4]]), np.array([[0, 1.0, 0]])
+    f = M.encode(p, n, a, v)[0]
+    raw = {"p": p[0], "n": n[0], "a": a[0], "v": v[0]}
+    seen = set()
+    for P, col in enumerate(M.COLUMNS):
+        seen.add(col if col[0] != "pad" else ("pad", P))
+        if col[0] == "sin":
+            assert np.isclose(f[P], np.sin(2 ** col[1] * np.pi * p[0, col[2]]), atol=1e-12)
+        elif col[0] == "cos":
+            assert np.isclose(f[P], np.cos(2 ** col[1] * np.pi * p[0, col[2]]), atol=1e-12)
+        elif col[0] == "pad":
+            assert f[P] == 0
+        else:
+            assert f[P] == raw[col[0]][col[1]]
+    # the layout is a permutation of the 84 features plus 12 pads
+    want = {(t, k, c) for t in ("sin", "cos") for k in range(M.FREQ) for c in range(3)}
+    want |= {(t, c) for t in "pnav" for c in range(3)}
+    assert {c for c in seen if c[0] != "pad"} == want and M.PAD.sum() == 12
+
+
+def test_encoder_lane_groups_share_code():
+    """Column P is computed by MFMA lane group (P >> 2) & 3; every group holds 18 sin/cos
+    columns of 3 consecutive octaves, so the GPU encoder runs one code path per lane."""
+    for g in range(4):
+        cols = [M.COLUMNS[P] for P in range(M.IN) if (P >> 2) & 3 == g]
+        trig = [c for c in cols if c[0] in ("sin", "cos")]
+        assert len(cols) == 24 and len(trig) == 18
+        assert {c[1] for c in trig} == {3 * g, 3 * g + 1, 3 * g + 2}
