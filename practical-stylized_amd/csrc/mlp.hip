@@ -47,10 +47,14 @@ __host__ __device__ constexpr int col_group(int P) { return (P >> 2) & 3; }
 __host__ __device__ constexpr int col_slot(int P) { return 8 * (P >> 5) + ((P & 31) >= 16 ? 4 : 0) + (P & 3); }
 __host__ __device__ constexpr bool col_is_pad(int P) { return col_group(P) >= 2 && col_slot(P) >= 18; }
 
-// sin / cos of 2^k pi x through the exactly reduced fraction of 2^(k-1) x
+// sin / cos of 2^k pi x = sin / cos of 2 pi fr, fr = the exactly reduced fraction of
+// 2^(k-1) x in [0, 1), on the hardware sine / cosine (v_sin_f32 / v_cos_f32 take the angle in
+// revolutions and need |arg| <= 256, which the reduction guarantees)
 __device__ __forceinline__ void octave_sincos(float x, int k, float& sn, float& cs) {
     const float t = ldexpf(x, k - 1);
-    sincospif(2.0f * (t - floorf(t)), &sn, &cs);
+    const float fr = t - floorf(t);
+    sn = __builtin_amdgcn_sinf(fr);
+    cs = __builtin_amdgcn_cosf(fr);
 }
 
 // feature of column P (reference path: the encode-only and FP64 reference kernels)
@@ -70,11 +74,10 @@ __device__ inline float mlp_feature(const VertexIn& x, int P) {
 }
 
 template <typename S>
-__device__ __forceinline__ VertexIn load_vertex(const S* rec, const float* anchors, uint32_t W, uint32_t H,
-                                                uint32_t L, uint32_t d, uint64_t sample) {
+__device__ __forceinline__ VertexIn load_vertex_at(const S* rec, const float* anchors, uint32_t W, uint32_t H,
+                                                   uint32_t L, uint32_t d, uint64_t s, uint64_t px, uint32_t x,
+                                                   uint32_t y) {
     const uint64_t plane = (uint64_t)W * H;
-    const uint64_t s = sample / plane, px = sample - s * plane;
-    const uint32_t x = (uint32_t)(px % W), y = (uint32_t)(px / W);
     const uint32_t abx = (W + 7) / 8, aby = (H + 7) / 8;
     auto pos = [&](uint32_t lv, float out[3]) {
         const S* r = rec + ((uint64_t)s * L + lv) * NLPF_SCREEN_PLANES * plane;
@@ -106,6 +109,14 @@ __device__ __forceinline__ VertexIn load_vertex(const S* rec, const float* ancho
     return v;
 }
 
+template <typename S>
+__device__ __forceinline__ VertexIn load_vertex(const S* rec, const float* anchors, uint32_t W, uint32_t H,
+                                                uint32_t L, uint32_t d, uint64_t sample) {
+    const uint64_t plane = (uint64_t)W * H;
+    const uint64_t s = sample / plane, px = sample - s * plane;
+    return load_vertex_at(rec, anchors, W, H, L, d, s, px, (uint32_t)(px % W), (uint32_t)(px / W));
+}
+
 // A fragment of W_l (rows = out features m0..m0+15, K-step s, permuted K order)
 __device__ __forceinline__ bf16x8 load_wfrag(const __bf16* w, int in_dim, int m0, int s, int lane) {
     const int m = m0 + (lane & 15), g = lane >> 4;
@@ -118,13 +129,23 @@ __device__ __forceinline__ bf16x8 load_wfrag(const __bf16* w, int in_dim, int m0
     return r;
 }
 
-// B fragments of the next layer from this layer's D tiles (ReLU applied)
+// B fragments of the next layer from this layer's D tiles (ReLU applied).  The ReLU runs
+// on the rounded bf16 pairs as a signed 16-bit max with 0: negative bf16 values (sign bit set,
+// -0 included) are negative int16, non-negative ones keep their order, and rounding commutes
+// with max(., 0).
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ bf16x8 relu_pack(const f32x4& a, const f32x4& b) {
     bf16x8 r;
     for (int i = 0; i < 4; ++i) {
-        r[i] = (__bf16)fmaxf(a[i], 0.0f);
-        r[4 + i] = (__bf16)fmaxf(b[i], 0.0f);
+        r[i] = (__bf16)a[i];
+        r[4 + i] = (__bf16)b[i];
     }
+    i16x2 v[4];
+    __builtin_memcpy(v, &r, sizeof(r));
+    const i16x2 zero = {0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = __builtin_elementwise_max(v[i], zero);
+    __builtin_memcpy(&r, v, sizeof(r));
     return r;
 }
 
@@ -259,10 +280,19 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
     const uint64_t nwaves = (uint64_t)gridDim.x * 4;
     const uint64_t per = 16 * MLP_QT;
     const uint64_t plane = (uint64_t)a.W * a.H;
+    // contiguous samples 0..n-1 on an image of >= 64 and < 2^24 pixels: the wave-uniform tile
+    // base is split into (slice, pixel) once, each lane adds its offset (< 64: at most one
+    // slice boundary), and the row comes from an FP32 reciprocal with one correction step
+    // (exact for pixel indices below 2^24)
+    const bool contiguous = !a.idx && !a.feats && plane >= 64 && plane < (1u << 24);
+    const float inv_w = 1.0f / (float)a.W;
     for (uint64_t base = wave * per; base < a.n; base += nwaves * per) {
         // keeps the (loop-invariant) LDS weight reads inside the loop: hoisted, the 46
         // fragments would hold 184 VGPRs for the whole loop
         asm volatile("" ::: "memory");
+        const uint64_t s_base = contiguous ? base / plane : 0;
+        const uint32_t px_base = contiguous ? (uint32_t)(base - s_base * plane) : 0;
+        uint32_t s_q[MLP_QT], px_q[MLP_QT];
         bf16x8 xin[MLP_QT][3];
 #pragma unroll
         for (int q = 0; q < MLP_QT; ++q) {
@@ -271,6 +301,27 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
                 if (a.feats) {
                     for (int s = 0; s < 3; ++s)
                         for (int j = 0; j < 8; ++j) xin[q][s][j] = (__bf16)a.feats[i * MLP_IN + phys_feature(s, g, j)];
+                } else if (contiguous) {
+                    uint32_t px = px_base + 16 * q + (lane & 15), sl = (uint32_t)s_base;
+                    if (px >= (uint32_t)plane) {
+                        px -= (uint32_t)plane;
+                        ++sl;
+                    }
+                    int y = (int)((float)px * inv_w);
+                    int x = (int)px - y * (int)a.W;
+                    if (x < 0) {
+                        --y;
+                        x += (int)a.W;
+                    } else if (x >= (int)a.W) {
+                        ++y;
+                        x -= (int)a.W;
+                    }
+                    s_q[q] = sl;
+                    px_q[q] = px;
+                    const VertexIn v =
+                        a.fp16 ? load_vertex_at((const __half*)a.rec, a.anchors, a.W, a.H, a.L, a.d, sl, px, x, y)
+                               : load_vertex_at((const float*)a.rec, nullptr, a.W, a.H, a.L, a.d, sl, px, x, y);
+                    encode_frags(v, lane, xin[q]);
                 } else {
                     const uint64_t sm = a.idx ? a.idx[i] : i;
                     const VertexIn v = a.fp16 ? load_vertex((const __half*)a.rec, a.anchors, a.W, a.H, a.L, a.d, sm)
@@ -290,8 +341,15 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
                 const uint64_t i = base + 16 * q + lane;
                 if (i < a.n) {
                     if (a.planar) {
-                        const uint64_t sm = a.idx ? a.idx[i] : i;
-                        const uint64_t sl = sm / plane, px = sm - sl * plane;
+                        uint64_t sl, px;
+                        if (contiguous) {
+                            sl = s_q[q];
+                            px = px_q[q];
+                        } else {
+                            const uint64_t sm = a.idx ? a.idx[i] : i;
+                            sl = sm / plane;
+                            px = sm - sl * plane;
+                        }
                         float* o = a.out + ((uint64_t)sl * a.out_levels + a.out_slot) * 3 * plane + px;
                         o[0] = y[q][0];
                         o[plane] = y[q][1];
