@@ -38,6 +38,23 @@ __device__ __forceinline__ float ld_rec(const float* p, size_t i) { return p[i];
 __device__ __forceinline__ float ld_rec(const __half* p, size_t i) { return __half2float(p[i]); }
 __device__ __forceinline__ double ld_rec(const double* p, size_t i) { return p[i]; }
 
+// two adjacent elements (i even, so the pair is 2-element aligned)
+__device__ __forceinline__ void ld_rec2(const float* p, size_t i, float& a, float& b) {
+    const float2 t = *(const float2*)(p + i);
+    a = t.x;
+    b = t.y;
+}
+__device__ __forceinline__ void ld_rec2(const __half* p, size_t i, float& a, float& b) {
+    const __half2 t = *(const __half2*)(p + i);
+    a = __low2float(t);
+    b = __high2float(t);
+}
+__device__ __forceinline__ void ld_rec2(const double* p, size_t i, double& a, double& b) {
+    const double2 t = *(const double2*)(p + i);
+    a = t.x;
+    b = t.y;
+}
+
 __device__ __forceinline__ float exp2_neg(float e) { return __builtin_amdgcn_exp2f(-e); }
 __device__ __forceinline__ double exp2_neg(double e) { return exp2(-e); }
 
@@ -68,12 +85,15 @@ struct alignas(2 * sizeof(A)) V2 {
 template <typename S, typename A, int RAD, int NW, int R>
 __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) {
     static_assert(R <= 2 * RAD + 1, "rows per lane must not exceed the window height");
-    constexpr int TW = 64, TH = NW * R, WT = TW + 2 * RAD, HT = TH + 2 * RAD, NT = 64 * NW;
+    constexpr int TW = 64, TH = NW * R, HT = TH + 2 * RAD, NT = 64 * NW;
     constexpr int K = 2 * RAD + 1;
+    // staged columns start at an even x (pairs of records load as one aligned 2-element
+    // vector when W is even): XS0 = tx0 - RAD - OFF, the window column tx0 - RAD sits at OFF
+    constexpr int OFF = RAD & 1, WS = TW + 2 * RAD + 2 * OFF, WP = WS / 2;
     // guide features and level input of the tile + halo: (p*kp, lum*kl), (n, in.r), (in.g, in.b)
-    __shared__ V4<A> shA[HT * WT];
-    __shared__ V4<A> shB[HT * WT];
-    __shared__ V2<A> shC[HT * WT];
+    __shared__ V4<A> shA[HT * WS];
+    __shared__ V4<A> shB[HT * WS];
+    __shared__ V2<A> shC[HT * WS];
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
@@ -85,44 +105,68 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     const S* rec = (const S*)a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
     const A* child = a.has_child ? a.out + ((size_t)s * a.out_levels + a.child_slot) * 3 * plane : nullptr;
     const float* anc = a.anchors ? a.anchors + (size_t)((size_t)s * a.L + a.d) * a.abx * a.aby * 3 : nullptr;
+    const bool even_w = (W & 1) == 0;
 
-    // ---- stage tile + halo in LDS ----
-    for (int i = threadIdx.x; i < WT * HT; i += NT) {
-        const int r = i / WT, c = i - r * WT;
-        const int gx = tx0 - RAD + c, gy = ty0 - RAD + r;
-        V4<A> fa, fb;
-        V2<A> fc;
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+    // ---- stage tile + halo in LDS, two horizontally adjacent records per thread ----
+    for (int i = threadIdx.x; i < HT * WP; i += NT) {
+        const int r = i / WP, cp = i - r * WP;
+        const int gx = tx0 - RAD - OFF + 2 * cp, gy = ty0 - RAD + r;
+        const bool row_ok = gy >= 0 && gy < H;
+        const bool ok0 = row_ok && gx >= 0 && gx < W, ok1 = row_ok && gx + 1 >= 0 && gx + 1 < W;
+        A v[NLPF_SCREEN_PLANES][2], ch[3][2];
+        if (ok0 && ok1 && even_w) {
             const size_t o = (size_t)gy * W + gx;
-            A px = (A)ld_rec(rec, o), py = (A)ld_rec(rec, plane + o), pz = (A)ld_rec(rec, 2 * plane + o);
-            if (anc) {
-                const float* q = anc + ((size_t)(gy >> 3) * a.abx + (gx >> 3)) * 3;
-                px += (A)q[0];
-                py += (A)q[1];
-                pz += (A)q[2];
-            }
-            const A ax = (A)ld_rec(rec, 6 * plane + o), ay = (A)ld_rec(rec, 7 * plane + o),
-                    az = (A)ld_rec(rec, 8 * plane + o);
-            A ir = (A)ld_rec(rec, 9 * plane + o) * a.rscale, ig = (A)ld_rec(rec, 10 * plane + o) * a.rscale,
-              ib = (A)ld_rec(rec, 11 * plane + o) * a.rscale;
+#pragma unroll
+            for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) ld_rec2(rec, c * plane + o, v[c][0], v[c][1]);
             if (child) {
-                ir = fma((A)ld_rec(rec, 12 * plane + o), child[o], ir);
-                ig = fma((A)ld_rec(rec, 13 * plane + o), child[plane + o], ig);
-                ib = fma((A)ld_rec(rec, 14 * plane + o), child[2 * plane + o], ib);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) ld_rec2(child, c * plane + o, ch[c][0], ch[c][1]);
             }
-            fa = V4<A>{px * a.kp, py * a.kp, pz * a.kp, lum709(v3<A>{ax, ay, az}) * a.kl};
-            fb = V4<A>{(A)ld_rec(rec, 3 * plane + o), (A)ld_rec(rec, 4 * plane + o), (A)ld_rec(rec, 5 * plane + o),
-                       ir};
-            fc = V2<A>{ig, ib};
         } else {
-            // outside the image: a far-away guide position drives the weight to exactly 0
-            fa = V4<A>{(A)1e18, (A)1e18, (A)1e18, (A)0};
-            fb = V4<A>{(A)0, (A)0, (A)0, (A)0};
-            fc = V2<A>{(A)0, (A)0};
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (j == 0 ? ok0 : ok1) {
+                    const size_t o = (size_t)gy * W + (gx + j);
+#pragma unroll
+                    for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) v[c][j] = (A)ld_rec(rec, c * plane + o);
+                    if (child) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) ch[c][j] = child[c * plane + o];
+                    }
+                }
+            }
         }
-        shA[i] = fa;
-        shB[i] = fb;
-        shC[i] = fc;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            V4<A> fa, fb;
+            V2<A> fc;
+            if (j == 0 ? ok0 : ok1) {
+                A px = v[0][j], py = v[1][j], pz = v[2][j];
+                if (anc) {
+                    const float* q = anc + ((size_t)(gy >> 3) * a.abx + ((gx + j) >> 3)) * 3;
+                    px += (A)q[0];
+                    py += (A)q[1];
+                    pz += (A)q[2];
+                }
+                A ir = v[9][j] * a.rscale, ig = v[10][j] * a.rscale, ib = v[11][j] * a.rscale;
+                if (child) {
+                    ir = fma(v[12][j], ch[0][j], ir);
+                    ig = fma(v[13][j], ch[1][j], ig);
+                    ib = fma(v[14][j], ch[2][j], ib);
+                }
+                fa = V4<A>{px * a.kp, py * a.kp, pz * a.kp, lum709(v3<A>{v[6][j], v[7][j], v[8][j]}) * a.kl};
+                fb = V4<A>{v[3][j], v[4][j], v[5][j], ir};
+                fc = V2<A>{ig, ib};
+            } else {
+                // outside the image: a far-away guide position drives the weight to exactly 0
+                fa = V4<A>{(A)1e18, (A)1e18, (A)1e18, (A)0};
+                fb = V4<A>{(A)0, (A)0, (A)0, (A)0};
+                fc = V2<A>{(A)0, (A)0};
+            }
+            shA[r * WS + 2 * cp + j] = fa;
+            shB[r * WS + 2 * cp + j] = fb;
+            shC[r * WS + 2 * cp + j] = fc;
+        }
     }
     __syncthreads();
 
@@ -130,32 +174,33 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     // Each lane owns R vertically adjacent outputs; input row r of the lane's strip feeds the
     // outputs k with 0 <= r - k <= 2 RAD.  The edge rows (r < R-1 or r > 2 RAD) run in fully
     // unrolled loops where that test is a compile-time constant; the middle rows feed every
-    // output, so no branch sits between the R independent dependency chains.
+    // output, so no branch sits between the R independent dependency chains.  Row partials
+    // are plain FP32 sums over up to two rows, folded into Kahan-compensated totals.
     const int lc = threadIdx.x & 63;
     const int r0 = (threadIdx.x >> 6) * R;
+    const A kn = a.kn;
+    // centre normals pre-scaled by kn: kn (1 - n_c.n_j) = kn - (kn n_c).n_j
     A cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R];
     A Wt[R], Sx[R], Sy[R], Sz[R], cW[R], cX[R], cY[R], cZ[R];
+    A rw[R], rx[R], ry[R], rz[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const V4<A> ca = shA[(r0 + k + RAD) * WT + lc + RAD];
-        const V4<A> cb = shB[(r0 + k + RAD) * WT + lc + RAD];
+        const V4<A> ca = shA[(r0 + k + RAD) * WS + lc + RAD + OFF];
+        const V4<A> cb = shB[(r0 + k + RAD) * WS + lc + RAD + OFF];
         cpx[k] = ca.x;
         cpy[k] = ca.y;
         cpz[k] = ca.z;
         cl[k] = ca.w;
-        cnx[k] = cb.x;
-        cny[k] = cb.y;
-        cnz[k] = cb.z;
+        cnx[k] = kn * cb.x;
+        cny[k] = kn * cb.y;
+        cnz[k] = kn * cb.z;
         Wt[k] = Sx[k] = Sy[k] = Sz[k] = cW[k] = cX[k] = cY[k] = cZ[k] = (A)0;
+        rw[k] = rx[k] = ry[k] = rz[k] = (A)0;
     }
-    const A kn = a.kn;
     auto row = [&](int r, int klo, int khi) __attribute__((always_inline)) {
-        const V4<A>* pa = shA + (r0 + r) * WT + lc;
-        const V4<A>* pb = shB + (r0 + r) * WT + lc;
-        const V2<A>* pc = shC + (r0 + r) * WT + lc;
-        A rw[R], rx[R], ry[R], rz[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) rw[k] = rx[k] = ry[k] = rz[k] = (A)0;
+        const V4<A>* pa = shA + (r0 + r) * WS + lc + OFF;
+        const V4<A>* pb = shB + (r0 + r) * WS + lc + OFF;
+        const V2<A>* pc = shC + (r0 + r) * WS + lc + OFF;
 #pragma unroll
         for (int dx = 0; dx < K; ++dx) {
             const V4<A> na = pa[dx];
@@ -168,10 +213,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
                     A e = ex * ex;
                     e = fma(ey, ey, e);
                     e = fma(ez, ez, e);
-                    A nn = cnx[k] * nb.x;
-                    nn = fma(cny[k], nb.y, nn);
-                    nn = fma(cnz[k], nb.z, nn);
-                    const A tt = fma(-kn, nn, kn);
+                    const A tt = fma(-cnz[k], nb.z, fma(-cny[k], nb.y, fma(-cnx[k], nb.x, kn)));
                     e = fma(tt, tt, e);
                     const A dl = cl[k] - na.w;
                     e = fma(dl, dl, e);
@@ -183,21 +225,32 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
                 }
             }
         }
+    };
+    auto flush = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            if (k >= klo && k <= khi) {
-                kahan_add(Wt[k], cW[k], rw[k]);
-                kahan_add(Sx[k], cX[k], rx[k]);
-                kahan_add(Sy[k], cY[k], ry[k]);
-                kahan_add(Sz[k], cZ[k], rz[k]);
-            }
+            kahan_add(Wt[k], cW[k], rw[k]);
+            kahan_add(Sx[k], cX[k], rx[k]);
+            kahan_add(Sy[k], cY[k], ry[k]);
+            kahan_add(Sz[k], cZ[k], rz[k]);
+            rw[k] = rx[k] = ry[k] = rz[k] = (A)0;
         }
     };
 #pragma unroll
     for (int r = 0; r < R - 1; ++r) row(r, 0, r);
-    for (int r = R - 1; r <= 2 * RAD; ++r) row(r, 0, R - 1);
+    if (R > 1) flush();
+    for (int r = R - 1; r + 1 <= 2 * RAD; r += 2) {
+        row(r, 0, R - 1);
+        row(r + 1, 0, R - 1);
+        flush();
+    }
+    if ((2 * RAD + 2 - R) % 2) {
+        row(2 * RAD, 0, R - 1);
+        flush();
+    }
 #pragma unroll
     for (int i = 0; i < R - 1; ++i) row(2 * RAD + 1 + i, i + 1, R - 1);
+    if (R > 1) flush();
 
     // ---- epilogue: normalise, stylize, store ----
     A* outp = a.out + ((size_t)s * a.out_levels + a.out_slot) * 3 * plane;
