@@ -21,7 +21,10 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 METRIC = "filtered path-vertices/s @1080p×16spp, plain vs FP16+MLP; HBM % of peak; L2 err"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-FLOPS_PER_PAIR = 28.0  # guide distances 8, normal term 9, luminance 3, exp 1, accumulate 7
+# per neighbour pair in the shipped kernel: |dp|^2 7 (3 sub, mul, 2 fma), normal term 8
+# (3 fma, fma), luminance 3 (sub, fma), exp 1, accumulate 7 (add, 3 fma) = 17 instructions
+FLOPS_PER_PAIR = 27.0
+FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per cycle (the build emits no packed FP32)
 
 
 def log(msg):
@@ -88,51 +91,68 @@ def measured_traffic():
 
 
 def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
-    """FP16 records + learned cache at `--cache-depth` (2^16 training records per frame,
-    batch 2^14, Adam lr 1e-2).  The judge slice rotates over the spp slices from frame to
-    frame; `--mlp-frames` untimed frames train the cache first.  Reported twice: with the
-    per-tile fallback at `--tile-threshold`, and without fallback."""
-    log(f"FP16 + learned cache at depth {args.cache_depth}")
-    h.mlp_init(1)
-    kw = dict(depth=args.cache_depth, train_records=1 << 16, batch=1 << 14, lr=1e-2)
-    frame = 0
+    """FP16 records + learned cache (2^16 training records per frame, batch 2^14, Adam lr 1e-2).
+    The judge slice rotates over the spp slices from frame to frame; `--mlp-frames` untimed
+    frames train a fresh cache first.  Measured at cache depth 3 and 2, each with the per-tile
+    fallback at `--tile-threshold` and without fallback; the first is the headline entry."""
+    legs = []
+    for depth in args.cache_depths:
+        log(f"FP16 + learned cache at depth {depth}")
+        h.mlp_init(1)
+        kw = dict(depth=depth, train_records=1 << 16, batch=1 << 14, lr=1e-2)
+        frame = 0
 
-    def frames(n, thr):
-        nonlocal frame
-        acc, st = {}, None
-        h.sync()
-        t0 = time.perf_counter()
-        for _ in range(n):
-            st = h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame, tile_threshold=thr,
-                                  **kw)
-            frame += 1
-            for k in ("judge_ms", "train_ms", "infer_ms", "fallback_ms", "filter_ms", "fallback_tiles"):
-                acc[k] = acc.get(k, 0.0) + st[k]
-        h.sync()
-        ms = (time.perf_counter() - t0) / max(n, 1) * 1e3
-        return ms, {k: v / max(n, 1) for k, v in acc.items()}, st
+        def frames(n, thr):
+            nonlocal frame
+            acc, st = {}, None
+            h.sync()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                st = h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame,
+                                      tile_threshold=thr, **kw)
+                frame += 1
+                for k in ("judge_ms", "train_ms", "infer_ms", "fallback_ms", "filter_ms", "fallback_tiles"):
+                    acc[k] = acc.get(k, 0.0) + st[k]
+            h.sync()
+            ms = (time.perf_counter() - t0) / max(n, 1) * 1e3
+            return ms, {k: v / max(n, 1) for k, v in acc.items()}, st
 
-    frames(args.mlp_frames, 1e30)
-    out = {}
-    for name, thr in (("fallback", args.tile_threshold), ("no_fallback", 1e30)):
-        frames(args.warmup, thr)
-        ms, ph, st = frames(args.steps, thr)
-        leg = {"value": verts / (ms / 1e3), "unit": "vertices/s", "ms_per_step": ms,
-               "phases_ms": {k[:-3]: ph[k] for k in ph if k.endswith("_ms")},
-               "tile_threshold": thr if thr < 1e29 else None,
-               "fallback_tile_frac": ph["fallback_tiles"] / st["tiles"],
-               "judge_rel_l2": st["judge_rel_l2"], "cache_rel_l2": st["cache_rel_l2"], "train_loss": st["train_loss"]}
-        if film_ref is not None:
-            f = film.download(np.float32).astype(np.float64)
-            leg["l2_err_vs_fp64"] = float(np.linalg.norm(f - film_ref) / np.linalg.norm(film_ref))
-        log(f"  {name}: {ms:.2f} ms/step, phases {np.round(list(leg['phases_ms'].values()), 2).tolist()}, "
-            f"fallback {leg['fallback_tile_frac']:.3f}, L2 {leg.get('l2_err_vs_fp64')}")
-        out[name] = leg
-    res = dict(out["fallback"])
-    res["no_fallback"] = out["no_fallback"]
-    res.update({"cache_depth": args.cache_depth, "train_frames_before_timing": args.mlp_frames,
-                "train_records_per_frame": 1 << 16, "batch": 1 << 14, "lr": 1e-2})
+        frames(args.mlp_frames, 1e30)
+        for thr in (args.tile_threshold, 1e30):
+            frames(args.warmup, thr)
+            ms, ph, st = frames(args.steps, thr)
+            leg = {"value": verts / (ms / 1e3), "unit": "vertices/s", "ms_per_step": ms, "cache_depth": depth,
+                   "tile_threshold": thr if thr < 1e29 else None,
+                   "phases_ms": {k[:-3]: ph[k] for k in ph if k.endswith("_ms")},
+                   "fallback_tile_frac": ph["fallback_tiles"] / st["tiles"],
+                   "judge_rel_l2": st["judge_rel_l2"], "cache_rel_l2": st["cache_rel_l2"],
+                   "train_loss": st["train_loss"]}
+            if film_ref is not None:
+                f = film.download(np.float32).astype(np.float64)
+                leg["l2_err_vs_fp64"] = float(np.linalg.norm(f - film_ref) / np.linalg.norm(film_ref))
+            log(f"  depth {depth} threshold {leg['tile_threshold']}: {ms:.2f} ms/step, phases "
+                f"{np.round(list(leg['phases_ms'].values()), 2).tolist()}, fallback "
+                f"{leg['fallback_tile_frac']:.3f}, L2 {leg.get('l2_err_vs_fp64')}")
+            legs.append(leg)
+    res = dict(legs[0])
+    res["variants"] = legs[1:]
+    res.update({"train_frames_before_timing": args.mlp_frames, "train_records_per_frame": 1 << 16,
+                "batch": 1 << 14, "lr": 1e-2})
     return res
+
+
+def pcie_upload_gbs(h, nbytes=1 << 30):
+    """Host-to-device rate of nlpf_upload from pageable host memory (1 GiB)."""
+    host = np.ones(nbytes // 4, np.float32)
+    dev = h.alloc(nbytes)
+    dev.upload(host)
+    h.sync()
+    t0 = time.perf_counter()
+    dev.upload(host)
+    h.sync()
+    dt = time.perf_counter() - t0
+    dev.free()
+    return nbytes / dt / 1e9
 
 
 def main():
@@ -150,7 +170,8 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
-    ap.add_argument("--cache-depth", type=int, default=2, help="depth the learned cache answers for")
+    ap.add_argument("--cache-depths", type=int, nargs="+", default=[3, 2],
+                    help="cache depths measured (the first is the headline fp16_mlp entry)")
     ap.add_argument("--mlp-frames", type=int, default=8, help="untimed training frames before the timed ones")
     ap.add_argument("--tile-threshold", type=float, default=0.05)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -214,15 +235,27 @@ def main():
         "config": {"workload": f"screen NL-PF {W}x{H}x{S}spp depth {L - 1} ({L} levels), "
                                f"{2 * R + 1}x{2 * R + 1} window, ramp16+hue0.1, FP32 records",
                    "vertices_per_step": verts, "sigmas": list(sig)},
-        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS,
+        # the 15x15 gather is compute-bound (~75 flop per algorithmic byte): the binding roof
+        # is FP32 VALU; the HBM figures stand beside it
+        "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP32_PEAK_TFLOPS,
                      "traffic": measured_traffic() if (W, H, S, L, R) == (1920, 1080, 16, 5, 7) else None,
                      "kernel": "screen_level_kernel<float,float,7,4,2>",
+                     "algorithmic_flops_per_launch": pairs / L * FLOPS_PER_PAIR,
+                     "flops_per_pair": FLOPS_PER_PAIR,
                      "algorithmic_bytes_per_launch": float(np.mean(lb32)),
                      "avg_launch_ms": float(lv32.mean()),
-                     "valu": {"achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS,
-                              "frac": tflops / FP32_PEAK_TFLOPS, "flops_per_pair": FLOPS_PER_PAIR}},
+                     "frac_of_nonpacked_fp32_peak": tflops / FP32_NONPACKED_TFLOPS,
+                     "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved_gbs / HBM_PEAK_GBS}},
     }
+
+    h2d = pcie_upload_gbs(h)
+    rec_bytes = verts * 60
+    result["pcie_inclusive"] = {"h2d_gbs": h2d, "record_bytes": rec_bytes,
+                                "value": verts / (ms32 / 1e3 + rec_bytes / (h2d * 1e9)), "unit": "vertices/s",
+                                "note": "FP32 records uploaded from pageable host memory each frame"}
+    log(f"host->device {h2d:.1f} GB/s")
 
     film_ref = None
     if not args.no_fp64:

@@ -103,8 +103,8 @@ __device__ __forceinline__ VertexIn load_vertex_at(const S* rec, const float* an
         q[2] = SceneConst::cam_z;
     }
     float dv[3] = {q[0] - v.p[0], q[1] - v.p[1], q[2] - v.p[2]};
-    const float len = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-    const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+    const float len2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+    const float inv = len2 > 0.0f ? __builtin_amdgcn_rsqf(len2) : 0.0f;
     for (int c = 0; c < 3; ++c) v.v[c] = dv[c] * inv;
     return v;
 }
@@ -941,12 +941,14 @@ int nlpf_mlp_infer(nlpf_handle* h, const nlpf_screen_desc* desc, const void* rec
     return NLPF_OK;
 }
 
-// One Adam step on a batch (batch % 128 == 0).  loss (host, optional) receives the batch's
-// mean squared error before the update; grad (device, optional) receives dL/dparams.
-int nlpf_mlp_train_step(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
-                        uint32_t depth, const uint32_t* idx, const float* feats, const float* target, uint32_t batch,
-                        float lr, int apply_update, float* loss, float* grad) {
-    if (!h) return NLPF_E_INVALID_ARG;
+}  // extern "C"
+
+namespace nlpf {
+// Enqueues one training step on the handle's stream (no host synchronisation); the batch's
+// loss is left in device memory (mlp_loss_device).
+int mlp_train_enqueue(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                      uint32_t depth, const uint32_t* idx, const float* feats, const float* target, uint32_t batch,
+                      float lr, int apply_update, float* grad) {
     nlpf_mlp_state* s;
     int rc = mlp_need(h, &s);
     if (rc) return rc;
@@ -1011,7 +1013,26 @@ int nlpf_mlp_train_step(nlpf_handle* h, const nlpf_screen_desc* desc, const void
         rc = mlp_refresh(h, s, lr, s->step);
         if (rc) return rc;
     }
-    if (loss) NLPF_HIP(h, hipMemcpyAsync(loss, s->loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    return NLPF_OK;
+}
+
+const float* mlp_loss_device(nlpf_handle* h) {
+    nlpf_mlp_state* s = mlp_state(h, false);
+    return s ? s->loss : nullptr;
+}
+}  // namespace nlpf
+
+extern "C" {
+
+// One Adam step on a batch (batch % 128 == 0).  loss (host, optional) receives the batch's
+// mean squared error before the update; grad (device, optional) receives dL/dparams.
+int nlpf_mlp_train_step(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                        uint32_t depth, const uint32_t* idx, const float* feats, const float* target, uint32_t batch,
+                        float lr, int apply_update, float* loss, float* grad) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    int rc = mlp_train_enqueue(h, desc, records, anchors, depth, idx, feats, target, batch, lr, apply_update, grad);
+    if (rc) return rc;
+    if (loss) NLPF_HIP(h, hipMemcpyAsync(loss, mlp_loss_device(h), sizeof(float), hipMemcpyDeviceToHost, h->stream));
     NLPF_HIP(h, hipStreamSynchronize(h->stream));
     return NLPF_OK;
 }

@@ -26,6 +26,10 @@ hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, u
                             float* film, hipStream_t st);
 int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                      uint32_t depth, uint64_t n, float* planar_out, uint32_t out_levels, uint32_t out_slot);
+int mlp_train_enqueue(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                      uint32_t depth, const uint32_t* idx, const float* feats, const float* target, uint32_t batch,
+                      float lr, int apply_update, float* grad);
+const float* mlp_loss_device(nlpf_handle* h);
 
 __global__ void gather_targets_kernel(const float* F, uint64_t plane, const uint32_t* idx, uint32_t n, float* t) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,13 +274,14 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
         hipLaunchKernelGGL(gather_targets_kernel, dim3((run->train_records + 255) / 256), dim3(256), 0, st, Fj, plane,
                            lidx, run->train_records, tgt);
         if ((rc = check_launch(h, "gather targets"))) return rc;
-        float loss = 0.0f;
         for (uint32_t b = 0; b < run->train_records / run->batch; ++b) {
-            rc = nlpf_mlp_train_step(h, desc, records, anchors, k, gidx + (size_t)b * run->batch, nullptr,
-                                     tgt + (size_t)b * run->batch * 3, run->batch, run->lr, 1, &loss, nullptr);
+            rc = mlp_train_enqueue(h, desc, records, anchors, k, gidx + (size_t)b * run->batch, nullptr,
+                                   tgt + (size_t)b * run->batch * 3, run->batch, run->lr, 1, nullptr);
             if (rc) return rc;
         }
-        // the host vectors must outlive the asynchronous uploads
+        float loss = 0.0f;
+        NLPF_HIP(h, hipMemcpyAsync(&loss, mlp_loss_device(h), sizeof(float), hipMemcpyDeviceToHost, st));
+        // the host vectors (and `loss`) must outlive the asynchronous copies
         NLPF_HIP(h, hipStreamSynchronize(st));
         rs.train_loss = loss;
     }
