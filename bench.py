@@ -141,6 +141,125 @@ def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
     return res
 
 
+def adjoint_leg(h, N):
+    """Config 3: forward + adjoint at 960x540 x 8 spp, 4 levels, 7x7 window; 19 parameters
+    (3 bandwidths + 16 ramp-stop values), L2 loss to a target rendered at perturbed parameters."""
+    W, H, S, L, R = 960, 540, 8, 4, 3
+    log("adjoint (config 3)")
+    rng = np.random.default_rng(3)
+    pos = np.sort(rng.random(16)).astype(np.float32)
+    val = np.sort(rng.random(16)).astype(np.float32)
+    table = N.style_table([N.ramp(pos, val), N.hue_shift(0.1, inner=0)])
+    sig = np.array([0.05, 0.9, 0.5])
+    desc = h.screen_desc(W, H, S, L, N.FP32, R)
+    rec = h.alloc(W * H * S * L * 60)
+    h.generate_screen_records(desc, rec, seed=3)
+    film = h.alloc(W * H * 12)
+    tt = table.copy()
+    tt[0]["p"][16:32] = (tt[0]["p"][16:32] + rng.normal(0, 0.05, 16)).astype(np.float32)
+    h.set_style(tt, [1])
+    h.set_sigmas(*(sig * np.exp(rng.normal(0, 0.05, 3))))
+    h.screen_run(desc, rec, None, film, None)
+    target = film.download(np.float32)
+    h.set_style(table, [1])
+    h.set_sigmas(*sig)
+    lv = h.alloc(S * L * 3 * W * H * 4)
+    grad = h.alloc(19 * 4)
+    dfilm = h.alloc(W * H * 12)
+    fwd, bwd = [], []
+    for it in range(4):
+        h.sync()
+        t0 = time.perf_counter()
+        h.screen_run(desc, rec, None, film, lv)
+        h.sync()
+        t1 = time.perf_counter()
+        f = film.download(np.float32)
+        dfilm.upload((2 * (f - target)).astype(np.float32))
+        h.sync()
+        t2 = time.perf_counter()
+        h.screen_backward(desc, rec, lv, dfilm, grad)
+        h.sync()
+        t3 = time.perf_counter()
+        if it:
+            fwd.append(t1 - t0)
+            bwd.append(t3 - t2)
+    g = grad.download(np.float32, 19)
+    for b in (rec, film, lv, grad, dfilm):
+        b.free()
+    verts = W * H * S * L
+    return {"workload": f"{W}x{H}x{S}spp, {L} levels, {2 * R + 1}x{2 * R + 1} window, ramp16+hue0.1, 19 parameters",
+            "forward_ms": 1e3 * float(np.mean(fwd)), "backward_ms": 1e3 * float(np.mean(bwd)),
+            "value": verts / float(np.mean(fwd) + np.mean(bwd)), "unit": "vertices/s (forward + adjoint)",
+            "grad_finite": bool(np.isfinite(g).all())}
+
+
+def large_leg(h, N):
+    """Config 5: 3840x2160 x 64 spp, depth 6 (7 levels), 15x15 window, toon (6 levels) at depth 1
+    wrapping the 16-stop ramp at depth 3; records generated and filtered in 16-spp chunks."""
+    W, H, S, L, R, chunks = 3840, 2160, 16, 7, 7, 4
+    rng = np.random.default_rng(2024)
+    pos = np.sort(rng.random(16)).astype(np.float32)
+    val = np.sort(rng.random(16)).astype(np.float32)
+    table = N.style_table([N.ramp(pos, val, depth_mask=N.depth_mask(3)),
+                           N.toon(1.0 / 6.0, inner=0, depth_mask=N.depth_mask(1))])
+    h.set_style(table, [1])
+    plane = W * H
+    verts = plane * S * L
+    out = {"workload": f"{W}x{H}x{S * chunks}spp in {chunks} chunks of {S} spp, {L} levels, 15x15 window, "
+                       "toon(6) at depth 1 over ramp16 at depth 3"}
+    for name, prec, rb in (("fp32", N.FP32, 60), ("fp16", N.FP16, 30)):
+        log(f"config 5 {name}")
+        desc = h.screen_desc(W, H, S, L, prec, R)
+        rec = h.alloc(verts * rb)
+        anc = h.alloc(S * L * ((W + 7) // 8) * ((H + 7) // 8) * 12) if prec == N.FP16 else None
+        film = h.alloc(plane * 12)
+        acc = np.zeros(plane * 3)
+        total = 0.0
+        for c in range(chunks):
+            h.generate_screen_records(desc, rec, anc, seed=100 + c)
+            h.sync()
+            t0 = time.perf_counter()
+            h.screen_run(desc, rec, anc, film, None)
+            h.sync()
+            total += time.perf_counter() - t0
+            acc += film.download(np.float32)
+        for b in (rec, anc, film):
+            if b is not None:
+                b.free()
+        bytes_alg = chunks * S * plane * ((rb + 12 + 12) * (L - 1) + rb + 12)
+        out[name] = {"ms_per_frame": total * 1e3, "value": chunks * verts / total, "unit": "vertices/s",
+                     "hbm_gbs_algorithmic": bytes_alg / total / 1e9,
+                     "hbm_frac": bytes_alg / total / 1e9 / HBM_PEAK_GBS,
+                     "film_finite": bool(np.isfinite(acc).all())}
+        log(f"  {name}: {total * 1e3:.1f} ms per 64-spp frame")
+    return out
+
+
+def world_leg(h_unused, N):
+    """Config 1 in world mode (the reference's hash-grid NL-PF): 256x256 x 4 paths per pixel,
+    max_depth 3, SMIS, no clusters, reuse cap above every neighbour count, 4-level toon at
+    depth 1, records traced on the GPU from the procedural box scene."""
+    W, H, B, L = 256, 256, 4, 4
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": B, "smis": True, "cluster": False, "reuse": [1 << 16]}
+    with N.Handle(cfg) as hw:
+        hw.set_style(N.style_table([N.toon(0.25, depth_mask=N.depth_mask(1))]), [0, 0, 0, 0, 0, 0, 0, -1])
+        desc = hw.world_desc(W, H, B, L)
+        bufs = hw.alloc_world(desc)
+        hw.generate_world_records(desc, bufs, seed=1)
+        hw.world_run(desc, bufs)
+        times = []
+        for _ in range(3):
+            hw.world_run(desc, bufs)
+            st = hw.stats()
+            times.append(st["total_ms"] / 1e3)  # HIP events around the depth loop and the film
+    pts = sum(st["points"][:L])
+    nb = sum(st["neighbors"][:L])
+    ms = 1e3 * float(np.median(times))
+    return {"workload": f"{W}x{H}x{B} paths, {L} levels, radius_scale default, SMIS, toon(4) at depth 1",
+            "ms_per_frame": ms, "vertices": int(pts), "value": pts / (ms / 1e3), "unit": "vertices/s",
+            "neighbours_per_vertex": nb / max(pts, 1)}
+
+
 def pcie_upload_gbs(h, nbytes=1 << 30):
     """Host-to-device rate of nlpf_upload from pageable host memory (1 GiB)."""
     host = np.ones(nbytes // 4, np.float32)
@@ -174,6 +293,7 @@ def main():
                     help="cache depths measured (the first is the headline fp16_mlp entry)")
     ap.add_argument("--mlp-frames", type=int, default=8, help="untimed training frames before the timed ones")
     ap.add_argument("--tile-threshold", type=float, default=0.05)
+    ap.add_argument("--no-extra", action="store_true", help="skip the config 3 / 5 / world-mode measurements")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one FP32 pass and one 1 GiB device copy, for rocprofv3 counter runs")
     args = ap.parse_args()
@@ -296,6 +416,15 @@ def main():
         anc.free()
     else:
         result["fp16_mlp"] = None
+
+    if not args.no_extra:
+        rec32.free()
+        result["adjoint_config3"] = adjoint_leg(h, N)
+        result["large_config5"] = large_leg(h, N)
+        result["world_config1"] = world_leg(h, N)
+        h.set_style(table, [root])
+        rec32 = h.alloc(verts * 60)
+        h.generate_screen_records(d32, rec32, seed=1)
 
     if not args.no_cpu:
         log("CPU baseline (oracle FP32 restatement)")

@@ -124,11 +124,101 @@ __device__ __forceinline__ void style_vjp(const BwdArgs& a, v3<float> F, v3<floa
     h = g;
 }
 
+// Tile + halo of one (slice, depth) image in LDS, as in the forward level kernel: two
+// horizontally adjacent records per thread (one aligned 2-element load per plane when W is
+// even), packed as (p*kp, lum*kl), (n, v.x), (v.y, v.z).  Pass A stages v = in = radiance +
+// throughput * out_{d+1}; pass B stages v = u.
+template <int RAD, int NW, int R, bool PASS_A>
+__device__ __forceinline__ void bwd_stage(const BwdArgs& a, uint32_t s, int tx0, int ty0, V4<float>* shA,
+                                          V4<float>* shB, V2<float>* shC) {
+    constexpr int TW = 64, TH = NW * R, HT = TH + 2 * RAD, NT = 64 * NW;
+    constexpr int OFF = RAD & 1, WS = TW + 2 * RAD + 2 * OFF, WP = WS / 2;
+    const int W = (int)a.W, H = (int)a.H;
+    const size_t plane = a.plane;
+    const float* rec = a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
+    const float* child = PASS_A && a.has_child ? a.level_out + ((size_t)s * a.L + a.d + 1) * 3 * plane : nullptr;
+    const float* up = PASS_A ? nullptr : a.u + (size_t)s * 3 * plane;
+    const bool even_w = (W & 1) == 0;
+    constexpr int NPL = PASS_A ? NLPF_SCREEN_PLANES : 9;  // planes read from the records
+    for (int i = threadIdx.x; i < HT * WP; i += NT) {
+        const int r = i / WP, cp = i - r * WP;
+        const int gx = tx0 - RAD - OFF + 2 * cp, gy = ty0 - RAD + r;
+        const bool row_ok = gy >= 0 && gy < H;
+        const bool ok0 = row_ok && gx >= 0 && gx < W, ok1 = row_ok && gx + 1 >= 0 && gx + 1 < W;
+        float v[NLPF_SCREEN_PLANES][2], ch[3][2];
+        if (ok0 && ok1 && even_w) {
+            const size_t o = (size_t)gy * W + gx;
+#pragma unroll
+            for (int c = 0; c < NPL; ++c) ld_rec2(rec, c * plane + o, v[c][0], v[c][1]);
+            if (PASS_A) {
+                if (child) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ld_rec2(child, c * plane + o, ch[c][0], ch[c][1]);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) ld_rec2(up, c * plane + o, ch[c][0], ch[c][1]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (j == 0 ? ok0 : ok1) {
+                    const size_t o = (size_t)gy * W + (gx + j);
+#pragma unroll
+                    for (int c = 0; c < NPL; ++c) v[c][j] = rec[c * plane + o];
+                    const float* src = PASS_A ? child : up;
+                    if (src) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) ch[c][j] = src[c * plane + o];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            V4<float> fa, fb;
+            V2<float> fc;
+            if (j == 0 ? ok0 : ok1) {
+                float vx, vy, vz;
+                if (PASS_A) {
+                    vx = v[9][j];
+                    vy = v[10][j];
+                    vz = v[11][j];
+                    if (child) {
+                        vx = fmaf(v[12][j], ch[0][j], vx);
+                        vy = fmaf(v[13][j], ch[1][j], vy);
+                        vz = fmaf(v[14][j], ch[2][j], vz);
+                    }
+                } else {
+                    vx = ch[0][j];
+                    vy = ch[1][j];
+                    vz = ch[2][j];
+                }
+                fa = V4<float>{v[0][j] * a.kp, v[1][j] * a.kp, v[2][j] * a.kp,
+                               lum709(v3<float>{v[6][j], v[7][j], v[8][j]}) * a.kl};
+                fb = V4<float>{v[3][j], v[4][j], v[5][j], vx};
+                fc = V2<float>{vy, vz};
+            } else {
+                fa = V4<float>{1e18f, 1e18f, 1e18f, 0.0f};
+                fb = V4<float>{0.0f, 0.0f, 0.0f, 0.0f};
+                fc = V2<float>{0.0f, 0.0f};
+            }
+            shA[r * WS + 2 * cp + j] = fa;
+            shB[r * WS + 2 * cp + j] = fb;
+            shC[r * WS + 2 * cp + j] = fc;
+        }
+    }
+}
+
+// Pass A.  Lanes own R vertically adjacent outputs; the middle rows of the window feed every
+// output without a branch, the edge rows are unrolled with compile-time output ranges.
 template <int RAD, int NW, int R>
 __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
-    constexpr int TW = 64, TH = NW * R, WT = TW + 2 * RAD, HT = TH + 2 * RAD, NT = 64 * NW;
-    constexpr int K = 2 * RAD + 1;
-    __shared__ float sh[10][HT][WT];
+    constexpr int TW = 64, TH = NW * R, HT = TH + 2 * RAD, K = 2 * RAD + 1;
+    constexpr int OFF = RAD & 1, WS = TW + 2 * RAD + 2 * OFF;
+    __shared__ V4<float> shA[HT * WS];
+    __shared__ V4<float> shB[HT * WS];
+    __shared__ V2<float> shC[HT * WS];
     __shared__ float red[NW][ADJ_NP];
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
@@ -137,126 +227,108 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
     const int tx0 = (int)(tile % a.tiles_x) * TW, ty0 = (int)(tile / a.tiles_x) * TH;
     const int W = (int)a.W, H = (int)a.H;
     const size_t plane = a.plane;
-    const float* rec = a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
-    const float* child = a.has_child ? a.level_out + ((size_t)s * a.L + a.d + 1) * 3 * plane : nullptr;
-
-    for (int i = threadIdx.x; i < WT * HT; i += NT) {
-        const int r = i / WT, c = i - r * WT;
-        const int gx = tx0 - RAD + c, gy = ty0 - RAD + r;
-        float f[10];
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const size_t o = (size_t)gy * W + gx;
-            float ir = rec[9 * plane + o], ig = rec[10 * plane + o], ib = rec[11 * plane + o];
-            if (child) {
-                ir = fmaf(rec[12 * plane + o], child[o], ir);
-                ig = fmaf(rec[13 * plane + o], child[plane + o], ig);
-                ib = fmaf(rec[14 * plane + o], child[2 * plane + o], ib);
-            }
-            f[0] = rec[o] * a.kp;
-            f[1] = rec[plane + o] * a.kp;
-            f[2] = rec[2 * plane + o] * a.kp;
-            f[3] = rec[3 * plane + o];
-            f[4] = rec[4 * plane + o];
-            f[5] = rec[5 * plane + o];
-            f[6] = lum709(v3<float>{rec[6 * plane + o], rec[7 * plane + o], rec[8 * plane + o]}) * a.kl;
-            f[7] = ir;
-            f[8] = ig;
-            f[9] = ib;
-        } else {
-            f[0] = f[1] = f[2] = 1e18f;
-            f[3] = f[4] = f[5] = f[6] = f[7] = f[8] = f[9] = 0.0f;
-        }
-#pragma unroll
-        for (int q = 0; q < 10; ++q) sh[q][r][c] = f[q];
-    }
+    bwd_stage<RAD, NW, R, true>(a, s, tx0, ty0, shA, shB, shC);
     __syncthreads();
 
     const int lc = threadIdx.x & 63;
     const int r0 = (threadIdx.x >> 6) * R;
+    const float kn = a.kn;
     float cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R];
     float Wt[R], Sx[R], Sy[R], Sz[R], cW[R], cX[R], cY[R], cZ[R];
+    float rw[R], rx[R], ry[R], rz[R];
     float Am[3][R], Bm[3][3][R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int rr = r0 + k + RAD, cc = lc + RAD;
-        cpx[k] = sh[0][rr][cc];
-        cpy[k] = sh[1][rr][cc];
-        cpz[k] = sh[2][rr][cc];
-        cnx[k] = sh[3][rr][cc];
-        cny[k] = sh[4][rr][cc];
-        cnz[k] = sh[5][rr][cc];
-        cl[k] = sh[6][rr][cc];
+        const V4<float> ca = shA[(r0 + k + RAD) * WS + lc + RAD + OFF];
+        const V4<float> cb = shB[(r0 + k + RAD) * WS + lc + RAD + OFF];
+        cpx[k] = ca.x;
+        cpy[k] = ca.y;
+        cpz[k] = ca.z;
+        cl[k] = ca.w;
+        cnx[k] = kn * cb.x;
+        cny[k] = kn * cb.y;
+        cnz[k] = kn * cb.z;
         Wt[k] = Sx[k] = Sy[k] = Sz[k] = cW[k] = cX[k] = cY[k] = cZ[k] = 0.0f;
+        rw[k] = rx[k] = ry[k] = rz[k] = 0.0f;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             Am[q][k] = 0.0f;
             Bm[q][0][k] = Bm[q][1][k] = Bm[q][2][k] = 0.0f;
         }
     }
-    const float kn = a.kn;
-    for (int r = 0; r < R + 2 * RAD; ++r) {
-        float rw[R], rx[R], ry[R], rz[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) rw[k] = rx[k] = ry[k] = rz[k] = 0.0f;
-        const int lr = r0 + r;
+    auto row = [&](int r, int klo, int khi) __attribute__((always_inline)) {
+        const V4<float>* pa = shA + (r0 + r) * WS + lc + OFF;
+        const V4<float>* pb = shB + (r0 + r) * WS + lc + OFF;
+        const V2<float>* pc = shC + (r0 + r) * WS + lc + OFF;
 #pragma unroll
         for (int dx = 0; dx < K; ++dx) {
-            const int lcol = lc + dx;
-            const float npx = sh[0][lr][lcol], npy = sh[1][lr][lcol], npz = sh[2][lr][lcol];
-            const float nnx = sh[3][lr][lcol], nny = sh[4][lr][lcol], nnz = sh[5][lr][lcol];
-            const float nl = sh[6][lr][lcol];
-            const float nir = sh[7][lr][lcol], nig = sh[8][lr][lcol], nib = sh[9][lr][lcol];
+            const V4<float> na = pa[dx];
+            const V4<float> nb = pb[dx];
+            const V2<float> nc = pc[dx];
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                if (r - k >= 0 && r - k < K) {
-                    const float ex = cpx[k] - npx, ey = cpy[k] - npy, ez = cpz[k] - npz;
+                if (k >= klo && k <= khi) {
+                    const float ex = cpx[k] - na.x, ey = cpy[k] - na.y, ez = cpz[k] - na.z;
                     float ep = ex * ex;
                     ep = fmaf(ey, ey, ep);
                     ep = fmaf(ez, ez, ep);
-                    float nn = cnx[k] * nnx;
-                    nn = fmaf(cny[k], nny, nn);
-                    nn = fmaf(cnz[k], nnz, nn);
-                    const float tt = fmaf(-kn, nn, kn);
+                    const float tt = fmaf(-cnz[k], nb.z, fmaf(-cny[k], nb.y, fmaf(-cnx[k], nb.x, kn)));
                     const float en = tt * tt;
-                    const float dl = cl[k] - nl;
+                    const float dl = cl[k] - na.w;
                     const float el = dl * dl;
                     const float w = exp2_neg(ep + en + el);
                     rw[k] += w;
-                    rx[k] = fmaf(w, nir, rx[k]);
-                    ry[k] = fmaf(w, nig, ry[k]);
-                    rz[k] = fmaf(w, nib, rz[k]);
+                    rx[k] = fmaf(w, nb.w, rx[k]);
+                    ry[k] = fmaf(w, nc.x, ry[k]);
+                    rz[k] = fmaf(w, nc.y, rz[k]);
                     const float wp = w * ep, wn = w * en, wl = w * el;
                     Am[0][k] += wp;
                     Am[1][k] += wn;
                     Am[2][k] += wl;
-                    Bm[0][0][k] = fmaf(wp, nir, Bm[0][0][k]);
-                    Bm[0][1][k] = fmaf(wp, nig, Bm[0][1][k]);
-                    Bm[0][2][k] = fmaf(wp, nib, Bm[0][2][k]);
-                    Bm[1][0][k] = fmaf(wn, nir, Bm[1][0][k]);
-                    Bm[1][1][k] = fmaf(wn, nig, Bm[1][1][k]);
-                    Bm[1][2][k] = fmaf(wn, nib, Bm[1][2][k]);
-                    Bm[2][0][k] = fmaf(wl, nir, Bm[2][0][k]);
-                    Bm[2][1][k] = fmaf(wl, nig, Bm[2][1][k]);
-                    Bm[2][2][k] = fmaf(wl, nib, Bm[2][2][k]);
+                    Bm[0][0][k] = fmaf(wp, nb.w, Bm[0][0][k]);
+                    Bm[0][1][k] = fmaf(wp, nc.x, Bm[0][1][k]);
+                    Bm[0][2][k] = fmaf(wp, nc.y, Bm[0][2][k]);
+                    Bm[1][0][k] = fmaf(wn, nb.w, Bm[1][0][k]);
+                    Bm[1][1][k] = fmaf(wn, nc.x, Bm[1][1][k]);
+                    Bm[1][2][k] = fmaf(wn, nc.y, Bm[1][2][k]);
+                    Bm[2][0][k] = fmaf(wl, nb.w, Bm[2][0][k]);
+                    Bm[2][1][k] = fmaf(wl, nc.x, Bm[2][1][k]);
+                    Bm[2][2][k] = fmaf(wl, nc.y, Bm[2][2][k]);
                 }
             }
         }
+    };
+    auto flush = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            if (r - k >= 0 && r - k < K) {
-                kahan_add(Wt[k], cW[k], rw[k]);
-                kahan_add(Sx[k], cX[k], rx[k]);
-                kahan_add(Sy[k], cY[k], ry[k]);
-                kahan_add(Sz[k], cZ[k], rz[k]);
-            }
+            kahan_add(Wt[k], cW[k], rw[k]);
+            kahan_add(Sx[k], cX[k], rx[k]);
+            kahan_add(Sy[k], cY[k], ry[k]);
+            kahan_add(Sz[k], cZ[k], rz[k]);
+            rw[k] = rx[k] = ry[k] = rz[k] = 0.0f;
         }
+    };
+#pragma unroll
+    for (int r = 0; r < R - 1; ++r) row(r, 0, r);
+    if (R > 1) flush();
+    for (int r = R - 1; r + 1 <= 2 * RAD; r += 2) {
+        row(r, 0, R - 1);
+        row(r + 1, 0, R - 1);
+        flush();
     }
+    if ((2 * RAD + 2 - R) % 2) {
+        row(2 * RAD, 0, R - 1);
+        flush();
+    }
+#pragma unroll
+    for (int i = 0; i < R - 1; ++i) row(2 * RAD + 1 + i, i + 1, R - 1);
+    if (R > 1) flush();
 
     float pg[ADJ_NP];
 #pragma unroll
     for (int q = 0; q < ADJ_NP; ++q) pg[q] = 0.0f;
     const float* gin = a.g_in + (size_t)s * 3 * plane;
-    float* up = a.u + (size_t)s * 3 * plane;
+    float* upw = a.u + (size_t)s * 3 * plane;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int ox = tx0 + lc, oy = ty0 + r0 + k;
@@ -276,9 +348,9 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
                 for (int c = 0; c < 3; ++c) acc = fmaf(hv[c], Bm[q][c][k] - Fv[c] * Am[q][k], acc);
                 pg[q] += acc * inv * (2.0f * 0.69314718056f / a.sig[q]);
             }
-            up[o] = hh.x * inv;
-            up[plane + o] = hh.y * inv;
-            up[2 * plane + o] = hh.z * inv;
+            upw[o] = hh.x * inv;
+            upw[plane + o] = hh.y * inv;
+            upw[2 * plane + o] = hh.z * inv;
         }
     }
     // deterministic block reduction: wave butterflies, then waves in order
@@ -297,91 +369,94 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
     }
 }
 
-// g_{d+1, j} = thr_j * sum_i w_ij u_i
+// Pass B: g_{d+1, j} = thr_j * sum_i w_ij u_i (the weights are symmetric and so are the windows)
 template <int RAD, int NW, int R>
-__global__ void __launch_bounds__(64 * NW) screen_bwd_b_kernel(BwdArgs a) {
-    constexpr int TW = 64, TH = NW * R, WT = TW + 2 * RAD, HT = TH + 2 * RAD, NT = 64 * NW;
-    constexpr int K = 2 * RAD + 1;
-    __shared__ float sh[10][HT][WT];
+__global__ void __launch_bounds__(64 * NW, 2) screen_bwd_b_kernel(BwdArgs a) {
+    constexpr int TW = 64, TH = NW * R, HT = TH + 2 * RAD, K = 2 * RAD + 1;
+    constexpr int OFF = RAD & 1, WS = TW + 2 * RAD + 2 * OFF;
+    __shared__ V4<float> shA[HT * WS];
+    __shared__ V4<float> shB[HT * WS];
+    __shared__ V2<float> shC[HT * WS];
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
     const uint32_t s = t / tiles, tile = t - s * tiles;
     const int tx0 = (int)(tile % a.tiles_x) * TW, ty0 = (int)(tile / a.tiles_x) * TH;
     const int W = (int)a.W, H = (int)a.H;
     const size_t plane = a.plane;
-    const float* rec = a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
-    const float* up = a.u + (size_t)s * 3 * plane;
-    for (int i = threadIdx.x; i < WT * HT; i += NT) {
-        const int r = i / WT, c = i - r * WT;
-        const int gx = tx0 - RAD + c, gy = ty0 - RAD + r;
-        float f[10];
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const size_t o = (size_t)gy * W + gx;
-            f[0] = rec[o] * a.kp;
-            f[1] = rec[plane + o] * a.kp;
-            f[2] = rec[2 * plane + o] * a.kp;
-            f[3] = rec[3 * plane + o];
-            f[4] = rec[4 * plane + o];
-            f[5] = rec[5 * plane + o];
-            f[6] = lum709(v3<float>{rec[6 * plane + o], rec[7 * plane + o], rec[8 * plane + o]}) * a.kl;
-            f[7] = up[o];
-            f[8] = up[plane + o];
-            f[9] = up[2 * plane + o];
-        } else {
-            f[0] = f[1] = f[2] = 1e18f;
-            f[3] = f[4] = f[5] = f[6] = f[7] = f[8] = f[9] = 0.0f;
-        }
-#pragma unroll
-        for (int q = 0; q < 10; ++q) sh[q][r][c] = f[q];
-    }
+    bwd_stage<RAD, NW, R, false>(a, s, tx0, ty0, shA, shB, shC);
     __syncthreads();
     const int lc = threadIdx.x & 63;
     const int r0 = (threadIdx.x >> 6) * R;
-    float cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R], Sx[R], Sy[R], Sz[R];
+    const float kn = a.kn;
+    float cpx[R], cpy[R], cpz[R], cnx[R], cny[R], cnz[R], cl[R];
+    float Sx[R], Sy[R], Sz[R], cX[R], cY[R], cZ[R], rx[R], ry[R], rz[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int rr = r0 + k + RAD, cc = lc + RAD;
-        cpx[k] = sh[0][rr][cc];
-        cpy[k] = sh[1][rr][cc];
-        cpz[k] = sh[2][rr][cc];
-        cnx[k] = sh[3][rr][cc];
-        cny[k] = sh[4][rr][cc];
-        cnz[k] = sh[5][rr][cc];
-        cl[k] = sh[6][rr][cc];
-        Sx[k] = Sy[k] = Sz[k] = 0.0f;
+        const V4<float> ca = shA[(r0 + k + RAD) * WS + lc + RAD + OFF];
+        const V4<float> cb = shB[(r0 + k + RAD) * WS + lc + RAD + OFF];
+        cpx[k] = ca.x;
+        cpy[k] = ca.y;
+        cpz[k] = ca.z;
+        cl[k] = ca.w;
+        cnx[k] = kn * cb.x;
+        cny[k] = kn * cb.y;
+        cnz[k] = kn * cb.z;
+        Sx[k] = Sy[k] = Sz[k] = cX[k] = cY[k] = cZ[k] = rx[k] = ry[k] = rz[k] = 0.0f;
     }
-    const float kn = a.kn;
-    for (int r = 0; r < R + 2 * RAD; ++r) {
-        const int lr = r0 + r;
+    auto row = [&](int r, int klo, int khi) __attribute__((always_inline)) {
+        const V4<float>* pa = shA + (r0 + r) * WS + lc + OFF;
+        const V4<float>* pb = shB + (r0 + r) * WS + lc + OFF;
+        const V2<float>* pc = shC + (r0 + r) * WS + lc + OFF;
 #pragma unroll
         for (int dx = 0; dx < K; ++dx) {
-            const int lcol = lc + dx;
-            const float npx = sh[0][lr][lcol], npy = sh[1][lr][lcol], npz = sh[2][lr][lcol];
-            const float nnx = sh[3][lr][lcol], nny = sh[4][lr][lcol], nnz = sh[5][lr][lcol];
-            const float nl = sh[6][lr][lcol];
-            const float ux = sh[7][lr][lcol], uy = sh[8][lr][lcol], uz = sh[9][lr][lcol];
+            const V4<float> na = pa[dx];
+            const V4<float> nb = pb[dx];
+            const V2<float> nc = pc[dx];
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                if (r - k >= 0 && r - k < K) {
-                    const float ex = cpx[k] - npx, ey = cpy[k] - npy, ez = cpz[k] - npz;
+                if (k >= klo && k <= khi) {
+                    const float ex = cpx[k] - na.x, ey = cpy[k] - na.y, ez = cpz[k] - na.z;
                     float e = ex * ex;
                     e = fmaf(ey, ey, e);
                     e = fmaf(ez, ez, e);
-                    float nn = cnx[k] * nnx;
-                    nn = fmaf(cny[k], nny, nn);
-                    nn = fmaf(cnz[k], nnz, nn);
-                    const float tt = fmaf(-kn, nn, kn);
+                    const float tt = fmaf(-cnz[k], nb.z, fmaf(-cny[k], nb.y, fmaf(-cnx[k], nb.x, kn)));
                     e = fmaf(tt, tt, e);
-                    const float dl = cl[k] - nl;
+                    const float dl = cl[k] - na.w;
                     e = fmaf(dl, dl, e);
                     const float w = exp2_neg(e);
-                    Sx[k] = fmaf(w, ux, Sx[k]);
-                    Sy[k] = fmaf(w, uy, Sy[k]);
-                    Sz[k] = fmaf(w, uz, Sz[k]);
+                    rx[k] = fmaf(w, nb.w, rx[k]);
+                    ry[k] = fmaf(w, nc.x, ry[k]);
+                    rz[k] = fmaf(w, nc.y, rz[k]);
                 }
             }
         }
+    };
+    auto flush = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            kahan_add(Sx[k], cX[k], rx[k]);
+            kahan_add(Sy[k], cY[k], ry[k]);
+            kahan_add(Sz[k], cZ[k], rz[k]);
+            rx[k] = ry[k] = rz[k] = 0.0f;
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < R - 1; ++r) row(r, 0, r);
+    if (R > 1) flush();
+    for (int r = R - 1; r + 1 <= 2 * RAD; r += 2) {
+        row(r, 0, R - 1);
+        row(r + 1, 0, R - 1);
+        flush();
     }
+    if ((2 * RAD + 2 - R) % 2) {
+        row(2 * RAD, 0, R - 1);
+        flush();
+    }
+#pragma unroll
+    for (int i = 0; i < R - 1; ++i) row(2 * RAD + 1 + i, i + 1, R - 1);
+    if (R > 1) flush();
+
+    const float* rec = a.rec + (size_t)((size_t)s * a.L + a.d) * NLPF_SCREEN_PLANES * plane;
     float* gout = a.g_out + (size_t)s * 3 * plane;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -403,13 +478,20 @@ __global__ void bwd_seed_kernel(const float* dfilm, float* g0, uint32_t S, uint6
     for (int c = 0; c < 3; ++c) g0[(s * 3 + c) * plane + px] = dfilm[3 * px + c] * inv;
 }
 
-// fixed-order Kahan sum of the per-workgroup partials
+// Fixed-order sum of the per-workgroup partials: one workgroup per parameter, thread t
+// Kahan-sums partials t, t+256, ... in order, then a fixed pairwise tree over the threads.
 __global__ void bwd_reduce_kernel(const float* partial, uint32_t nblocks, float* out) {
-    const int q = threadIdx.x;
-    if (q >= ADJ_NP) return;
+    __shared__ float sh[256];
+    const int q = blockIdx.x;
     float s = 0.0f, c = 0.0f;
-    for (uint32_t b = 0; b < nblocks; ++b) kahan_add(s, c, partial[(size_t)b * ADJ_NP + q]);
-    out[q] = s;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) kahan_add(s, c, partial[(size_t)b * ADJ_NP + q]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sh[0];
 }
 
 template <int RAD>
@@ -527,7 +609,8 @@ extern "C" int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc
         if (d + 1 < L) NLPF_HIP(h, screen_bwd_level(a, desc->radius, true, st));
         std::swap(gin, gout);
     }
-    hipLaunchKernelGGL(bwd_reduce_kernel, dim3(1), dim3(64), 0, st, partial, nb, grad_params);
+    // the caller's array holds 3 + 16 * (number of ramp nodes) floats
+    hipLaunchKernelGGL(bwd_reduce_kernel, dim3(3 + 16 * nr), dim3(256), 0, st, partial, nb, grad_params);
     if ((rc = check_launch(h, "backward reduce"))) return rc;
     rc = record_event(h, 1);
     if (rc) return rc;
