@@ -39,6 +39,7 @@ __device__ __forceinline__ int phys_feature(int s, int g, int j) {
 // g = (P >> 2) & 3 of the MFMA B operand, as its slot u = 8 (P / 32) + j (j as in
 // phys_feature); the columns are assigned so every lane group runs the same code:
 //   slots u < 18    sin (u even) / cos (u odd) of 2^k pi p_c, k = 3 g + u / 6, c = (u % 6) / 2
+//                   (octave 3 g from the hardware sine / cosine, the next two by doubling)
 //   slots 18..23    g = 0: clamped p, n;  g = 1: albedo, view;  g = 2, 3: zero padding
 struct VertexIn {
     float p[3], n[3], a[3], v[3];
@@ -57,13 +58,23 @@ __device__ __forceinline__ void octave_sincos(float x, int k, float& sn, float& 
     cs = __builtin_amdgcn_cosf(fr);
 }
 
-// feature of column P (reference path: the encode-only and FP64 reference kernels)
+// the next octave from the current one: sin 2x = (2 sin x) cos x, cos 2x = 1 - (2 sin x) sin x
+__device__ __forceinline__ void octave_double(float& sn, float& cs) {
+    const float t = 2.0f * sn;
+    const float s2 = t * cs;
+    cs = fmaf(-t, sn, 1.0f);
+    sn = s2;
+}
+
+// feature of column P (reference path: the encode-only and FP64 reference kernels); octave
+// 3 g + i comes from octave 3 g and i doublings, exactly as encode_frags computes it
 __device__ inline float mlp_feature(const VertexIn& x, int P) {
     const int g = col_group(P), u = col_slot(P);
     if (u < 18) {
         const int c = (u % 6) >> 1;
         float sn, cs;
-        octave_sincos(c == 0 ? x.p[0] : c == 1 ? x.p[1] : x.p[2], 3 * g + u / 6, sn, cs);
+        octave_sincos(c == 0 ? x.p[0] : c == 1 ? x.p[1] : x.p[2], 3 * g, sn, cs);
+        for (int i = 0; i < u / 6; ++i) octave_double(sn, cs);
         return (u & 1) ? cs : sn;
     }
     const int r = u - 18;
@@ -134,19 +145,23 @@ __device__ __forceinline__ bf16x8 load_wfrag(const __bf16* w, int in_dim, int m0
 // -0 included) are negative int16, non-negative ones keep their order, and rounding commutes
 // with max(., 0).
 typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// two floats -> two bf16 (round to nearest even) in one v_cvt_pk_bf16_f32
+__device__ __forceinline__ bf16x2 cvt2(float x, float y) { return __builtin_convertvector((f32x2){x, y}, bf16x2); }
+__device__ __forceinline__ bf16x8 pack8(bf16x2 a, bf16x2 b, bf16x2 c, bf16x2 d) {
+    return __builtin_shufflevector(__builtin_shufflevector(a, b, 0, 1, 2, 3), __builtin_shufflevector(c, d, 0, 1, 2, 3),
+                                   0, 1, 2, 3, 4, 5, 6, 7);
+}
+__device__ __forceinline__ bf16x2 relu2(bf16x2 v) {
+    i16x2 t;
+    __builtin_memcpy(&t, &v, sizeof(t));
+    t = __builtin_elementwise_max(t, (i16x2){0, 0});
+    __builtin_memcpy(&v, &t, sizeof(t));
+    return v;
+}
 __device__ __forceinline__ bf16x8 relu_pack(const f32x4& a, const f32x4& b) {
-    bf16x8 r;
-    for (int i = 0; i < 4; ++i) {
-        r[i] = (__bf16)a[i];
-        r[4 + i] = (__bf16)b[i];
-    }
-    i16x2 v[4];
-    __builtin_memcpy(v, &r, sizeof(r));
-    const i16x2 zero = {0, 0};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = __builtin_elementwise_max(v[i], zero);
-    __builtin_memcpy(&r, v, sizeof(r));
-    return r;
+    return pack8(relu2(cvt2(a[0], a[1])), relu2(cvt2(a[2], a[3])), relu2(cvt2(b[0], b[1])), relu2(cvt2(b[2], b[3])));
 }
 
 struct MlpParamsDev {
@@ -227,9 +242,16 @@ __device__ __forceinline__ void encode_frags(const VertexIn& v, int lane, bf16x8
     const int g = lane >> 4;
     float u[24];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) {
+        float sn, cs;
+        octave_sincos(v.p[c], 3 * g, sn, cs);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) octave_sincos(v.p[c], 3 * g + i, u[6 * i + 2 * c], u[6 * i + 2 * c + 1]);
+        for (int i = 0; i < 3; ++i) {
+            if (i) octave_double(sn, cs);
+            u[6 * i + 2 * c] = sn;
+            u[6 * i + 2 * c + 1] = cs;
+        }
+    }
     const bool g0 = g == 0, g1 = g == 1;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -238,8 +260,8 @@ __device__ __forceinline__ void encode_frags(const VertexIn& v, int lane, bf16x8
     }
 #pragma unroll
     for (int s = 0; s < 3; ++s)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) x[s][j] = (__bf16)u[8 * s + j];
+        x[s] = pack8(cvt2(u[8 * s], u[8 * s + 1]), cvt2(u[8 * s + 2], u[8 * s + 3]), cvt2(u[8 * s + 4], u[8 * s + 5]),
+                     cvt2(u[8 * s + 6], u[8 * s + 7]));
 }
 
 struct MlpInferArgs {
@@ -254,7 +276,24 @@ struct MlpInferArgs {
     float* out;           // [n][3], or planar [slice][out_levels][3][W*H] (slot out_slot) when planar != 0
     int planar;
     uint32_t out_levels, out_slot;
+    uint64_t sample0;     // contiguous samples: sample0 .. sample0 + n - 1
 };
+
+// all weight fragments and biases into LDS (once per workgroup)
+__device__ __forceinline__ void stage_weights(const MlpParamsDev& P, bf16x8 (*wl)[64], f32x4* bl) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int f = wv; f < MLP_NFRAG; f += 4) {
+        const int l = f < 12 ? 0 : (f < 44 ? 1 + (f - 12) / 8 : MLP_LAYERS - 1);
+        const int r = f < 12 ? f : (f < 44 ? (f - 12) % 8 : f - 44);
+        const int ks = l == 0 ? 3 : 2;
+        wl[f][lane] = load_wfrag(P.w[l], mlp_in(l), 16 * (r / ks), r % ks, lane);
+    }
+    for (int i = threadIdx.x; i < MLP_NBIAS; i += blockDim.x) {
+        const int l = i / MLP_HID;
+        ((float*)bl)[i] = P.b[l][i - MLP_HID * l];
+    }
+    __syncthreads();
+}
 
 // Persistent inference: the workgroup stages all weight fragments and biases in LDS once
 // (47 KB), then its 4 waves loop over tiles of 16*MLP_QT samples.
@@ -262,17 +301,7 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
     __shared__ bf16x8 wl[MLP_NFRAG][64];
     __shared__ f32x4 bl[MLP_NBIAS / 4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4;
-    for (int f = wv; f < MLP_NFRAG; f += 4) {
-        const int l = f < 12 ? 0 : (f < 44 ? 1 + (f - 12) / 8 : MLP_LAYERS - 1);
-        const int r = f < 12 ? f : (f < 44 ? (f - 12) % 8 : f - 44);
-        const int ks = l == 0 ? 3 : 2;
-        wl[f][lane] = load_wfrag(a.P.w[l], mlp_in(l), 16 * (r / ks), r % ks, lane);
-    }
-    for (int i = threadIdx.x; i < MLP_NBIAS; i += blockDim.x) {
-        const int l = i / MLP_HID;
-        ((float*)bl)[i] = a.P.b[l][i - MLP_HID * l];
-    }
-    __syncthreads();
+    stage_weights(a.P, wl, bl);
     const auto wf = [&](int l, int t, int s) { return wl[frag_index(l, t, s)][lane]; };
     const auto bf = [&](int l, int t) { return bl[(bias_index(l) + 16 * t + 4 * g) >> 2]; };
 
@@ -290,8 +319,8 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
         // keeps the (loop-invariant) LDS weight reads inside the loop: hoisted, the 46
         // fragments would hold 184 VGPRs for the whole loop
         asm volatile("" ::: "memory");
-        const uint64_t s_base = contiguous ? base / plane : 0;
-        const uint32_t px_base = contiguous ? (uint32_t)(base - s_base * plane) : 0;
+        const uint64_t s_base = contiguous ? (a.sample0 + base) / plane : 0;
+        const uint32_t px_base = contiguous ? (uint32_t)(a.sample0 + base - s_base * plane) : 0;
         uint32_t s_q[MLP_QT], px_q[MLP_QT];
         bf16x8 xin[MLP_QT][3];
 #pragma unroll
@@ -303,10 +332,9 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
                         for (int j = 0; j < 8; ++j) xin[q][s][j] = (__bf16)a.feats[i * MLP_IN + phys_feature(s, g, j)];
                 } else if (contiguous) {
                     uint32_t px = px_base + 16 * q + (lane & 15), sl = (uint32_t)s_base;
-                    if (px >= (uint32_t)plane) {
-                        px -= (uint32_t)plane;
-                        ++sl;
-                    }
+                    const bool wrap = px >= (uint32_t)plane;
+                    px = wrap ? px - (uint32_t)plane : px;
+                    sl = wrap ? sl + 1 : sl;
                     int y = (int)((float)px * inv_w);
                     int x = (int)px - y * (int)a.W;
                     if (x < 0) {
@@ -323,7 +351,7 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
                                : load_vertex_at((const float*)a.rec, nullptr, a.W, a.H, a.L, a.d, sl, px, x, y);
                     encode_frags(v, lane, xin[q]);
                 } else {
-                    const uint64_t sm = a.idx ? a.idx[i] : i;
+                    const uint64_t sm = a.idx ? a.idx[i] : a.sample0 + i;
                     const VertexIn v = a.fp16 ? load_vertex((const __half*)a.rec, a.anchors, a.W, a.H, a.L, a.d, sm)
                                               : load_vertex((const float*)a.rec, nullptr, a.W, a.H, a.L, a.d, sm);
                     encode_frags(v, lane, xin[q]);
@@ -346,7 +374,7 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
                             sl = s_q[q];
                             px = px_q[q];
                         } else {
-                            const uint64_t sm = a.idx ? a.idx[i] : i;
+                            const uint64_t sm = a.idx ? a.idx[i] : a.sample0 + i;
                             sl = sm / plane;
                             px = sm - sl * plane;
                         }
@@ -820,7 +848,17 @@ static MlpInferArgs infer_args(nlpf_mlp_state* s, const nlpf_screen_desc* desc, 
     a.feats = feats;
     a.n = n;
     a.out = out;
+    a.sample0 = 0;
     return a;
+}
+
+// persistent grid of at most 3 workgroups per CU (LDS-bound)
+static void launch_infer(nlpf_handle* h, const MlpInferArgs& a) {
+    const uint64_t plane = (uint64_t)a.W * a.H;
+    const uint64_t per_block = 4 * 16 * MLP_QT;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((a.n + per_block - 1) / per_block, 768);
+    (void)plane;
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, h->stream, a);
 }
 
 static int check_rec_args(nlpf_handle* h, const nlpf_screen_desc* desc, const void* rec, const float* feats,
@@ -928,11 +966,9 @@ int nlpf_mlp_infer(nlpf_handle* h, const nlpf_screen_desc* desc, const void* rec
     if (rc) return rc;
     if (n == 0) return NLPF_OK;
     MlpInferArgs a = infer_args(s, desc, records, anchors, depth, idx, feats, n, out_rgb);
-    const uint64_t per_block = 4 * 16 * MLP_QT;
-    const uint64_t blocks = std::min<uint64_t>((n + per_block - 1) / per_block, 768);  // 3 per CU (LDS), persistent
     rc = record_event(h, 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3((uint32_t)blocks), dim3(256), 0, h->stream, a);
+    launch_infer(h, a);
     NLPF_HIP(h, hipGetLastError());
     rc = record_event(h, 1);
     if (rc) return rc;
@@ -1070,20 +1106,21 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
 
 namespace nlpf {
 // internal entry points used by the learned-cache NL-PF pass (mlp_run.hip)
-// estimates for samples 0..n-1 written into slot `out_slot` of planar level data
+// estimates for samples sample0..sample0+n-1 written into slot `out_slot` of planar level data
 // [slice][out_levels][3][W*H]
 int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
-                     uint32_t depth, uint64_t n, float* planar_out, uint32_t out_levels, uint32_t out_slot) {
+                     uint32_t depth, uint64_t sample0, uint64_t n, float* planar_out, uint32_t out_levels,
+                     uint32_t out_slot) {
     nlpf_mlp_state* s;
     int rc = mlp_need(h, &s);
     if (rc) return rc;
+    if (n == 0) return NLPF_OK;
     MlpInferArgs a = infer_args(s, desc, records, anchors, depth, nullptr, nullptr, n, planar_out);
     a.planar = 1;
     a.out_levels = out_levels;
     a.out_slot = out_slot;
-    const uint64_t per_block = 4 * 16 * MLP_QT;
-    const uint64_t blocks = std::min<uint64_t>((n + per_block - 1) / per_block, 768);  // 3 per CU (LDS), persistent
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3((uint32_t)blocks), dim3(256), 0, h->stream, a);
+    a.sample0 = sample0;
+    launch_infer(h, a);
     return check_launch(h, "mlp infer");
 }
 }  // namespace nlpf

@@ -25,7 +25,8 @@ hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant
 hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
                             float* film, hipStream_t st);
 int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
-                     uint32_t depth, uint64_t n, float* planar_out, uint32_t out_levels, uint32_t out_slot);
+                     uint32_t depth, uint64_t sample0, uint64_t n, float* planar_out, uint32_t out_levels,
+                     uint32_t out_slot);
 int mlp_train_enqueue(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                       uint32_t depth, const uint32_t* idx, const float* feats, const float* target, uint32_t batch,
                       float lr, int apply_update, float* grad);
@@ -288,7 +289,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     rc = record_event(h, 2);
     if (rc) return rc;
     // 3. inference at depth k for every slice, then the depth-k stylization
-    rc = mlp_infer_planar(h, desc, records, anchors, k, plane * S, lv, L, k);
+    rc = mlp_infer_planar(h, desc, records, anchors, k, 0, plane * S, lv, L, k);
     if (rc) return rc;
     hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, slot(sj, k), Fj, W, H, tiles_x, nd_fit);
     if ((rc = check_launch(h, "cache fit"))) return rc;

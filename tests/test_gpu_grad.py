@@ -106,3 +106,20 @@ def test_adjoint_matches_fp64_gpu_finite_differences():
             # h = 1e-3 (the step the north-star check names): many vertices straddle kinks at
             # this size, so the agreement is at the 1% level of the largest component
             assert err <= 1e-2 * max(abs(fd[q][0]), 0.05 * scale), (q, fd[q], g[q])
+
+
+def test_gradient_array_bounds_one_ramp():
+    """With one ramp node the adjoint writes exactly 3 + 16 floats: a guard band after them stays
+    untouched (regression: the reduction used to write the two-ramp maximum of 35)."""
+    with N.Handle({"type": "pf", "mode": "screen"}) as h:
+        W, H, S, L, R = 64, 24, 1, 2, 3
+        desc, rec, table, target, film = problem(h, W, H, S, L, R, seed=5)
+        lv = h.alloc(S * L * 3 * W * H * 4)
+        h.screen_run(desc, rec, None, film, lv)
+        f = film.download(np.float32).reshape(H, W, 3).astype(np.float64)
+        dfilm = h.to_device((2 * (f - target)).astype(np.float32))
+        guard = np.full(64, 12345.0, np.float32)
+        buf = h.to_device(np.concatenate([np.zeros(19, np.float32), guard]))
+        h.screen_backward(desc, rec, lv, dfilm, buf)
+        out = buf.download(np.float32, 19 + 64)
+        assert np.isfinite(out[:19]).all() and np.array_equal(out[19:], guard)
