@@ -237,14 +237,12 @@ __device__ __forceinline__ void mlp_forward_wave(const WF& wf, const BF& bf, bf1
     }
 }
 
-// encoded input fragments of the lane's sample for the 3 K-steps (slot u = 8 s + j)
-__device__ __forceinline__ void encode_frags(const VertexIn& v, int lane, bf16x8 (&x)[3]) {
-    const int g = lane >> 4;
-    float u[24];
+// slots 0..17 of lane group g: sin / cos of octaves 3g, 3g+1, 3g+2 of each coordinate
+__device__ __forceinline__ void encode_trig(const float (&p)[3], int g, float (&u)[18]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         float sn, cs;
-        octave_sincos(v.p[c], 3 * g, sn, cs);
+        octave_sincos(p[c], 3 * g, sn, cs);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             if (i) octave_double(sn, cs);
@@ -252,16 +250,38 @@ __device__ __forceinline__ void encode_frags(const VertexIn& v, int lane, bf16x8
             u[6 * i + 2 * c + 1] = cs;
         }
     }
-    const bool g0 = g == 0, g1 = g == 1;
+}
+
+// the raw slots 18..23 as three bf16 pairs: (p0, p1), (p2, n0), (n1, n2) for lane group 0,
+// (a0, a1), (a2, v0), (v1, v2) for lane group 1 (p clamped to [-2, 2])
+__device__ __forceinline__ void raw_pairs(const VertexIn& v, bf16x2 (&r0)[3], bf16x2 (&r1)[3]) {
+    const float c0 = fminf(fmaxf(v.p[0], -2.0f), 2.0f), c1 = fminf(fmaxf(v.p[1], -2.0f), 2.0f),
+                c2 = fminf(fmaxf(v.p[2], -2.0f), 2.0f);
+    r0[0] = cvt2(c0, c1);
+    r0[1] = cvt2(c2, v.n[0]);
+    r0[2] = cvt2(v.n[1], v.n[2]);
+    r1[0] = cvt2(v.a[0], v.a[1]);
+    r1[1] = cvt2(v.a[2], v.v[0]);
+    r1[2] = cvt2(v.v[1], v.v[2]);
+}
+
+__device__ __forceinline__ void assemble_frags(const float (&u)[18], const bf16x2 (&raw)[3], bf16x8 (&x)[3]) {
+    x[0] = pack8(cvt2(u[0], u[1]), cvt2(u[2], u[3]), cvt2(u[4], u[5]), cvt2(u[6], u[7]));
+    x[1] = pack8(cvt2(u[8], u[9]), cvt2(u[10], u[11]), cvt2(u[12], u[13]), cvt2(u[14], u[15]));
+    x[2] = pack8(cvt2(u[16], u[17]), raw[0], raw[1], raw[2]);
+}
+
+// encoded input fragments of the lane's sample for the 3 K-steps (slot u = 8 s + j)
+__device__ __forceinline__ void encode_frags(const VertexIn& v, int lane, bf16x8 (&x)[3]) {
+    const int g = lane >> 4;
+    float u[18];
+    encode_trig(v.p, g, u);
+    bf16x2 r0[3], r1[3], raw[3];
+    raw_pairs(v, r0, r1);
+    const bf16x2 z = cvt2(0.0f, 0.0f);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        u[18 + c] = g0 ? fminf(fmaxf(v.p[c], -2.0f), 2.0f) : (g1 ? v.a[c] : 0.0f);
-        u[21 + c] = g0 ? v.n[c] : (g1 ? v.v[c] : 0.0f);
-    }
-#pragma unroll
-    for (int s = 0; s < 3; ++s)
-        x[s] = pack8(cvt2(u[8 * s], u[8 * s + 1]), cvt2(u[8 * s + 2], u[8 * s + 3]), cvt2(u[8 * s + 4], u[8 * s + 5]),
-                     cvt2(u[8 * s + 6], u[8 * s + 7]));
+    for (int k = 0; k < 3; ++k) raw[k] = g == 0 ? r0[k] : (g == 1 ? r1[k] : z);
+    assemble_frags(u, raw, x);
 }
 
 struct MlpInferArgs {
@@ -309,10 +329,10 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
     const uint64_t nwaves = (uint64_t)gridDim.x * 4;
     const uint64_t per = 16 * MLP_QT;
     const uint64_t plane = (uint64_t)a.W * a.H;
-    // contiguous samples 0..n-1 on an image of >= 64 and < 2^24 pixels: the wave-uniform tile
-    // base is split into (slice, pixel) once, each lane adds its offset (< 64: at most one
-    // slice boundary), and the row comes from an FP32 reciprocal with one correction step
-    // (exact for pixel indices below 2^24)
+    // contiguous samples on an image of >= 64 and < 2^24 pixels: the wave-uniform tile base is
+    // split into (slice, pixel) once, each lane adds its offset (< 64: at most one slice
+    // boundary), and the row comes from an FP32 reciprocal with one correction step (exact for
+    // pixel indices below 2^24)
     const bool contiguous = !a.idx && !a.feats && plane >= 64 && plane < (1u << 24);
     const float inv_w = 1.0f / (float)a.W;
     for (uint64_t base = wave * per; base < a.n; base += nwaves * per) {
@@ -321,44 +341,74 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
         asm volatile("" ::: "memory");
         const uint64_t s_base = contiguous ? (a.sample0 + base) / plane : 0;
         const uint32_t px_base = contiguous ? (uint32_t)(a.sample0 + base - s_base * plane) : 0;
-        uint32_t s_q[MLP_QT], px_q[MLP_QT];
         bf16x8 xin[MLP_QT][3];
+        if (contiguous) {
+            // each lane loads its own sample (base + lane) once; the 16-lane row of tile q then
+            // reads position and raw-feature pairs of its column's sample from row q
+            float pc[3] = {0.0f, 0.0f, 0.0f};
+            bf16x2 r0[3], r1[3];
 #pragma unroll
-        for (int q = 0; q < MLP_QT; ++q) {
-            const uint64_t i = base + 16 * q + (lane & 15);
-            if (i < a.n) {
-                if (a.feats) {
-                    for (int s = 0; s < 3; ++s)
-                        for (int j = 0; j < 8; ++j) xin[q][s][j] = (__bf16)a.feats[i * MLP_IN + phys_feature(s, g, j)];
-                } else if (contiguous) {
-                    uint32_t px = px_base + 16 * q + (lane & 15), sl = (uint32_t)s_base;
-                    const bool wrap = px >= (uint32_t)plane;
-                    px = wrap ? px - (uint32_t)plane : px;
-                    sl = wrap ? sl + 1 : sl;
-                    int y = (int)((float)px * inv_w);
-                    int x = (int)px - y * (int)a.W;
-                    if (x < 0) {
-                        --y;
-                        x += (int)a.W;
-                    } else if (x >= (int)a.W) {
-                        ++y;
-                        x -= (int)a.W;
-                    }
-                    s_q[q] = sl;
-                    px_q[q] = px;
-                    const VertexIn v =
-                        a.fp16 ? load_vertex_at((const __half*)a.rec, a.anchors, a.W, a.H, a.L, a.d, sl, px, x, y)
-                               : load_vertex_at((const float*)a.rec, nullptr, a.W, a.H, a.L, a.d, sl, px, x, y);
-                    encode_frags(v, lane, xin[q]);
-                } else {
-                    const uint64_t sm = a.idx ? a.idx[i] : a.sample0 + i;
-                    const VertexIn v = a.fp16 ? load_vertex((const __half*)a.rec, a.anchors, a.W, a.H, a.L, a.d, sm)
-                                              : load_vertex((const float*)a.rec, nullptr, a.W, a.H, a.L, a.d, sm);
-                    encode_frags(v, lane, xin[q]);
+            for (int k = 0; k < 3; ++k) r0[k] = r1[k] = cvt2(0.0f, 0.0f);
+            if (base + lane < a.n) {
+                uint32_t px = px_base + lane, sl = (uint32_t)s_base;
+                const bool wrap = px >= (uint32_t)plane;
+                px = wrap ? px - (uint32_t)plane : px;
+                sl = wrap ? sl + 1 : sl;
+                int yy = (int)((float)px * inv_w);
+                int xx = (int)px - yy * (int)a.W;
+                const bool lo = xx < 0, hi = xx >= (int)a.W;
+                yy = lo ? yy - 1 : (hi ? yy + 1 : yy);
+                xx = lo ? xx + (int)a.W : (hi ? xx - (int)a.W : xx);
+                const VertexIn v =
+                    a.fp16 ? load_vertex_at((const __half*)a.rec, a.anchors, a.W, a.H, a.L, a.d, sl, px, xx, yy)
+                           : load_vertex_at((const float*)a.rec, nullptr, a.W, a.H, a.L, a.d, sl, px, xx, yy);
+                pc[0] = v.p[0];
+                pc[1] = v.p[1];
+                pc[2] = v.p[2];
+                raw_pairs(v, r0, r1);
+            }
+            uint32_t rb[6];
+            __builtin_memcpy(&rb[0], r0, sizeof(r0));
+            __builtin_memcpy(&rb[3], r1, sizeof(r1));
+#pragma unroll
+            for (int q = 0; q < MLP_QT; ++q) {
+                const int src = 16 * q + (lane & 15);
+                float p[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) p[c] = __shfl(pc[c], src);
+                uint32_t t[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) t[k] = (uint32_t)__shfl((int)rb[k], src);
+                float u[18];
+                encode_trig(p, g, u);
+                bf16x2 raw[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t bits = g == 0 ? t[k] : (g == 1 ? t[3 + k] : 0u);
+                    __builtin_memcpy(&raw[k], &bits, sizeof(bits));
                 }
-            } else {
-                for (int s = 0; s < 3; ++s)
-                    for (int j = 0; j < 8; ++j) xin[q][s][j] = (__bf16)0.0f;
+                assemble_frags(u, raw, xin[q]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < MLP_QT; ++q) {
+                const uint64_t i = base + 16 * q + (lane & 15);
+                if (i < a.n) {
+                    if (a.feats) {
+                        for (int s = 0; s < 3; ++s)
+                            for (int j = 0; j < 8; ++j)
+                                xin[q][s][j] = (__bf16)a.feats[i * MLP_IN + phys_feature(s, g, j)];
+                    } else {
+                        const uint64_t sm = a.idx ? a.idx[i] : a.sample0 + i;
+                        const VertexIn v = a.fp16
+                                               ? load_vertex((const __half*)a.rec, a.anchors, a.W, a.H, a.L, a.d, sm)
+                                               : load_vertex((const float*)a.rec, nullptr, a.W, a.H, a.L, a.d, sm);
+                        encode_frags(v, lane, xin[q]);
+                    }
+                } else {
+                    for (int s = 0; s < 3; ++s)
+                        for (int j = 0; j < 8; ++j) xin[q][s][j] = (__bf16)0.0f;
+                }
             }
         }
         f32x4 y[MLP_QT];
@@ -371,8 +421,10 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
                     if (a.planar) {
                         uint64_t sl, px;
                         if (contiguous) {
-                            sl = s_q[q];
-                            px = px_q[q];
+                            const uint32_t pq = px_base + 16 * q + lane;
+                            const bool wrap = pq >= (uint32_t)plane;
+                            px = wrap ? pq - (uint32_t)plane : pq;
+                            sl = wrap ? s_base + 1 : s_base;
                         } else {
                             const uint64_t sm = a.idx ? a.idx[i] : a.sample0 + i;
                             sl = sm / plane;
