@@ -293,9 +293,9 @@ struct MlpInferArgs {
     const uint32_t* idx;  // sample list (slice * W*H + pixel) or nullptr for 0..n-1
     const float* feats;   // optional raw features [n][96] instead of records
     uint64_t n;
-    float* out;           // [n][3], or planar [slice][out_levels][3][W*H] (slot out_slot) when planar != 0
+    float* out;           // [n][3], or planar [slice - out_slice0][out_levels][3][W*H] (slot out_slot)
     int planar;
-    uint32_t out_levels, out_slot;
+    uint32_t out_levels, out_slot, out_slice0;
     uint64_t sample0;     // contiguous samples: sample0 .. sample0 + n - 1
 };
 
@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(256, 2) mlp_infer_kernel(MlpInferArgs a) {
                             sl = sm / plane;
                             px = sm - sl * plane;
                         }
-                        float* o = a.out + ((uint64_t)sl * a.out_levels + a.out_slot) * 3 * plane + px;
+                        float* o = a.out + ((uint64_t)(sl - a.out_slice0) * a.out_levels + a.out_slot) * 3 * plane + px;
                         o[0] = y[q][0];
                         o[plane] = y[q][1];
                         o[2 * plane] = y[q][2];
@@ -901,6 +901,7 @@ static MlpInferArgs infer_args(nlpf_mlp_state* s, const nlpf_screen_desc* desc, 
     a.n = n;
     a.out = out;
     a.sample0 = 0;
+    a.out_slice0 = 0;
     return a;
 }
 
@@ -1159,10 +1160,10 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
 namespace nlpf {
 // internal entry points used by the learned-cache NL-PF pass (mlp_run.hip)
 // estimates for samples sample0..sample0+n-1 written into slot `out_slot` of planar level data
-// [slice][out_levels][3][W*H]
+// [slice - out_slice0][out_levels][3][W*H] (the samples' slices must be >= out_slice0)
 int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                      uint32_t depth, uint64_t sample0, uint64_t n, float* planar_out, uint32_t out_levels,
-                     uint32_t out_slot) {
+                     uint32_t out_slot, uint32_t out_slice0) {
     nlpf_mlp_state* s;
     int rc = mlp_need(h, &s);
     if (rc) return rc;
@@ -1172,6 +1173,7 @@ int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* r
     a.out_levels = out_levels;
     a.out_slot = out_slot;
     a.sample0 = sample0;
+    a.out_slice0 = out_slice0;
     launch_infer(h, a);
     return check_launch(h, "mlp infer");
 }

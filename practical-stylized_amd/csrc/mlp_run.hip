@@ -3,18 +3,19 @@
 // Per frame (one call):
 //   1. judge slice  levels D..k of one spp slice filtered exactly (plain NL-PF); the unstylized
 //                   depth-k values F_k are the training targets; levels k-1..0 from the exact
-//                   depth-k output give the slice's exact level-0 image E;
+//                   depth-k output give the slice's exact images (kept as its final result);
 //   2. train        `train_records` judge vertices, Adam steps of `batch` on BF16 MFMA;
 //   3. infer        the network estimates F_k for every slice (levels > k are not read),
 //                   stylized at depth k like a filtered value;
-//   4. judge        levels k-1..0 of the judge slice rerun on the estimate give P; per 64x8
-//                   tile, the relative L2 of P against E is the error the cache puts into the
-//                   image;
+//   4. judge        levels k-1..0 of the judge slice rerun on its estimate (in a separate
+//                   image) give P; per 64x8 tile, the relative L2 of P against the exact
+//                   level-0 image E is the error the cache puts into the image;
 //   5. fallback     tiles above the threshold are filtered exactly at depths D..k in every
-//                   slice: a level-0 tile depends on depth-d tiles up to d tiles away (window
-//                   radius <= 7 < tile height 8), so the depth-d mask is the level-0 mask
-//                   dilated d times;
-//   6. levels k-1..0 plain NL-PF on the mixed depth-k output, then the film.
+//                   other slice: a level-0 tile depends on depth-d tiles up to d tiles away
+//                   (window radius <= 7 < tile height 8), so the depth-d mask is the level-0
+//                   mask dilated d times;
+//   6. levels k-1..0 plain NL-PF on the mixed depth-k output of the other slices, then the
+//      film over all slices.
 #include "common.hpp"
 #include "screen_impl.hpp"
 #include "scene.hpp"
@@ -26,7 +27,7 @@ hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, u
                             float* film, hipStream_t st);
 int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                      uint32_t depth, uint64_t sample0, uint64_t n, float* planar_out, uint32_t out_levels,
-                     uint32_t out_slot);
+                     uint32_t out_slot, uint32_t out_slice0);
 int mlp_train_enqueue(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                       uint32_t depth, const uint32_t* idx, const float* feats, const float* target, uint32_t batch,
                       float lr, int apply_update, float* grad);
@@ -163,8 +164,9 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     const size_t idx_bytes = (size_t)run->train_records * sizeof(uint32_t);
     const size_t mask_bytes = (size_t)L * ntiles;
     const size_t nd_bytes = (size_t)ntiles * 2 * sizeof(double);
-    int rc = ensure_scratch(h, al(lv_bytes) + 3 * al(f_bytes) + al(tgt_bytes) + 2 * al(idx_bytes) + al(mask_bytes) +
-                                   2 * al(nd_bytes));
+    const size_t p_bytes = (size_t)(k + 1) * f_bytes;
+    int rc = ensure_scratch(h, al(lv_bytes) + al(f_bytes) + al(p_bytes) + al(tgt_bytes) + 2 * al(idx_bytes) +
+                                   al(mask_bytes) + 2 * al(nd_bytes));
     if (rc) return rc;
     char* base = (char*)h->scratch;
     auto take = [&](size_t b) {
@@ -174,8 +176,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     };
     float* lv = level_out ? level_out : (float*)take(lv_bytes);
     float* Fj = (float*)take(f_bytes);  // judge slice: exact unstylized F_k (training targets)
-    float* Xj = (float*)take(f_bytes);  // judge slice: exact stylized depth-k output
-    float* Ej = (float*)take(f_bytes);  // judge slice: exact level-0 output
+    float* Pj = (float*)take(p_bytes);  // judge slice on its estimate: levels 0..k [k+1][3][W*H]
     float* tgt = (float*)take(tgt_bytes);
     uint32_t* lidx = (uint32_t*)take(idx_bytes);  // pixel ids within the judge slice
     uint32_t* gidx = (uint32_t*)take(idx_bytes);  // sample ids s*W*H + pixel
@@ -224,22 +225,35 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     aj.rec = slice_rec(sj);
     aj.anchors = slice_anc(sj);
     aj.out = slot(sj, 0);
-    nlpf_screen_desc d1 = *desc;
-    d1.spp = 1;
-    // stylize depth k of `n` slices starting at slice s0 (planar level data, records of those slices)
-    auto style_k = [&](uint32_t s0, uint32_t n) -> int {
-        if (root < 0) return NLPF_OK;
+    // stylize slot k of `n` slices of planar level data io ([n][out_levels][3][W*H]) whose
+    // records start at slice s0
+    auto style_k = [&](float* io, uint32_t out_levels, uint32_t s0, uint32_t n) -> int {
+        if (root < 0 || n == 0) return NLPF_OK;
         const uint64_t cnt = plane * n;
-        float* io = slot(s0, 0);
         if (fp16)
             hipLaunchKernelGGL(style_planar_kernel<__half>, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, io,
-                               n, plane, L, k, h->d_nodes, root, k, (const __half*)slice_rec(s0), slice_anc(s0), W, H,
-                               L, need_pos);
+                               n, plane, out_levels, k, h->d_nodes, root, k, (const __half*)slice_rec(s0),
+                               slice_anc(s0), W, H, L, need_pos);
         else
             hipLaunchKernelGGL(style_planar_kernel<float>, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, io,
-                               n, plane, L, k, h->d_nodes, root, k, (const float*)slice_rec(s0), nullptr, W, H, L,
-                               need_pos);
+                               n, plane, out_levels, k, h->d_nodes, root, k, (const float*)slice_rec(s0), nullptr, W,
+                               H, L, need_pos);
         return check_launch(h, "style depth k");
+    };
+    // one level over the slices other than the judge slice (two contiguous ranges)
+    auto level_others = [&](ScreenArgs<float> a, uint32_t d) -> hipError_t {
+        const uint32_t r0[2] = {0, sj + 1}, rn[2] = {sj, S - 1 - sj};
+        for (int r = 0; r < 2; ++r) {
+            if (!rn[r]) continue;
+            ScreenArgs<float> b = a;
+            b.S = rn[r];
+            b.rec = slice_rec(r0[r]);
+            b.anchors = slice_anc(r0[r]);
+            b.out = slot(r0[r], 0);
+            const hipError_t e = level(b, d);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     };
 
     nlpf_mlp_run_stats rs{};
@@ -253,10 +267,8 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
         NLPF_HIP(h, level(af, k));
     }
     NLPF_HIP(h, hipMemcpyAsync(Fj, slot(sj, k), f_bytes, hipMemcpyDeviceToDevice, st));
-    if ((rc = style_k(sj, 1))) return rc;
-    NLPF_HIP(h, hipMemcpyAsync(Xj, slot(sj, k), f_bytes, hipMemcpyDeviceToDevice, st));
+    if ((rc = style_k(slot(sj, 0), L, sj, 1))) return rc;
     for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
-    NLPF_HIP(h, hipMemcpyAsync(Ej, slot(sj, 0), f_bytes, hipMemcpyDeviceToDevice, st));
     rc = record_event(h, 1);
     if (rc) return rc;
     // 2. online training on `train_records` judge vertices (a PCG stream of `seed` picks them)
@@ -288,17 +300,39 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     }
     rc = record_event(h, 2);
     if (rc) return rc;
-    // 3. inference at depth k for every slice, then the depth-k stylization
-    rc = mlp_infer_planar(h, desc, records, anchors, k, 0, plane * S, lv, L, k);
+    // 3. inference at depth k: the other slices into their level slots, the judge slice into
+    //    slot k of its separate image; then the depth-k stylization
+    rc = mlp_infer_planar(h, desc, records, anchors, k, 0, (uint64_t)sj * plane, lv, L, k, 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, slot(sj, k), Fj, W, H, tiles_x, nd_fit);
+    rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)(sj + 1) * plane, (uint64_t)(S - 1 - sj) * plane,
+                          lv, L, k, 0);
+    if (rc) return rc;
+    rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)sj * plane, plane, Pj, k + 1, k, sj);
+    if (rc) return rc;
+    float* Pk = Pj + (size_t)k * 3 * plane;
+    hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pk, Fj, W, H, tiles_x, nd_fit);
     if ((rc = check_launch(h, "cache fit"))) return rc;
-    if ((rc = style_k(0, S))) return rc;
+    if ((rc = style_k(lv, L, 0, sj))) return rc;
+    if ((rc = style_k(slot(sj + 1, 0), L, sj + 1, S - 1 - sj))) return rc;
+    if ((rc = style_k(Pj, k + 1, sj, 1))) return rc;
     rc = record_event(h, 3);
     if (rc) return rc;
-    // 4. judge: levels k-1..0 of the judge slice on the estimate, per-tile error against E
-    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
-    hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, slot(sj, 0), Ej, W, H, tiles_x, nd_img);
+    // 4. judge: levels k-1..0 of the judge slice on its estimate, per-tile error against its
+    //    exact level-0 image
+    {
+        ScreenArgs<float> ap = aj;
+        ap.out = Pj;
+        ap.out_levels = k + 1;
+        for (int d = (int)k - 1; d >= 0; --d) {
+            ScreenArgs<float> b = ap;
+            b.d = (uint32_t)d;
+            b.out_slot = (uint32_t)d;
+            b.child_slot = (uint32_t)d + 1;
+            b.has_child = 1;  // slot k holds the estimate
+            NLPF_HIP(h, fp16 ? screen_level_fp16(b, desc->radius, 0, st) : screen_level_fp32(b, desc->radius, 0, st));
+        }
+    }
+    hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pj, slot(sj, 0), W, H, tiles_x, nd_img);
     if ((rc = check_launch(h, "tile judge"))) return rc;
     std::vector<double> hi(2 * (size_t)ntiles), hf(2 * (size_t)ntiles);
     NLPF_HIP(h, hipMemcpyAsync(hi.data(), nd_img, nd_bytes, hipMemcpyDeviceToHost, st));
@@ -322,9 +356,8 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     rs.cache_rel_l2 = fd_ > 0.0 ? (float)std::sqrt(fn_ / fd_) : 0.0f;
     rs.tiles = ntiles;
     rs.fallback_tiles = nfall;
-    // 5. exact depth-k output for the judge slice, exact filtering of the fallback tiles
-    NLPF_HIP(h, hipMemcpyAsync(slot(sj, k), Xj, f_bytes, hipMemcpyDeviceToDevice, st));
-    if (nfall) {
+    // 5. exact filtering of the fallback tiles in the other slices (the judge slice is exact)
+    if (nfall && S > 1) {
         std::vector<uint8_t> hm((size_t)L * ntiles, 0);
         for (uint32_t d = 1; d < L; ++d) {
             dilate_tiles(m, tiles_x, tiles_y);
@@ -336,14 +369,14 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
         for (int d = (int)L - 1; d >= (int)k; --d) {
             ScreenArgs<float> am = a0;
             am.tile_mask = masks + (size_t)d * ntiles;
-            NLPF_HIP(h, level(am, (uint32_t)d));
+            NLPF_HIP(h, level_others(am, (uint32_t)d));
         }
         NLPF_HIP(h, hipStreamSynchronize(st));  // hm outlives its upload
     }
     rc = record_event(h, 4);
     if (rc) return rc;
-    // 6. plain levels k-1..0 on the mixed depth-k output, then the film
-    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(a0, (uint32_t)d));
+    // 6. plain levels k-1..0 on the mixed depth-k output of the other slices, then the film
+    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level_others(a0, (uint32_t)d));
     NLPF_HIP(h, screen_film_fp32(lv, S, L, 0, plane, film, st));
     rc = record_event(h, 5);
     if (rc) return rc;
