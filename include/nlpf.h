@@ -114,7 +114,10 @@ typedef struct nlpf_handle nlpf_handle;
  * [0.1,0.05], reuse [128,64], vram 2048, smis true, no_first_hit_stylization false,
  * disable_flag_check false, cluster true, scene_bounds null) plus the screen-mode fields
  * mode ("world"|"screen", default "world"), pixel_radius (7), sigma_pos (0.05),
- * sigma_normal (0.9), sigma_lum (0.5), precision ("fp32"|"fp16"|"fp64"), seed (0).
+ * sigma_normal (0.9), sigma_lum (0.5), precision ("fp32"|"fp16"|"fp64"), seed (0),
+ * hash_table_size (64 Mi buckets) and world_kernel ("auto"|"thread"|"wave": world-mode filter
+ * with one thread or one wavefront per vertex, same results; "auto" picks by the expected
+ * neighbour count).
  * Unknown keys are ignored, as serde does for the reference (no deny_unknown_fields). */
 int nlpf_create(const char* config_json, nlpf_handle** out);
 int nlpf_destroy(nlpf_handle* h);

@@ -32,6 +32,9 @@ struct Config {
     uint64_t seed = 0;
     // hash-grid bucket count; the reference fixes 64 Mi heads (lm_path_filtering.rs:1667)
     uint64_t hash_table_size = 64ull * 1024 * 1024;
+    // world-mode filter kernel: "auto" (by the expected neighbour count), "thread" (one thread
+    // per vertex) or "wave" (one wavefront per vertex); the results are the same
+    std::string world_kernel = "auto";
 
     // per-depth vectors repeat their last element (lm_path_filtering.rs:1337-1347, 1584-1586)
     float radius_scale_at(uint32_t d) const {

@@ -336,6 +336,12 @@ bool parse_config(const char* json, Config& cfg, std::string& err) {
                 return false;
             }
             c.hash_table_size = u;
+        } else if (k == "world_kernel") {
+            if (v.kind != JVal::Str || (v.str != "auto" && v.str != "thread" && v.str != "wave")) {
+                err = "world_kernel must be \"auto\", \"thread\" or \"wave\"";
+                return false;
+            }
+            c.world_kernel = v.str;
         }
         // other keys (e.g. "final_gather" in configs/teaser/nl-pf.json) are ignored like serde does
     }
@@ -364,7 +370,8 @@ std::string config_to_json(const Config& c) {
     o << ",\"mode\":\"" << c.mode << "\",\"pixel_radius\":" << c.pixel_radius << ",\"sigma_pos\":" << c.sigma_pos
       << ",\"sigma_normal\":" << c.sigma_normal << ",\"sigma_lum\":" << c.sigma_lum << ",\"precision\":\""
       << (c.precision == NLPF_FP16 ? "fp16" : c.precision == NLPF_FP64 ? "fp64" : "fp32")
-      << "\",\"seed\":" << c.seed << ",\"hash_table_size\":" << c.hash_table_size << "}";
+      << "\",\"seed\":" << c.seed << ",\"hash_table_size\":" << c.hash_table_size << ",\"world_kernel\":\""
+      << c.world_kernel << "\"}";
     return o.str();
 }
 

@@ -112,11 +112,14 @@ __global__ void w_keys_kernel(const float* rec, uint64_t N, const uint32_t* pts,
 }
 
 __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint32_t n, const float* rec, uint64_t N,
-                                uint32_t* bstart, uint32_t* bend, float4* pos_sorted) {
+                                uint32_t* bstart, uint32_t* bend, float4* pos_sorted, uint32_t* occupied) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const uint32_t k = keys[e];
-    if (e == 0 || keys[e - 1] != k) bstart[k] = e;
+    if (e == 0 || keys[e - 1] != k) {
+        bstart[k] = e;
+        atomicAdd(occupied, 1u);
+    }
     if (e == n - 1 || keys[e + 1] != k) bend[k] = e + 1;
     const uint32_t i = ids[e];
     pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
@@ -171,7 +174,9 @@ struct FilterArgs {
     unsigned long long* nbr_count;
 };
 
-__global__ void __launch_bounds__(256) w_filter_kernel(FilterArgs a) {
+// One thread per vertex: for small neighbour sets (a few per vertex), where a wavefront per
+// vertex would leave most lanes idle.
+__global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
     const uint32_t tl = blockIdx.x * blockDim.x + threadIdx.x;
     if (tl >= a.count) return;
     const uint64_t N = a.N;
@@ -274,6 +279,180 @@ __global__ void __launch_bounds__(256) w_filter_kernel(FilterArgs a) {
     a.rad_out[3 * (uint64_t)c + 1] = L.y;
     a.rad_out[3 * (uint64_t)c + 2] = L.z;
     a.rng[tl] = rng;
+}
+
+// One wavefront per vertex (4 per workgroup).
+//   gather   the 27-cell walk is uniform across the wave; the lanes test 64 bucket entries at a
+//            time, a ballot and a prefix count append the accepted ones in bucket order, and
+//            lane 0 replays the reservoir draws (:813-821) for the entries beyond the cap, in
+//            order, so the RNG stream is the sequential one; the neighbour list lives in the
+//            wave's LDS slice when the cap fits (WF_CAP), else in the global list buffer;
+//   reuse    lanes take neighbours k = lane, lane + 64, ...; the SMIS denominator of neighbour
+//            k loops over every reused neighbour m in order with a wave-uniform index;
+//   reduce   per-lane Kahan sums, then a fixed shuffle tree (deterministic).
+constexpr uint32_t WF_CAP = 1024;
+__global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
+    __shared__ uint32_t lbuf[4][WF_CAP];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t tl = blockIdx.x * 4 + wv;
+    if (tl >= a.count) return;  // uniform per wave; no workgroup barrier below
+    const uint64_t N = a.N;
+    const uint32_t c = a.pts[a.off + tl];
+    const float3 p = make_float3(pl(a.rec, N, 0, c), pl(a.rec, N, 1, c), pl(a.rec, N, 2, c));
+    const float3 ns = make_float3(pl(a.rec, N, 3, c), pl(a.rec, N, 4, c), pl(a.rec, N, 5, c));
+    const float3 ng = make_float3(pl(a.rec, N, 6, c), pl(a.rec, N, 7, c), pl(a.rec, N, 8, c));
+    const float3 wo = make_float3(pl(a.rec, N, 9, c), pl(a.rec, N, 10, c), pl(a.rec, N, 11, c));
+    const float3 R = make_float3(pl(a.rec, N, 12, c), pl(a.rec, N, 13, c), pl(a.rec, N, 14, c));
+    const float3 le = make_float3(pl(a.rec, N, 15, c), pl(a.rec, N, 16, c), pl(a.rec, N, 17, c));
+    const uint32_t myflags = a.new_flags[c];
+    const uint32_t mycluster = a.cluster_id ? a.cluster_id[c] : 0u;
+    Pcg32 rng = a.rng[tl];  // advanced by lane 0 only
+    const bool in_lds = a.K <= WF_CAP;
+    uint32_t* gbuf = a.nbuf + (uint64_t)tl * a.K;
+    uint32_t* lb = lbuf[wv];
+    auto put = [&](uint32_t i, uint32_t j) {
+        if (in_lds)
+            lb[i] = j;
+        else
+            gbuf[i] = j;
+    };
+
+    float3 q = p;
+    if (a.cluster_id) q = make_float3(pl(a.rec, N, 0, mycluster), pl(a.rec, N, 1, mycluster), pl(a.rec, N, 2, mycluster));
+    uint32_t cell[3];
+    get_cell(a.g, q.x, q.y, q.z, cell);
+    uint32_t cnt = 0;  // wave-uniform
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)a.g.res[0] || cy >= (int)a.g.res[1] || cz >= (int)a.g.res[2])
+                    continue;
+                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
+                const uint32_t e0 = a.bstart[b], e1 = a.bend[b];
+                for (uint32_t base = e0; base < e1; base += 64) {
+                    const uint32_t e = base + (uint32_t)lane;
+                    bool ok = e < e1;
+                    uint32_t j = 0;
+                    if (ok) {
+                        const float4 qe = a.pos_sorted[e];
+                        j = __float_as_uint(qe.w);
+                        ok = dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius;
+                        if (ok && a.cluster_id && a.cluster_id[j] != mycluster) ok = false;
+                        if (ok && a.flag_check && (uint32_t)a.new_flags[j] != myflags) ok = false;
+                    }
+                    const uint64_t mask = __ballot(ok);
+                    const uint32_t total = (uint32_t)__popcll(mask);
+                    const uint32_t pos = cnt + (uint32_t)__popcll(mask & below);
+                    if (ok && pos < a.K) put(pos, j);  // the slots before the cap fill in order
+                    if (cnt + total > a.K) {
+                        // reservoir over the cap, entry by entry in bucket order (after the
+                        // in-order stores above have completed)
+                        if (!in_lds) __threadfence_block();
+                        uint64_t rest = mask;
+                        uint32_t at = cnt;
+                        while (rest) {
+                            const int src = __ffsll((long long)rest) - 1;
+                            rest &= rest - 1;
+                            const uint32_t jj = (uint32_t)__shfl((int)j, src);
+                            if (at >= a.K && lane == 0) {
+                                const uint32_t r = pcg_next(rng) % (at + 1);
+                                if (r < a.K) put(r, jj);
+                            }
+                            ++at;
+                        }
+                    }
+                    cnt += total;
+                }
+            }
+    // make the list written by all lanes visible to the whole wave
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t used = cnt < a.K ? cnt : a.K;
+    const float inv_used = used ? 1.0f / (float)used : 0.0f;
+    auto get = [&](uint32_t i) -> uint32_t { return in_lds ? lb[i] : gbuf[i]; };
+    // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
+    // the exact sum (a plain running sum drifts by ~n ulp)
+    float dsum[3] = {0, 0, 0}, dc[3] = {0, 0, 0}, isum[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
+    for (uint32_t k0 = 0; k0 < used; k0 += 64) {
+        const uint32_t k = k0 + (uint32_t)lane;
+        const bool active = k < used;
+        float3 wi = make_float3(0.0f, 0.0f, 0.0f);
+        float3 reused = make_float3(0.0f, 0.0f, 0.0f);
+        bool ind_ok = false;
+        float bpdf = 0.0f;
+        if (active) {
+            const uint32_t j = get(k);
+            const NbrRec& r = a.pk[j];
+            const float4 dw = r.dwi_pdf;
+            if (dw.w > 0.0f) {
+                float fcos;
+                const float dpdf = diffuse_eval(ns, ng, wo, make_float3(dw.x, dw.y, dw.z), fcos);
+                const float4 di = r.dirr_mis;
+                const float w = dw.w / (dw.w + dpdf);  // mis_weight, balance heuristic (:305-316)
+                const float sc = fcos * di.w * w / dw.w * inv_used;
+                kahan_add(dsum[0], dc[0], R.x * sc * di.x);
+                kahan_add(dsum[1], dc[1], R.y * sc * di.y);
+                kahan_add(dsum[2], dc[2], R.z * sc * di.z);
+            }
+            const float4 iw = r.iwi_mis;
+            wi = make_float3(iw.x, iw.y, iw.z);
+            if (sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z) != 0.0f) {
+                ind_ok = true;
+                float fcos;
+                bpdf = diffuse_eval(ns, ng, wo, wi, fcos);
+                const float4 ir = r.iirr;
+                reused = make_float3(R.x * fcos * ir.x * iw.w, R.y * fcos * ir.y * iw.w, R.z * fcos * ir.z * iw.w);
+            }
+        }
+        if (a.smis) {
+            // sum_m pdf_m(wi_k) over every reused neighbour m, in order
+            float sum_pdf = 0.0f, sc = 0.0f;
+            for (uint32_t m = 0; m < used; ++m) {
+                const uint32_t jm = __builtin_amdgcn_readfirstlane(get(m));
+                const NbrRec& o = a.pk[jm];
+                const float4 a0 = o.ns_wox, a1 = o.ng_woy, a2 = o.woz;
+                float fc;
+                kahan_add(sum_pdf, sc,
+                          diffuse_eval(make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
+                                       make_float3(a0.w, a1.w, a2.x), wi, fc));
+            }
+            if (active && ind_ok) {
+                kahan_add(isum[0], ic[0], reused.x / sum_pdf);
+                kahan_add(isum[1], ic[1], reused.y / sum_pdf);
+                kahan_add(isum[2], ic[2], reused.z / sum_pdf);
+            }
+        } else if (active && ind_ok && bpdf > 0.0f) {
+            kahan_add(isum[0], ic[0], reused.x / bpdf * inv_used);
+            kahan_add(isum[1], ic[1], reused.y / bpdf * inv_used);
+            kahan_add(isum[2], ic[2], reused.z / bpdf * inv_used);
+        }
+    }
+    // fixed-order reduction over the lanes
+    float v[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        v[i] = dsum[i] - dc[i];
+        v[3 + i] = isum[i] - ic[i];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) v[i] += __shfl_xor(v[i], off);
+    if (lane == 0) {
+        atomicAdd(a.nbr_count, (unsigned long long)cnt);
+        v3<float> L{le.x + v[0] + v[3], le.y + v[1] + v[4], le.z + v[2] + v[5]};
+        if (a.stylize) {
+            const uint32_t m = a.material[c];
+            const int root = m < a.n_materials ? a.roots[m] : -1;
+            if (root >= 0) L = apply_style<float>(a.nodes, root, (uint32_t)a.depth, myflags, L, v3<float>{p.x, p.y, p.z});
+        }
+        a.rad_out[3 * (uint64_t)c] = L.x;
+        a.rad_out[3 * (uint64_t)c + 1] = L.y;
+        a.rad_out[3 * (uint64_t)c + 2] = L.z;
+        a.rng[tl] = rng;
+    }
 }
 
 // parent (depth d-1) indirect record <- child's filtered radiance (kernel_propagate :973-998,
@@ -641,8 +820,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             tb = w.cub_bytes;
             NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.keys_sorted, w.pts, w.ids_sorted,
                                                            (int)n, 0, (int)table_bits, st));
+            NLPF_HIP(h, hipMemsetAsync(w.counters + 4, 0, sizeof(uint32_t), st));
             hipLaunchKernelGGL(w_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, w.ids_sorted, n, recd, N,
-                               w.bstart, w.bend, w.pos_sorted);
+                               w.bstart, w.bend, w.pos_sorted, w.counters + 4);
             hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind,
                                new_flags + (uint64_t)d * N, w.pk);
             if ((rc = check_launch(h, "world grid"))) return rc;
@@ -683,6 +863,15 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     if (n_uncl == 0) break;
                 }
             }
+            // kernel choice: records lie on surfaces, so about 9 of the 27 visited cells are occupied
+            // and a vertex meets ~9 n / (occupied buckets) candidates; a wavefront per vertex pays
+            // off from ~16 of them (with clusters, the cluster cap bounds the set)
+            const uint32_t occupied = read_u32(h, w.counters + 4, rc);
+            if (rc) return rc;
+            const double cand = 9.0 * (double)n / (double)std::max<uint32_t>(occupied, 1u);
+            const bool wave_kernel = cfg.world_kernel == "wave" ||
+                                     (cfg.world_kernel == "auto" &&
+                                      std::min<double>(cand, cfg.cluster ? (double)K + 1.0 : cand) >= 16.0);
             const uint64_t batch = std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
             for (uint64_t off = 0; off < n; off += batch) {
                 const uint32_t cnt = (uint32_t)std::min<uint64_t>(batch, n - off);
@@ -712,7 +901,10 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 fa.stylize = (d > 0 || !cfg.no_first_hit_stylization) && !h->h_roots.empty();
                 fa.rad_out = rad;
                 fa.nbr_count = w.nbr + d;
-                hipLaunchKernelGGL(w_filter_kernel, grid1(cnt), dim3(256), 0, st, fa);
+                if (wave_kernel)
+                    hipLaunchKernelGGL(w_filter_wave_kernel, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
+                else
+                    hipLaunchKernelGGL(w_filter_thread_kernel, grid1(cnt), dim3(256), 0, st, fa);
             }
             if ((rc = check_launch(h, "world filter"))) return rc;
             hipLaunchKernelGGL(w_clear_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bstart, w.bend);

@@ -58,6 +58,8 @@ def test_struct_layouts_match_header(tmp_path):
     ('{"pixel_radius":9}', "pixel_radius"),
     ('{"sigma_pos":0}', "sigma_pos"),
     ('{"precision":"bf16"}', "precision"),
+    ('{"world_kernel":"warp"}', "world_kernel"),
+    ('{"hash_table_size":0}', "hash_table_size"),
 ])
 def test_invalid_config_rejected_before_gpu(cfg, msg):
     lib = N.load()

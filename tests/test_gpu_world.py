@@ -84,11 +84,12 @@ def check(film, rad, ofilm, orad, host, L, tol=1e-5, sens=None):
     assert (ef <= bound).all(), (ef.max(), int(np.argmax(ef / bound)))
 
 
+@pytest.mark.parametrize("kernel", ["thread", "wave"])
 @pytest.mark.parametrize("smis", [True, False])
-def test_world_parity_config1_like(smis):
+def test_world_parity_config1_like(smis, kernel):
     W, H, branch, L = 48, 40, 4, 4
     cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "smis": smis, "cluster": False,
-           "radius_scale": [2.0, 3.0], "reuse": [1 << 16]}
+           "radius_scale": [2.0, 3.0], "reuse": [1 << 16], "world_kernel": kernel}
     nodes, roots = styles()
     film, rad, host, st = run_world(cfg, W, H, branch, L, nodes, roots)
     ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots)
@@ -99,12 +100,14 @@ def test_world_parity_config1_like(smis):
     assert st["neighbors"][0] > st["points"][0]  # reuse actually happens
 
 
-def test_world_reservoir_and_batches_match_oracle():
+@pytest.mark.parametrize("kernel", ["thread", "wave"])
+def test_world_reservoir_and_batches_match_oracle(kernel):
     """reuse cap below the neighbour count (reservoir path) and a vram budget forcing several
     batches; the reservoir RNG streams are part of the deterministic contract."""
     W, H, branch, L = 32, 32, 2, 3
     cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "smis": True, "cluster": False,
-           "radius_scale": [3.0], "reuse": [6, 4], "vram": 1, "no_first_hit_stylization": True}
+           "radius_scale": [3.0], "reuse": [6, 4], "vram": 1, "no_first_hit_stylization": True,
+           "world_kernel": kernel}
     film, rad, host, st = run_world(cfg, W, H, branch, L)
     ofilm, orad, _ = oracle_world(host, W, H, branch, L, cfg)
     check(film, rad, ofilm, orad, host, L)
