@@ -275,6 +275,13 @@ int nlpf_mlp_train_step(nlpf_handle* h, const nlpf_screen_desc* desc, const void
                         const float* anchors, uint32_t depth, const uint32_t* idx,
                         const float* feats, const float* target, uint32_t batch, float lr,
                         int apply_update, float* loss, float* grad);
+/* Inference weights as an exponential moving average of the trained parameters:
+ * ema_t = decay ema_{t-1} + (1 - decay) theta_t after every applied Adam step, used
+ * bias-corrected (ema_t / (1 - decay^t)); decay 0 (the default) uses the parameters themselves.
+ * Setting it (and nlpf_mlp_init / nlpf_mlp_set_params) restarts the average at the current
+ * parameters.  nlpf_mlp_infer, nlpf_mlp_reference and the cache pass use these weights,
+ * training uses the parameters. */
+int nlpf_mlp_set_ema(nlpf_handle* h, float decay);
 /* FP64 VALU evaluation of the same network (bf16 weights and activations as on MFMA, FP64
  * sums): the error oracle of the learned path. */
 int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,

@@ -684,6 +684,13 @@ struct MlpAdamArgs {
     __bf16* wt16;  // per layer [in][out]
     float lr, beta1, beta2, eps;
     int step;  // 1-based
+    // optional exponential moving average of the parameters for inference (ema == nullptr: off):
+    // ema_t updates so far including this one (0: reset), bias-corrected copy in ema_hat / w16_ema
+    float* ema;
+    float* ema_hat;
+    __bf16* w16_ema;
+    float ema_decay;
+    int ema_t;
 };
 __device__ __forceinline__ bool mlp_locate(int p, int& l, int& o, int& i, bool& is_bias) {
     for (l = 0; l < MLP_LAYERS; ++l) {
@@ -732,6 +739,19 @@ __global__ void mlp_adam_kernel(MlpAdamArgs a) {
         const int off = mlp_w16off(l);
         a.w16[off + o * mlp_in(l) + i] = (__bf16)th;
         a.wt16[off + i * mlp_out(l) + o] = (__bf16)th;
+    }
+    if (a.ema) {
+        float eh = th;
+        if (a.ema_t == 0) {
+            a.ema[p] = 0.0f;
+        } else {
+            const float e = a.ema_decay * a.ema[p] + (1.0f - a.ema_decay) * th;
+            a.ema[p] = e;
+            eh = e / (1.0f - powf(a.ema_decay, (float)a.ema_t));
+        }
+        if (pad) eh = 0.0f;
+        a.ema_hat[p] = eh;
+        if (!bias) a.w16_ema[mlp_w16off(l) + o * mlp_in(l) + i] = (__bf16)eh;
     }
 }
 
@@ -805,6 +825,12 @@ struct nlpf_mlp_state {
     __bf16* w16 = nullptr;
     __bf16* wt16 = nullptr;
     int step = 0;
+    // inference weights: an exponential moving average of the parameters when ema_decay > 0
+    float* ema = nullptr;
+    float* ema_hat = nullptr;
+    __bf16* w16_ema = nullptr;
+    float ema_decay = 0.0f;
+    int ema_t = 0;
     // training workspace
     uint32_t cap = 0;
     __bf16* act = nullptr;
@@ -839,7 +865,7 @@ void mlp_release(nlpf_handle* h) {
         if (g_mlp[i].first == h) {
             nlpf_mlp_state* s = g_mlp[i].second;
             mlp_free_train(s);
-            void* ps[] = {s->param, s->m, s->v, s->grad, s->w16, s->wt16};
+            void* ps[] = {s->param, s->m, s->v, s->grad, s->w16, s->wt16, s->ema, s->ema_hat, s->w16_ema};
             for (void* p : ps)
                 if (p) (void)hipFree(p);
             delete s;
@@ -860,6 +886,19 @@ static MlpParamsDev mlp_dev_params(nlpf_mlp_state* s) {
     }
     return P;
 }
+// weights used by inference: the moving average when it is on
+static MlpParamsDev mlp_infer_params(nlpf_mlp_state* s) {
+    MlpParamsDev P = mlp_dev_params(s);
+    if (s->ema_decay > 0.0f) {
+        int off = 0;
+        for (int l = 0; l < MLP_LAYERS; ++l) {
+            P.w[l] = s->w16_ema + off;
+            P.b[l] = s->ema_hat + mlp_boff(l);
+            off += mlp_in(l) * mlp_out(l);
+        }
+    }
+    return P;
+}
 
 static int mlp_refresh(nlpf_handle* h, nlpf_mlp_state* s, float lr, int step) {
     MlpAdamArgs a;
@@ -874,6 +913,13 @@ static int mlp_refresh(nlpf_handle* h, nlpf_mlp_state* s, float lr, int step) {
     a.beta2 = 0.999f;
     a.eps = 1e-8f;
     a.step = step;
+    a.ema = s->ema_decay > 0.0f ? s->ema : nullptr;
+    a.ema_hat = s->ema_hat;
+    a.w16_ema = s->w16_ema;
+    a.ema_decay = s->ema_decay;
+    if (step == 0) s->ema_t = 0;
+    else s->ema_t += 1;
+    a.ema_t = s->ema_t;
     hipLaunchKernelGGL(mlp_adam_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, h->stream, a);
     return check_launch(h, "mlp adam");
 }
@@ -888,7 +934,7 @@ static int mlp_need(nlpf_handle* h, nlpf_mlp_state** out) {
 static MlpInferArgs infer_args(nlpf_mlp_state* s, const nlpf_screen_desc* desc, const void* rec, const float* anc,
                                uint32_t depth, const uint32_t* idx, const float* feats, uint64_t n, float* out) {
     MlpInferArgs a{};
-    a.P = mlp_dev_params(s);
+    a.P = mlp_infer_params(s);
     a.rec = rec;
     a.anchors = desc && desc->precision == NLPF_FP16 ? anc : nullptr;
     a.fp16 = desc && desc->precision == NLPF_FP16;
@@ -941,6 +987,9 @@ int nlpf_mlp_init(nlpf_handle* h, uint64_t seed) {
         NLPF_HIP(h, hipMalloc(&s->grad, sizeof(float) * MLP_NPARAM));
         NLPF_HIP(h, hipMalloc(&s->w16, sizeof(__bf16) * MLP_NW16));
         NLPF_HIP(h, hipMalloc(&s->wt16, sizeof(__bf16) * MLP_NW16));
+        NLPF_HIP(h, hipMalloc(&s->ema, sizeof(float) * MLP_NPARAM));
+        NLPF_HIP(h, hipMalloc(&s->ema_hat, sizeof(float) * MLP_NPARAM));
+        NLPF_HIP(h, hipMalloc(&s->w16_ema, sizeof(__bf16) * MLP_NW16));
     }
     // He-uniform for the ReLU layers, half the range for the linear output layer; biases 0
     std::vector<float> p(MLP_NPARAM, 0.0f);
@@ -961,6 +1010,19 @@ int nlpf_mlp_init(nlpf_handle* h, uint64_t seed) {
     NLPF_HIP(h, hipMemset(s->grad, 0, sizeof(float) * MLP_NPARAM));
     s->step = 0;
     int rc = mlp_refresh(h, s, 0.0f, 0);
+    if (rc) return rc;
+    NLPF_HIP(h, hipStreamSynchronize(h->stream));
+    return NLPF_OK;
+}
+
+int nlpf_mlp_set_ema(nlpf_handle* h, float decay) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    nlpf_mlp_state* s;
+    int rc = mlp_need(h, &s);
+    if (rc) return rc;
+    if (!(decay >= 0.0f && decay < 1.0f)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "EMA decay must be in [0, 1)");
+    s->ema_decay = decay;
+    rc = mlp_refresh(h, s, 0.0f, 0);  // restarts the average at the current parameters
     if (rc) return rc;
     NLPF_HIP(h, hipStreamSynchronize(h->stream));
     return NLPF_OK;
@@ -1137,8 +1199,9 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
     if (rc) return rc;
     if (n == 0) return NLPF_OK;
     MlpRefArgs a;
-    a.w16 = s->w16;
-    a.param = s->param;
+    // the weights inference uses (the moving average when it is on)
+    a.w16 = s->ema_decay > 0.0f ? s->w16_ema : s->w16;
+    a.param = s->ema_decay > 0.0f ? s->ema_hat : s->param;
     a.rec = records;
     a.anchors = desc->precision == NLPF_FP16 ? anchors : nullptr;
     a.fp16 = desc->precision == NLPF_FP16;

@@ -130,6 +130,7 @@ _SIGS = [
     ("nlpf_mlp_train_step", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p, C.c_uint32,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_int,
                                       C.POINTER(C.c_float), C.c_void_p]),
+    ("nlpf_mlp_set_ema", C.c_int, [C.c_void_p, C.c_float]),
     ("nlpf_mlp_reference", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p, C.c_uint32,
                                      C.c_void_p, C.c_uint64, C.c_void_p]),
     ("nlpf_screen_run_mlp", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
@@ -326,6 +327,9 @@ class Handle:
     def mlp_set_params(self, params):
         p = np.ascontiguousarray(params, np.float32)
         self._check(self.lib.nlpf_mlp_set_params(self.h, p.ctypes.data_as(C.c_void_p), len(p)))
+
+    def mlp_set_ema(self, decay):
+        self._check(self.lib.nlpf_mlp_set_ema(self.h, decay))
 
     def mlp_encode(self, desc, records, anchors, depth, idx, n, feats):
         self._check(self.lib.nlpf_mlp_encode(self.h, C.byref(desc), _ptr(records), _ptr(anchors), depth,

@@ -138,3 +138,40 @@ def test_training_reduces_loss(h):
         losses.append(h.mlp_train_step(None, None, None, 0, None, df.ptr + k * B * M.IN * 4,
                                        dt.ptr + k * B * 12, B, 1e-2))
     assert np.mean(losses[-20:]) < 0.1 * np.mean(losses[:5]), (losses[:5], losses[-5:])
+
+
+def test_ema_inference_weights(h):
+    """With a moving average on, inference uses the bias-corrected EMA of the parameter
+    trajectory (checked against the FP64 oracle's forward on the numpy EMA of the downloaded
+    parameters), the FP64 reference kernel uses the same weights, and turning it off returns
+    to the parameters themselves."""
+    rng = np.random.default_rng(5)
+    params = random_params(rng)
+    h.mlp_set_params(params)
+    decay = 0.9
+    h.mlp_set_ema(decay)
+    B = 1024
+    feats = random_feats(rng, B)
+    target = rng.normal(0, 1, (B, 3)).astype(np.float32)
+    df, dt = h.to_device(feats), h.to_device(target)
+    ema = np.zeros(M.NPARAM)
+    for t in range(1, 7):
+        h.mlp_train_step(None, None, None, 0, None, df, dt, B, 1e-2)
+        ema = decay * ema + (1 - decay) * h.mlp_get_params().astype(np.float64)
+    ema_hat = ema / (1 - decay ** 6)
+    raw = h.mlp_get_params()
+    out = h.alloc(B * 12)
+    h.mlp_infer(None, None, None, 0, None, df, B, out)
+    y = out.download(np.float32, 3 * B).reshape(B, 3).astype(np.float64)
+    ref_ema, _ = M.forward(ema_hat, feats)
+    ref_raw, _ = M.forward(raw, feats)
+    scale = np.abs(ref_ema).max()
+    e_ema = np.median(np.abs(y - ref_ema)) / scale
+    e_raw = np.median(np.abs(y - ref_raw)) / scale
+    assert e_ema < 1e-3 and e_raw > 10 * e_ema, (e_ema, e_raw)
+    h.mlp_set_ema(0.0)
+    h.mlp_infer(None, None, None, 0, None, df, B, out)
+    y0 = out.download(np.float32, 3 * B).reshape(B, 3).astype(np.float64)
+    assert np.median(np.abs(y0 - ref_raw)) / scale < 1e-5
+    with pytest.raises(N.NlpfError):
+        h.mlp_set_ema(1.0)
