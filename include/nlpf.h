@@ -119,16 +119,21 @@ typedef struct nlpf_handle nlpf_handle;
  * with one thread or one wavefront per vertex, same results; "auto" picks by the expected
  * neighbour count).
  * Unknown keys are ignored, as serde does for the reference (no deny_unknown_fields). */
+/* Replaces PathFiltering::new / Config deserialisation (lm_path_filtering.rs:264-298, 1317)
+ * and the device setup of lm_path_filtering::render (:1802-1813). */
 int nlpf_create(const char* config_json, nlpf_handle** out);
 int nlpf_destroy(nlpf_handle* h);
 /* Message of the last failing call on h; h == NULL gives the last nlpf_create failure. */
 const char* nlpf_last_error(const nlpf_handle* h);
 /* Parsed configuration echoed back as JSON (for host-side mirrors and tests). */
 int nlpf_get_config_json(const nlpf_handle* h, char* buf, size_t cap);
-/* Overwrite the screen-mode bandwidths (sigma_pos, sigma_normal, sigma_lum). */
+/* Overwrite the screen-mode bandwidths (sigma_pos, sigma_normal, sigma_lum); no reference
+ * counterpart (screen mode is an addition). */
 int nlpf_set_sigmas(nlpf_handle* h, float sigma_pos, float sigma_normal, float sigma_lum);
 
-/* ---- device memory (hipMalloc / hipMemcpy on the handle's device and stream) ---- */
+/* ---- device memory (hipMalloc / hipMemcpy on the handle's device and stream) ----
+ * Stand in for the LuisaCompute device buffers the reference allocates per render
+ * (lm_path_filtering.rs:1319-1560: Buffer<Vertex>, per-depth vertex buffers, film). */
 int nlpf_alloc(nlpf_handle* h, size_t bytes, void** dptr);
 int nlpf_free(nlpf_handle* h, void* dptr);
 int nlpf_upload(nlpf_handle* h, void* dst_device, const void* src_host, size_t bytes);
@@ -176,6 +181,10 @@ typedef struct nlpf_screen_desc {
     int32_t radius;
 } nlpf_screen_desc;
 
+/* Screen-mode pass: no reference function (the north-star workload defines it); it takes the
+ * place of the depth loop PathFiltering::render runs (lm_path_filtering.rs:1673-1774:
+ * kernel_filter :999-1060 per depth, kernel_propagate :973-998 to the parent, kernel_splat +
+ * kernel_update_film :428-506) with an image-space window instead of the hash grid. */
 int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
                     const float* anchors, void* film_rgb, void* level_out);
 
@@ -191,7 +200,9 @@ int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc, const voi
                          const void* level_out, const float* dL_dfilm, float* grad_params);
 
 /* Kernel time (ms, HIP events on the handle's stream) of the last nlpf_screen_run, total
- * and per level (levels entries, deepest level first in execution order, indexed by depth). */
+ * and per level (levels entries, deepest level first in execution order, indexed by depth).
+ * The reference only logs timings and the average neighbour count
+ * (lm_path_filtering.rs:1767-1773, 1798). */
 typedef struct nlpf_stats {
     double total_ms;
     double level_ms[32];
@@ -240,7 +251,7 @@ int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const float* rec
                    const int32_t* path_depth, float* film_rgb, float* radiance_out);
 
 /* Trace the procedural scene (below) into world-mode records, one path per (pixel, branch),
- * as kernel_mk_trace does: camera ray, NEE to the quad light with its radiance stylized at
+ * as kernel_mk_trace does (lm_path_filtering.rs:1062-1218, area-light NEE light/area.rs:50-131): camera ray, NEE to the quad light with its radiance stylized at
  * depth+1 by material `light material`, cosine BSDF sampling, StylizationFlags updated by
  * each hit's material root. Materials: 0 floor 1 ceiling 2 back 3 left 4 right 5 sphere0
  * 6 sphere1 7 light; wall/sphere albedos U[albedo_lo, albedo_hi] per material from `seed`. */
@@ -331,7 +342,9 @@ int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc, const void
  * streams keyed by (seed, record index). FP16 output also writes the anchors. */
 int nlpf_generate_screen_records(nlpf_handle* h, const nlpf_screen_desc* desc,
                                  const nlpf_gen_desc* gen, void* records, float* anchors);
-/* Convert FP32 records to FP16 records + anchors (same layout rules as above). */
+/* Convert FP32 records to FP16 records + anchors (same layout rules as above).  The synthetic
+ * generator and the conversion have no reference counterpart: the reference's records come
+ * from kernel_mk_trace (lm_path_filtering.rs:1062-1218) and are FP32 only. */
 int nlpf_convert_records_fp16(nlpf_handle* h, const nlpf_screen_desc* desc_fp32,
                               const float* records_fp32, void* records_fp16, float* anchors);
 
