@@ -48,15 +48,17 @@ int or_screen_run_f32(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius
                       double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
                       float* film, float* level_out, int threads);
 
-/* World-mode NL-PF (cluster = false), FP64; see oracle_world.c. */
+/* World-mode NL-PF, FP64; see oracle_world.c.  cluster != 0 runs the deterministic clustering
+ * (cluster_out, optional, receives every level's cluster ids [L][N], 0xFFFFFFFF off-level). */
 int or_world_neighbor_counts(const float* pos, uint32_t n, const float pmin[3], const float pmax[3], float radius,
                              uint64_t table, uint32_t* out_count);
 int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const float aabb_min[3],
                  const float aabb_max[3], const float* rec, const uint8_t* flags, const uint8_t* new_flags,
                  const uint16_t* material, const int32_t* path_depth, const float* radius_scale, uint32_t n_rs,
                  const uint32_t* reuse, uint32_t n_reuse, uint64_t vram_mb, int smis, int no_first_hit_stylization,
-                 int disable_flag_check, uint64_t seed, uint64_t table, const or_node* nodes, const int32_t* roots,
-                 uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out);
+                 int disable_flag_check, int cluster, uint64_t seed, uint64_t table, const or_node* nodes,
+                 const int32_t* roots, uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out,
+                 uint32_t* cluster_out);
 
 /* FP64 reference gradient of the screen-mode NL-PF; returns the number of parameters
  * (3 + 16 * ramp nodes) written to grad, or -1. */

@@ -39,7 +39,7 @@ def load():
     lib.or_world_neighbor_counts.argtypes = [vp, u32, vp, vp, C.c_float, u64, vp]
     lib.or_world_neighbor_counts.restype = i32
     lib.or_world_run.argtypes = [u32, u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, u32, u64, i32, i32,
-                                 i32, u64, u64, vp, vp, u32, vp, vp, vp]
+                                 i32, i32, u64, u64, vp, vp, u32, vp, vp, vp, vp]
     lib.or_world_run.restype = i32
     lib.or_screen_grad_f64.argtypes = [u32, u32, u32, u32, i32, vp, vp, vp, u32, i32, vp, vp, vp]
     lib.or_screen_grad_f64.restype = i32
@@ -117,9 +117,10 @@ def world_neighbor_counts(pos, pmin, pmax, radius, table):
 
 
 def world_run(rec, flags, new_flags, material, path_depth, W, H, branch, L, aabb_min, aabb_max, cfg,
-              nodes=None, roots=(), want_radiance=False):
+              nodes=None, roots=(), want_radiance=False, want_clusters=False):
     """cfg: dict with the "pf" fields (radius_scale, reuse, vram, smis, no_first_hit_stylization,
-    disable_flag_check, seed, hash_table_size)."""
+    disable_flag_check, cluster, seed, hash_table_size).  want_clusters adds the cluster ids
+    [L][N] (0xFFFFFFFF off-level) to the returned tuple."""
     lib = load()
     N = W * H * branch
     rec = np.ascontiguousarray(rec, np.float32)
@@ -127,6 +128,7 @@ def world_run(rec, flags, new_flags, material, path_depth, W, H, branch, L, aabb
     film = np.empty((H, W, 3), np.float64)
     rad = np.empty((L, N, 3), np.float64) if want_radiance else None
     nbr = np.zeros(32, np.uint64)
+    cl = np.empty((L, N), np.uint32) if want_clusters else None
     rs = np.ascontiguousarray(cfg.get("radius_scale", [0.1, 0.05]), np.float32)
     ru = np.ascontiguousarray(cfg.get("reuse", [128, 64]), np.uint32)
     if nodes is None or len(nodes) == 0:
@@ -142,11 +144,15 @@ def world_run(rec, flags, new_flags, material, path_depth, W, H, branch, L, aabb
                           _p(np.ascontiguousarray(path_depth, np.int32)), _p(rs), len(rs), _p(ru), len(ru),
                           int(cfg.get("vram", 2048)), int(cfg.get("smis", True)),
                           int(cfg.get("no_first_hit_stylization", False)), int(cfg.get("disable_flag_check", False)),
-                          int(cfg.get("seed", 0)), int(cfg.get("hash_table_size", 64 * 1024 * 1024)),
-                          _p(nodes_arr), _p(roots), len(roots), _p(film), _p(rad), _p(nbr))
+                          int(cfg.get("cluster", True)), int(cfg.get("seed", 0)),
+                          int(cfg.get("hash_table_size", 64 * 1024 * 1024)),
+                          _p(nodes_arr), _p(roots), len(roots), _p(film), _p(rad), _p(nbr), _p(cl))
     if rc != 0:
         raise MemoryError("oracle allocation failed")
-    return (film, rad, nbr) if want_radiance else film
+    out = (film, rad, nbr) if want_radiance else (film,)
+    if want_clusters:
+        out = out + (cl,)
+    return out if len(out) > 1 else out[0]
 
 
 def screen_grad(records, W, H, S, L, radius, sigmas, nodes, root, dfilm):

@@ -17,7 +17,17 @@
  * Determinism choices shared with the library (the reference's order is scheduling-dependent):
  * grid entries of one bucket are visited in ascending path id; the reservoir RNG of the
  * thread handling the t-th point of a batch is Pcg32 new_seq_offset(t, splitmix64(seed)),
- * carried across depths.  Clustering is not restated here (cluster = false only).
+ * carried across depths.
+ *
+ * Clustering (cluster = true) is the library's deterministic replacement of the reference's
+ * racy CAS seeding / joining (:532-623, 1675-1710), restated sequentially:
+ *   repeat: every bucket's smallest unclustered path id becomes a seed;
+ *     repeat: every unclustered vertex proposes to the first seed (27-cell visit order) within
+ *       the radius whose cluster has fewer than K joiners; each seed accepts its proposals in
+ *       ascending vertex order until it has K joiners;  until no vertex proposes;
+ *   until every vertex is clustered.
+ * The filter then gathers around the seed's position and keeps the members of the vertex's
+ * own cluster (:782-790, 807-811).
  */
 #include <math.h>
 #include <stdint.h>
@@ -151,6 +161,95 @@ int or_world_neighbor_counts(const float* pos, uint32_t n, const float pmin[3], 
     return 0;
 }
 
+/* first entry e of bucket b in the sorted (bucket << 32 | path) keys */
+static uint32_t bucket_lo(const uint64_t* kv, uint32_t n, uint64_t b) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if ((kv[mid] >> 32) < b) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* deterministic clustering of one level (see the header comment); cid[i] = seed of i */
+static void cluster_level(lvl_t lv, const float pmin[3], const float pmax[3], const uint32_t res[3], uint64_t table,
+                          float radius, uint32_t K, const uint32_t* pts, uint32_t n, const uint64_t* kv,
+                          uint32_t* cid, uint32_t* csize, uint64_t* prop) {
+    const uint32_t UNSET = 0xFFFFFFFFu;
+    for (uint32_t e = 0; e < n; ++e) {
+        cid[pts[e]] = UNSET;
+        csize[pts[e]] = 0;
+    }
+    for (;;) {
+        /* seeds: the smallest unclustered path id of each bucket (its entries ascend by path id;
+           a bucket can hold colliding cells, as on the GPU) */
+        uint32_t nseed = 0;
+        for (uint32_t e = 0; e < n; ++e) {
+            const uint32_t i = pts[e];
+            if (cid[i] != UNSET) continue;
+            int32_t c[3];
+            cell_of(pmin, pmax, res, (float)F(lv, 0, i), (float)F(lv, 1, i), (float)F(lv, 2, i), c);
+            const uint64_t b = bucket_of(pmin, pmax, res, table, c);
+            uint32_t smallest = UNSET;
+            for (uint32_t f = bucket_lo(kv, n, b); f < n && (kv[f] >> 32) == b; ++f) {
+                const uint32_t j = (uint32_t)kv[f];
+                if (cid[j] == UNSET) {
+                    smallest = j;
+                    break;
+                }
+            }
+            if (smallest == i) prop[nseed++] = i;
+        }
+        for (uint32_t s = 0; s < nseed; ++s) cid[prop[s]] = (uint32_t)prop[s];
+        uint32_t n_uncl = 0;
+        for (;;) {
+            uint32_t np = 0;
+            n_uncl = 0;
+            for (uint32_t e = 0; e < n; ++e) {
+                const uint32_t i = pts[e];
+                if (cid[i] != UNSET) continue;
+                ++n_uncl;
+                const float px = (float)F(lv, 0, i), py = (float)F(lv, 1, i), pz = (float)F(lv, 2, i);
+                int32_t c[3];
+                cell_of(pmin, pmax, res, px, py, pz, c);
+                int done = 0;
+                for (int dz = -1; dz <= 1 && !done; ++dz)
+                    for (int dy = -1; dy <= 1 && !done; ++dy)
+                        for (int dx = -1; dx <= 1 && !done; ++dx) {
+                            int32_t nc[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+                            if (nc[0] < 0 || nc[1] < 0 || nc[2] < 0 || nc[0] >= (int)res[0] ||
+                                nc[1] >= (int)res[1] || nc[2] >= (int)res[2])
+                                continue;
+                            const uint64_t b = bucket_of(pmin, pmax, res, table, nc);
+                            for (uint32_t f = bucket_lo(kv, n, b); f < n && (kv[f] >> 32) == b; ++f) {
+                                const uint32_t j = (uint32_t)kv[f];
+                                const float ddx = (float)F(lv, 0, j) - px, ddy = (float)F(lv, 1, j) - py,
+                                            ddz = (float)F(lv, 2, j) - pz;
+                                if (!(sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) <= radius)) continue;
+                                if (cid[j] == j && csize[j] < K) {
+                                    prop[np++] = ((uint64_t)j << 32) | i;
+                                    done = 1;
+                                    break;
+                                }
+                            }
+                        }
+            }
+            if (np == 0) break;
+            qsort(prop, np, sizeof(uint64_t), cmp_u64);
+            for (uint32_t a = 0; a < np;) {
+                const uint32_t seed = (uint32_t)(prop[a] >> 32);
+                uint32_t z = a;
+                while (z < np && (uint32_t)(prop[z] >> 32) == seed) ++z;
+                const uint32_t size0 = csize[seed], room = K > size0 ? K - size0 : 0u;
+                for (uint32_t r = 0; r < z - a && r < room; ++r) cid[(uint32_t)prop[a + r]] = seed;
+                csize[seed] = size0 + ((z - a) < room ? (z - a) : room);
+                a = z;
+            }
+        }
+        if (n_uncl == 0) break;
+    }
+}
+
 /*
  * Whole world-mode pass.  Arguments mirror nlpf_world_run; cfg_* carry the "pf" config.
  * film [H][W][3] (double), radiance_out optional [L][N][3] (double).
@@ -159,14 +258,20 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                  const float aabb_max[3], const float* rec, const uint8_t* flags, const uint8_t* new_flags,
                  const uint16_t* material, const int32_t* path_depth, const float* radius_scale, uint32_t n_rs,
                  const uint32_t* reuse, uint32_t n_reuse, uint64_t vram_mb, int smis, int no_first_hit_stylization,
-                 int disable_flag_check, uint64_t seed, uint64_t table, const or_node* nodes, const int32_t* roots,
-                 uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out) {
+                 int disable_flag_check, int cluster, uint64_t seed, uint64_t table, const or_node* nodes,
+                 const int32_t* roots, uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out,
+                 uint32_t* cluster_out) {
     const uint64_t npix = (uint64_t)W * H, N = npix * branch;
     double* ind = (double*)calloc(8 * N, sizeof(double));
     double* rad = (double*)calloc(3 * N * L, sizeof(double));
     uint32_t* pts = (uint32_t*)malloc(sizeof(uint32_t) * N);
     uint64_t* kv = (uint64_t*)malloc(sizeof(uint64_t) * N);
-    if (!ind || !rad || !pts || !kv) return -1;
+    uint32_t* cid = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    uint32_t* csize = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    uint64_t* prop = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    if (!ind || !rad || !pts || !kv || !cid || !csize || !prop) return -1;
+    if (cluster_out)
+        for (uint64_t i = 0; i < (uint64_t)L * N; ++i) cluster_out[i] = 0xFFFFFFFFu;
     uint32_t kmax = 1;
     for (uint32_t d = 0; d < L; ++d) {
         uint32_t K = d < n_reuse ? reuse[d] : reuse[n_reuse - 1];
@@ -204,6 +309,7 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
         }
         qsort(kv, n, sizeof(uint64_t), cmp_u64);
         const uint32_t K = d < (int)n_reuse ? reuse[d] : reuse[n_reuse - 1];
+        if (cluster) cluster_level(lv, aabb_min, aabb_max, res, table, radius, K, pts, n, kv, cid, csize, prop);
         uint64_t batch = vram_mb * 1024 * 1024 / ((uint64_t)K * 4);
         if (batch < 1) batch = 1;
         uint64_t nbr_total = 0;
@@ -211,7 +317,8 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
         for (uint32_t e = 0; e < n; ++e) {
             const uint32_t c = pts[e];
             const uint32_t tl = (uint32_t)(e % batch);
-            const float qx = (float)F(lv, 0, c), qy = (float)F(lv, 1, c), qz = (float)F(lv, 2, c);
+            const uint32_t q = cluster ? cid[c] : c;  /* the seed's position when clustered */
+            const float qx = (float)F(lv, 0, q), qy = (float)F(lv, 1, q), qz = (float)F(lv, 2, q);
             const double ns[3] = {F(lv, 3, c), F(lv, 4, c), F(lv, 5, c)};
             const double ng[3] = {F(lv, 6, c), F(lv, 7, c), F(lv, 8, c)};
             const double wo[3] = {F(lv, 9, c), F(lv, 10, c), F(lv, 11, c)};
@@ -238,6 +345,7 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                             float ddx = (float)F(lv, 0, j) - qx, ddy = (float)F(lv, 1, j) - qy,
                                   ddz = (float)F(lv, 2, j) - qz;
                             if (!(sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) <= radius)) continue;
+                            if (cluster && cid[j] != q) continue;
                             if (!disable_flag_check && new_flags[(uint64_t)d * N + j] != myf) continue;
                             if (cnt < K) {
                                 buf[cnt] = j;
@@ -292,6 +400,8 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
             for (int ch = 0; ch < 3; ++ch) radd[3 * (uint64_t)c + ch] = Lr[ch];
         }
         if (nbr_out) nbr_out[d] = nbr_total;
+        if (cluster && cluster_out)
+            for (uint32_t e = 0; e < n; ++e) cluster_out[(uint64_t)d * N + pts[e]] = cid[pts[e]];
         /* propagate to depth d-1 */
         memset(ind, 0, sizeof(double) * 8 * N);
         if (d > 0) {
@@ -339,5 +449,8 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
     free(kv);
     free(rng);
     free(buf);
+    free(cid);
+    free(csize);
+    free(prop);
     return 0;
 }

@@ -144,10 +144,14 @@ def test_world_toon_depth1_mostly_exact():
     assert (e <= 1e-5).mean() >= 0.99, (e <= 1e-5).mean()
 
 
-def test_world_cluster_mode_runs_and_is_deterministic():
+@pytest.mark.parametrize("kernel", ["thread", "wave"])
+def test_world_cluster_mode_matches_oracle(kernel):
+    """cluster = true: deterministic clustering (bucket-minimum seeds, ordered joining up to K),
+    the gather around the seed restricted to the vertex's cluster, the reservoir over the K+1
+    members; bitwise reproducible, and equal to the oracle's restatement."""
     W, H, branch, L = 32, 32, 2, 3
     cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "cluster": True, "radius_scale": [3.0],
-           "reuse": [16]}
+           "reuse": [16], "world_kernel": kernel}
     f1, r1, host, st = run_world(cfg, W, H, branch, L)
     f2, r2, _, _ = run_world(cfg, W, H, branch, L)
     assert np.array_equal(f1, f2) and np.array_equal(r1, r2)
@@ -156,3 +160,7 @@ def test_world_cluster_mode_runs_and_is_deterministic():
     for d in range(L):
         if st["points"][d]:
             assert st["neighbors"][d] <= st["points"][d] * 17
+    ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg)
+    assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    sens = sensitivity(host, W, H, branch, L, cfg, None, (), ofilm, orad)
+    check(f1, r1, ofilm, orad, host, L, sens=sens)

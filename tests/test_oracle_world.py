@@ -95,7 +95,7 @@ def make_records(rng, W, H, branch, L, same_normal=False, tight=False):
 
 
 def numpy_world(rec, depth, W, H, branch, L, radius_scale, smis, nodes=None, roots=(), flags=None, new_flags=None,
-                material=None, no_first_hit_stylization=True):
+                material=None, no_first_hit_stylization=True, clusters=None):
     """Independent restatement for cnt <= K (no reservoir) and no bucket collisions: the
     neighbour set of a vertex is every depth-d vertex within the radius with equal flags."""
     Np = W * H * branch
@@ -112,7 +112,12 @@ def numpy_world(rec, depth, W, H, branch, L, radius_scale, smis, nodes=None, roo
         P = rec[d, 0:3, pts].astype(np.float32)  # (n, 3)
         dist = np.sqrt(((P[:, None, :] - P[None, :, :]) ** 2).sum(-1, dtype=np.float32))
         for a, c in enumerate(pts):
-            nb = pts[(dist[a] <= radius) & (nf[d, pts] == nf[d, c])]
+            if clusters is None:
+                nb = pts[(dist[a] <= radius) & (nf[d, pts] == nf[d, c])]
+            else:  # the gather runs around the seed and keeps the vertex's own cluster
+                seed = clusters[d, c]
+                sa = int(np.nonzero(pts == seed)[0][0])
+                nb = pts[(dist[sa] <= radius) & (clusters[d, pts] == seed) & (nf[d, pts] == nf[d, c])]
             ns, ng, wo, R = r[3:6, c], r[6:9, c], r[9:12, c], r[12:15, c]
 
             def ev(ns_, ng_, wo_, wi_):
@@ -161,7 +166,7 @@ def test_matches_numpy_restatement(oracle, smis):
     rng = np.random.default_rng(7 + smis)
     W, H, branch, L = 6, 5, 2, 3
     rec, depth = make_records(rng, W, H, branch, L)
-    cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "smis": smis, "no_first_hit_stylization": True}
+    cfg = {"cluster": False, "radius_scale": [8.0], "reuse": [1 << 20], "smis": smis, "no_first_hit_stylization": True}
     Np = W * H * branch
     z8 = np.zeros((L, Np), np.uint8)
     z16 = np.zeros((L, Np), np.uint16)
@@ -188,7 +193,7 @@ def test_flags_gate_and_stylization(oracle):
     nodes = N.style_table([N.rex(0.4), N.hue_shift(0.2), N.enable_if(1, inner=1, else_=0)])
     roots = [2, 0]
     for nfhs in (True, False):
-        cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "smis": True, "no_first_hit_stylization": nfhs}
+        cfg = {"cluster": False, "radius_scale": [8.0], "reuse": [1 << 20], "smis": True, "no_first_hit_stylization": nfhs}
         film, rad, _ = oracle.world_run(rec, nf, nf, mat, depth, W, H, branch, L, AMIN, AMAX, cfg, nodes=nodes,
                                         roots=roots, want_radiance=True)
         ref = numpy_world(rec, depth, W, H, branch, L, 8.0, True, nodes=nodes, roots=roots, new_flags=nf,
@@ -206,7 +211,7 @@ def test_lambertian_closed_forms(oracle):
     rec, _ = make_records(rng, W, H, branch, L, same_normal=True, tight=True)
     depth = np.full(W * H, 1, np.int32)
     rec[1, 25] = 0.0  # no direct light at the top level: its output is le
-    cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "no_first_hit_stylization": True}
+    cfg = {"cluster": False, "radius_scale": [8.0], "reuse": [1 << 20], "no_first_hit_stylization": True}
     Np = W * H
     z8 = np.zeros((L, Np), np.uint8)
     z16 = np.zeros((L, Np), np.uint16)
@@ -233,7 +238,7 @@ def test_reservoir_cap_and_film_restylization(oracle):
     z8 = np.zeros((L, Np), np.uint8)
     z16 = np.zeros((L, Np), np.uint16)
     # K = 1: every vertex reuses exactly one of the 27 candidates
-    cfg = {"radius_scale": [8.0], "reuse": [1], "smis": False, "no_first_hit_stylization": True}
+    cfg = {"cluster": False, "radius_scale": [8.0], "reuse": [1], "smis": False, "no_first_hit_stylization": True}
     _, rad, nbr = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg, want_radiance=True)
     assert nbr[0] == Np * Np  # every vertex sees all of them
     r = rec[0].astype(np.float64)
@@ -243,9 +248,111 @@ def test_reservoir_cap_and_film_restylization(oracle):
         assert np.isclose(single[c], rad[0, c], rtol=1e-10).all(1).any()
     # film re-applies depth-0 stylization to the branch mean (kernel_update_film :451-470)
     nodes = N.style_table([N.rex(0.5)])
-    cfg = {"radius_scale": [8.0], "reuse": [1 << 20], "smis": True, "no_first_hit_stylization": False}
+    cfg = {"cluster": False, "radius_scale": [8.0], "reuse": [1 << 20], "smis": True, "no_first_hit_stylization": False}
     film, rad, _ = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg, nodes=nodes,
                                     roots=[0], want_radiance=True)
     mean = rad[0].reshape(branch, W * H, 3).mean(0)
     again = oracle.stylize(nodes, [0], mean, depth=0)
     assert np.allclose(film.reshape(-1, 3), again, rtol=1e-12)
+
+
+def numpy_clusters(rec, depth, L, radius_scale, K, table=64 * 1024 * 1024):
+    """Independent restatement of the deterministic clustering: bucket-minimum seeds, then
+    rounds of proposals to the first seed with room (27-cell order, bucket entries by path id),
+    each seed accepting its proposals in ascending vertex order up to K joiners."""
+    UNSET = 0xFFFFFFFF
+    ext = (AMAX - AMIN).astype(np.float32)
+    avg = np.float32((ext[0] + ext[1] + ext[2]) / np.float32(3))
+    out = np.full((L, rec.shape[2]), UNSET, np.uint32)
+    for d in range(L):
+        pts = np.nonzero(depth >= d)[0]
+        radius = np.float32(avg * np.float32(0.01) * np.float32(radius_scale))
+        res = np.floor(ext / radius).astype(np.uint32)
+        P = rec[d, 0:3].T.astype(np.float32)
+        cell = {int(i): cells(P[i], res).astype(np.int64) for i in pts}
+        bucket = {i: int(teschner(cell[i], table)) for i in cell}
+        members = {}
+        for i in sorted(cell):
+            members.setdefault(bucket[i], []).append(i)
+        cid = {i: UNSET for i in cell}
+        size = {i: 0 for i in cell}
+        while True:
+            seeds = [min([j for j in m if cid[j] == UNSET], default=None) for m in members.values()]
+            for sd in seeds:
+                if sd is not None:
+                    cid[sd] = sd
+            while True:
+                props, n_uncl = [], 0
+                for i in sorted(cell):
+                    if cid[i] != UNSET:
+                        continue
+                    n_uncl += 1
+                    found = None
+                    for dz in (-1, 0, 1):
+                        for dy in (-1, 0, 1):
+                            for dx in (-1, 0, 1):
+                                if found is not None:
+                                    continue
+                                nc = cell[i] + np.array([dx, dy, dz])
+                                if (nc < 0).any() or (nc >= res).any():
+                                    continue
+                                for j in members.get(int(teschner(nc, table)), []):
+                                    dd = np.sqrt(np.float32(((P[j] - P[i]) ** 2).sum(dtype=np.float32)))
+                                    if dd <= radius and cid[j] == j and size[j] < K:
+                                        found = j
+                                        break
+                    if found is not None:
+                        props.append((found, i))
+                if not props:
+                    break
+                props.sort()
+                for sd in sorted(set(p[0] for p in props)):
+                    mine = [i for (t, i) in props if t == sd]
+                    room = max(K - size[sd], 0)
+                    for i in mine[:room]:
+                        cid[i] = sd
+                    size[sd] += min(len(mine), room)
+            if n_uncl == 0:
+                break
+        for i, c in cid.items():
+            out[d, i] = c
+    return out
+
+
+@pytest.mark.parametrize("smis,K", [(True, 1), (False, 1), (True, 64), (False, 64)])
+def test_clustering_matches_restatement_and_invariants(oracle, smis, K):
+    """cluster = true: the oracle's cluster ids equal the independent restatement; every vertex
+    sits in exactly one cluster whose seed is within the radius and has at most K joiners.  With
+    a cap the clusters never fill (K = 64, no reservoir draws), the filtered radiance equals the
+    numpy restatement gathering each vertex's own cluster."""
+    rng = np.random.default_rng(17)
+    W, H, branch, L, rs = 8, 8, 2, 2, 8.0
+    rec, depth = make_records(rng, W, H, branch, L)
+    Np = W * H * branch
+    z8 = np.zeros((L, Np), np.uint8)
+    z16 = np.zeros((L, Np), np.uint16)
+    cfg = {"cluster": True, "radius_scale": [rs], "reuse": [K], "smis": smis, "no_first_hit_stylization": True}
+    film, rad, nbr, cl = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg,
+                                          want_radiance=True, want_clusters=True)
+    expect = numpy_clusters(rec, depth, L, rs, K)
+    assert np.array_equal(cl, expect)
+    ext = (AMAX - AMIN).astype(np.float32)
+    radius = np.float32(np.float32((ext[0] + ext[1] + ext[2]) / np.float32(3)) * np.float32(0.01) * np.float32(rs))
+    for d in range(L):
+        pts = np.nonzero(depth >= d)[0]
+        c = cl[d, pts]
+        assert (c != 0xFFFFFFFF).all() and (cl[d, c] == c).all()  # seeds are their own cluster
+        P = rec[d, 0:3].T
+        dd = np.sqrt(((P[pts] - P[c]) ** 2).sum(-1, dtype=np.float32))
+        assert (dd <= radius).all()
+        seeds, counts = np.unique(c, return_counts=True)
+        assert (counts <= K + 1).all() and len(seeds) < len(pts)  # capped, and some clusters merge
+        if K == 1:
+            assert (counts == K + 1).any()  # the cap binds
+    if K == 1:
+        return
+    assert max(np.unique(cl[d][depth >= d], return_counts=True)[1].max() for d in range(L)) <= K
+    ref = numpy_world(rec, depth, W, H, branch, L, rs, smis, clusters=cl)
+    for d in range(L):
+        m = depth >= d
+        assert np.allclose(rad[d][m], ref[d][m], rtol=1e-12, atol=1e-14), d
