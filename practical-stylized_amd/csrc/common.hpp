@@ -52,6 +52,9 @@ struct Timer {
 
 }  // namespace nlpf
 
+struct nlpf_mlp_state;
+struct nlpf_world_state;
+
 struct nlpf_handle {
     nlpf::Config cfg;
     int device = 0;
@@ -68,6 +71,10 @@ struct nlpf_handle {
     // reusable device scratch
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // learned-cache and world-mode state, created on first use (owned by the handle, so
+    // different handles never share state)
+    nlpf_mlp_state* mlp = nullptr;
+    nlpf_world_state* world = nullptr;
 };
 
 #define NLPF_FAIL(h, code, msg)          \

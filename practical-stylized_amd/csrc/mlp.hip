@@ -841,13 +841,9 @@ struct nlpf_mlp_state {
 };
 
 namespace nlpf {
-static std::vector<std::pair<nlpf_handle*, nlpf_mlp_state*>> g_mlp;
 nlpf_mlp_state* mlp_state(nlpf_handle* h, bool create) {
-    for (auto& kv : g_mlp)
-        if (kv.first == h) return kv.second;
-    if (!create) return nullptr;
-    g_mlp.emplace_back(h, new nlpf_mlp_state());
-    return g_mlp.back().second;
+    if (!h->mlp && create) h->mlp = new nlpf_mlp_state();
+    return h->mlp;
 }
 static void mlp_free_train(nlpf_mlp_state* s) {
     void* ps[] = {s->act, s->dz, s->y, s->loss_part, s->loss};
@@ -861,17 +857,14 @@ static void mlp_free_train(nlpf_mlp_state* s) {
     s->cap = 0;
 }
 void mlp_release(nlpf_handle* h) {
-    for (size_t i = 0; i < g_mlp.size(); ++i)
-        if (g_mlp[i].first == h) {
-            nlpf_mlp_state* s = g_mlp[i].second;
-            mlp_free_train(s);
-            void* ps[] = {s->param, s->m, s->v, s->grad, s->w16, s->wt16, s->ema, s->ema_hat, s->w16_ema};
-            for (void* p : ps)
-                if (p) (void)hipFree(p);
-            delete s;
-            g_mlp.erase(g_mlp.begin() + i);
-            return;
-        }
+    nlpf_mlp_state* s = h->mlp;
+    if (!s) return;
+    mlp_free_train(s);
+    void* ps[] = {s->param, s->m, s->v, s->grad, s->w16, s->wt16, s->ema, s->ema_hat, s->w16_ema};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    delete s;
+    h->mlp = nullptr;
 }
 static constexpr int MLP_NW16 = MLP_NPARAM - (5 * MLP_HID + MLP_OUT);
 

@@ -656,21 +656,15 @@ struct nlpf_world_state {
 
 namespace nlpf {
 // workspace owned by the handle (allocated on first use, grown as needed)
-static std::vector<std::pair<nlpf_handle*, nlpf_world_state*>> g_world;
 static nlpf_world_state* world_state(nlpf_handle* h) {
-    for (auto& kv : g_world)
-        if (kv.first == h) return kv.second;
-    g_world.emplace_back(h, new nlpf_world_state());
-    return g_world.back().second;
+    if (!h->world) h->world = new nlpf_world_state();
+    return h->world;
 }
 void world_release(nlpf_handle* h) {
-    for (size_t i = 0; i < g_world.size(); ++i)
-        if (g_world[i].first == h) {
-            world_free(g_world[i].second->w);
-            delete g_world[i].second;
-            g_world.erase(g_world.begin() + i);
-            return;
-        }
+    if (!h->world) return;
+    world_free(h->world->w);
+    delete h->world;
+    h->world = nullptr;
 }
 
 template <typename T>
