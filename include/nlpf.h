@@ -8,7 +8,8 @@
  *
  * Conventions
  *  - every entry point returns an int status (NLPF_OK == 0); on failure
- *    nlpf_last_error(h) (or nlpf_last_error(NULL) for nlpf_create) holds a message;
+ *    nlpf_last_error(h) (or nlpf_last_error(NULL) for nlpf_create and
+ *    nlpf_write_image_host) holds a message;
  *  - buffers are device pointers owned by the caller, or allocated through
  *    nlpf_alloc (hipMalloc on the handle's device);
  *  - one HIP stream per handle; calls on one handle are ordered on that stream;
@@ -36,6 +37,7 @@ extern "C" {
 #define NLPF_E_HIP 4
 #define NLPF_E_UNSUPPORTED 5
 #define NLPF_E_NUMERIC 6
+#define NLPF_E_IO 7
 
 /* ---- storage precision of record planes ---- */
 #define NLPF_FP32 0
@@ -123,7 +125,8 @@ typedef struct nlpf_handle nlpf_handle;
  * and the device setup of lm_path_filtering::render (:1802-1813). */
 int nlpf_create(const char* config_json, nlpf_handle** out);
 int nlpf_destroy(nlpf_handle* h);
-/* Message of the last failing call on h; h == NULL gives the last nlpf_create failure. */
+/* Message of the last failing call on h; h == NULL gives the last failure of a call without a
+ * handle (nlpf_create, nlpf_write_image_host). */
 const char* nlpf_last_error(const nlpf_handle* h);
 /* Parsed configuration echoed back as JSON (for host-side mirrors and tests). */
 int nlpf_get_config_json(const nlpf_handle* h, char* buf, size_t cap);
@@ -236,8 +239,13 @@ typedef struct nlpf_gen_desc {
  * flags / new_flags: StylizationFlags bytes [L][N]; material: stylization material [L][N];
  * path_depth[t]: last depth with a vertex, -1 for a path whose camera ray missed.
  * The indirect record of depth d is produced by the filter of depth d+1 (kernel_propagate,
- * :973-998).  Output film_rgb [H][W][3] = film after kernel_splat + kernel_update_film
- * (:428-506, depth-0 re-stylization of the spp mean included); radiance_out (optional)
+ * :973-998).  film: with desc->film_accumulate == 0, [H][W][3] receives the pass's film after
+ * kernel_splat + kernel_update_film (:428-506, depth-0 re-stylization of the spp mean included),
+ * i.e. the mean of the pass's samples per pixel (0 where no path hit).  With film_accumulate != 0,
+ * film is a film buffer (NLPF_FILM_FLOATS, below) and the pass adds its samples to it as
+ * Film::add_sample does (one sample of weight 1 per path that hit, film.rs:187-222), so repeated
+ * passes accumulate as the reference's `repeats` loop does (:1629-1783); nlpf_film_resolve then gives
+ * the image.  Per pixel the pass's samples are added in branch order.  radiance_out (optional)
  * [L][N][3] = every vertex's filtered, stylized radiance (v.radiance). Config fields used:
  * max_depth, branch, radius_scale, reuse, vram (batching), smis, cluster,
  * no_first_hit_stylization, disable_flag_check, scene_bounds, seed, hash_table_size. */
@@ -245,10 +253,11 @@ typedef struct nlpf_gen_desc {
 typedef struct nlpf_world_desc {
     uint32_t width, height, branch, levels;
     float aabb_min[3], aabb_max[3]; /* scene AABB, used unless the config sets scene_bounds */
+    uint32_t film_accumulate;       /* 0: film is [H][W][3]; else a film buffer (see above) */
 } nlpf_world_desc;
 int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const float* records,
                    const uint8_t* flags, const uint8_t* new_flags, const uint16_t* material,
-                   const int32_t* path_depth, float* film_rgb, float* radiance_out);
+                   const int32_t* path_depth, float* film, float* radiance_out);
 
 /* Trace the procedural scene (below) into world-mode records, one path per (pixel, branch),
  * as kernel_mk_trace does (lm_path_filtering.rs:1062-1218, area-light NEE light/area.rs:50-131): camera ray, NEE to the quad light with its radiance stylized at
@@ -258,6 +267,49 @@ int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const float* rec
 int nlpf_generate_world_records(nlpf_handle* h, const nlpf_world_desc* desc,
                                 const nlpf_gen_desc* gen, float* records, uint8_t* flags,
                                 uint8_t* new_flags, uint16_t* material, int32_t* path_depth);
+
+/* ---- film (crates/akari_render/src/film.rs) ----
+ * A film buffer is one device buffer of NLPF_FILM_FLOATS(width, height) floats laid out as the
+ * reference's Film::data for the sRGB representation (film.rs:66-67, 88-93, 108-110):
+ *   [0, 3P)   pixels:  rgb of pixel i at 3i .. 3i+2  (P = width * height)
+ *   [3P, 6P)  splat:   rgb of pixel i at 3P + 3i ..
+ *   [6P, 7P)  weights: weight of pixel i at 6P + i
+ * Samples are (x, y) positions in pixels (xy[2n]), rgb[3n] and an optional weight[n] (NULL
+ * means 1); a sample lands in pixel (floor x, floor y) (Film::linear_index, film.rs:150-160).
+ * The reference adds with float atomics (film.rs:180-184, 210-219), so its sums depend on
+ * scheduling; here a pixel's samples are added in input order, onto the film's current values,
+ * so results are the sequential sums and repeat bit for bit.  Samples outside the film (or at a
+ * NaN position), which the reference only asserts against in debug builds, are skipped and
+ * counted in *dropped (optional).  width * height < 2^31, n < 2^31. */
+#define NLPF_FILM_FLOATS(w, h) (7ull * (uint64_t)(w) * (uint64_t)(h))
+/* Film::clear (film.rs:224-227): every value to 0. */
+int nlpf_film_clear(nlpf_handle* h, float* film, uint32_t width, uint32_t height);
+/* Film::add_sample (film.rs:187-222): c = remove_nan(rgb) * w; pixels += c, weights += w
+ * (remove_nan zeroes a colour with any NaN channel, color.rs:319-327). */
+int nlpf_film_add_samples(nlpf_handle* h, float* film, uint32_t width, uint32_t height,
+                          const float* xy, const float* rgb, const float* weight, uint64_t n,
+                          uint64_t* dropped);
+/* Film::add_splat (film.rs:162-186): c = remove_nan(rgb) * w, a NaN channel of c set to 0;
+ * splat += c. */
+int nlpf_film_add_splats(nlpf_handle* h, float* film, uint32_t width, uint32_t height,
+                         const float* xy, const float* rgb, const float* weight, uint64_t n,
+                         uint64_t* dropped);
+/* Film::copy_to_rgba_image (film.rs:120-147, 229-246): rgba[4P] (device) gets
+ * pixels / (weight == 0 ? 1 : weight) + splat * splat_scale, sRGB-encoded (color.rs:536-539)
+ * unless hdr, alpha 1. */
+int nlpf_film_resolve(nlpf_handle* h, const float* film, uint32_t width, uint32_t height,
+                      float splat_scale, int hdr, float* rgba);
+
+/* ---- image output (crates/akari_render/src/util/mod.rs:57-121) ----
+ * util::write_image: a path ending in ".exr" gets the R, G, B channels as 32-bit floats in an
+ * uncompressed scanline OpenEXR file; any other path gets an 8-bit RGB PNG of
+ * (linear_to_srgb(c) * 255).clamp(0, 255) truncated (color.rs:590-596, util/mod.rs:85-93).
+ * The parent directory is created.  rgba[4 * width * height] is row-major, top row first.
+ * nlpf_write_image reads rgba from the device (after the handle's earlier work); the _host
+ * variant reads host memory and needs no GPU. */
+int nlpf_write_image(nlpf_handle* h, const float* rgba, uint32_t width, uint32_t height,
+                     const char* path);
+int nlpf_write_image_host(const float* rgba, uint32_t width, uint32_t height, const char* path);
 
 /* ---- learned inner-expectation cache (radiance-cache MLP) ----
  * 84 inputs (clamped position, sin/cos of 2^k pi p for k = 0..11 per coordinate, normal,

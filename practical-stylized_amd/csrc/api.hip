@@ -103,6 +103,13 @@ using namespace nlpf;
 static std::mutex g_create_err_mu;
 static std::string g_create_err;
 
+namespace nlpf {
+void set_global_error(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(g_create_err_mu);
+    g_create_err = msg;
+}
+}  // namespace nlpf
+
 extern "C" {
 
 int nlpf_create(const char* config_json, nlpf_handle** out) {

@@ -93,6 +93,8 @@ struct nlpf_handle {
     } while (0)
 
 namespace nlpf {
+// error message of a call that has no handle, read by nlpf_last_error(NULL)
+void set_global_error(const std::string& msg);
 // grow-only scratch owned by the handle
 int ensure_scratch(nlpf_handle* h, size_t bytes);
 // record the last kernel failure (launch errors are sticky per stream)

@@ -477,12 +477,15 @@ __global__ void w_fill_kernel(float* p, uint64_t n, float v) {
     if (i < n) p[i] = v;
 }
 
-// kernel_splat + kernel_update_film + Film::add_sample (:428-506, film.rs:196-229) per pixel,
-// branches summed in order
+// kernel_splat + kernel_update_film + Film::add_sample (:428-506, film.rs:187-222) per pixel,
+// branches summed in order. accumulate == 0: film[3 npix] gets the mean of the pass's samples;
+// otherwise film is a 7-plane film buffer and the samples (weight 1 each) are added onto its
+// pixel and weight planes.
 __global__ void w_film_kernel(const float* rad0, const float* rec0, const int32_t* path_depth, const uint8_t* flags0,
                               const uint16_t* material0, uint64_t npix, uint32_t branch, uint64_t N,
                               const nlpf_style_node* nodes, const int32_t* roots, uint32_t n_materials,
-                              int stylize, float* film) {
+                              int stylize, int accumulate, float* film) {
+#pragma clang fp contract(off)
     const uint64_t px = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (px >= npix) return;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -495,6 +498,12 @@ __global__ void w_film_kernel(const float* rad0, const float* rec0, const int32_
     }
     const float scale = 1.0f / (float)branch;
     float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
+    if (accumulate) {
+        ax = film[3 * px];
+        ay = film[3 * px + 1];
+        az = film[3 * px + 2];
+        wsum = film[6 * npix + px];
+    }
     for (uint32_t b = 0; b < branch; ++b) {
         const uint64_t t = (uint64_t)b * npix + px;
         if (path_depth[t] < 0) continue;
@@ -511,6 +520,13 @@ __global__ void w_film_kernel(const float* rad0, const float* rec0, const int32_
         ay += L.y;
         az += L.z;
         wsum += 1.0f;
+    }
+    if (accumulate) {
+        film[3 * px] = ax;
+        film[3 * px + 1] = ay;
+        film[3 * px + 2] = az;
+        film[6 * npix + px] = wsum;
+        return;
     }
     film[3 * px] = wsum > 0.0f ? ax / wsum : 0.0f;
     film[3 * px + 1] = wsum > 0.0f ? ay / wsum : 0.0f;
@@ -917,7 +933,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     // level 0 radiance: when radiance_out is NULL, w.rad still holds depth 0 (last level filtered)
     hipLaunchKernelGGL(w_film_kernel, grid1(npix), dim3(256), 0, st, rad0, rec, path_depth, flags, material, npix,
                        desc->branch, N, h->d_nodes, h->d_roots, (uint32_t)h->h_roots.size(),
-                       (int)(!cfg.no_first_hit_stylization && !h->h_roots.empty()), film);
+                       (int)(!cfg.no_first_hit_stylization && !h->h_roots.empty()), (int)(desc->film_accumulate != 0),
+                       film);
     if ((rc = check_launch(h, "world film"))) return rc;
     rc = record_event(h, 1 + L);
     if (rc) return rc;

@@ -73,7 +73,8 @@ class _Stats(C.Structure):
 
 class _WorldDesc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("branch", C.c_uint32),
-                ("levels", C.c_uint32), ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3)]
+                ("levels", C.c_uint32), ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3),
+                ("film_accumulate", C.c_uint32)]
 
 
 class _MlpRunDesc(C.Structure):
@@ -89,6 +90,12 @@ class _MlpRunStats(C.Structure):
 
 
 WORLD_PLANES = 28
+NLPF_E_IO = 7
+
+
+def film_floats(width, height):
+    """Floats of a film buffer: | pixels rgb | splat rgb | weights | (film.rs:66-67, 108-110)."""
+    return 7 * int(width) * int(height)
 
 
 _lib = None
@@ -139,6 +146,15 @@ _SIGS = [
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_generate_world_records", C.c_int, [C.c_void_p, C.POINTER(_WorldDesc), C.POINTER(_GenDesc),
                                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("nlpf_film_clear", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("nlpf_film_add_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("nlpf_film_add_splats", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("nlpf_film_resolve", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
+                                    C.c_void_p]),
+    ("nlpf_write_image", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_char_p]),
+    ("nlpf_write_image_host", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_char_p]),
 ]
 
 
@@ -160,6 +176,19 @@ def load(path=LIB_PATH):
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def write_image_host(rgba, path):
+    """util::write_image (crates/akari_render/src/util/mod.rs:57-121) of a host image
+    rgba[H, W, 4]: ".exr" -> float RGB OpenEXR, otherwise 8-bit sRGB PNG. Needs no GPU."""
+    lib = load()
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    if rgba.ndim != 3 or rgba.shape[2] != 4:
+        raise ValueError("rgba must be [H, W, 4]")
+    H, W = rgba.shape[:2]
+    rc = lib.nlpf_write_image_host(rgba.ctypes.data_as(C.c_void_p), W, H, os.fsencode(path))
+    if rc != NLPF_OK:
+        raise NlpfError(rc, lib.nlpf_last_error(None).decode())
 
 
 def _ptr(x):
@@ -370,8 +399,9 @@ class Handle:
 
     # ---- world mode ----
     @staticmethod
-    def world_desc(W, H, branch, L, aabb_min=(-0.5, -0.5, -0.5), aabb_max=(0.5, 0.5, 0.5)):
-        return _WorldDesc(W, H, branch, L, (C.c_float * 3)(*aabb_min), (C.c_float * 3)(*aabb_max))
+    def world_desc(W, H, branch, L, aabb_min=(-0.5, -0.5, -0.5), aabb_max=(0.5, 0.5, 0.5), film_accumulate=0):
+        return _WorldDesc(W, H, branch, L, (C.c_float * 3)(*aabb_min), (C.c_float * 3)(*aabb_max),
+                          int(film_accumulate))
 
     def alloc_world(self, desc):
         n = desc.width * desc.height * desc.branch
@@ -401,20 +431,90 @@ class Handle:
                 "material": bufs["material"].download(np.uint16, L * n),
                 "path_depth": bufs["path_depth"].download(np.int32, n)}
 
-    def world_run(self, desc, bufs, want_radiance=False):
+    def world_run(self, desc, bufs, want_radiance=False, film=None):
+        """One world-mode pass. With desc.film_accumulate == 0 returns (film [H, W, 3], radiance);
+        otherwise `film` must be a film buffer (film_alloc) that the pass adds its samples to,
+        and the first element returned is None."""
         n = desc.width * desc.height * desc.branch
         L = desc.levels
-        film = self.alloc(desc.width * desc.height * 12)
+        own = film is None
+        if desc.film_accumulate and own:
+            raise ValueError("film_accumulate needs a film buffer")
+        if own:
+            film = self.alloc(desc.width * desc.height * 12)
         rad = self.alloc(L * n * 12) if want_radiance else None
         self._check(self.lib.nlpf_world_run(
             self.h, C.byref(desc), _ptr(bufs["records"]), _ptr(bufs["flags"]), _ptr(bufs["new_flags"]),
             _ptr(bufs["material"]), _ptr(bufs["path_depth"]), _ptr(film), _ptr(rad)))
-        f = film.download(np.float32).reshape(desc.height, desc.width, 3)
+        f = None
+        if own:
+            f = film.download(np.float32).reshape(desc.height, desc.width, 3)
+            film.free()
         r = rad.download(np.float32).reshape(L, n, 3) if want_radiance else None
-        film.free()
         if rad is not None:
             rad.free()
         return f, r
+
+    # ---- film (crates/akari_render/src/film.rs) ----
+    def film_alloc(self, W, H, clear=True):
+        f = self.alloc(4 * film_floats(W, H))
+        if clear:
+            self.film_clear(f, W, H)
+        return f
+
+    def film_clear(self, film, W, H):
+        self._check(self.lib.nlpf_film_clear(self.h, _ptr(film), W, H))
+
+    def _film_add(self, fn, film, W, H, xy, rgb, weight):
+        n = int(xy.nbytes // 8) if isinstance(xy, np.ndarray) else None
+        bufs = []
+
+        def dev(a, dt, k):
+            if a is None or isinstance(a, (DeviceBuffer, int)):
+                return a
+            b = self.to_device(np.ascontiguousarray(a, dt).reshape(-1, k))
+            bufs.append(b)
+            return b
+        if n is None:
+            raise TypeError("xy must be a host array [n, 2]")
+        d = C.c_uint64(0)
+        rc = fn(self.h, _ptr(film), W, H, _ptr(dev(xy, np.float32, 2)), _ptr(dev(rgb, np.float32, 3)),
+                _ptr(dev(weight, np.float32, 1)), n, C.byref(d))
+        for b in bufs:
+            b.free()
+        self._check(rc)
+        return d.value
+
+    def film_add_samples(self, film, W, H, xy, rgb, weight=None):
+        """Film::add_sample for host samples xy[n,2], rgb[n,3], weight[n] (None = 1); returns the
+        number of samples that fell outside the film."""
+        return self._film_add(self.lib.nlpf_film_add_samples, film, W, H, xy, rgb, weight)
+
+    def film_add_splats(self, film, W, H, xy, rgb, weight=None):
+        return self._film_add(self.lib.nlpf_film_add_splats, film, W, H, xy, rgb, weight)
+
+    def film_resolve(self, film, W, H, splat_scale=1.0, hdr=True, rgba=None):
+        """Film::copy_to_rgba_image; returns the host image [H, W, 4] unless a device rgba is given."""
+        own = rgba is None
+        if own:
+            rgba = self.alloc(16 * W * H)
+        self._check(self.lib.nlpf_film_resolve(self.h, _ptr(film), W, H, float(splat_scale), int(bool(hdr)),
+                                               _ptr(rgba)))
+        if not own:
+            return None
+        out = rgba.download(np.float32).reshape(H, W, 4)
+        rgba.free()
+        return out
+
+    def film_download(self, film, W, H):
+        """(pixels [H,W,3], splat [H,W,3], weights [H,W]) of a film buffer."""
+        a = film.download(np.float32, film_floats(W, H))
+        P = W * H
+        return a[:3 * P].reshape(H, W, 3), a[3 * P:6 * P].reshape(H, W, 3), a[6 * P:].reshape(H, W)
+
+    def write_image(self, rgba, W, H, path):
+        """util::write_image of a device image rgba (e.g. film_resolve's output buffer)."""
+        self._check(self.lib.nlpf_write_image(self.h, _ptr(rgba), W, H, os.fsencode(path)))
 
 
 # ---- stylization node builders ----

@@ -32,8 +32,9 @@ def test_every_declared_symbol_is_exported():
 def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include "nlpf.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){'
-                   'printf("%zu %zu %zu %zu %zu\\n", sizeof(nlpf_style_node), sizeof(nlpf_screen_desc),'
-                   ' sizeof(nlpf_gen_desc), sizeof(nlpf_stats), offsetof(nlpf_style_node, p));return 0;}\n')
+                   'printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(nlpf_style_node), sizeof(nlpf_screen_desc),'
+                   ' sizeof(nlpf_gen_desc), sizeof(nlpf_stats), offsetof(nlpf_style_node, p),'
+                   ' sizeof(nlpf_world_desc), offsetof(nlpf_world_desc, film_accumulate));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
@@ -43,6 +44,8 @@ def test_struct_layouts_match_header(tmp_path):
     assert sizes[2] == C.sizeof(N._GenDesc)
     assert sizes[3] == C.sizeof(N._Stats)
     assert sizes[4] == N.STYLE_NODE_DTYPE.fields["p"][1]
+    assert sizes[5] == C.sizeof(N._WorldDesc)
+    assert sizes[6] == N._WorldDesc.film_accumulate.offset
 
 
 @pytest.mark.parametrize("cfg,msg", [
