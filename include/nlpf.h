@@ -8,8 +8,8 @@
  *
  * Conventions
  *  - every entry point returns an int status (NLPF_OK == 0); on failure
- *    nlpf_last_error(h) (or nlpf_last_error(NULL) for nlpf_create and
- *    nlpf_write_image_host) holds a message;
+ *    nlpf_last_error(h) (or nlpf_last_error(NULL) for the calls without a handle:
+ *    nlpf_create, nlpf_style_from_scene, nlpf_write_image_host) holds a message;
  *  - buffers are device pointers owned by the caller, or allocated through
  *    nlpf_alloc (hipMalloc on the handle's device);
  *  - one HIP stream per handle; calls on one handle are ordered on that stream;
@@ -63,7 +63,7 @@ extern "C" {
  *
  * Parameter layout p[] per type:
  *   TOON       p[0]=color_band_width
- *   PALETTE    p[0]=indirect_scale p[1]=gamma p[2+3k..4+3k]=entry k rgb (count<=16)
+ *   PALETTE    p[0]=indirect_scale p[1]=gamma p[2+3k..4+3k]=entry k rgb (count<=18)
  *   SATURATION p[0]=power p[1]=indirect_scale; max_depth field
  *   REX        p[0]=blend
  *   EIGHT_BIT  p[0]=indirect_scale
@@ -86,8 +86,8 @@ extern "C" {
 #define NLPF_STY_SET_IDX 10
 
 #define NLPF_STY_MAX_PARAMS 56
-#define NLPF_STY_MAX_NODES 64
-#define NLPF_STY_MAX_MATERIALS 64
+#define NLPF_STY_MAX_NODES 1024
+#define NLPF_STY_MAX_MATERIALS 65536
 
 typedef struct nlpf_style_node {
     int32_t type;
@@ -126,7 +126,7 @@ typedef struct nlpf_handle nlpf_handle;
 int nlpf_create(const char* config_json, nlpf_handle** out);
 int nlpf_destroy(nlpf_handle* h);
 /* Message of the last failing call on h; h == NULL gives the last failure of a call without a
- * handle (nlpf_create, nlpf_write_image_host). */
+ * handle (nlpf_create, nlpf_style_from_scene, nlpf_write_image_host). */
 const char* nlpf_last_error(const nlpf_handle* h);
 /* Parsed configuration echoed back as JSON (for host-side mirrors and tests). */
 int nlpf_get_config_json(const nlpf_handle* h, char* buf, size_t cap);
@@ -150,6 +150,19 @@ int nlpf_sync(nlpf_handle* h);
  * material_roots[m] = root node of material m's stylization tree (-1 = unstylized). */
 int nlpf_set_style(nlpf_handle* h, const nlpf_style_node* nodes, uint32_t n_nodes,
                    const int32_t* material_roots, uint32_t n_materials);
+
+/* Build a style table from a reference scene file (akari_scenegraph JSON, the scene format the
+ * reference loads; akari_scenegraph/src/scene.rs:121-129, shader.rs:24-28): material m's root is
+ * the lowering of the shader graph of the material named materials[m]. Lowering follows the SVM
+ * compiler (svm/compiler.rs:337-477), starting from the graph's output node:
+ *  - sty_* nodes become nodes of this table, and their inputs must be constant
+ *    (float / rgb / float3 nodes);
+ *  - any other surface node ends the chain as the identity;
+ *  - identical subtrees share one node.
+ * On entry *n_nodes is the capacity of nodes[]; on return it is the number written. Host only
+ * (no GPU, no handle); errors via nlpf_last_error(NULL). Pass the result to nlpf_set_style. */
+int nlpf_style_from_scene(const char* scene_json, const char* const* materials, uint32_t n_materials,
+                          nlpf_style_node* nodes, uint32_t* n_nodes, int32_t* material_roots);
 
 /* Apply SurfaceClosure::apply_stylization (svm/surface/mod.rs:1824-1842) of material
  * `material[i]` (or material 0 when material == NULL) at path depth depth[i] (or `depth0`

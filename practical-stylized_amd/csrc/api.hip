@@ -224,8 +224,8 @@ int nlpf_sync(nlpf_handle* h) {
 int nlpf_set_style(nlpf_handle* h, const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* roots,
                    uint32_t n_materials) {
     if (!h) return NLPF_E_INVALID_ARG;
-    if (n_nodes > NLPF_STY_MAX_NODES) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many stylization nodes (max 64)");
-    if (n_materials > NLPF_STY_MAX_MATERIALS) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many materials (max 64)");
+    if (n_nodes > NLPF_STY_MAX_NODES) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many stylization nodes (max 1024)");
+    if (n_materials > NLPF_STY_MAX_MATERIALS) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many materials (max 65536)");
     if ((n_nodes && !nodes) || (n_materials && !roots)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "NULL style table");
     std::vector<nlpf_style_node> nv(nodes, nodes + n_nodes);
     for (uint32_t i = 0; i < n_nodes; ++i) {
@@ -236,8 +236,8 @@ int nlpf_set_style(nlpf_handle* h, const nlpf_style_node* nodes, uint32_t n_node
             NLPF_FAIL(h, NLPF_E_INVALID_ARG, "node index out of range at node " + std::to_string(i));
         if ((n.type == NLPF_STY_ENABLE_IF || n.type == NLPF_STY_SET_IDX) && n.idx > 3)
             NLPF_FAIL(h, NLPF_E_INVALID_ARG, "style index must be 0..3 at node " + std::to_string(i));
-        if (n.type == NLPF_STY_PALETTE && n.count > 16)
-            NLPF_FAIL(h, NLPF_E_INVALID_ARG, "palette holds at most 16 entries");
+        if (n.type == NLPF_STY_PALETTE && n.count > 18)
+            NLPF_FAIL(h, NLPF_E_INVALID_ARG, "palette holds at most 18 entries");
         if (n.type == NLPF_STY_RAMP) {
             if (n.count == 0 || n.count > 16) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "ramp needs 1..16 stops");
             for (uint32_t k = 1; k < n.count; ++k)

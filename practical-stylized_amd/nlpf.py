@@ -155,6 +155,8 @@ _SIGS = [
                                     C.c_void_p]),
     ("nlpf_write_image", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_char_p]),
     ("nlpf_write_image_host", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_char_p]),
+    ("nlpf_style_from_scene", C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_void_p,
+                                        C.POINTER(C.c_uint32), C.c_void_p]),
 ]
 
 
@@ -189,6 +191,30 @@ def write_image_host(rgba, path):
     rc = lib.nlpf_write_image_host(rgba.ctypes.data_as(C.c_void_p), W, H, os.fsencode(path))
     if rc != NLPF_OK:
         raise NlpfError(rc, lib.nlpf_last_error(None).decode())
+
+
+def style_from_scene(scene, materials=None, capacity=1024):
+    """Style table of a reference scene file (akari_scenegraph JSON): returns (nodes, roots,
+    materials), roots[m] being the root of the material named materials[m]. `scene` is a path
+    or the JSON text; `materials` defaults to every material of the scene in ascending name
+    order (the order of the reference's Collection, a BTreeMap). Needs no GPU."""
+    lib = load()
+    if isinstance(scene, (bytes, str)) and not str(scene).lstrip().startswith("{"):
+        with open(scene, "rb") as f:
+            text = f.read()
+    else:
+        text = scene.encode() if isinstance(scene, str) else bytes(scene)
+    if materials is None:
+        materials = sorted(json.loads(text)["materials"].keys(), key=lambda k: k.encode())
+    names = (C.c_char_p * max(1, len(materials)))(*[m.encode() for m in materials])
+    nodes = np.zeros(capacity, STYLE_NODE_DTYPE)
+    roots = np.zeros(max(1, len(materials)), np.int32)
+    n = C.c_uint32(capacity)
+    rc = lib.nlpf_style_from_scene(text, names, len(materials), nodes.ctypes.data_as(C.c_void_p), C.byref(n),
+                                   roots.ctypes.data_as(C.c_void_p))
+    if rc != NLPF_OK:
+        raise NlpfError(rc, lib.nlpf_last_error(None).decode())
+    return nodes[:n.value].copy(), roots[:len(materials)].copy(), list(materials)
 
 
 def _ptr(x):
@@ -575,7 +601,7 @@ def palette_hex(entries):
 
 def palette(colors, gamma=1.0, indirect_scale=1.0, inner=-1, depth_mask=ALL_DEPTHS):
     colors = np.asarray(colors, np.float32).reshape(-1, 3)
-    assert len(colors) <= 16
+    assert len(colors) <= 18
     return node("PALETTE", inner=inner, depth_mask=depth_mask, count=len(colors),
                 params=np.concatenate([[indirect_scale, gamma], colors.ravel()]))
 

@@ -115,3 +115,36 @@ def test_style_table_validation(h):
         h.set_style(N.style_table([N.ramp([0.5, 0.2], [0.1, 0.2])]), [0])  # descending stops
     with pytest.raises(N.NlpfError):
         h.set_style(N.style_table([N.toon(0.0)]), [0])
+
+
+def test_reference_scene_style_table(h):
+    """The style table imported from the reference's stylized scene (veach-ajar-sty-v2), over all
+    33 materials, random style flags and depths. Outputs within 1e-5 of the oracle, except where a
+    quantiser (toon band, palette entry) flips under a 4-ulp change of the input, which is then
+    checked against the oracle at that perturbed input."""
+    import os
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "veach_ajar_sty_materials.json")
+    nodes, roots, mats = N.style_from_scene(fixture)
+    rng = np.random.default_rng(7)
+    n = 20000
+    x = (rng.random((n, 3)) * rng.choice([0.2, 1.0, 3.0], (n, 1))).astype(np.float32)
+    flags = rng.choice([0, N.FLAG_STYLE2, N.FLAG_STYLE3, N.FLAG_STYLE4, N.FLAG_STYLE2 | N.FLAG_STYLE4],
+                       n).astype(np.uint8)
+    material = rng.integers(0, len(mats), n).astype(np.uint16)
+    depth = rng.integers(0, 4, n).astype(np.uint32)
+    h.set_style(nodes, roots)
+    g = h.stylize(x, depth=depth, flags=flags, material=material)
+    xo = x.astype(np.float64)
+    o = O.stylize(nodes, roots, xo, depth=depth, flags=flags, material=material)
+    bad = pixel_rel_err(g, o) > 1e-5
+    assert bad.mean() < 0.01
+    idx = np.flatnonzero(bad)
+    explained = np.zeros(len(idx), bool)
+    for e in (4.0, -4.0):
+        xp = (x[idx] * np.float32(1 + e * 2.0 ** -24)).astype(np.float64)
+        op = O.stylize(nodes, roots, xp, depth=depth[idx], flags=flags[idx], material=material[idx])
+        explained |= pixel_rel_err(g[idx], op) <= 1e-5
+    assert explained.all(), idx[~explained][:5]
+    # materials without stylization pass through
+    plain = np.isin(material, [i for i, r in enumerate(roots) if r < 0])
+    assert plain.any() and np.array_equal(g[plain], x[plain])
