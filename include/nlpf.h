@@ -9,7 +9,8 @@
  * Conventions
  *  - every entry point returns an int status (NLPF_OK == 0); on failure
  *    nlpf_last_error(h) (or nlpf_last_error(NULL) for the calls without a handle:
- *    nlpf_create, nlpf_style_from_scene, nlpf_write_image_host) holds a message;
+ *    nlpf_create, nlpf_style_from_scene, nlpf_write_world_scene, nlpf_write_image_host)
+ *    holds a message;
  *  - buffers are device pointers owned by the caller, or allocated through
  *    nlpf_alloc (hipMalloc on the handle's device);
  *  - one HIP stream per handle; calls on one handle are ordered on that stream;
@@ -126,7 +127,7 @@ typedef struct nlpf_handle nlpf_handle;
 int nlpf_create(const char* config_json, nlpf_handle** out);
 int nlpf_destroy(nlpf_handle* h);
 /* Message of the last failing call on h; h == NULL gives the last failure of a call without a
- * handle (nlpf_create, nlpf_style_from_scene, nlpf_write_image_host). */
+ * handle (nlpf_create, nlpf_style_from_scene, nlpf_write_world_scene, nlpf_write_image_host). */
 const char* nlpf_last_error(const nlpf_handle* h);
 /* Parsed configuration echoed back as JSON (for host-side mirrors and tests). */
 int nlpf_get_config_json(const nlpf_handle* h, char* buf, size_t cap);
@@ -323,6 +324,27 @@ int nlpf_film_resolve(nlpf_handle* h, const float* film, uint32_t width, uint32_
 int nlpf_write_image(nlpf_handle* h, const float* rgba, uint32_t width, uint32_t height,
                      const char* path);
 int nlpf_write_image_host(const float* rgba, uint32_t width, uint32_t height, const char* path);
+
+/* ---- the procedural scene as a reference scene file ----
+ * Writes the scene that nlpf_generate_world_records traces as an akari_scenegraph scene
+ * (akari_scenegraph/src/scene.rs:121-129, 368-383): `path` gets the JSON, and "<stem>.bin"
+ * next to it gets the one buffer it names (resolved relative to the JSON, scene.rs:641-690).
+ *  - Geometry is triangle meshes, one instance per material: exact quads for the walls and the
+ *    light, and UV spheres of sphere_segments segments (even, 4..4096) with analytic vertex
+ *    normals.
+ *  - Materials are named "mat<m>_<name>", so that they sort in material-id order. Each holds a
+ *    diffuse BSDF of the generator's reflectance for `gen` (the light instead holds an emission
+ *    node of colour 1 and strength 15), wrapped in the sty_* nodes lifted from the style table
+ *    (nodes, roots); nlpf_style_from_scene reads the same table back. Hue shift, ramp and
+ *    depth-gated nodes have no reference counterpart: NLPF_E_UNSUPPORTED.
+ *  - The camera is the generator's pinhole, in the reference's PerspectiveCamera conventions
+ *    (camera/mod.rs:130-152). Rendering the file with a box pixel filter of radius 1 matches
+ *    the generator's jitter.
+ * Host only; errors via nlpf_last_error(NULL). No reference counterpart (the reference's scenes
+ * come from its Blender exporter). */
+int nlpf_write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
+                           const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* material_roots,
+                           uint32_t n_materials, uint32_t sphere_segments);
 
 /* ---- learned inner-expectation cache (radiance-cache MLP) ----
  * 84 inputs (clamped position, sin/cos of 2^k pi p for k = 0..11 per coordinate, normal,

@@ -204,15 +204,7 @@ __device__ __forceinline__ int mat_root(const WGenArgs& g, int m) {
     return (g.roots && (uint32_t)m < g.n_materials) ? g.roots[m] : -1;
 }
 
-__device__ v3<float> material_albedo(int m, uint64_t key, float lo, float hi) {
-    Pcg32 r = pcg_seq_offset(0xA1BED0ull + (uint64_t)m, key);
-    const float a = lo + (hi - lo) * pcg_uniform(r);
-    const float b = lo + (hi - lo) * pcg_uniform(r);
-    const float c = lo + (hi - lo) * pcg_uniform(r);
-    return v3<float>{a, b, c};
-}
-
-#define LIGHT_EMISSION 15.0f
+#define LIGHT_EMISSION NLPF_LIGHT_EMISSION
 #define LIGHT_AREA (4.0f * SceneConst::lh * SceneConst::lh)
 
 __global__ void gen_world_kernel(WGenArgs g) {
@@ -262,7 +254,8 @@ __global__ void gen_world_kernel(WGenArgs g) {
         (void)u0;
         const float b1 = pcg_uniform(rng), b2 = pcg_uniform(rng);
         bool cont = false;
-        if (d + 1 < g.L) {
+        // the light is an emission-only surface: no lobe to sample, so the path ends there
+        if (d + 1 < g.L && mat != NLPF_MAT_LIGHT) {
             // next-event estimation to the quad light (sample_light :1260-1316, area.rs:51-110)
             const v3<float> lp{-SceneConst::lh + 2.0f * SceneConst::lh * u1, SceneConst::ly,
                                -SceneConst::lh + 2.0f * SceneConst::lh * u2};

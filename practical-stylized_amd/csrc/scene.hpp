@@ -40,6 +40,20 @@ NLPF_HD float pcg_uniform_open0(Pcg32& g) { return (float)((pcg_next(g) >> 8) + 
 #define NLPF_SCENE_SKY 100.0f
 #define NLPF_MAT_LIGHT 7
 #define NLPF_MAT_SKY 8
+#define NLPF_SCENE_MATERIALS 8
+// radiance of the quad light's lower face (emission colour 1, strength 15)
+#define NLPF_LIGHT_EMISSION 15.0f
+
+// diffuse reflectance of material m: three U[lo, hi) draws from the stream of the material,
+// keyed by the scene seed; the light is a pure emitter (reflectance 0)
+NLPF_HD v3<float> material_albedo(int m, uint64_t key, float lo, float hi) {
+    if (m == NLPF_MAT_LIGHT) return v3<float>{0.0f, 0.0f, 0.0f};
+    Pcg32 r = pcg_seq_offset(0xA1BED0ull + (uint64_t)m, key);
+    const float a = lo + (hi - lo) * pcg_uniform(r);
+    const float b = lo + (hi - lo) * pcg_uniform(r);
+    const float c = lo + (hi - lo) * pcg_uniform(r);
+    return v3<float>{a, b, c};
+}
 
 struct Hit {
     float t;

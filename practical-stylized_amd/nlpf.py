@@ -157,6 +157,8 @@ _SIGS = [
     ("nlpf_write_image_host", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_char_p]),
     ("nlpf_style_from_scene", C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_void_p,
                                         C.POINTER(C.c_uint32), C.c_void_p]),
+    ("nlpf_write_world_scene", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_GenDesc), C.c_void_p,
+                                         C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]),
 ]
 
 
@@ -215,6 +217,25 @@ def style_from_scene(scene, materials=None, capacity=1024):
     if rc != NLPF_OK:
         raise NlpfError(rc, lib.nlpf_last_error(None).decode())
     return nodes[:n.value].copy(), roots[:len(materials)].copy(), list(materials)
+
+
+SCENE_MATERIALS = ["floor", "ceiling", "back", "left", "right", "sphere0", "sphere1", "light"]
+
+
+def write_world_scene(path, W, H, seed=1, albedo_lo=0.2, albedo_hi=0.8, nodes=None, roots=(),
+                      sphere_segments=128):
+    """The procedural scene of generate_world_records as a reference scene file (JSON + .bin).
+    Needs no GPU."""
+    lib = load()
+    g = _GenDesc(seed, 0.0, 0.0, 0.0, 0.0, albedo_lo, albedo_hi)
+    nodes = np.zeros(0, STYLE_NODE_DTYPE) if nodes is None else np.ascontiguousarray(nodes, STYLE_NODE_DTYPE)
+    roots = np.ascontiguousarray(roots, np.int32)
+    rc = lib.nlpf_write_world_scene(os.fsencode(path), W, H, C.byref(g),
+                                    nodes.ctypes.data_as(C.c_void_p) if len(nodes) else None, len(nodes),
+                                    roots.ctypes.data_as(C.c_void_p) if len(roots) else None, len(roots),
+                                    sphere_segments)
+    if rc != NLPF_OK:
+        raise NlpfError(rc, lib.nlpf_last_error(None).decode())
 
 
 def _ptr(x):
