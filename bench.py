@@ -21,10 +21,15 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 METRIC = "filtered path-vertices/s @1080p×16spp, plain vs FP16+MLP; HBM % of peak; L2 err"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# per neighbour pair in the shipped kernel: |dp|^2 7 (3 sub, mul, 2 fma), normal term 8
-# (3 fma, fma), luminance 3 (sub, fma), exp 1, accumulate 7 (add, 3 fma) = 17 instructions
+# algorithmic flops per neighbour pair: |dp|^2 7 (3 sub, mul, 2 fma), normal term 8 (3 fma,
+# fma), luminance 3 (sub, fma), exp 1, accumulate 7 (add, 3 fma)
 FLOPS_PER_PAIR = 27.0
 FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per cycle (the build emits no packed FP32)
+# VALU issue slots per pair of the shipped kernel's inner loop (probes/isa_mix.py on
+# screen_level_kernel<float,float,7,4,2>: 1105 VALU per 60 pairs, one v_exp_f32 each), with
+# v_exp_f32 at half rate (probes/valu_rate.hip)
+ISSUE_SLOTS_PER_PAIR = 19.42
+VALU_ISSUE_PEAK = 256 * 4 * 16 * 2.4e9  # lane-instructions/s: 256 CUs x 4 SIMD16 at 2.4 GHz
 
 
 def log(msg):
@@ -398,6 +403,10 @@ def main():
                      "algorithmic_bytes_per_launch": float(np.mean(lb32)),
                      "avg_launch_ms": float(lv32.mean()),
                      "frac_of_nonpacked_fp32_peak": tflops / FP32_NONPACKED_TFLOPS,
+                     "valu_issue": {"slots_per_pair": ISSUE_SLOTS_PER_PAIR,
+                                    "achieved": pairs / (lv32.sum() / 1e3) * ISSUE_SLOTS_PER_PAIR,
+                                    "peak": VALU_ISSUE_PEAK, "unit": "lane-instructions/s",
+                                    "frac": pairs / (lv32.sum() / 1e3) * ISSUE_SLOTS_PER_PAIR / VALU_ISSUE_PEAK},
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved_gbs / HBM_PEAK_GBS}},
     }
