@@ -166,3 +166,27 @@ def test_invalid_screen_arguments(h):
     desc = h.screen_desc(8, 8, 1, 1, N.FP16, 3)
     with pytest.raises(N.NlpfError):  # FP16 without anchors
         h.screen_run(desc, 1234, None, 5678, None)
+
+
+@pytest.mark.parametrize("radius,W,H,S,L", [(7, 5, 3, 1, 2), (7, 1, 1, 2, 2), (2, 33, 9, 2, 3), (7, 66, 9, 1, 2),
+                                            (5, 129, 17, 1, 2)])
+def test_small_and_ragged_images(h, radius, W, H, S, L):
+    """Images narrower or shorter than the window, a single pixel, odd widths and tiles that
+    straddle the right and bottom edges, FP32 and FP16, against the oracle."""
+    table, root = ramp_hue_table(2)
+    h.set_style(table, [root])
+    h.set_sigmas(*SIG)
+    desc, rec, _ = gen_screen(h, W, H, S, L, N.FP32, seed=W + H, radius=radius)
+    film, lv = run_screen(h, desc, rec)
+    host = host_records_fp32(rec, W, H, S, L)
+    of, olv = O.screen_run(host, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+    check_levels(lv, olv, 1e-5)
+    assert pixel_rel_err(film, of).max() <= 1e-5
+    rec.free()
+    desc, rec, anc = gen_screen(h, W, H, S, L, N.FP16, seed=W + H, radius=radius)
+    film, lv = run_screen(h, desc, rec, anc)
+    dec = decode_fp16(rec, anc, W, H, S, L)
+    of, olv = O.screen_run(dec, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+    check_levels(lv, olv, 1e-5)
+    rec.free()
+    anc.free()

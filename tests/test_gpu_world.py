@@ -164,3 +164,32 @@ def test_world_cluster_mode_matches_oracle(kernel):
     assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
     sens = sensitivity(host, W, H, branch, L, cfg, None, (), ofilm, orad)
     check(f1, r1, ofilm, orad, host, L, sens=sens)
+
+
+def test_world_single_level_matches_oracle():
+    """max_depth 0: one level, no propagation, only the first hits are filtered."""
+    W, H, branch, L = 40, 30, 3, 1
+    cfg = {"type": "pf", "max_depth": 0, "branch": branch, "cluster": False, "radius_scale": [2.0],
+           "reuse": [1 << 16]}
+    nodes, roots = styles()
+    film, rad, host, st = run_world(cfg, W, H, branch, L, nodes, roots, seed=12)
+    ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots)
+    assert st["neighbors"][0] == int(onbr[0]) > 0
+    check(film, rad, ofilm, orad, host, L)
+
+
+def test_world_pass_without_hits():
+    """Every camera ray missed: no points at any depth, an all-zero film and no NaN."""
+    W, H, branch, L = 16, 8, 2, 3
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "cluster": True}
+    n = W * H * branch
+    host = {"records": np.zeros(L * 28 * n, np.float32), "flags": np.zeros(L * n, np.uint8),
+            "new_flags": np.zeros(L * n, np.uint8), "material": np.zeros(L * n, np.uint16),
+            "path_depth": np.full(n, -1, np.int32)}
+    with N.Handle(cfg) as h:
+        desc = h.world_desc(W, H, branch, L, AMIN, AMAX)
+        bufs = h.upload_world(desc, host)
+        film, rad = h.world_run(desc, bufs, want_radiance=True)
+        st = h.stats()
+    assert (film == 0).all() and (rad == 0).all()
+    assert st["vertices"] == 0 and all(v == 0 for v in st["neighbors"][:L])
