@@ -19,17 +19,9 @@ namespace nlpf {
 
 #define WP NLPF_WORLD_PLANES
 constexpr float kInvPi = 0.318309886183790671538f;
-
-// per-path reuse payload, gathered once per level so neighbour reads are 16-byte loads
-struct __align__(16) NbrRec {
-    float4 dwi_pdf;   // direct.wi, direct.pdf
-    float4 dirr_mis;  // direct.irradiance, direct.mis
-    float4 iwi_mis;   // indirect.wi, indirect.mis
-    float4 iirr;      // indirect.irradiance, flags-equal key in w (as bits)
-    float4 ns_wox;    // ns, wo.x
-    float4 ng_woy;    // ng, wo.y
-    float4 woz;       // wo.z, -, -, -
-};
+}  // namespace nlpf
+#include "world_bsdf.hpp"
+namespace nlpf {
 
 struct Grid {
     float pmin[3], pmax[3];
@@ -54,37 +46,10 @@ __device__ __forceinline__ void get_cell(const Grid& g, float px, float py, floa
     }
 }
 
-// dot product without fused multiply-adds: the reference decides hemisphere and leak tests
-// in plain FP32, so every sign test below rounds exactly like that arithmetic
-__device__ __forceinline__ float dot_rn(float3 a, float3 b) {
-    return __fadd_rn(__fadd_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)), __fmul_rn(a.z, b.z));
-}
-// SurfaceClosure::check_wo_wi_valid (svm/surface/mod.rs:1701-1716)
-__device__ __forceinline__ bool wo_wi_valid(float3 ns, float3 ng, float3 wo, float3 wi) {
-    auto sgn = [](float x) { return x > 0.0f ? 1.0f : -1.0f; };
-    const float flipped = sgn(dot_rn(ng, ns));
-    return (sgn(flipped * dot_rn(wo, ns)) * sgn(dot_rn(wo, ng)) > 0.0f) &&
-           (sgn(flipped * dot_rn(wi, ns)) * sgn(dot_rn(wi, ng)) > 0.0f);
-}
-// DiffuseBsdf::evaluate_impl (svm/surface/diffuse.rs:22-41) through SurfaceClosure::evaluate
-// (mod.rs:1725-1744): returns the pdf; fcos = |cos_i| / pi (multiply by R for f*cos)
-__device__ __forceinline__ float diffuse_eval(float3 ns, float3 ng, float3 wo, float3 wi, float& fcos) {
-    fcos = 0.0f;
-    if (!wo_wi_valid(ns, ng, wo, wi)) return 0.0f;
-    const float co = dot_rn(wo, ns);
-    const float ci = dot_rn(wi, ns);
-    if (!(__fmul_rn(co, ci) >= 0.0f)) return 0.0f;
-    // |cos| near grazing is a cancellation-prone difference of products: take its magnitude
-    // from an FP64 dot (the FP32 value above only decides the hemisphere, as in the reference)
-    const double cd = (double)wi.x * ns.x + (double)wi.y * ns.y + (double)wi.z * ns.z;
-    const float pdf = (float)(fabs(cd) * 0.318309886183790671538);
-    fcos = pdf;
-    return pdf;
-}
-
 // |d| without fused multiply-adds, so the radius test rounds exactly like plain IEEE code
 __device__ __forceinline__ float dist3(float x, float y, float z) {
-    return sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
+#pragma clang fp contract(off)
+    return sqrtf((x * x + y * y) + z * z);
 }
 
 template <typename T>
@@ -164,7 +129,9 @@ struct FilterArgs {
     Grid g;
     uint32_t K;
     uint32_t* nbuf;
+    uint32_t stride;  // list slots per vertex in nbuf
     Pcg32* rng;
+    int use_rng;      // 0 when no vertex of the launch can exceed the reuse cap: no draws
     const nlpf_style_node* nodes;
     const int32_t* roots;
     uint32_t n_materials;
@@ -189,8 +156,8 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
     const float3 le = make_float3(pl(a.rec, N, 15, c), pl(a.rec, N, 16, c), pl(a.rec, N, 17, c));
     const uint32_t myflags = a.new_flags[c];
     const uint32_t mycluster = a.cluster_id ? a.cluster_id[c] : 0u;
-    Pcg32 rng = a.rng[tl];
-    uint32_t* buf = a.nbuf + (uint64_t)tl * a.K;
+    Pcg32 rng = a.use_rng ? a.rng[tl] : Pcg32{0ull, 1ull};
+    uint32_t* buf = a.nbuf + (uint64_t)tl * a.stride;
 
     float3 q = p;
     if (a.cluster_id) q = make_float3(pl(a.rec, N, 0, mycluster), pl(a.rec, N, 1, mycluster), pl(a.rec, N, 2, mycluster));
@@ -249,15 +216,11 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
         const float4 ir = r.iirr;
         const float3 reused = make_float3(R.x * fcos * ir.x * iw.w, R.y * fcos * ir.y * iw.w, R.z * fcos * ir.z * iw.w);
         if (a.smis) {
-            float sum_pdf = 0.0f, sc = 0.0f;
-            for (uint32_t m = 0; m < used; ++m) {
-                const NbrRec& o = a.pk[buf[m]];
-                const float4 a0 = o.ns_wox, a1 = o.ng_woy, a2 = o.woz;
-                float fc;
-                kahan_add(sum_pdf, sc,
-                          diffuse_eval(make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
-                                       make_float3(a0.w, a1.w, a2.x), wi, fc));
-            }
+            // sum_m pdf_m(wi) over every reused neighbour m, in order, in FP64
+            const double wx = wi.x, wy = wi.y, wz = wi.z;
+            double sum_cos = 0.0;
+            for (uint32_t m = 0; m < used; ++m) sum_cos += smis_term(smis_rec(a.pk[buf[m]]), wi, wx, wy, wz);
+            const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);
             kahan_add(isum[0], ic[0], reused.x / sum_pdf);
             kahan_add(isum[1], ic[1], reused.y / sum_pdf);
             kahan_add(isum[2], ic[2], reused.z / sum_pdf);
@@ -278,21 +241,23 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
     a.rad_out[3 * (uint64_t)c] = L.x;
     a.rad_out[3 * (uint64_t)c + 1] = L.y;
     a.rad_out[3 * (uint64_t)c + 2] = L.z;
-    a.rng[tl] = rng;
+    if (a.use_rng) a.rng[tl] = rng;
 }
 
 // One wavefront per vertex (4 per workgroup).
 //   gather   the 27-cell walk is uniform across the wave; the lanes test 64 bucket entries at a
 //            time, a ballot and a prefix count append the accepted ones in bucket order, and
 //            lane 0 replays the reservoir draws (:813-821) for the entries beyond the cap, in
-//            order, so the RNG stream is the sequential one; the neighbour list lives in the
-//            wave's LDS slice when the cap fits (WF_CAP), else in the global list buffer;
-//   reuse    lanes take neighbours k = lane, lane + 64, ...; the SMIS denominator of neighbour
-//            k loops over every reused neighbour m in order with a wave-uniform index;
+//            order, so the RNG stream is the sequential one; the first WF_CAP list entries
+//            live in the wave's LDS slice, the rest in the global list buffer;
+//   reuse    lanes take neighbours k = lane, lane + 64, ...; for the SMIS denominators the
+//            reused neighbours' pdf inputs are staged in LDS 64 at a time, and every lane sums
+//            pdf_m(wi_k) over them in order, in FP64, with a wave-uniform index m;
 //   reduce   per-lane Kahan sums, then a fixed shuffle tree (deterministic).
 constexpr uint32_t WF_CAP = 1024;
 __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     __shared__ uint32_t lbuf[4][WF_CAP];
+    __shared__ SmisRec sbuf[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t tl = blockIdx.x * 4 + wv;
     if (tl >= a.count) return;  // uniform per wave; no workgroup barrier below
@@ -306,12 +271,12 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     const float3 le = make_float3(pl(a.rec, N, 15, c), pl(a.rec, N, 16, c), pl(a.rec, N, 17, c));
     const uint32_t myflags = a.new_flags[c];
     const uint32_t mycluster = a.cluster_id ? a.cluster_id[c] : 0u;
-    Pcg32 rng = a.rng[tl];  // advanced by lane 0 only
-    const bool in_lds = a.K <= WF_CAP;
-    uint32_t* gbuf = a.nbuf + (uint64_t)tl * a.K;
+    Pcg32 rng = a.use_rng ? a.rng[tl] : Pcg32{0ull, 1ull};  // advanced by lane 0 only
+    uint32_t* gbuf = a.nbuf + (uint64_t)tl * a.stride;
     uint32_t* lb = lbuf[wv];
+    SmisRec* sb = sbuf[wv];
     auto put = [&](uint32_t i, uint32_t j) {
-        if (in_lds)
+        if (i < WF_CAP)
             lb[i] = j;
         else
             gbuf[i] = j;
@@ -323,55 +288,77 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     get_cell(a.g, q.x, q.y, q.z, cell);
     uint32_t cnt = 0;  // wave-uniform
     const uint64_t below = (1ull << lane) - 1ull;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
-                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)a.g.res[0] || cy >= (int)a.g.res[1] || cz >= (int)a.g.res[2])
-                    continue;
-                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
-                const uint32_t e0 = a.bstart[b], e1 = a.bend[b];
-                for (uint32_t base = e0; base < e1; base += 64) {
-                    const uint32_t e = base + (uint32_t)lane;
-                    bool ok = e < e1;
-                    uint32_t j = 0;
-                    if (ok) {
-                        const float4 qe = a.pos_sorted[e];
-                        j = __float_as_uint(qe.w);
-                        ok = dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius;
-                        if (ok && a.cluster_id && a.cluster_id[j] != mycluster) ok = false;
-                        if (ok && a.flag_check && (uint32_t)a.new_flags[j] != myflags) ok = false;
-                    }
-                    const uint64_t mask = __ballot(ok);
-                    const uint32_t total = (uint32_t)__popcll(mask);
-                    const uint32_t pos = cnt + (uint32_t)__popcll(mask & below);
-                    if (ok && pos < a.K) put(pos, j);  // the slots before the cap fill in order
-                    if (cnt + total > a.K) {
-                        // reservoir over the cap, entry by entry in bucket order (after the
-                        // in-order stores above have completed)
-                        if (!in_lds) __threadfence_block();
-                        uint64_t rest = mask;
-                        uint32_t at = cnt;
-                        while (rest) {
-                            const int src = __ffsll((long long)rest) - 1;
-                            rest &= rest - 1;
-                            const uint32_t jj = (uint32_t)__shfl((int)j, src);
-                            if (at >= a.K && lane == 0) {
-                                const uint32_t r = pcg_next(rng) % (at + 1);
-                                if (r < a.K) put(r, jj);
-                            }
-                            ++at;
-                        }
-                    }
-                    cnt += total;
+    // lane c < 27 owns cell c of the walk (dz outer, dx inner) and loads its bucket range
+    uint32_t c_e0 = 0, c_n = 0;
+    if (lane < 27) {
+        const int cx = (int)cell[0] + lane % 3 - 1, cy = (int)cell[1] + (lane / 3) % 3 - 1,
+                  cz = (int)cell[2] + lane / 9 - 1;
+        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
+            const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
+            c_e0 = a.bstart[b];
+            c_n = a.bend[b] - c_e0;
+        }
+    }
+    // inclusive prefix of the cell sizes over the lanes: the walk's candidates, cell by cell
+    // and in bucket order within a cell, are the flat sequence 0 .. total-1
+    uint32_t c_end = c_n;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)c_end, off);
+        if (lane >= off) c_end += v;
+    }
+    const uint32_t total_cand = (uint32_t)__shfl((int)c_end, 26);
+    for (uint32_t base = 0; base < total_cand; base += 64) {
+        const uint32_t f = base + (uint32_t)lane;
+        bool ok = f < total_cand;
+        uint32_t j = 0;
+        if (ok) {
+            // the cell holding candidate f: the first c whose running end exceeds f
+            uint32_t e = 0, start = 0;
+            bool found = false;
+            for (int c = 0; c < 27; ++c) {
+                const uint32_t ce = (uint32_t)__builtin_amdgcn_readlane((int)c_end, c);
+                if (!found && f < ce) {
+                    found = true;
+                    e = (uint32_t)__builtin_amdgcn_readlane((int)c_e0, c) + (f - start);
                 }
+                start = ce;
             }
+            const float4 qe = a.pos_sorted[e];
+            j = __float_as_uint(qe.w);
+            ok = dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius;
+            if (ok && a.cluster_id && a.cluster_id[j] != mycluster) ok = false;
+            if (ok && a.flag_check && (uint32_t)a.new_flags[j] != myflags) ok = false;
+        }
+        const uint64_t mask = __ballot(ok);
+        const uint32_t total = (uint32_t)__popcll(mask);
+        const uint32_t pos = cnt + (uint32_t)__popcll(mask & below);
+        if (ok && pos < a.K) put(pos, j);  // the slots before the cap fill in order
+        if (cnt + total > a.K) {
+            // reservoir over the cap, entry by entry in walk order (after the in-order stores
+            // above have completed)
+            if (cnt + total > WF_CAP) __threadfence_block();
+            uint64_t rest = mask;
+            uint32_t at = cnt;
+            while (rest) {
+                const int src = __ffsll((long long)rest) - 1;
+                rest &= rest - 1;
+                const uint32_t jj = (uint32_t)__shfl((int)j, src);
+                if (at >= a.K && lane == 0) {
+                    const uint32_t r = pcg_next(rng) % (at + 1);
+                    if (r < a.K) put(r, jj);
+                }
+                ++at;
+            }
+        }
+        cnt += total;
+    }
     // make the list written by all lanes visible to the whole wave
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const uint32_t used = cnt < a.K ? cnt : a.K;
     const float inv_used = used ? 1.0f / (float)used : 0.0f;
-    auto get = [&](uint32_t i) -> uint32_t { return in_lds ? lb[i] : gbuf[i]; };
+    auto get = [&](uint32_t i) -> uint32_t { return i < WF_CAP ? lb[i] : gbuf[i]; };
     // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
     // the exact sum (a plain running sum drifts by ~n ulp)
     float dsum[3] = {0, 0, 0}, dc[3] = {0, 0, 0}, isum[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
@@ -407,17 +394,21 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
             }
         }
         if (a.smis) {
-            // sum_m pdf_m(wi_k) over every reused neighbour m, in order
-            float sum_pdf = 0.0f, sc = 0.0f;
-            for (uint32_t m = 0; m < used; ++m) {
-                const uint32_t jm = __builtin_amdgcn_readfirstlane(get(m));
-                const NbrRec& o = a.pk[jm];
-                const float4 a0 = o.ns_wox, a1 = o.ng_woy, a2 = o.woz;
-                float fc;
-                kahan_add(sum_pdf, sc,
-                          diffuse_eval(make_float3(a0.x, a0.y, a0.z), make_float3(a1.x, a1.y, a1.z),
-                                       make_float3(a0.w, a1.w, a2.x), wi, fc));
+            // sum_m pdf_m(wi_k) over every reused neighbour m, in order, in FP64
+            const double wx = wi.x, wy = wi.y, wz = wi.z;
+            double sum_cos = 0.0;
+            for (uint32_t m0 = 0; m0 < used; m0 += 64) {
+                const uint32_t mn = used - m0 < 64 ? used - m0 : 64;
+                // every lane has read the previous chunk before it is overwritten
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if ((uint32_t)lane < mn) sb[lane] = smis_rec(a.pk[get(m0 + (uint32_t)lane)]);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
+                for (uint32_t m = 0; m < mn; ++m) sum_cos += smis_term(sb[m], wi, wx, wy, wz);
             }
+            const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);
             if (active && ind_ok) {
                 kahan_add(isum[0], ic[0], reused.x / sum_pdf);
                 kahan_add(isum[1], ic[1], reused.y / sum_pdf);
@@ -451,7 +442,7 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
         a.rad_out[3 * (uint64_t)c] = L.x;
         a.rad_out[3 * (uint64_t)c + 1] = L.y;
         a.rad_out[3 * (uint64_t)c + 2] = L.z;
-        a.rng[tl] = rng;
+        if (a.use_rng) a.rng[tl] = rng;
     }
 }
 
@@ -470,6 +461,30 @@ __global__ void w_propagate_kernel(const float* rec_child, const float* rec_pare
     ind[5 * N + i] = rad[3 * (uint64_t)i + 2];
     ind[6 * N + i] = pl(rec_child, N, 18, i);
     ind[7 * N + i] = pl(rec_parent, N, 27, i);
+}
+
+// upper bound of a point's neighbour count: the entries of the buckets of its 27-cell walk
+// (around its cluster seed when clustering), i.e. the candidates the filter tests
+__global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+                                  const uint32_t* cluster_id, const uint32_t* bstart, const uint32_t* bend,
+                                  uint32_t* out_max) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t c = pts[t];
+    const uint32_t qc = cluster_id ? cluster_id[c] : c;
+    uint32_t cell[3];
+    get_cell(g, pl(rec, N, 0, qc), pl(rec, N, 1, qc), pl(rec, N, 2, qc), cell);
+    uint32_t total = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] || cz >= (int)g.res[2])
+                    continue;
+                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
+                total += bend[b] - bstart[b];
+            }
+    atomicMax(out_max, total);
 }
 
 __global__ void w_fill_kernel(float* p, uint64_t n, float v) {
@@ -882,7 +897,19 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             const bool wave_kernel = cfg.world_kernel == "wave" ||
                                      (cfg.world_kernel == "auto" &&
                                       std::min<double>(cand, cfg.cluster ? (double)K + 1.0 : cand) >= 16.0);
-            const uint64_t batch = std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
+            // The reference filters in batches of vram / (4 K) points (:1714-1744); batching is
+            // observable only through the reservoir streams, which are per batch slot. When no
+            // point's candidate count exceeds K there are no draws, and the whole depth runs as
+            // one launch with list slots sized by that count.
+            NLPF_HIP(h, hipMemsetAsync(w.counters + 5, 0, sizeof(uint32_t), st));
+            hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                               cfg.cluster ? w.cluster_id : nullptr, w.bstart, w.bend, w.counters + 5);
+            if ((rc = check_launch(h, "world candidates"))) return rc;
+            const uint32_t max_cand = read_u32(h, w.counters + 5, rc);
+            if (rc) return rc;
+            const bool single = max_cand <= K && (uint64_t)n * std::max<uint32_t>(max_cand, 1u) <= w.nbuf_elems;
+            const uint64_t batch =
+                single ? n : std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
             for (uint64_t off = 0; off < n; off += batch) {
                 const uint32_t cnt = (uint32_t)std::min<uint64_t>(batch, n - off);
                 FilterArgs fa;
@@ -901,7 +928,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 fa.g = g;
                 fa.K = K;
                 fa.nbuf = w.nbuf;
+                fa.stride = single ? std::max<uint32_t>(max_cand, 1u) : K;
                 fa.rng = w.rng;
+                fa.use_rng = single ? 0 : 1;
                 fa.nodes = h->d_nodes;
                 fa.roots = h->d_roots;
                 fa.n_materials = (uint32_t)h->h_roots.size();
