@@ -90,6 +90,14 @@ __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint3
     pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
 }
 
+// largest bucket (entries from one bucket start to its end), after w_ranges_kernel
+__global__ void w_bucket_max_kernel(const uint32_t* keys, uint32_t n, const uint32_t* bend, uint32_t* out_max) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t k = keys[e];
+    if (e == 0 || keys[e - 1] != k) atomicMax(out_max, bend[k] - e);
+}
+
 __global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint32_t* bstart, uint32_t* bend) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -113,6 +121,10 @@ __global__ void w_pack_kernel(const float* rec, uint64_t N, const uint32_t* pts,
     r.woz = make_float4(pl(rec, N, 11, i), 0.0f, 0.0f, 0.0f);
     pk[i] = r;
 }
+
+// per-depth neighbour counters: a kernel adds to slot blockIdx % NBR_SLOTS, so hundreds of
+// thousands of per-vertex additions do not serialize on one address; the host sums the slots
+constexpr uint32_t NBR_SLOTS = 256;
 
 struct FilterArgs {
     const float* rec;
@@ -138,7 +150,7 @@ struct FilterArgs {
     int depth;
     int smis, flag_check, stylize;
     float* rad_out;     // [N][3] level radiance (v.radiance)
-    unsigned long long* nbr_count;
+    unsigned long long* nbr_count;  // NBR_SLOTS counters of this depth (spread to avoid contention)
 };
 
 // One thread per vertex: for small neighbour sets (a few per vertex), where a wavefront per
@@ -188,7 +200,7 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
                     ++cnt;
                 }
             }
-    atomicAdd(a.nbr_count, (unsigned long long)cnt);
+    atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)cnt);
     const uint32_t used = cnt < a.K ? cnt : a.K;
     const float inv_used = 1.0f / (float)used;
     // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
@@ -432,7 +444,7 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) v[i] += __shfl_xor(v[i], off);
     if (lane == 0) {
-        atomicAdd(a.nbr_count, (unsigned long long)cnt);
+        atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)cnt);
         v3<float> L{le.x + v[0] + v[3], le.y + v[1] + v[4], le.z + v[2] + v[5]};
         if (a.stylize) {
             const uint32_t m = a.material[c];
@@ -724,7 +736,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.nbuf, nbuf_elems);
     rc |= dalloc(h, &w.prop, N);
     rc |= dalloc(h, &w.prop_sorted, N);
-    rc |= dalloc(h, &w.nbr, 32);
+    rc |= dalloc(h, &w.nbr, 32 * NBR_SLOTS);
     rc |= dalloc(h, &w.pos_sorted, N);
     rc |= dalloc(h, &w.pk, N);
     rc |= dalloc(h, &w.ind, 8 * N);
@@ -804,7 +816,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     const uint32_t table_bits = std::max(1, (int)std::ceil(std::log2((double)cfg.hash_table_size)));
     hipStream_t st = h->stream;
 
-    NLPF_HIP(h, hipMemsetAsync(w.nbr, 0, sizeof(unsigned long long) * 32, st));
+    NLPF_HIP(h, hipMemsetAsync(w.nbr, 0, sizeof(unsigned long long) * 32 * NBR_SLOTS, st));
     hipLaunchKernelGGL(w_rng_init_kernel, grid1(batch_max), dim3(256), 0, st, w.rng, (uint32_t)batch_max,
                        splitmix64(cfg.seed));
     hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
@@ -899,15 +911,24 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                       std::min<double>(cand, cfg.cluster ? (double)K + 1.0 : cand) >= 16.0);
             // The reference filters in batches of vram / (4 K) points (:1714-1744); batching is
             // observable only through the reservoir streams, which are per batch slot. When no
-            // point's candidate count exceeds K there are no draws, and the whole depth runs as
-            // one launch with list slots sized by that count.
-            NLPF_HIP(h, hipMemsetAsync(w.counters + 5, 0, sizeof(uint32_t), st));
-            hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                               cfg.cluster ? w.cluster_id : nullptr, w.bstart, w.bend, w.counters + 5);
-            if ((rc = check_launch(h, "world candidates"))) return rc;
-            const uint32_t max_cand = read_u32(h, w.counters + 5, rc);
+            // point's candidate count can exceed K there are no draws, and the whole depth runs
+            // as one launch with list slots sized by that count: first bounded by 27 x the
+            // largest bucket, else counted exactly.
+            NLPF_HIP(h, hipMemsetAsync(w.counters + 5, 0, 2 * sizeof(uint32_t), st));
+            hipLaunchKernelGGL(w_bucket_max_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bend,
+                               w.counters + 6);
+            if ((rc = check_launch(h, "world bucket size"))) return rc;
+            const uint64_t bound = 27ull * read_u32(h, w.counters + 6, rc);
             if (rc) return rc;
-            const bool single = max_cand <= K && (uint64_t)n * std::max<uint32_t>(max_cand, 1u) <= w.nbuf_elems;
+            uint64_t max_cand = bound;
+            if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
+                hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                                   cfg.cluster ? w.cluster_id : nullptr, w.bstart, w.bend, w.counters + 5);
+                if ((rc = check_launch(h, "world candidates"))) return rc;
+                max_cand = read_u32(h, w.counters + 5, rc);
+                if (rc) return rc;
+            }
+            const bool single = max_cand <= K && (uint64_t)n * std::max<uint64_t>(max_cand, 1) <= w.nbuf_elems;
             const uint64_t batch =
                 single ? n : std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
             for (uint64_t off = 0; off < n; off += batch) {
@@ -928,7 +949,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 fa.g = g;
                 fa.K = K;
                 fa.nbuf = w.nbuf;
-                fa.stride = single ? std::max<uint32_t>(max_cand, 1u) : K;
+                fa.stride = single ? (uint32_t)std::max<uint64_t>(max_cand, 1) : K;
                 fa.rng = w.rng;
                 fa.use_rng = single ? 0 : 1;
                 fa.nodes = h->d_nodes;
@@ -939,7 +960,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 fa.flag_check = !cfg.disable_flag_check;
                 fa.stylize = (d > 0 || !cfg.no_first_hit_stylization) && !h->h_roots.empty();
                 fa.rad_out = rad;
-                fa.nbr_count = w.nbr + d;
+                fa.nbr_count = w.nbr + (uint64_t)d * NBR_SLOTS;
                 if (wave_kernel)
                     hipLaunchKernelGGL(w_filter_wave_kernel, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
                 else
@@ -967,9 +988,13 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     if ((rc = check_launch(h, "world film"))) return rc;
     rc = record_event(h, 1 + L);
     if (rc) return rc;
-    unsigned long long nb[32];
-    NLPF_HIP(h, hipMemcpyAsync(nb, w.nbr, sizeof(nb), hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> slots(32 * NBR_SLOTS);
+    NLPF_HIP(h, hipMemcpyAsync(slots.data(), w.nbr, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost,
+                               st));
     NLPF_HIP(h, hipStreamSynchronize(st));
+    unsigned long long nb[32] = {};
+    for (uint32_t d = 0; d < 32; ++d)
+        for (uint32_t k = 0; k < NBR_SLOTS; ++k) nb[d] += slots[(uint64_t)d * NBR_SLOTS + k];
     stt.total_ms = event_ms(h, 0, 1 + L);
     for (uint32_t i = 0; i < L; ++i) stt.level_ms[L - 1 - i] = event_ms(h, i, i + 1);
     stt.film_ms = event_ms(h, L, L + 1);
