@@ -331,6 +331,8 @@ def main():
     ap.add_argument("--quality-frames", type=int, default=8,
                     help="untimed frames after the timed ones whose film error vs FP64 is averaged")
     ap.add_argument("--no-extra", action="store_true", help="skip the config 3 / 5 / world-mode measurements")
+    ap.add_argument("--world-only", action="store_true",
+                    help="only the world-mode measurement (config 1), printed as its own JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one FP32 pass and one 1 GiB device copy, for rocprofv3 counter runs")
     args = ap.parse_args()
@@ -342,6 +344,10 @@ def main():
         sys.exit(2)
 
     import nlpf as N
+
+    if args.world_only:
+        print(json.dumps({"world_config1": world_leg(None, N)}), flush=True)
+        return
 
     W, H, S, L, R = args.width, args.height, args.spp, args.max_depth + 1, args.radius
     plane = W * H

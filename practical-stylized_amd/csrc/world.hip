@@ -176,30 +176,37 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
     uint32_t cell[3];
     get_cell(a.g, q.x, q.y, q.z, cell);
     uint32_t cnt = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
-                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)a.g.res[0] || cy >= (int)a.g.res[1] || cz >= (int)a.g.res[2])
-                    continue;
-                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
-                const uint32_t e1 = a.bend[b];
-                for (uint32_t e = a.bstart[b]; e < e1; ++e) {
-                    const float4 qe = a.pos_sorted[e];
-                    if (!(dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius)) continue;
-                    const uint32_t j = __float_as_uint(qe.w);
-                    if (a.cluster_id && a.cluster_id[j] != mycluster) continue;
-                    if (a.flag_check && (uint32_t)a.new_flags[j] != myflags) continue;
-                    // reservoir over the reuse cap (:813-821)
-                    if (cnt < a.K) {
-                        buf[cnt] = j;
-                    } else {
-                        const uint32_t r = pcg_next(rng) % (cnt + 1);
-                        if (r < a.K) buf[r] = j;
-                    }
-                    ++cnt;
-                }
+    // the 27 bucket ranges are loaded up front (one round trip), then walked in order
+    uint32_t r0[27], r1[27];
+#pragma unroll
+    for (int c = 0; c < 27; ++c) {
+        const int cx = (int)cell[0] + c % 3 - 1, cy = (int)cell[1] + (c / 3) % 3 - 1, cz = (int)cell[2] + c / 9 - 1;
+        r0[c] = r1[c] = 0;
+        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
+            const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
+            r0[c] = a.bstart[b];
+            r1[c] = a.bend[b];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 27; ++c) {
+        const uint32_t e1 = r1[c];
+        for (uint32_t e = r0[c]; e < e1; ++e) {
+            const float4 qe = a.pos_sorted[e];
+            if (!(dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius)) continue;
+            const uint32_t j = __float_as_uint(qe.w);
+            if (a.cluster_id && a.cluster_id[j] != mycluster) continue;
+            if (a.flag_check && (uint32_t)a.new_flags[j] != myflags) continue;
+            // reservoir over the reuse cap (:813-821)
+            if (cnt < a.K) {
+                buf[cnt] = j;
+            } else {
+                const uint32_t r = pcg_next(rng) % (cnt + 1);
+                if (r < a.K) buf[r] = j;
             }
+            ++cnt;
+        }
+    }
     atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)cnt);
     const uint32_t used = cnt < a.K ? cnt : a.K;
     const float inv_used = 1.0f / (float)used;
