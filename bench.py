@@ -288,9 +288,15 @@ def world_leg(h_unused, N):
             times.append(st["total_ms"] / 1e3)
         pts = sum(st["points"][:L])
         nb = sum(st["neighbors"][:L])
+        pairs = sum(st["smis_pairs"][:L])
         ms = 1e3 * float(np.median(times))
+        # the O(K^2) SMIS denominators dominate: their pdf evaluations per second, and the VALU
+        # issue they imply at the filter's ~23.5 instructions per evaluation (inner loop of
+        # w_filter_wave_kernel), over the whole frame time
         out["radius_scale_2"] = {"ms_per_frame": ms, "value": pts / (ms / 1e3), "unit": "vertices/s",
-                                 "neighbours_per_vertex": nb / max(pts, 1)}
+                                 "neighbours_per_vertex": nb / max(pts, 1),
+                                 "smis_pairs_per_frame": pairs, "smis_pairs_per_s": pairs / (ms / 1e3),
+                                 "smis_valu_issue_frac": pairs * 23.5 / (ms / 1e3) / VALU_ISSUE_PEAK}
     return out
 
 

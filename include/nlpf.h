@@ -231,6 +231,9 @@ typedef struct nlpf_stats {
      * average, lm_path_filtering.rs:1767-1773) */
     uint64_t points[32];
     uint64_t neighbors[32];
+    /* world mode with SMIS: sum over the depth's points of (reused neighbours)^2, the pdf
+     * evaluations of the SMIS denominators */
+    uint64_t smis_pairs[32];
 } nlpf_stats;
 int nlpf_get_stats(const nlpf_handle* h, nlpf_stats* out);
 

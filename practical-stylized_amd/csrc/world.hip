@@ -150,7 +150,8 @@ struct FilterArgs {
     int depth;
     int smis, flag_check, stylize;
     float* rad_out;     // [N][3] level radiance (v.radiance)
-    unsigned long long* nbr_count;  // NBR_SLOTS counters of this depth (spread to avoid contention)
+    unsigned long long* nbr_count;   // NBR_SLOTS counters of this depth (spread to avoid contention)
+    unsigned long long* pair_count;  // NBR_SLOTS counters of SMIS pdf evaluations
 };
 
 // One thread per vertex: for small neighbour sets (a few per vertex), where a wavefront per
@@ -209,6 +210,7 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
     }
     atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)cnt);
     const uint32_t used = cnt < a.K ? cnt : a.K;
+    if (a.smis) atomicAdd(a.pair_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)used * used);
     const float inv_used = 1.0f / (float)used;
     // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
     // the exact sum (a plain running sum drifts by ~n ulp)
@@ -452,6 +454,7 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
         for (int i = 0; i < 6; ++i) v[i] += __shfl_xor(v[i], off);
     if (lane == 0) {
         atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)cnt);
+        if (a.smis) atomicAdd(a.pair_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)used * used);
         v3<float> L{le.x + v[0] + v[3], le.y + v[1] + v[4], le.z + v[2] + v[5]};
         if (a.stylize) {
             const uint32_t m = a.material[c];
@@ -743,7 +746,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.nbuf, nbuf_elems);
     rc |= dalloc(h, &w.prop, N);
     rc |= dalloc(h, &w.prop_sorted, N);
-    rc |= dalloc(h, &w.nbr, 32 * NBR_SLOTS);
+    rc |= dalloc(h, &w.nbr, 64 * NBR_SLOTS);  // neighbours, then SMIS pairs, per depth
     rc |= dalloc(h, &w.pos_sorted, N);
     rc |= dalloc(h, &w.pk, N);
     rc |= dalloc(h, &w.ind, 8 * N);
@@ -823,7 +826,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     const uint32_t table_bits = std::max(1, (int)std::ceil(std::log2((double)cfg.hash_table_size)));
     hipStream_t st = h->stream;
 
-    NLPF_HIP(h, hipMemsetAsync(w.nbr, 0, sizeof(unsigned long long) * 32 * NBR_SLOTS, st));
+    NLPF_HIP(h, hipMemsetAsync(w.nbr, 0, sizeof(unsigned long long) * 64 * NBR_SLOTS, st));
     hipLaunchKernelGGL(w_rng_init_kernel, grid1(batch_max), dim3(256), 0, st, w.rng, (uint32_t)batch_max,
                        splitmix64(cfg.seed));
     hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
@@ -968,6 +971,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 fa.stylize = (d > 0 || !cfg.no_first_hit_stylization) && !h->h_roots.empty();
                 fa.rad_out = rad;
                 fa.nbr_count = w.nbr + (uint64_t)d * NBR_SLOTS;
+                fa.pair_count = w.nbr + (uint64_t)(32 + d) * NBR_SLOTS;
                 if (wave_kernel)
                     hipLaunchKernelGGL(w_filter_wave_kernel, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
                 else
@@ -995,18 +999,22 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     if ((rc = check_launch(h, "world film"))) return rc;
     rc = record_event(h, 1 + L);
     if (rc) return rc;
-    std::vector<unsigned long long> slots(32 * NBR_SLOTS);
+    std::vector<unsigned long long> slots(64 * NBR_SLOTS);
     NLPF_HIP(h, hipMemcpyAsync(slots.data(), w.nbr, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost,
                                st));
     NLPF_HIP(h, hipStreamSynchronize(st));
-    unsigned long long nb[32] = {};
+    unsigned long long nb[32] = {}, np[32] = {};
     for (uint32_t d = 0; d < 32; ++d)
-        for (uint32_t k = 0; k < NBR_SLOTS; ++k) nb[d] += slots[(uint64_t)d * NBR_SLOTS + k];
+        for (uint32_t k = 0; k < NBR_SLOTS; ++k) {
+            nb[d] += slots[(uint64_t)d * NBR_SLOTS + k];
+            np[d] += slots[(uint64_t)(32 + d) * NBR_SLOTS + k];
+        }
     stt.total_ms = event_ms(h, 0, 1 + L);
     for (uint32_t i = 0; i < L; ++i) stt.level_ms[L - 1 - i] = event_ms(h, i, i + 1);
     stt.film_ms = event_ms(h, L, L + 1);
     for (uint32_t d = 0; d < L && d < 32; ++d) {
         stt.neighbors[d] = nb[d];
+        stt.smis_pairs[d] = np[d];
         stt.vertices += stt.points[d];
     }
     h->stats = stt;

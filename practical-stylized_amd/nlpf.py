@@ -68,7 +68,7 @@ class _Stats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("level_ms", C.c_double * 32), ("film_ms", C.c_double),
                 ("vertices", C.c_uint64), ("bytes_algorithmic", C.c_uint64),
                 ("flops_algorithmic", C.c_double), ("points", C.c_uint64 * 32),
-                ("neighbors", C.c_uint64 * 32)]
+                ("neighbors", C.c_uint64 * 32), ("smis_pairs", C.c_uint64 * 32)]
 
 
 class _WorldDesc(C.Structure):
@@ -442,7 +442,7 @@ class Handle:
         return {"total_ms": s.total_ms, "level_ms": list(s.level_ms), "film_ms": s.film_ms,
                 "vertices": s.vertices, "bytes_algorithmic": s.bytes_algorithmic,
                 "flops_algorithmic": s.flops_algorithmic, "points": list(s.points),
-                "neighbors": list(s.neighbors)}
+                "neighbors": list(s.neighbors), "smis_pairs": list(s.smis_pairs)}
 
     # ---- world mode ----
     @staticmethod
