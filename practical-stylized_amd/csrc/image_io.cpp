@@ -223,13 +223,18 @@ int nlpf_write_image_host(const float* rgba, uint32_t width, uint32_t height, co
         set_global_error(err);
         return NLPF_E_INVALID_ARG;
     }
-    const int rc = write_image(rgba, width, height, path, err);
+    int rc;
+    try {
+        rc = write_image(rgba, width, height, path, err);
+    } catch (const std::bad_alloc&) {
+        err = "write_image: out of host memory";
+        rc = NLPF_E_OOM;
+    }
     if (rc) set_global_error(err);
     return rc;
 }
 
-int nlpf_write_image(nlpf_handle* h, const float* rgba, uint32_t width, uint32_t height, const char* path) {
-    if (!h) return NLPF_E_INVALID_ARG;
+static int write_image_device(nlpf_handle* h, const float* rgba, uint32_t width, uint32_t height, const char* path) {
     std::string err;
     if (!image_args_ok(rgba, width, height, path, err)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, err);
     std::vector<float> host(4ull * width * height);
@@ -238,6 +243,15 @@ int nlpf_write_image(nlpf_handle* h, const float* rgba, uint32_t width, uint32_t
     const int rc = write_image(host.data(), width, height, path, err);
     if (rc) h->err = err;
     return rc;
+}
+
+int nlpf_write_image(nlpf_handle* h, const float* rgba, uint32_t width, uint32_t height, const char* path) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    try {
+        return write_image_device(h, rgba, width, height, path);
+    } catch (const std::bad_alloc&) {
+        NLPF_FAIL(h, NLPF_E_OOM, "write_image: out of host memory");
+    }
 }
 
 }  // extern "C"

@@ -65,7 +65,15 @@ struct JParser {
         ++s;
         return true;
     }
+    int depth = 0;  // nesting of the value being parsed
     bool parse(JVal& v) {
+        if (depth >= 256) return fail("JSON nested deeper than 256 levels");
+        ++depth;
+        const bool ok = parse_value(v);
+        --depth;
+        return ok;
+    }
+    bool parse_value(JVal& v) {
         ws();
         if (*s == '{') {
             ++s;

@@ -313,8 +313,8 @@ struct Lowering {
 
 using namespace nlpf;
 
-extern "C" int nlpf_style_from_scene(const char* scene_json, const char* const* materials, uint32_t n_materials,
-                                     nlpf_style_node* nodes, uint32_t* n_nodes, int32_t* roots) {
+static int style_from_scene(const char* scene_json, const char* const* materials, uint32_t n_materials,
+                            nlpf_style_node* nodes, uint32_t* n_nodes, int32_t* roots) {
     if (!scene_json || (n_materials && (!materials || !roots)) || !n_nodes || (*n_nodes && !nodes)) {
         set_global_error("nlpf_style_from_scene: NULL argument");
         return NLPF_E_INVALID_ARG;
@@ -360,4 +360,14 @@ extern "C" int nlpf_style_from_scene(const char* scene_json, const char* const* 
     if (!table.empty()) memcpy(nodes, table.data(), sizeof(nlpf_style_node) * table.size());
     *n_nodes = (uint32_t)table.size();
     return NLPF_OK;
+}
+
+extern "C" int nlpf_style_from_scene(const char* scene_json, const char* const* materials, uint32_t n_materials,
+                                     nlpf_style_node* nodes, uint32_t* n_nodes, int32_t* roots) {
+    try {
+        return style_from_scene(scene_json, materials, n_materials, nodes, n_nodes, roots);
+    } catch (const std::bad_alloc&) {
+        set_global_error("nlpf_style_from_scene: out of host memory");
+        return NLPF_E_OOM;
+    }
 }

@@ -284,9 +284,9 @@ const char* kMatNames[NLPF_SCENE_MATERIALS] = {"floor", "ceiling", "back", "left
 
 using namespace nlpf;
 
-extern "C" int nlpf_write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
-                                      const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* roots,
-                                      uint32_t n_materials, uint32_t sphere_segments) {
+static int write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
+                             const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* roots,
+                             uint32_t n_materials, uint32_t sphere_segments) {
     auto fail = [](int code, const std::string& m) {
         set_global_error("nlpf_write_world_scene: " + m);
         return code;
@@ -403,4 +403,18 @@ extern "C" int nlpf_write_world_scene(const char* path, uint32_t width, uint32_t
         if (fclose(fh) != 0 || w != want) return fail(NLPF_E_IO, "short write to " + fp);
     }
     return NLPF_OK;
+}
+
+extern "C" int nlpf_write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
+                                      const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* roots,
+                                      uint32_t n_materials, uint32_t sphere_segments) {
+    try {
+        return write_world_scene(path, width, height, gen, nodes, n_nodes, roots, n_materials, sphere_segments);
+    } catch (const std::bad_alloc&) {
+        set_global_error("nlpf_write_world_scene: out of host memory");
+        return NLPF_E_OOM;
+    } catch (const std::filesystem::filesystem_error& e) {
+        set_global_error(std::string("nlpf_write_world_scene: ") + e.what());
+        return NLPF_E_IO;
+    }
 }

@@ -219,3 +219,6 @@ def test_errors():
         N.style_from_scene(FIXTURE, capacity=3)
     with pytest.raises(N.NlpfError, match="scene JSON"):
         N.style_from_scene("{", ["m"])
+    # a hostile nesting depth is an error, not a stack overflow
+    with pytest.raises(N.NlpfError, match="nested deeper"):
+        N.style_from_scene('{"materials":' + "[" * 100000 + "]" * 100000 + "}", ["m"])
