@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 METRIC = "filtered path-vertices/s @1080p×16spp, plain vs FP16+MLP; HBM % of peak; L2 err"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# algorithmic flops per neighbour pair: |dp|^2 7 (3 sub, mul, 2 fma), normal term 8 (3 fma,
+# algorithmic flops per neighbour pair: |dp|^2 8 (3 sub, mul, 2 fma), normal term 8 (3 fma,
 # fma), luminance 3 (sub, fma), exp 1, accumulate 7 (add, 3 fma)
 FLOPS_PER_PAIR = 27.0
 FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per cycle (the build emits no packed FP32)
@@ -250,21 +250,40 @@ def large_leg(h, N):
     return out
 
 
-def world_leg(h_unused, N):
+def world_cpu(host, W, H, B, L, cfg, nodes, roots, film, pts):
+    """The FP64 world-mode oracle (OpenMP over reservoir slots; grid build, sorts and
+    propagation single-threaded) on the whole frame's records, beside the GPU pass, and the
+    GPU film's relative L2 error against it."""
+    import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    ofilm = O.world_run(host["records"], host["flags"], host["new_flags"], host["material"], host["path_depth"],
+                        W, H, B, L, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), cfg, nodes=nodes, roots=roots)
+    dt = time.perf_counter() - t0
+    err = float(np.linalg.norm(film.astype(np.float64) - ofilm) / max(np.linalg.norm(ofilm), 1e-30))
+    return {"value": pts / dt, "unit": "vertices/s", "cores": threads, "kind": "port",
+            "sample": f"the whole frame ({pts} vertices), FP64 oracle, {threads} threads, {dt:.1f} s",
+            "film_rel_l2_fp32_gpu_vs_fp64": err}
+
+
+def world_leg(h_unused, N, cpu=True):
     """Config 1 in world mode (the reference's hash-grid NL-PF): 256x256 x 4 paths per pixel,
     max_depth 3, SMIS, no clusters, reuse cap above every neighbour count, 4-level toon at
     depth 1, records traced on the GPU from the procedural box scene."""
     W, H, B, L = 256, 256, 4, 4
     cfg = {"type": "pf", "max_depth": L - 1, "branch": B, "smis": True, "cluster": False, "reuse": [1 << 16]}
+    nodes = N.style_table([N.toon(0.25, depth_mask=N.depth_mask(1))])
+    roots = [0, 0, 0, 0, 0, 0, 0, -1]
     with N.Handle(cfg) as hw:
-        hw.set_style(N.style_table([N.toon(0.25, depth_mask=N.depth_mask(1))]), [0, 0, 0, 0, 0, 0, 0, -1])
+        hw.set_style(nodes, roots)
         desc = hw.world_desc(W, H, B, L)
         bufs = hw.alloc_world(desc)
         hw.generate_world_records(desc, bufs, seed=1)
         hw.world_run(desc, bufs)
         times = []
         for _ in range(3):
-            hw.world_run(desc, bufs)
+            film, _ = hw.world_run(desc, bufs)
             st = hw.stats()
             times.append(st["total_ms"] / 1e3)  # HIP events around the depth loop and the film
         pts = sum(st["points"][:L])
@@ -273,23 +292,29 @@ def world_leg(h_unused, N):
         out = {"workload": f"{W}x{H}x{B} paths, {L} levels, radius_scale default, SMIS, toon(4) at depth 1",
                "ms_per_frame": ms, "vertices": int(pts), "value": pts / (ms / 1e3), "unit": "vertices/s",
                "neighbours_per_vertex": nb / max(pts, 1)}
+        host = hw.download_world(desc, bufs) if cpu else None
+    if cpu:
+        log("world mode: FP64 oracle on the same frame (CPU)")
+        out["cpu_oracle"] = world_cpu(host, W, H, B, L, cfg, nodes, roots, film, pts)
+        out["cpu_oracle"]["gpu_speedup"] = out["value"] / out["cpu_oracle"]["value"]
     # the same records with a 20x larger reuse radius: the filter itself dominates
     cfg2 = dict(cfg, radius_scale=[2.0])
     with N.Handle(cfg2) as hw:
-        hw.set_style(N.style_table([N.toon(0.25, depth_mask=N.depth_mask(1))]), [0, 0, 0, 0, 0, 0, 0, -1])
+        hw.set_style(nodes, roots)
         desc = hw.world_desc(W, H, B, L)
         bufs = hw.alloc_world(desc)
         hw.generate_world_records(desc, bufs, seed=1)
         hw.world_run(desc, bufs)
         times = []
         for _ in range(3):
-            hw.world_run(desc, bufs)
+            film, _ = hw.world_run(desc, bufs)
             st = hw.stats()
             times.append(st["total_ms"] / 1e3)
         pts = sum(st["points"][:L])
         nb = sum(st["neighbors"][:L])
         pairs = sum(st["smis_pairs"][:L])
         ms = 1e3 * float(np.median(times))
+        host = hw.download_world(desc, bufs) if cpu else None
         # the O(K^2) SMIS denominators dominate: their pdf evaluations per second, and the VALU
         # issue they imply at the filter's ~23.5 instructions per evaluation (inner loop of
         # w_filter_wave_kernel), over the whole frame time
@@ -297,6 +322,11 @@ def world_leg(h_unused, N):
                                  "neighbours_per_vertex": nb / max(pts, 1),
                                  "smis_pairs_per_frame": pairs, "smis_pairs_per_s": pairs / (ms / 1e3),
                                  "smis_valu_issue_frac": pairs * 23.5 / (ms / 1e3) / VALU_ISSUE_PEAK}
+    if cpu:
+        log("world mode, radius_scale 2: FP64 oracle on the same frame (CPU)")
+        o = world_cpu(host, W, H, B, L, cfg2, nodes, roots, film, pts)
+        o["gpu_speedup"] = out["radius_scale_2"]["value"] / o["value"]
+        out["radius_scale_2"]["cpu_oracle"] = o
     return out
 
 
@@ -352,7 +382,7 @@ def main():
     import nlpf as N
 
     if args.world_only:
-        print(json.dumps({"world_config1": world_leg(None, N)}), flush=True)
+        print(json.dumps({"world_config1": world_leg(None, N, cpu=not args.no_cpu)}), flush=True)
         return
 
     W, H, S, L, R = args.width, args.height, args.spp, args.max_depth + 1, args.radius
@@ -474,7 +504,7 @@ def main():
         rec32.free()
         result["adjoint_config3"] = adjoint_leg(h, N)
         result["large_config5"] = large_leg(h, N)
-        result["world_config1"] = world_leg(h, N)
+        result["world_config1"] = world_leg(h, N, cpu=not args.no_cpu)
         h.set_style(table, [root])
         rec32 = h.alloc(verts * 60)
         h.generate_screen_records(d32, rec32, seed=1)

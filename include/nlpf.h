@@ -225,8 +225,8 @@ typedef struct nlpf_stats {
     double level_ms[32];
     double film_ms;
     uint64_t vertices;
-    uint64_t bytes_algorithmic;
-    double flops_algorithmic;
+    uint64_t bytes_algorithmic;  /* records read once, level outputs written once, read once */
+    double flops_algorithmic;    /* 27 per neighbour pair (window^2 pairs per vertex) */
     /* world mode: points gathered and neighbours visited per depth (the reference logs the
      * average, lm_path_filtering.rs:1767-1773) */
     uint64_t points[32];

@@ -286,8 +286,7 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
     }
     if (batch_max > N) batch_max = N;
     pcg_t* rng = (pcg_t*)malloc(sizeof(pcg_t) * batch_max);
-    uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * kmax);
-    if (!rng || !buf) return -1;
+    if (!rng) return -1;
     const uint64_t key = splitmix(seed);
     for (uint64_t t = 0; t < batch_max; ++t) rng[t] = pcg_seq(t, key);
     float ext[3];
@@ -314,91 +313,105 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
         if (batch < 1) batch = 1;
         uint64_t nbr_total = 0;
         double* radd = rad + (uint64_t)d * 3 * N;
-        for (uint32_t e = 0; e < n; ++e) {
-            const uint32_t c = pts[e];
-            const uint32_t tl = (uint32_t)(e % batch);
-            const uint32_t q = cluster ? cid[c] : c;  /* the seed's position when clustered */
-            const float qx = (float)F(lv, 0, q), qy = (float)F(lv, 1, q), qz = (float)F(lv, 2, q);
-            const double ns[3] = {F(lv, 3, c), F(lv, 4, c), F(lv, 5, c)};
-            const double ng[3] = {F(lv, 6, c), F(lv, 7, c), F(lv, 8, c)};
-            const double wo[3] = {F(lv, 9, c), F(lv, 10, c), F(lv, 11, c)};
-            const double R[3] = {F(lv, 12, c), F(lv, 13, c), F(lv, 14, c)};
-            const uint8_t myf = new_flags[(uint64_t)d * N + c];
-            int32_t cell[3];
-            cell_of(aabb_min, aabb_max, res, qx, qy, qz, cell);
-            uint32_t cnt = 0;
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        int32_t nc[3] = {cell[0] + dx, cell[1] + dy, cell[2] + dz};
-                        if (nc[0] < 0 || nc[1] < 0 || nc[2] < 0 || nc[0] >= (int)res[0] || nc[1] >= (int)res[1] ||
-                            nc[2] >= (int)res[2])
-                            continue;
-                        uint64_t b = bucket_of(aabb_min, aabb_max, res, table, nc);
-                        uint32_t lo = 0, hi = n;
-                        while (lo < hi) {
-                            uint32_t mid = (lo + hi) / 2;
-                            if ((kv[mid] >> 32) < b) lo = mid + 1; else hi = mid;
-                        }
-                        for (uint32_t k = lo; k < n && (kv[k] >> 32) == b; ++k) {
-                            uint32_t j = (uint32_t)kv[k];
-                            float ddx = (float)F(lv, 0, j) - qx, ddy = (float)F(lv, 1, j) - qy,
-                                  ddz = (float)F(lv, 2, j) - qz;
-                            if (!(sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) <= radius)) continue;
-                            if (cluster && cid[j] != q) continue;
-                            if (!disable_flag_check && new_flags[(uint64_t)d * N + j] != myf) continue;
-                            if (cnt < K) {
-                                buf[cnt] = j;
-                            } else {
-                                uint32_t r = pcg_next(&rng[tl]) % (cnt + 1);
-                                if (r < K) buf[r] = j;
+        /* Points are independent except for the reservoir stream of batch slot tl, which the
+           points e = tl, tl + batch, ... draw from in that order: slots run in parallel, each
+           slot's points in order, so the results do not depend on the thread count. */
+        const uint64_t nslots = batch < n ? batch : n;
+        int alloc_fail = 0;
+#pragma omp parallel reduction(+ : nbr_total) reduction(| : alloc_fail)
+        {
+            uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * kmax);
+            if (!buf) alloc_fail = 1;
+#pragma omp for schedule(dynamic, 64)
+            for (uint64_t tl = 0; tl < nslots; ++tl) {
+                for (uint64_t e = tl; e < n && buf; e += batch) {
+                    const uint32_t c = pts[e];
+                    const uint32_t q = cluster ? cid[c] : c;  /* the seed's position when clustered */
+                    const float qx = (float)F(lv, 0, q), qy = (float)F(lv, 1, q), qz = (float)F(lv, 2, q);
+                    const double ns[3] = {F(lv, 3, c), F(lv, 4, c), F(lv, 5, c)};
+                    const double ng[3] = {F(lv, 6, c), F(lv, 7, c), F(lv, 8, c)};
+                    const double wo[3] = {F(lv, 9, c), F(lv, 10, c), F(lv, 11, c)};
+                    const double R[3] = {F(lv, 12, c), F(lv, 13, c), F(lv, 14, c)};
+                    const uint8_t myf = new_flags[(uint64_t)d * N + c];
+                    int32_t cell[3];
+                    cell_of(aabb_min, aabb_max, res, qx, qy, qz, cell);
+                    uint32_t cnt = 0;
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                int32_t nc[3] = {cell[0] + dx, cell[1] + dy, cell[2] + dz};
+                                if (nc[0] < 0 || nc[1] < 0 || nc[2] < 0 || nc[0] >= (int)res[0] ||
+                                    nc[1] >= (int)res[1] || nc[2] >= (int)res[2])
+                                    continue;
+                                uint64_t b = bucket_of(aabb_min, aabb_max, res, table, nc);
+                                uint32_t lo = 0, hi = n;
+                                while (lo < hi) {
+                                    uint32_t mid = (lo + hi) / 2;
+                                    if ((kv[mid] >> 32) < b) lo = mid + 1; else hi = mid;
+                                }
+                                for (uint32_t k = lo; k < n && (kv[k] >> 32) == b; ++k) {
+                                    uint32_t j = (uint32_t)kv[k];
+                                    float ddx = (float)F(lv, 0, j) - qx, ddy = (float)F(lv, 1, j) - qy,
+                                          ddz = (float)F(lv, 2, j) - qz;
+                                    if (!(sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) <= radius)) continue;
+                                    if (cluster && cid[j] != q) continue;
+                                    if (!disable_flag_check && new_flags[(uint64_t)d * N + j] != myf) continue;
+                                    if (cnt < K) {
+                                        buf[cnt] = j;
+                                    } else {
+                                        uint32_t r = pcg_next(&rng[tl]) % (cnt + 1);
+                                        if (r < K) buf[r] = j;
+                                    }
+                                    ++cnt;
+                                }
                             }
-                            ++cnt;
+                    nbr_total += cnt;
+                    const uint32_t used = cnt < K ? cnt : K;
+                    double direct[3] = {0, 0, 0}, indirect[3] = {0, 0, 0};
+                    for (uint32_t k = 0; k < used; ++k) {
+                        const uint32_t j = buf[k];
+                        const double dpdf = F(lv, 25, j);
+                        if (dpdf > 0) {
+                            double wi[3] = {F(lv, 19, j), F(lv, 20, j), F(lv, 21, j)}, fc;
+                            double bp = diffuse(ns, ng, wo, wi, &fc);
+                            double w = dpdf / (dpdf + bp);
+                            for (int ch = 0; ch < 3; ++ch)
+                                direct[ch] += R[ch] * fc * F(lv, 22 + ch, j) * F(lv, 26, j) * w / dpdf / (double)used;
+                        }
+                        double wi[3] = {ind[j], ind[N + j], ind[2 * N + j]};
+                        if (sqrt(wi[0] * wi[0] + wi[1] * wi[1] + wi[2] * wi[2]) == 0) continue;
+                        double fc;
+                        double bp = diffuse(ns, ng, wo, wi, &fc);
+                        double reused[3];
+                        for (int ch = 0; ch < 3; ++ch) reused[ch] = R[ch] * fc * ind[(3 + ch) * N + j] * ind[6 * N + j];
+                        if (smis) {
+                            double sum = 0;
+                            for (uint32_t m = 0; m < used; ++m) {
+                                uint32_t o = buf[m];
+                                double ons[3] = {F(lv, 3, o), F(lv, 4, o), F(lv, 5, o)};
+                                double ong[3] = {F(lv, 6, o), F(lv, 7, o), F(lv, 8, o)};
+                                double owo[3] = {F(lv, 9, o), F(lv, 10, o), F(lv, 11, o)}, f2;
+                                sum += diffuse(ons, ong, owo, wi, &f2);
+                            }
+                            for (int ch = 0; ch < 3; ++ch) indirect[ch] += reused[ch] / sum;
+                        } else if (bp > 0) {
+                            for (int ch = 0; ch < 3; ++ch) indirect[ch] += reused[ch] / bp / (double)used;
                         }
                     }
-            nbr_total += cnt;
-            const uint32_t used = cnt < K ? cnt : K;
-            double direct[3] = {0, 0, 0}, indirect[3] = {0, 0, 0};
-            for (uint32_t k = 0; k < used; ++k) {
-                const uint32_t j = buf[k];
-                const double dpdf = F(lv, 25, j);
-                if (dpdf > 0) {
-                    double wi[3] = {F(lv, 19, j), F(lv, 20, j), F(lv, 21, j)}, fc;
-                    double bp = diffuse(ns, ng, wo, wi, &fc);
-                    double w = dpdf / (dpdf + bp);
-                    for (int ch = 0; ch < 3; ++ch)
-                        direct[ch] += R[ch] * fc * F(lv, 22 + ch, j) * F(lv, 26, j) * w / dpdf / (double)used;
-                }
-                double wi[3] = {ind[j], ind[N + j], ind[2 * N + j]};
-                if (sqrt(wi[0] * wi[0] + wi[1] * wi[1] + wi[2] * wi[2]) == 0) continue;
-                double fc;
-                double bp = diffuse(ns, ng, wo, wi, &fc);
-                double reused[3];
-                for (int ch = 0; ch < 3; ++ch) reused[ch] = R[ch] * fc * ind[(3 + ch) * N + j] * ind[6 * N + j];
-                if (smis) {
-                    double sum = 0;
-                    for (uint32_t m = 0; m < used; ++m) {
-                        uint32_t o = buf[m];
-                        double ons[3] = {F(lv, 3, o), F(lv, 4, o), F(lv, 5, o)};
-                        double ong[3] = {F(lv, 6, o), F(lv, 7, o), F(lv, 8, o)};
-                        double owo[3] = {F(lv, 9, o), F(lv, 10, o), F(lv, 11, o)}, f2;
-                        sum += diffuse(ons, ong, owo, wi, &f2);
+                    double Lr[3];
+                    for (int ch = 0; ch < 3; ++ch) Lr[ch] = F(lv, 15 + ch, c) + direct[ch] + indirect[ch];
+                    if (d > 0 || !no_first_hit_stylization) {
+                        uint16_t m = material[(uint64_t)d * N + c];
+                        int root = m < n_materials ? roots[m] : -1;
+                        double p[3] = {F(lv, 0, c), F(lv, 1, c), F(lv, 2, c)};
+                        if (root >= 0) or_apply_style(nodes, root, (uint32_t)d, myf, Lr, p, Lr);
                     }
-                    for (int ch = 0; ch < 3; ++ch) indirect[ch] += reused[ch] / sum;
-                } else if (bp > 0) {
-                    for (int ch = 0; ch < 3; ++ch) indirect[ch] += reused[ch] / bp / (double)used;
+                    for (int ch = 0; ch < 3; ++ch) radd[3 * (uint64_t)c + ch] = Lr[ch];
                 }
             }
-            double Lr[3];
-            for (int ch = 0; ch < 3; ++ch) Lr[ch] = F(lv, 15 + ch, c) + direct[ch] + indirect[ch];
-            if (d > 0 || !no_first_hit_stylization) {
-                uint16_t m = material[(uint64_t)d * N + c];
-                int root = m < n_materials ? roots[m] : -1;
-                double p[3] = {F(lv, 0, c), F(lv, 1, c), F(lv, 2, c)};
-                if (root >= 0) or_apply_style(nodes, root, (uint32_t)d, myf, Lr, p, Lr);
-            }
-            for (int ch = 0; ch < 3; ++ch) radd[3 * (uint64_t)c + ch] = Lr[ch];
+            free(buf);
         }
+        if (alloc_fail) return -1;
         if (nbr_out) nbr_out[d] = nbr_total;
         if (cluster && cluster_out)
             for (uint32_t e = 0; e < n; ++e) cluster_out[(uint64_t)d * N + pts[e]] = cid[pts[e]];
@@ -448,7 +461,6 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
     free(pts);
     free(kv);
     free(rng);
-    free(buf);
     free(cid);
     free(csize);
     free(prop);

@@ -398,8 +398,10 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
     const double out_b = 3.0 * asz;
     // every record read once, every level output written once and (below the top) read once
     st.bytes_algorithmic = (uint64_t)((double)st.vertices * (rec_b + out_b) + (double)plane * S * (L - 1) * out_b);
+    // per neighbour pair: |dp|^2 8 (3 sub, mul, 2 FMA), normal term 8 (4 FMA), luminance 3
+    // (sub, FMA), exp 1, accumulate 7 (weight-sum add, 3 FMA)
     const double K = 2.0 * desc->radius + 1.0;
-    st.flops_algorithmic = (double)st.vertices * K * K * 30.0;
+    st.flops_algorithmic = (double)st.vertices * K * K * 27.0;
     h->stats = st;
     return NLPF_OK;
 }

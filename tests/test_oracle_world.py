@@ -356,3 +356,36 @@ def test_clustering_matches_restatement_and_invariants(oracle, smis, K):
     for d in range(L):
         m = depth >= d
         assert np.allclose(rad[d][m], ref[d][m], rtol=1e-12, atol=1e-14), d
+
+
+def test_result_independent_of_thread_count(oracle, tmp_path):
+    """The oracle filters reservoir slots in parallel and each slot's points in order. With
+    more points than one batch (vram 1 MB at K = 4 gives 65536 slots) and neighbour counts
+    above K, its output must equal a one-thread run bit for bit."""
+    import os
+    import subprocess
+    import sys
+    rng = np.random.default_rng(11)
+    W, H, branch, L = 96, 96, 8, 1
+    rec, _ = make_records(rng, W, H, branch, L)
+    Np = W * H * branch
+    depth = np.zeros(Np, np.int32)
+    z8 = np.zeros((L, Np), np.uint8)
+    z16 = np.zeros((L, Np), np.uint16)
+    cfg = {"cluster": False, "radius_scale": [1.2], "reuse": [4], "vram": 1, "smis": True,
+           "no_first_hit_stylization": True}
+    film, rad, nbr = oracle.world_run(rec, z8, z8, z16, depth, W, H, branch, L, AMIN, AMAX, cfg, want_radiance=True)
+    assert nbr[0] > 4 * Np  # the reservoir draws
+    np.save(tmp_path / "rec.npy", rec)
+    script = (
+        "import sys, numpy as np, oracle as O\n"
+        f"rec = np.load(sys.argv[1]); Np = {Np}; L = {L}\n"
+        "z8 = np.zeros((L, Np), np.uint8); z16 = np.zeros((L, Np), np.uint16)\n"
+        f"f, r, n = O.world_run(rec, z8, z8, z16, np.zeros(Np, np.int32), {W}, {H}, {branch}, L,\n"
+        "    np.full(3, -0.5, np.float32), np.full(3, 0.5, np.float32), " + repr(cfg) + ", want_radiance=True)\n"
+        "np.save(sys.argv[2], r)\n")
+    env = dict(os.environ, OMP_NUM_THREADS="1",
+               PYTHONPATH=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    subprocess.run([sys.executable, "-c", script, str(tmp_path / "rec.npy"), str(tmp_path / "rad1.npy")],
+                   check=True, env=env)
+    assert np.array_equal(np.load(tmp_path / "rad1.npy"), rad)
