@@ -67,6 +67,33 @@ def timed_runs(h, desc, rec, anc, film, steps, warmup):
     return wall / steps * 1e3, lv_ms / steps
 
 
+def window_sweep(h, N, W, H, S, L, recs, steps=3, warmup=1):
+    """Level-kernel rate for every window radius 1..7 on the same records: per-pair work falls
+    with the square of the window while the bytes per vertex stay fixed, so the small windows
+    show how close the kernel's memory path comes to the HBM roof. recs: {"fp32": (rec,
+    None), "fp16": (rec, anchors)}."""
+    film = h.alloc(W * H * 12)
+    plane = W * H
+    out = {}
+    for name, (rec, anc) in recs.items():
+        prec, rec_b = (N.FP32, 60) if name == "fp32" else (N.FP16, 30)
+        rows = []
+        for r in range(1, 8):
+            desc = h.screen_desc(W, H, S, L, prec, r)
+            _, lv = timed_runs(h, desc, rec, anc, film, steps, warmup)
+            t = lv.sum() / 1e3
+            gbs = sum(level_bytes(S, plane, L, rec_b)) / 1e9 / t
+            tfl = plane * S * L * (2 * r + 1) ** 2 * FLOPS_PER_PAIR / t / 1e12
+            rows.append({"radius": r, "window": f"{2 * r + 1}x{2 * r + 1}", "level_ms_sum": lv.sum(),
+                         "vertices_per_s": plane * S * L / t, "hbm_gbs_algorithmic": gbs,
+                         "hbm_frac": gbs / HBM_PEAK_GBS, "tflops": tfl,
+                         "fp32_nonpacked_frac": tfl / FP32_NONPACKED_TFLOPS})
+            log(f"  {name} radius {r}: {lv.sum():.2f} ms, {gbs:.0f} GB/s, {tfl:.1f} TFLOP/s")
+        out[name] = rows
+    film.free()
+    return out
+
+
 def cpu_baseline(h, rec, W, H, S, L, radius, sig, table, root, slices):
     """FP32 OpenMP restatement (oracle/) on the first `slices` spp slices of the same records."""
     import oracle as O
@@ -367,6 +394,7 @@ def main():
     ap.add_argument("--quality-frames", type=int, default=8,
                     help="untimed frames after the timed ones whose film error vs FP64 is averaged")
     ap.add_argument("--no-extra", action="store_true", help="skip the config 3 / 5 / world-mode measurements")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the window-radius sweep of the level kernel")
     ap.add_argument("--world-only", action="store_true",
                     help="only the world-mode measurement (config 1), printed as its own JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -495,6 +523,9 @@ def main():
             fp16["l2_err_vs_fp64"] = float(np.linalg.norm(f16 - film_ref) / np.linalg.norm(film_ref))
         result["fp16"] = fp16
         result["fp16_mlp"] = None if args.no_mlp else mlp_leg(h, args, d16, rec16, anc, film16, film_ref, verts, S)
+        if not args.no_sweep:
+            log("window sweep, radius 1..7")
+            result["window_sweep"] = window_sweep(h, N, W, H, S, L, {"fp32": (rec32, None), "fp16": (rec16, anc)})
         rec16.free()
         anc.free()
     else:

@@ -12,13 +12,14 @@ hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant
 hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, int variant, hipStream_t st);
 int screen_variants_fp32();
 
-// tile variant of the screen gather: NLPF_SCREEN_VARIANT overrides the default (for tuning)
+// tile variant of the screen gather: -1 takes each radius's default; NLPF_SCREEN_VARIANT
+// forces one variant for every radius (for tuning)
 static int screen_variant() {
     static int v = -2;
     if (v == -2) {
         const char* e = getenv("NLPF_SCREEN_VARIANT");
-        v = e ? atoi(e) : 0;
-        if (v < 0 || v >= screen_variants_fp32()) v = 0;
+        v = e ? atoi(e) : -1;
+        if (v < 0 || v >= screen_variants_fp32()) v = -1;
     }
     return v;
 }

@@ -1,22 +1,29 @@
 // Screen-mode level kernels, fp16 record storage (float accumulation), radius 1..7,
-// tile variants (waves per workgroup, rows per lane) selected at run time.
+// tile variants (waves per workgroup, rows per lane) selected at run time; variant -1
+// takes the radius's default.
 #include "screen_impl.hpp"
 
 namespace nlpf {
 
+// Variants: 4 waves per workgroup with 2 (variant 0, 64x8 tiles), 4 (1) or 3 (2) rows per
+// lane. Default per radius: the fastest at 1920x1080x16 spp x 5 levels on MI355X, FP32 and
+// FP16 records alike (also timed and never fastest: 8 or 2 waves with 2 rows, 2 waves with 4).
+// The 2-row tiles reach 256 VGPRs and spill to AGPRs at radius 2, 5 and 6 (one wave per
+// SIMD); those radii take 3 or 4 rows per lane.
+static constexpr int kDefaultVariant[8] = {0, 2, 2, 2, 0, 1, 2, 0};
+
 template <int RAD>
 static hipError_t dispatch_fp16(const ScreenArgs<float>& a, int variant, hipStream_t st) {
     constexpr int K = 2 * RAD + 1;
+    if (variant < 0) variant = kDefaultVariant[RAD];
     switch (variant) {
         case 1: return launch_screen_level<__half, float, RAD, 4, (4 < K ? 4 : K)>(a, st);
-        case 0: return launch_screen_level<__half, float, RAD, 4, (2 < K ? 2 : K)>(a, st);
-        case 2: return launch_screen_level<__half, float, RAD, 8, (2 < K ? 2 : K)>(a, st);
-        case 3: return launch_screen_level<__half, float, RAD, 2, (4 < K ? 4 : K)>(a, st);
-        default: return launch_screen_level<__half, float, RAD, 4, (2 < K ? 2 : K)>(a, st);
+        case 2: return launch_screen_level<__half, float, RAD, 4, 3>(a, st);
+        default: return launch_screen_level<__half, float, RAD, 4, 2>(a, st);
     }
 }
 
-int screen_variants_fp16() { return 4; }
+int screen_variants_fp16() { return 3; }
 
 hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st) {
     switch (radius) {

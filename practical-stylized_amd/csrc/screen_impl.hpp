@@ -31,7 +31,7 @@ struct ScreenArgs {
     uint32_t abx, aby;  // anchor blocks per row / column (fp16)
     A kp, kn, kl;
     A rscale;  // radiance decode factor (FP16 records store radiance / NLPF_FP16_RADIANCE_SCALE)
-    const uint8_t* tile_mask;  // optional [tiles_y][tiles_x]: 0 = skip the tile (all slices)
+    const uint8_t* tile_mask;  // optional [ceil(H/8)][ceil(W/64)]: 0 = skip the 64x8 tile (all slices)
 };
 
 __device__ __forceinline__ float ld_rec(const float* p, size_t i) { return p[i]; }
@@ -301,6 +301,7 @@ __global__ void screen_film_kernel(const A* out, uint32_t S, uint32_t out_levels
 template <typename S, typename A, int RAD, int NW, int R>
 hipError_t launch_screen_level(const ScreenArgs<A>& a0, hipStream_t st) {
     constexpr int TW = 64, TH = NW * R;
+    if (a0.tile_mask && TH != 8) return hipErrorInvalidValue;  // masks cover 64x8 tiles
     ScreenArgs<A> a = a0;
     a.tiles_x = (a.W + TW - 1) / TW;
     a.tiles_y = (a.H + TH - 1) / TH;

@@ -71,12 +71,12 @@ def predict(h, desc, rec, slices):
     return np.moveaxis(y, -1, 1)  # (slices, 3, H, W)
 
 
-def check_upper_levels(host, lv, film, table, root, tol=1e-5):
+def check_upper_levels(host, lv, film, table, root, tol=1e-5, radius=R):
     """Levels k-1..0 and the film from the FP64 oracle run on the GPU's depth-k output."""
     mod = host[:, :K].copy()
     thr = mod[:, K - 1, 12:15].astype(np.float64)
     mod[:, K - 1, 9:12] = (mod[:, K - 1, 9:12].astype(np.float64) + thr * lv[:, K]).astype(np.float32)
-    of, olv = O.screen_run(mod, W, H, S, K, R, SIG, nodes=table, root=root, want_levels=True)
+    of, olv = O.screen_run(mod, W, H, S, K, radius, SIG, nodes=table, root=root, want_levels=True)
     e = pixel_rel_err(np.moveaxis(lv[:, :K], 2, -1), np.moveaxis(olv, 2, -1))
     assert e.max() <= tol, e.max()
     assert pixel_rel_err(film, of).max() <= tol
@@ -130,8 +130,12 @@ def dilate(mask, times):
     return binary_dilation(mask, structure=np.ones((3, 3), bool), iterations=times) if times else mask
 
 
-def test_partial_fallback_selects_tiles(h, recs):
-    desc, rec, host = recs
+@pytest.mark.parametrize("radius", [R, 3])
+def test_partial_fallback_selects_tiles(h, recs, radius):
+    """At radius 3 the plain levels run 64x12 tiles while the masked fallback launches run the
+    64x8 tiles the mask describes."""
+    _, rec, host = recs
+    desc = h.screen_desc(W, H, S, L, N.FP32, radius)
     table, root = style(h, False)
     h.mlp_init(11)
     sj = 0
@@ -142,7 +146,7 @@ def test_partial_fallback_selects_tiles(h, recs):
     mod = host[sj:sj + 1, :K].copy()
     thr_k = mod[0, K - 1, 12:15].astype(np.float64)
     mod[0, K - 1, 9:12] = (mod[0, K - 1, 9:12].astype(np.float64) + thr_k * pred[sj]).astype(np.float32)
-    _, plv = O.screen_run(mod, W, H, 1, K, R, SIG, want_levels=True)
+    _, plv = O.screen_run(mod, W, H, 1, K, radius, SIG, want_levels=True)
     err = tile_rel_l2(plv[0, 0], lv0[sj, 0])
     e = np.sort(err.ravel())
     gaps = np.diff(e)
@@ -169,7 +173,7 @@ def test_partial_fallback_selects_tiles(h, recs):
             assert e_fb[pm].max() <= 1e-6  # fallback tiles: exact (their halo was refiltered)
             if (~pm).any():
                 assert e_ml[~pm].max() <= 1e-6  # the rest: the network's estimate
-        check_upper_levels(host, lv, film, table, root)
+        check_upper_levels(host, lv, film, table, root, radius=radius)
         # the film inside the level-0 fallback tiles is the plain NL-PF's
         p0 = np.repeat(np.repeat(mask0, 8, 0), 64, 1)[:H, :W]
         assert pixel_rel_err(film[p0], film0[p0]).max() <= 1e-5

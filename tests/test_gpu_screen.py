@@ -61,6 +61,31 @@ def test_fp32_parity_ramp_hue(h, radius, W, H, S, L):
     rec.free()
 
 
+@pytest.mark.parametrize("radius", range(1, 8))
+def test_every_radius_fp32_fp16(h, radius):
+    """Each radius runs its own default tile shape (64x8, 64x12 or 64x16 outputs per
+    workgroup); the image's edges cut every tile shape's last row and column of tiles."""
+    W, H, S, L = 131, 37, 2, 2
+    table, root = ramp_hue_table(radius)
+    h.set_style(table, [root])
+    h.set_sigmas(*SIG)
+    desc, rec, _ = gen_screen(h, W, H, S, L, N.FP32, seed=100 + radius, radius=radius)
+    film, lv = run_screen(h, desc, rec)
+    host = host_records_fp32(rec, W, H, S, L)
+    of, olv = O.screen_run(host, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+    check_levels(lv, olv, 1e-5)
+    assert pixel_rel_err(film, of).max() <= 1e-5
+    rec.free()
+    desc, rec, anc = gen_screen(h, W, H, S, L, N.FP16, seed=100 + radius, radius=radius)
+    film, lv = run_screen(h, desc, rec, anc)
+    dec = decode_fp16(rec, anc, W, H, S, L)
+    of, olv = O.screen_run(dec, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+    check_levels(lv, olv, 1e-5)
+    assert pixel_rel_err(film, of).max() <= 1e-5
+    rec.free()
+    anc.free()
+
+
 def test_fp32_parity_unstylized_and_determinism(h):
     h.set_style(N.style_table([N.toon(0.25)]), [-1])
     W, H, S, L = 130, 50, 2, 3
