@@ -10,6 +10,7 @@
 // log2(e)/(2 sigma^2) factors are folded into kp, kn, kl, so one v_exp_f32 per pair remains.
 // Row partial sums (2r+1 terms) are folded into Kahan-compensated totals.
 #pragma once
+#include <type_traits>
 #include "common.hpp"
 #include "style.hpp"
 
@@ -91,9 +92,11 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     // vector when W is even): XS0 = tx0 - RAD - OFF, the window column tx0 - RAD sits at OFF
     constexpr int OFF = RAD & 1, WS = TW + 2 * RAD + 2 * OFF, WP = WS / 2;
     // guide features and level input of the tile + halo: (p*kp, lum*kl), (n, in.r), (in.g, in.b)
-    __shared__ V4<A> shA[HT * WS];
-    __shared__ V4<A> shB[HT * WS];
-    __shared__ V2<A> shC[HT * WS];
+    constexpr int NPAIR = HT * WP, NIT = (NPAIR + NT - 1) / NT;
+    constexpr bool HALF = std::is_same<S, __half>::value;  // FP16 records carry position anchors
+    __shared__ V4<A> shA[HT * WS + 2];
+    __shared__ V4<A> shB[HT * WS + 2];
+    __shared__ V2<A> shC[HT * WS + 2];
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
@@ -108,48 +111,25 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     const bool even_w = (W & 1) == 0;
 
     // ---- stage tile + halo in LDS, two horizontally adjacent records per thread ----
-    for (int i = threadIdx.x; i < HT * WP; i += NT) {
-        const int r = i / WP, cp = i - r * WP;
-        const int gx = tx0 - RAD - OFF + 2 * cp, gy = ty0 - RAD + r;
-        const bool row_ok = gy >= 0 && gy < H;
-        const bool ok0 = row_ok && gx >= 0 && gx < W, ok1 = row_ok && gx + 1 >= 0 && gx + 1 < W;
-        A v[NLPF_SCREEN_PLANES][2], ch[3][2];
-        if (ok0 && ok1 && even_w) {
-            const size_t o = (size_t)gy * W + gx;
-#pragma unroll
-            for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) ld_rec2(rec, c * plane + o, v[c][0], v[c][1]);
-            if (child) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) ld_rec2(child, c * plane + o, ch[c][0], ch[c][1]);
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                if (j == 0 ? ok0 : ok1) {
-                    const size_t o = (size_t)gy * W + (gx + j);
-#pragma unroll
-                    for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) v[c][j] = (A)ld_rec(rec, c * plane + o);
-                    if (child) {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) ch[c][j] = child[c * plane + o];
-                    }
-                }
-            }
-        }
+    // put(): guide features and level input of one loaded record pair into LDS slots
+    // idx, idx + 1; records outside the image get a far-away guide position, which drives
+    // their weight to exactly 0
+    auto put = [&](int idx, bool ok0, bool ok1, int gy, int gx, const A (&v)[NLPF_SCREEN_PLANES][2],
+                   const A (&ch)[3][2], bool has_child) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             V4<A> fa, fb;
             V2<A> fc;
             if (j == 0 ? ok0 : ok1) {
                 A px = v[0][j], py = v[1][j], pz = v[2][j];
-                if (anc) {
+                if (HALF) {
                     const float* q = anc + ((size_t)(gy >> 3) * a.abx + ((gx + j) >> 3)) * 3;
                     px += (A)q[0];
                     py += (A)q[1];
                     pz += (A)q[2];
                 }
                 A ir = v[9][j] * a.rscale, ig = v[10][j] * a.rscale, ib = v[11][j] * a.rscale;
-                if (child) {
+                if (has_child) {
                     ir = fma(v[12][j], ch[0][j], ir);
                     ig = fma(v[13][j], ch[1][j], ig);
                     ib = fma(v[14][j], ch[2][j], ib);
@@ -158,14 +138,79 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
                 fb = V4<A>{v[3][j], v[4][j], v[5][j], ir};
                 fc = V2<A>{ig, ib};
             } else {
-                // outside the image: a far-away guide position drives the weight to exactly 0
                 fa = V4<A>{(A)1e18, (A)1e18, (A)1e18, (A)0};
                 fb = V4<A>{(A)0, (A)0, (A)0, (A)0};
                 fc = V2<A>{(A)0, (A)0};
             }
-            shA[r * WS + 2 * cp + j] = fa;
-            shB[r * WS + 2 * cp + j] = fb;
-            shC[r * WS + 2 * cp + j] = fc;
+            shA[idx + j] = fa;
+            shB[idx + j] = fb;
+            shC[idx + j] = fc;
+        }
+    };
+    if (HALF && even_w) {
+        // FP16 records, even width and even staged columns: the two records of a pair are both
+        // inside or both outside the image. Every pair loads from its position clamped into
+        // the image (one aligned 2-element load per plane) and out-of-image pairs take the
+        // sentinel, so the loop body is branch-free and the loads of consecutive iterations are
+        // in flight together. Threads past the last pair write two spare LDS slots. (FP16
+        // passes run 1.5-3 % faster this way; the same form with FP32 records ran 4-11 %
+        // slower than the loop below, so FP32 and FP64 keep that loop.)
+        auto stage_even = [&](auto has_child_c) __attribute__((always_inline)) {
+            constexpr bool HC = decltype(has_child_c)::value;
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int i0 = (int)threadIdx.x + it * NT;
+                const bool live = NPAIR % NT == 0 || i0 < NPAIR;
+                const int i = live ? i0 : NPAIR - 1;
+                const int r = i / WP, cp = i - r * WP;
+                const int gx = tx0 - RAD - OFF + 2 * cp, gy = ty0 - RAD + r;
+                const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
+                const int gxc = min(max(gx, 0), W - 2), gyc = min(max(gy, 0), H - 1);
+                const size_t o = (size_t)gyc * W + gxc;
+                A v[NLPF_SCREEN_PLANES][2], ch[3][2] = {};
+#pragma unroll
+                for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) ld_rec2(rec, c * plane + o, v[c][0], v[c][1]);
+                if (HC) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ld_rec2(child, c * plane + o, ch[c][0], ch[c][1]);
+                }
+                put(live ? r * WS + 2 * cp : HT * WS, ok, ok, gyc, gxc, v, ch, HC);
+            }
+        };
+        if (child)
+            stage_even(std::true_type{});
+        else
+            stage_even(std::false_type{});
+    } else {
+        for (int i = threadIdx.x; i < NPAIR; i += NT) {
+            const int r = i / WP, cp = i - r * WP;
+            const int gx = tx0 - RAD - OFF + 2 * cp, gy = ty0 - RAD + r;
+            const bool row_ok = gy >= 0 && gy < H;
+            const bool ok0 = row_ok && gx >= 0 && gx < W, ok1 = row_ok && gx + 1 >= 0 && gx + 1 < W;
+            A v[NLPF_SCREEN_PLANES][2], ch[3][2];
+            if (ok0 && ok1 && even_w) {
+                const size_t o = (size_t)gy * W + gx;
+#pragma unroll
+                for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) ld_rec2(rec, c * plane + o, v[c][0], v[c][1]);
+                if (child) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ld_rec2(child, c * plane + o, ch[c][0], ch[c][1]);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (j == 0 ? ok0 : ok1) {
+                        const size_t o = (size_t)gy * W + (gx + j);
+#pragma unroll
+                        for (int c = 0; c < NLPF_SCREEN_PLANES; ++c) v[c][j] = (A)ld_rec(rec, c * plane + o);
+                        if (child) {
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) ch[c][j] = child[c * plane + o];
+                        }
+                    }
+                }
+            }
+            put(r * WS + 2 * cp, ok0, ok1, gy, gx, v, ch, child != nullptr);
         }
     }
     __syncthreads();
@@ -302,6 +347,7 @@ template <typename S, typename A, int RAD, int NW, int R>
 hipError_t launch_screen_level(const ScreenArgs<A>& a0, hipStream_t st) {
     constexpr int TW = 64, TH = NW * R;
     if (a0.tile_mask && TH != 8) return hipErrorInvalidValue;  // masks cover 64x8 tiles
+    if (std::is_same<S, __half>::value && !a0.anchors) return hipErrorInvalidValue;
     ScreenArgs<A> a = a0;
     a.tiles_x = (a.W + TW - 1) / TW;
     a.tiles_y = (a.H + TH - 1) / TH;
