@@ -152,6 +152,11 @@ struct FilterArgs {
     float* rad_out;     // [N][3] level radiance (v.radiance)
     unsigned long long* nbr_count;   // NBR_SLOTS counters of this depth (spread to avoid contention)
     unsigned long long* pair_count;  // NBR_SLOTS counters of SMIS pdf evaluations
+    // SMIS denominators shared within a cluster (wave kernel, clustering on): 0 = each vertex
+    // sums its own; 1 = build: the first entry of each uncapped list sums them for the whole
+    // list into smis_sum (no radiance); 2 = use smis_sum for uncapped lists
+    int smis_phase;
+    double* smis_sum;  // [N] sum over the list of cos_m(wi_j), by reused vertex j
 };
 
 // One thread per vertex: for small neighbour sets (a few per vertex), where a wavefront per
@@ -380,6 +385,39 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     const uint32_t used = cnt < a.K ? cnt : a.K;
     const float inv_used = used ? 1.0f / (float)used : 0.0f;
     auto get = [&](uint32_t i) -> uint32_t { return i < WF_CAP ? lb[i] : gbuf[i]; };
+    // With clustering, an uncapped list (cnt <= K) is every member of the vertex's cluster with
+    // its flags, in the walk order around the cluster seed: each vertex j of the list has this
+    // same list. Its first entry sums pdf_m(wi_j) over it once for every j (in the order used
+    // below, so the sums are the ones each member would form), and the members look them up.
+    const bool shared = a.smis_phase != 0 && cnt <= a.K;
+    if (a.smis_phase == 1) {
+        if (!shared || used == 0 || get(0) != c) return;  // uniform per wave
+        for (uint32_t k0 = 0; k0 < used; k0 += 64) {
+            const uint32_t k = k0 + (uint32_t)lane;
+            uint32_t j = 0;
+            float3 wi = make_float3(0.0f, 0.0f, 0.0f);
+            if (k < used) {
+                j = get(k);
+                const float4 iw = a.pk[j].iwi_mis;
+                wi = make_float3(iw.x, iw.y, iw.z);
+            }
+            const double wx = wi.x, wy = wi.y, wz = wi.z;
+            double sum_cos = 0.0;
+            for (uint32_t m0 = 0; m0 < used; m0 += 64) {
+                const uint32_t mn = used - m0 < 64 ? used - m0 : 64;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if ((uint32_t)lane < mn) sb[lane] = smis_rec(a.pk[get(m0 + (uint32_t)lane)]);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
+                for (uint32_t m = 0; m < mn; ++m) sum_cos += smis_term(sb[m], wi, wx, wy, wz);
+            }
+            if (k < used) a.smis_sum[j] = sum_cos;
+        }
+        if (lane == 0) atomicAdd(a.pair_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)used * used);
+        return;
+    }
     // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
     // the exact sum (a plain running sum drifts by ~n ulp)
     float dsum[3] = {0, 0, 0}, dc[3] = {0, 0, 0}, isum[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
@@ -418,16 +456,20 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
             // sum_m pdf_m(wi_k) over every reused neighbour m, in order, in FP64
             const double wx = wi.x, wy = wi.y, wz = wi.z;
             double sum_cos = 0.0;
-            for (uint32_t m0 = 0; m0 < used; m0 += 64) {
-                const uint32_t mn = used - m0 < 64 ? used - m0 : 64;
-                // every lane has read the previous chunk before it is overwritten
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-                __builtin_amdgcn_wave_barrier();
-                if ((uint32_t)lane < mn) sb[lane] = smis_rec(a.pk[get(m0 + (uint32_t)lane)]);
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-                __builtin_amdgcn_wave_barrier();
+            if (shared) {
+                if (active && ind_ok) sum_cos = a.smis_sum[get(k)];
+            } else {
+                for (uint32_t m0 = 0; m0 < used; m0 += 64) {
+                    const uint32_t mn = used - m0 < 64 ? used - m0 : 64;
+                    // every lane has read the previous chunk before it is overwritten
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    if ((uint32_t)lane < mn) sb[lane] = smis_rec(a.pk[get(m0 + (uint32_t)lane)]);
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
 #pragma unroll 4
-                for (uint32_t m = 0; m < mn; ++m) sum_cos += smis_term(sb[m], wi, wx, wy, wz);
+                    for (uint32_t m = 0; m < mn; ++m) sum_cos += smis_term(sb[m], wi, wx, wy, wz);
+                }
             }
             const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);
             if (active && ind_ok) {
@@ -454,7 +496,8 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
         for (int i = 0; i < 6; ++i) v[i] += __shfl_xor(v[i], off);
     if (lane == 0) {
         atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)cnt);
-        if (a.smis) atomicAdd(a.pair_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)used * used);
+        if (a.smis && !shared)
+            atomicAdd(a.pair_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)used * used);
         v3<float> L{le.x + v[0] + v[3], le.y + v[1] + v[4], le.z + v[2] + v[5]};
         if (a.stylize) {
             const uint32_t m = a.material[c];
@@ -606,12 +649,27 @@ __global__ void w_cluster_reset_seed_kernel(const float* rec, uint64_t N, const 
     get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
     bucket_seed[cell_bucket(c[0], c[1], c[2], g.table)] = 0xFFFFFFFFu;
 }
+// seed index of one seeding round over the bucket-sorted entries: seed_rank[e] = number of
+// seed entries before e (exclusive scan of these flags), seed_entry[seed_rank[e]] = e
+__global__ void w_seed_flags_kernel(const uint32_t* ids_sorted, uint32_t n, const uint32_t* cluster_id,
+                                    uint32_t* flags) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > n) return;
+    flags[e] = e < n && cluster_id[ids_sorted[e]] == ids_sorted[e] ? 1u : 0u;
+}
+__global__ void w_seed_scatter_kernel(const uint32_t* flags, const uint32_t* seed_rank, uint32_t n,
+                                      uint32_t* seed_entry) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n && flags[e]) seed_entry[seed_rank[e]] = e;
+}
 // every unclustered vertex proposes to the first seed (visit order of :60-123) within the
 // radius that still has room; key = seed << 32 | vertex so a sort groups proposals by seed
-// with the smallest vertex ids first
+// with the smallest vertex ids first. Only seeds can take a proposal, so the walk visits a
+// bucket's seed entries (seed_entry[seed_rank[start] .. seed_rank[end]), in bucket order).
 __global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
                                          const float4* pos_sorted, const uint32_t* bstart, const uint32_t* bend,
-                                         const uint32_t* cluster_id, const uint32_t* cluster_size, uint32_t cap,
+                                         const uint32_t* seed_rank, const uint32_t* seed_entry,
+                                         const uint32_t* cluster_size, uint32_t cap, const uint32_t* cluster_id,
                                          unsigned long long* prop, uint32_t* n_prop, uint32_t* n_unclustered) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -628,11 +686,12 @@ __global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uin
                 if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] || cz >= (int)g.res[2])
                     continue;
                 const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
-                for (uint32_t e = bstart[b]; e < bend[b]; ++e) {
-                    const float4 qe = pos_sorted[e];
+                const uint32_t s1 = seed_rank[bend[b]];
+                for (uint32_t si = seed_rank[bstart[b]]; si < s1; ++si) {
+                    const float4 qe = pos_sorted[seed_entry[si]];
                     if (!(dist3(qe.x - px, qe.y - py, qe.z - pz) <= g.radius)) continue;
                     const uint32_t j = __float_as_uint(qe.w);
-                    if (cluster_id[j] == j && cluster_size[j] < cap) {
+                    if (cluster_size[j] < cap) {
                         const uint32_t slot = atomicAdd(n_prop, 1u);
                         prop[slot] = ((unsigned long long)j << 32) | i;
                         return;
@@ -640,15 +699,26 @@ __global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uin
                 }
             }
 }
+// first index of the run of proposals to `seed` that holds index e (binary search: the keys
+// are sorted, seed in the high word)
+__device__ __forceinline__ uint32_t run_start(const unsigned long long* prop_sorted, uint32_t e, uint32_t seed) {
+    uint32_t lo = 0, hi = e;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint32_t)(prop_sorted[mid] >> 32) < seed)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 // accept the first (cap - size) proposals of each seed, in ascending vertex order
 __global__ void w_cluster_accept_kernel(const unsigned long long* prop_sorted, uint32_t n, uint32_t* cluster_id,
                                         uint32_t* cluster_size, uint32_t cap, uint32_t* n_accepted) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const uint32_t seed = (uint32_t)(prop_sorted[e] >> 32);
-    uint32_t first = e;
-    while (first > 0 && (uint32_t)(prop_sorted[first - 1] >> 32) == seed) --first;
-    const uint32_t rank = e - first;
+    const uint32_t rank = e - run_start(prop_sorted, e, seed);
     const uint32_t size0 = cluster_size[seed];
     if (size0 + rank < cap) {
         cluster_id[(uint32_t)prop_sorted[e]] = seed;
@@ -661,9 +731,7 @@ __global__ void w_cluster_size_kernel(const unsigned long long* prop_sorted, uin
     if (e >= n) return;
     const uint32_t seed = (uint32_t)(prop_sorted[e] >> 32);
     if (e + 1 < n && (uint32_t)(prop_sorted[e + 1] >> 32) == seed) return;  // last of its run
-    uint32_t first = e;
-    while (first > 0 && (uint32_t)(prop_sorted[first - 1] >> 32) == seed) --first;
-    const uint32_t proposals = e - first + 1;
+    const uint32_t proposals = e - run_start(prop_sorted, e, seed) + 1;
     const uint32_t size0 = cluster_size[seed];
     const uint32_t room = cap > size0 ? cap - size0 : 0u;
     cluster_size[seed] = size0 + (proposals < room ? proposals : room);
@@ -687,13 +755,16 @@ struct WorldWork {
     NbrRec* pk = nullptr;
     float *ind = nullptr, *rad = nullptr;
     Pcg32* rng = nullptr;
+    uint32_t *seed_flags = nullptr, *seed_rank = nullptr, *seed_entry = nullptr;
+    double* smis_sum = nullptr;
     void* cub = nullptr;
 };
 
 static void world_free(WorldWork& w) {
     void* ps[] = {w.flag, w.pts, w.keys, w.keys_sorted, w.ids_sorted, w.bstart, w.bend, w.bucket_seed,
                   w.cluster_id, w.cluster_size, w.counters, w.nbuf, w.prop, w.prop_sorted, w.nbr,
-                  w.pos_sorted, w.pk, w.ind, w.rad, w.rng, w.cub};
+                  w.pos_sorted, w.pk, w.ind, w.rad, w.rng, w.seed_flags, w.seed_rank, w.seed_entry,
+                  w.smis_sum, w.cub};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     w = WorldWork{};
@@ -752,15 +823,20 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.ind, 8 * N);
     rc |= dalloc(h, &w.rad, 3 * N);
     rc |= dalloc(h, &w.rng, rng_n);
+    rc |= dalloc(h, &w.seed_flags, N + 1);
+    rc |= dalloc(h, &w.seed_rank, N + 1);
+    rc |= dalloc(h, &w.seed_entry, N);
+    rc |= dalloc(h, &w.smis_sum, N);
     if (rc) return rc;
     // temp storage for the hipcub primitives at the largest size
-    size_t b1 = 0, b2 = 0, b3 = 0;
+    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
     hipcub::CountingInputIterator<uint32_t> it(0);
     NLPF_HIP(h, hipcub::DeviceSelect::Flagged(nullptr, b1, it, w.flag, w.pts, w.counters, (int)N, h->stream));
     NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(nullptr, b2, w.keys, w.keys_sorted, w.pts, w.ids_sorted, (int)N, 0,
                                                    32, h->stream));
     NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(nullptr, b3, w.prop, w.prop_sorted, (int)N, 0, 64, h->stream));
-    w.cub_bytes = std::max(b1, std::max(b2, b3));
+    NLPF_HIP(h, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, w.seed_flags, w.seed_rank, (int)(N + 1), h->stream));
+    w.cub_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
     NLPF_HIP(h, hipMalloc(&w.cub, w.cub_bytes));
     // empty bucket ranges; the filter of every depth restores the entries it set
     NLPF_HIP(h, hipMemsetAsync(w.bstart, 0, sizeof(uint32_t) * table, h->stream));
@@ -885,12 +961,20 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                        w.cluster_id, w.bucket_seed);
                     hipLaunchKernelGGL(w_cluster_reset_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
                                        w.bucket_seed);
+                    // seed entries of every bucket, for the propose walks of this round
+                    hipLaunchKernelGGL(w_seed_flags_kernel, grid1((uint64_t)n + 1), dim3(256), 0, st, w.ids_sorted, n,
+                                       w.cluster_id, w.seed_flags);
+                    tb = w.cub_bytes;
+                    NLPF_HIP(h, hipcub::DeviceScan::ExclusiveSum(w.cub, tb, w.seed_flags, w.seed_rank, (int)(n + 1),
+                                                                 st));
+                    hipLaunchKernelGGL(w_seed_scatter_kernel, grid1(n), dim3(256), 0, st, w.seed_flags, w.seed_rank, n,
+                                       w.seed_entry);
                     uint32_t n_uncl = 0;
                     for (int inner = 0;; ++inner) {
                         NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * 4, st));
                         hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                           w.pos_sorted, w.bstart, w.bend, w.cluster_id, w.cluster_size, K, w.prop,
-                                           w.counters + 1, w.counters + 2);
+                                           w.pos_sorted, w.bstart, w.bend, w.seed_rank, w.seed_entry, w.cluster_size,
+                                           K, w.cluster_id, w.prop, w.counters + 1, w.counters + 2);
                         if ((rc = check_launch(h, "world cluster"))) return rc;
                         uint32_t hc[4];
                         NLPF_HIP(h, hipMemcpyAsync(hc, w.counters, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -941,41 +1025,48 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             const bool single = max_cand <= K && (uint64_t)n * std::max<uint64_t>(max_cand, 1) <= w.nbuf_elems;
             const uint64_t batch =
                 single ? n : std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
-            for (uint64_t off = 0; off < n; off += batch) {
-                const uint32_t cnt = (uint32_t)std::min<uint64_t>(batch, n - off);
-                FilterArgs fa;
-                fa.rec = recd;
-                fa.N = N;
-                fa.pts = w.pts;
-                fa.off = (uint32_t)off;
-                fa.count = cnt;
-                fa.new_flags = new_flags + (uint64_t)d * N;
-                fa.material = material + (uint64_t)d * N;
-                fa.cluster_id = cfg.cluster ? w.cluster_id : nullptr;
-                fa.pk = w.pk;
-                fa.pos_sorted = w.pos_sorted;
-                fa.bstart = w.bstart;
-                fa.bend = w.bend;
-                fa.g = g;
-                fa.K = K;
-                fa.nbuf = w.nbuf;
-                fa.stride = single ? (uint32_t)std::max<uint64_t>(max_cand, 1) : K;
-                fa.rng = w.rng;
-                fa.use_rng = single ? 0 : 1;
-                fa.nodes = h->d_nodes;
-                fa.roots = h->d_roots;
-                fa.n_materials = (uint32_t)h->h_roots.size();
-                fa.depth = d;
-                fa.smis = cfg.smis;
-                fa.flag_check = !cfg.disable_flag_check;
-                fa.stylize = (d > 0 || !cfg.no_first_hit_stylization) && !h->h_roots.empty();
-                fa.rad_out = rad;
-                fa.nbr_count = w.nbr + (uint64_t)d * NBR_SLOTS;
-                fa.pair_count = w.nbr + (uint64_t)(32 + d) * NBR_SLOTS;
-                if (wave_kernel)
-                    hipLaunchKernelGGL(w_filter_wave_kernel, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
-                else
-                    hipLaunchKernelGGL(w_filter_thread_kernel, grid1(cnt), dim3(256), 0, st, fa);
+            // clustering + SMIS on the wave kernel: a build pass over every batch fills the
+            // shared SMIS denominators before the filter pass reads them
+            const bool shared_smis = cfg.cluster && cfg.smis && wave_kernel;
+            for (int phase = shared_smis ? 1 : 0; phase <= (shared_smis ? 2 : 0); ++phase) {
+                for (uint64_t off = 0; off < n; off += batch) {
+                    const uint32_t cnt = (uint32_t)std::min<uint64_t>(batch, n - off);
+                    FilterArgs fa;
+                    fa.rec = recd;
+                    fa.N = N;
+                    fa.pts = w.pts;
+                    fa.off = (uint32_t)off;
+                    fa.count = cnt;
+                    fa.new_flags = new_flags + (uint64_t)d * N;
+                    fa.material = material + (uint64_t)d * N;
+                    fa.cluster_id = cfg.cluster ? w.cluster_id : nullptr;
+                    fa.pk = w.pk;
+                    fa.pos_sorted = w.pos_sorted;
+                    fa.bstart = w.bstart;
+                    fa.bend = w.bend;
+                    fa.g = g;
+                    fa.K = K;
+                    fa.nbuf = w.nbuf;
+                    fa.stride = single ? (uint32_t)std::max<uint64_t>(max_cand, 1) : K;
+                    fa.rng = w.rng;
+                    fa.use_rng = single ? 0 : 1;
+                    fa.nodes = h->d_nodes;
+                    fa.roots = h->d_roots;
+                    fa.n_materials = (uint32_t)h->h_roots.size();
+                    fa.depth = d;
+                    fa.smis = cfg.smis;
+                    fa.flag_check = !cfg.disable_flag_check;
+                    fa.stylize = (d > 0 || !cfg.no_first_hit_stylization) && !h->h_roots.empty();
+                    fa.rad_out = rad;
+                    fa.nbr_count = w.nbr + (uint64_t)d * NBR_SLOTS;
+                    fa.pair_count = w.nbr + (uint64_t)(32 + d) * NBR_SLOTS;
+                    fa.smis_phase = phase;
+                    fa.smis_sum = w.smis_sum;
+                    if (wave_kernel)
+                        hipLaunchKernelGGL(w_filter_wave_kernel, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
+                    else
+                        hipLaunchKernelGGL(w_filter_thread_kernel, grid1(cnt), dim3(256), 0, st, fa);
+                }
             }
             if ((rc = check_launch(h, "world filter"))) return rc;
             hipLaunchKernelGGL(w_clear_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bstart, w.bend);

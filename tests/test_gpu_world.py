@@ -144,26 +144,48 @@ def test_world_toon_depth1_mostly_exact():
     assert (e <= 1e-5).mean() >= 0.99, (e <= 1e-5).mean()
 
 
-@pytest.mark.parametrize("kernel", ["thread", "wave"])
-def test_world_cluster_mode_matches_oracle(kernel):
+@pytest.mark.parametrize("kernel,styled", [("thread", False), ("wave", False), ("wave", True)])
+def test_world_cluster_mode_matches_oracle(kernel, styled):
     """cluster = true: deterministic clustering (bucket-minimum seeds, ordered joining up to K),
     the gather around the seed restricted to the vertex's cluster, the reservoir over the K+1
-    members; bitwise reproducible, and equal to the oracle's restatement."""
+    members; bitwise reproducible, and equal to the oracle's restatement. The wave kernel
+    shares the SMIS denominators of uncapped lists within a cluster; with the style table,
+    members of one cluster carry different flags and so form different lists."""
     W, H, branch, L = 32, 32, 2, 3
     cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "cluster": True, "radius_scale": [3.0],
            "reuse": [16], "world_kernel": kernel}
-    f1, r1, host, st = run_world(cfg, W, H, branch, L)
-    f2, r2, _, _ = run_world(cfg, W, H, branch, L)
+    nodes, roots = styles() if styled else (None, ())
+    f1, r1, host, st = run_world(cfg, W, H, branch, L, nodes, roots)
+    f2, r2, _, _ = run_world(cfg, W, H, branch, L, nodes, roots)
     assert np.array_equal(f1, f2) and np.array_equal(r1, r2)
     assert np.isfinite(f1).all()
     # clusters cap the neighbour sets: at most reuse + 1 members (seed + K joiners)
     for d in range(L):
         if st["points"][d]:
             assert st["neighbors"][d] <= st["points"][d] * 17
+    ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots)
+    assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    sens = sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad)
+    check(f1, r1, ofilm, orad, host, L, sens=sens)
+
+
+def test_world_large_clusters_share_smis_sums():
+    """Large clusters under a reuse cap they never reach, so every list is uncapped: the SMIS
+    denominators are formed once per list (by its first entry) and looked up by every member.
+    A list of n members then costs n^2 pdf evaluations once instead of n^2 per member, so the
+    evaluations equal the neighbour visits. (Unstylized: with the style table this density
+    leaves 0.1 % of the depth-0 vertices up to 1.8e-5 from the FP64 oracle, the same with
+    and without the shared sums, as FP32 error of depth 1 is carried up.)"""
+    W, H, branch, L = 48, 40, 4, 3
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "cluster": True, "radius_scale": [8.0],
+           "reuse": [1024], "world_kernel": "wave"}
+    film, rad, host, st = run_world(cfg, W, H, branch, L)
     ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg)
     assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    assert st["neighbors"][0] > 16 * st["points"][0]  # large lists
+    assert [st["smis_pairs"][d] for d in range(L)] == [st["neighbors"][d] for d in range(L)]
     sens = sensitivity(host, W, H, branch, L, cfg, None, (), ofilm, orad)
-    check(f1, r1, ofilm, orad, host, L, sens=sens)
+    check(film, rad, ofilm, orad, host, L, sens=sens)
 
 
 def test_world_single_level_matches_oracle():
