@@ -98,6 +98,16 @@ __global__ void w_bucket_max_kernel(const uint32_t* keys, uint32_t n, const uint
     if (e == 0 || keys[e - 1] != k) atomicMax(out_max, bend[k] - e);
 }
 
+// largest cluster (seed + joiners) of the depth: with clustering, a bound on every vertex's
+// candidate count (its candidates are members of its own cluster)
+__global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint32_t* cluster_id,
+                                     const uint32_t* cluster_size, uint32_t* out_max) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = pts[t];
+    if (cluster_id[i] == i) atomicMax(out_max, cluster_size[i] + 1u);
+}
+
 __global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint32_t* bstart, uint32_t* bend) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -1007,12 +1017,16 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             // observable only through the reservoir streams, which are per batch slot. When no
             // point's candidate count can exceed K there are no draws, and the whole depth runs
             // as one launch with list slots sized by that count: first bounded by 27 x the
-            // largest bucket, else counted exactly.
+            // largest bucket (by the largest cluster when clustering), else counted exactly.
             NLPF_HIP(h, hipMemsetAsync(w.counters + 5, 0, 2 * sizeof(uint32_t), st));
-            hipLaunchKernelGGL(w_bucket_max_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bend,
-                               w.counters + 6);
+            if (cfg.cluster)
+                hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, w.pts, n, w.cluster_id,
+                                   w.cluster_size, w.counters + 6);
+            else
+                hipLaunchKernelGGL(w_bucket_max_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bend,
+                                   w.counters + 6);
             if ((rc = check_launch(h, "world bucket size"))) return rc;
-            const uint64_t bound = 27ull * read_u32(h, w.counters + 6, rc);
+            const uint64_t bound = (cfg.cluster ? 1ull : 27ull) * read_u32(h, w.counters + 6, rc);
             if (rc) return rc;
             uint64_t max_cand = bound;
             if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
