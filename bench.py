@@ -320,6 +320,24 @@ def world_leg(h_unused, N, cpu=True):
                "ms_per_frame": ms, "vertices": int(pts), "value": pts / (ms / 1e3), "unit": "vertices/s",
                "neighbours_per_vertex": nb / max(pts, 1)}
         host = hw.download_world(desc, bufs) if cpu else None
+    # the same records under the reference's default "pf" settings: clustering on, reuse
+    # [128, 64], default radius
+    with N.Handle({"type": "pf", "max_depth": L - 1, "branch": B}) as hw:
+        hw.set_style(nodes, roots)
+        desc = hw.world_desc(W, H, B, L)
+        bufs = hw.alloc_world(desc)
+        hw.generate_world_records(desc, bufs, seed=1)
+        hw.world_run(desc, bufs)
+        times = []
+        for _ in range(3):
+            hw.world_run(desc, bufs)
+            times.append(hw.stats()["total_ms"])
+        st = hw.stats()
+        ms_d = float(np.median(times))
+        pts_d = sum(st["points"][:L])
+        out["reference_defaults"] = {"config": "cluster, reuse [128, 64], radius_scale [0.1, 0.05], SMIS",
+                                     "ms_per_frame": ms_d, "value": pts_d / (ms_d / 1e3), "unit": "vertices/s",
+                                     "neighbours_per_vertex": sum(st["neighbors"][:L]) / max(pts_d, 1)}
     if cpu:
         log("world mode: FP64 oracle on the same frame (CPU)")
         out["cpu_oracle"] = world_cpu(host, W, H, B, L, cfg, nodes, roots, film, pts)
