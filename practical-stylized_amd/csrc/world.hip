@@ -62,6 +62,30 @@ __device__ __forceinline__ void kahan_add(T& sum, T& comp, T v) {
 
 __device__ __forceinline__ float pl(const float* rec, uint64_t N, int c, uint64_t i) { return rec[(uint64_t)c * N + i]; }
 
+// Wavefront-wide max / sum (every lane of the wavefront must reach the call): a kernel then
+// issues one atomic per wavefront on a shared counter instead of one per lane, which the
+// memory system would serialize.
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+    return v;
+}
+
+// counts[d] = paths with a vertex at depth d (path_depth >= d), d < L
+__global__ void w_depth_count_kernel(const int32_t* path_depth, uint64_t N, uint32_t L, uint32_t* counts) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int pd = i < N ? path_depth[i] : -1;
+    for (uint32_t d = 0; d < L; ++d) {
+        const uint32_t c = wave_sum(pd >= (int)d ? 1u : 0u);
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(counts + d, c);
+    }
+}
+
 __global__ void w_flag_kernel(const int32_t* path_depth, uint64_t N, int d, uint8_t* flag) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) flag[i] = path_depth[i] >= d ? 1 : 0;
@@ -79,23 +103,31 @@ __global__ void w_keys_kernel(const float* rec, uint64_t N, const uint32_t* pts,
 __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint32_t n, const float* rec, uint64_t N,
                                 uint32_t* bstart, uint32_t* bend, float4* pos_sorted, uint32_t* occupied) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const uint32_t k = keys[e];
-    if (e == 0 || keys[e - 1] != k) {
-        bstart[k] = e;
-        atomicAdd(occupied, 1u);
+    uint32_t head = 0;
+    if (e < n) {
+        const uint32_t k = keys[e];
+        if (e == 0 || keys[e - 1] != k) {
+            bstart[k] = e;
+            head = 1;
+        }
+        if (e == n - 1 || keys[e + 1] != k) bend[k] = e + 1;
+        const uint32_t i = ids[e];
+        pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
     }
-    if (e == n - 1 || keys[e + 1] != k) bend[k] = e + 1;
-    const uint32_t i = ids[e];
-    pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
+    head = wave_sum(head);
+    if ((threadIdx.x & 63) == 0 && head) atomicAdd(occupied, head);
 }
 
 // largest bucket (entries from one bucket start to its end), after w_ranges_kernel
 __global__ void w_bucket_max_kernel(const uint32_t* keys, uint32_t n, const uint32_t* bend, uint32_t* out_max) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const uint32_t k = keys[e];
-    if (e == 0 || keys[e - 1] != k) atomicMax(out_max, bend[k] - e);
+    uint32_t size = 0;
+    if (e < n) {
+        const uint32_t k = keys[e];
+        if (e == 0 || keys[e - 1] != k) size = bend[k] - e;
+    }
+    size = wave_max(size);
+    if ((threadIdx.x & 63) == 0 && size) atomicMax(out_max, size);
 }
 
 // largest cluster (seed + joiners) of the depth: with clustering, a bound on every vertex's
@@ -103,9 +135,13 @@ __global__ void w_bucket_max_kernel(const uint32_t* keys, uint32_t n, const uint
 __global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint32_t* cluster_id,
                                      const uint32_t* cluster_size, uint32_t* out_max) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t i = pts[t];
-    if (cluster_id[i] == i) atomicMax(out_max, cluster_size[i] + 1u);
+    uint32_t members = 0;
+    if (t < n) {
+        const uint32_t i = pts[t];
+        if (cluster_id[i] == i) members = cluster_size[i] + 1u;
+    }
+    members = wave_max(members);
+    if ((threadIdx.x & 63) == 0 && members) atomicMax(out_max, members);
 }
 
 __global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint32_t* bstart, uint32_t* bend) {
@@ -544,22 +580,25 @@ __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* 
                                   const uint32_t* cluster_id, const uint32_t* bstart, const uint32_t* bend,
                                   uint32_t* out_max) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t c = pts[t];
-    const uint32_t qc = cluster_id ? cluster_id[c] : c;
-    uint32_t cell[3];
-    get_cell(g, pl(rec, N, 0, qc), pl(rec, N, 1, qc), pl(rec, N, 2, qc), cell);
     uint32_t total = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
-                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] || cz >= (int)g.res[2])
-                    continue;
-                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
-                total += bend[b] - bstart[b];
-            }
-    atomicMax(out_max, total);
+    if (t < n) {
+        const uint32_t c = pts[t];
+        const uint32_t qc = cluster_id ? cluster_id[c] : c;
+        uint32_t cell[3];
+        get_cell(g, pl(rec, N, 0, qc), pl(rec, N, 1, qc), pl(rec, N, 2, qc), cell);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
+                    if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] ||
+                        cz >= (int)g.res[2])
+                        continue;
+                    const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
+                    total += bend[b] - bstart[b];
+                }
+    }
+    total = wave_max(total);
+    if ((threadIdx.x & 63) == 0 && total) atomicMax(out_max, total);
 }
 
 __global__ void w_fill_kernel(float* p, uint64_t n, float v) {
@@ -682,32 +721,43 @@ __global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uin
                                          const uint32_t* cluster_size, uint32_t cap, const uint32_t* cluster_id,
                                          unsigned long long* prop, uint32_t* n_prop, uint32_t* n_unclustered) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t i = pts[t];
-    if (cluster_id[i] != 0xFFFFFFFFu) return;
-    atomicAdd(n_unclustered, 1u);
-    const float px = pl(rec, N, 0, i), py = pl(rec, N, 1, i), pz = pl(rec, N, 2, i);
-    uint32_t cell[3];
-    get_cell(g, px, py, pz, cell);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
-                if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] || cz >= (int)g.res[2])
-                    continue;
-                const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
-                const uint32_t s1 = seed_rank[bend[b]];
-                for (uint32_t si = seed_rank[bstart[b]]; si < s1; ++si) {
-                    const float4 qe = pos_sorted[seed_entry[si]];
-                    if (!(dist3(qe.x - px, qe.y - py, qe.z - pz) <= g.radius)) continue;
-                    const uint32_t j = __float_as_uint(qe.w);
-                    if (cluster_size[j] < cap) {
-                        const uint32_t slot = atomicAdd(n_prop, 1u);
-                        prop[slot] = ((unsigned long long)j << 32) | i;
-                        return;
+    const uint32_t i = t < n ? pts[t] : 0u;
+    const bool open = t < n && cluster_id[i] == 0xFFFFFFFFu;
+    uint32_t seed = 0xFFFFFFFFu;
+    if (open) {
+        const float px = pl(rec, N, 0, i), py = pl(rec, N, 1, i), pz = pl(rec, N, 2, i);
+        uint32_t cell[3];
+        get_cell(g, px, py, pz, cell);
+        for (int cz = -1; cz <= 1 && seed == 0xFFFFFFFFu; ++cz)
+            for (int cy = -1; cy <= 1 && seed == 0xFFFFFFFFu; ++cy)
+                for (int cx = -1; cx <= 1 && seed == 0xFFFFFFFFu; ++cx) {
+                    const int x = (int)cell[0] + cx, y = (int)cell[1] + cy, z = (int)cell[2] + cz;
+                    if (x < 0 || y < 0 || z < 0 || x >= (int)g.res[0] || y >= (int)g.res[1] || z >= (int)g.res[2])
+                        continue;
+                    const uint32_t b = cell_bucket((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table);
+                    const uint32_t s1 = seed_rank[bend[b]];
+                    for (uint32_t si = seed_rank[bstart[b]]; si < s1; ++si) {
+                        const float4 qe = pos_sorted[seed_entry[si]];
+                        if (!(dist3(qe.x - px, qe.y - py, qe.z - pz) <= g.radius)) continue;
+                        const uint32_t j = __float_as_uint(qe.w);
+                        if (cluster_size[j] < cap) {
+                            seed = j;
+                            break;
+                        }
                     }
                 }
-            }
+    }
+    // one counter update per wavefront; proposal slots from a ballot prefix count (the
+    // proposals are sorted afterwards, so their order here does not matter)
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t opened = wave_sum(open ? 1u : 0u);
+    if (lane == 0 && opened) atomicAdd(n_unclustered, opened);
+    const bool has = seed != 0xFFFFFFFFu;
+    const uint64_t mask = __ballot(has);
+    uint32_t base = 0;
+    if (lane == 0 && mask) base = atomicAdd(n_prop, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, 0);
+    if (has) prop[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = ((unsigned long long)seed << 32) | i;
 }
 // first index of the run of proposals to `seed` that holds index e (binary search: the keys
 // are sorted, seed in the high word)
@@ -724,7 +774,7 @@ __device__ __forceinline__ uint32_t run_start(const unsigned long long* prop_sor
 }
 // accept the first (cap - size) proposals of each seed, in ascending vertex order
 __global__ void w_cluster_accept_kernel(const unsigned long long* prop_sorted, uint32_t n, uint32_t* cluster_id,
-                                        uint32_t* cluster_size, uint32_t cap, uint32_t* n_accepted) {
+                                        uint32_t* cluster_size, uint32_t cap) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const uint32_t seed = (uint32_t)(prop_sorted[e] >> 32);
@@ -732,7 +782,6 @@ __global__ void w_cluster_accept_kernel(const unsigned long long* prop_sorted, u
     const uint32_t size0 = cluster_size[seed];
     if (size0 + rank < cap) {
         cluster_id[(uint32_t)prop_sorted[e]] = seed;
-        atomicAdd(n_accepted, 1u);
     }
 }
 __global__ void w_cluster_size_kernel(const unsigned long long* prop_sorted, uint32_t n, uint32_t* cluster_size,
@@ -823,7 +872,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.bucket_seed, table);
     rc |= dalloc(h, &w.cluster_id, N);
     rc |= dalloc(h, &w.cluster_size, N);
-    rc |= dalloc(h, &w.counters, 8);
+    rc |= dalloc(h, &w.counters, 40);  // 0..7 per-depth scratch counters, 8..39 points per depth
     rc |= dalloc(h, &w.nbuf, nbuf_elems);
     rc |= dalloc(h, &w.prop, N);
     rc |= dalloc(h, &w.prop_sorted, N);
@@ -922,6 +971,14 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     rc = record_event(h, 0);
     if (rc) return rc;
 
+    // points of every depth (paths whose last vertex is at depth >= d), read back once
+    NLPF_HIP(h, hipMemsetAsync(w.counters + 8, 0, 32 * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(w_depth_count_kernel, grid1(N), dim3(256), 0, st, path_depth, N, L, w.counters + 8);
+    if ((rc = check_launch(h, "world depth counts"))) return rc;
+    uint32_t level_points[32] = {};
+    NLPF_HIP(h, hipMemcpyAsync(level_points, w.counters + 8, sizeof(level_points), hipMemcpyDeviceToHost, st));
+    NLPF_HIP(h, hipStreamSynchronize(st));
+
     nlpf_stats stt{};
     const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
     const float avg_ext = (ext[0] + ext[1] + ext[2]) / 3.0f;
@@ -932,8 +989,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
         hipcub::CountingInputIterator<uint32_t> it(0);
         size_t tb = w.cub_bytes;
         NLPF_HIP(h, hipcub::DeviceSelect::Flagged(w.cub, tb, it, w.flag, w.pts, w.counters, (int)N, st));
-        const uint32_t n = read_u32(h, w.counters, rc);
-        if (rc) return rc;
+        const uint32_t n = level_points[d];
         stt.points[d] = n;
         if (n > 0) {
             Grid g;
@@ -996,7 +1052,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                         NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(w.cub, tb, w.prop, w.prop_sorted, (int)n_prop, 0,
                                                                       64, st));
                         hipLaunchKernelGGL(w_cluster_accept_kernel, grid1(n_prop), dim3(256), 0, st, w.prop_sorted,
-                                           n_prop, w.cluster_id, w.cluster_size, K, w.counters + 3);
+                                           n_prop, w.cluster_id, w.cluster_size, K);
                         hipLaunchKernelGGL(w_cluster_size_kernel, grid1(n_prop), dim3(256), 0, st, w.prop_sorted, n_prop,
                                            w.cluster_size, K);
                         if (inner > 1 << 20) NLPF_FAIL(h, NLPF_E_NUMERIC, "cluster joining did not converge");
@@ -1007,12 +1063,6 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             // kernel choice: records lie on surfaces, so about 9 of the 27 visited cells are occupied
             // and a vertex meets ~9 n / (occupied buckets) candidates; a wavefront per vertex pays
             // off from ~16 of them (with clusters, the cluster cap bounds the set)
-            const uint32_t occupied = read_u32(h, w.counters + 4, rc);
-            if (rc) return rc;
-            const double cand = 9.0 * (double)n / (double)std::max<uint32_t>(occupied, 1u);
-            const bool wave_kernel = cfg.world_kernel == "wave" ||
-                                     (cfg.world_kernel == "auto" &&
-                                      std::min<double>(cand, cfg.cluster ? (double)K + 1.0 : cand) >= 16.0);
             // The reference filters in batches of vram / (4 K) points (:1714-1744); batching is
             // observable only through the reservoir streams, which are per batch slot. When no
             // point's candidate count can exceed K there are no draws, and the whole depth runs
@@ -1026,8 +1076,18 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 hipLaunchKernelGGL(w_bucket_max_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bend,
                                    w.counters + 6);
             if ((rc = check_launch(h, "world bucket size"))) return rc;
-            const uint64_t bound = (cfg.cluster ? 1ull : 27ull) * read_u32(h, w.counters + 6, rc);
-            if (rc) return rc;
+            uint32_t hc[3];  // occupied buckets, (candidate count), largest bucket or cluster
+            NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + 4, sizeof(hc), hipMemcpyDeviceToHost, st));
+            NLPF_HIP(h, hipStreamSynchronize(st));
+            const uint32_t occupied = hc[0];
+            const uint64_t bound = (cfg.cluster ? 1ull : 27ull) * hc[2];
+            // kernel choice: records lie on surfaces, so about 9 of the 27 visited cells are
+            // occupied and a vertex meets ~9 n / (occupied buckets) candidates; a wavefront per
+            // vertex pays off from ~16 of them (with clusters, the cluster cap bounds the set)
+            const double cand = 9.0 * (double)n / (double)std::max<uint32_t>(occupied, 1u);
+            const bool wave_kernel = cfg.world_kernel == "wave" ||
+                                     (cfg.world_kernel == "auto" &&
+                                      std::min<double>(cand, cfg.cluster ? (double)K + 1.0 : cand) >= 16.0);
             uint64_t max_cand = bound;
             if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
                 hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
