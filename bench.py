@@ -338,6 +338,28 @@ def world_leg(h_unused, N, cpu=True):
         out["reference_defaults"] = {"config": "cluster, reuse [128, 64], radius_scale [0.1, 0.05], SMIS",
                                      "ms_per_frame": ms_d, "value": pts_d / (ms_d / 1e3), "unit": "vertices/s",
                                      "neighbours_per_vertex": sum(st["neighbors"][:L]) / max(pts_d, 1)}
+    # clustering on and off at three radii under the reference's default reuse [128, 64]
+    sweep = []
+    for rs in (None, 2.0, 4.0):
+        for cluster in (True, False):
+            c = {"type": "pf", "max_depth": L - 1, "branch": B, "cluster": cluster}
+            if rs is not None:
+                c["radius_scale"] = [rs]
+            with N.Handle(c) as hw:
+                hw.set_style(nodes, roots)
+                desc = hw.world_desc(W, H, B, L)
+                bufs = hw.alloc_world(desc)
+                hw.generate_world_records(desc, bufs, seed=1)
+                hw.world_run(desc, bufs)
+                times = []
+                for _ in range(2):
+                    hw.world_run(desc, bufs)
+                    times.append(hw.stats()["total_ms"])
+                st = hw.stats()
+                sweep.append({"radius_scale": rs or "default", "cluster": cluster, "ms_per_frame": min(times),
+                              "neighbours_per_vertex": sum(st["neighbors"][:L]) / max(sum(st["points"][:L]), 1),
+                              "smis_pairs": sum(st["smis_pairs"][:L])})
+    out["cluster_sweep"] = sweep
     if cpu:
         log("world mode: FP64 oracle on the same frame (CPU)")
         out["cpu_oracle"] = world_cpu(host, W, H, B, L, cfg, nodes, roots, film, pts)

@@ -76,16 +76,6 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// counts[d] = paths with a vertex at depth d (path_depth >= d), d < L
-__global__ void w_depth_count_kernel(const int32_t* path_depth, uint64_t N, uint32_t L, uint32_t* counts) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int pd = i < N ? path_depth[i] : -1;
-    for (uint32_t d = 0; d < L; ++d) {
-        const uint32_t c = wave_sum(pd >= (int)d ? 1u : 0u);
-        if ((threadIdx.x & 63) == 0 && c) atomicAdd(counts + d, c);
-    }
-}
-
 __global__ void w_flag_kernel(const int32_t* path_depth, uint64_t N, int d, uint8_t* flag) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) flag[i] = path_depth[i] >= d ? 1 : 0;
@@ -872,7 +862,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.bucket_seed, table);
     rc |= dalloc(h, &w.cluster_id, N);
     rc |= dalloc(h, &w.cluster_size, N);
-    rc |= dalloc(h, &w.counters, 40);  // 0..7 per-depth scratch counters, 8..39 points per depth
+    rc |= dalloc(h, &w.counters, 8);
     rc |= dalloc(h, &w.nbuf, nbuf_elems);
     rc |= dalloc(h, &w.prop, N);
     rc |= dalloc(h, &w.prop_sorted, N);
@@ -971,14 +961,6 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     rc = record_event(h, 0);
     if (rc) return rc;
 
-    // points of every depth (paths whose last vertex is at depth >= d), read back once
-    NLPF_HIP(h, hipMemsetAsync(w.counters + 8, 0, 32 * sizeof(uint32_t), st));
-    hipLaunchKernelGGL(w_depth_count_kernel, grid1(N), dim3(256), 0, st, path_depth, N, L, w.counters + 8);
-    if ((rc = check_launch(h, "world depth counts"))) return rc;
-    uint32_t level_points[32] = {};
-    NLPF_HIP(h, hipMemcpyAsync(level_points, w.counters + 8, sizeof(level_points), hipMemcpyDeviceToHost, st));
-    NLPF_HIP(h, hipStreamSynchronize(st));
-
     nlpf_stats stt{};
     const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
     const float avg_ext = (ext[0] + ext[1] + ext[2]) / 3.0f;
@@ -989,7 +971,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
         hipcub::CountingInputIterator<uint32_t> it(0);
         size_t tb = w.cub_bytes;
         NLPF_HIP(h, hipcub::DeviceSelect::Flagged(w.cub, tb, it, w.flag, w.pts, w.counters, (int)N, st));
-        const uint32_t n = level_points[d];
+        const uint32_t n = read_u32(h, w.counters, rc);
+        if (rc) return rc;
         stt.points[d] = n;
         if (n > 0) {
             Grid g;
