@@ -281,7 +281,10 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
             // sum_m pdf_m(wi) over every reused neighbour m, in order, in FP64
             const double wx = wi.x, wy = wi.y, wz = wi.z;
             double sum_cos = 0.0;
-            for (uint32_t m = 0; m < used; ++m) sum_cos += smis_term(smis_rec(a.pk[buf[m]]), wi, wx, wy, wz);
+            for (uint32_t m = 0; m < used; ++m) {
+                const SmisRec r = smis_rec(a.pk[buf[m]]);
+                sum_cos = smis_add(sum_cos, r, r.bits, wi, wx, wy, wz);
+            }
             const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);
             kahan_add(isum[0], ic[0], reused.x / sum_pdf);
             kahan_add(isum[1], ic[1], reused.y / sum_pdf);
@@ -447,7 +450,14 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll 4
-                for (uint32_t m = 0; m < mn; ++m) sum_cos += smis_term(sb[m], wi, wx, wy, wz);
+                for (uint32_t m = 0; m < mn; ++m) {
+                    // records are broadcast from LDS: their flags are wave-uniform
+                    const uint32_t bits = (uint32_t)__builtin_amdgcn_readfirstlane((int)sb[m].bits);
+                    if (bits & SMIS_SAME)
+                        sum_cos = smis_add_same(sum_cos, sb[m], bits, wi, wx, wy, wz);
+                    else
+                        sum_cos = smis_add(sum_cos, sb[m], bits, wi, wx, wy, wz);
+                }
             }
             if (k < used) a.smis_sum[j] = sum_cos;
         }
@@ -504,7 +514,14 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                     __builtin_amdgcn_wave_barrier();
 #pragma unroll 4
-                    for (uint32_t m = 0; m < mn; ++m) sum_cos += smis_term(sb[m], wi, wx, wy, wz);
+                    for (uint32_t m = 0; m < mn; ++m) {
+                    // records are broadcast from LDS: their flags are wave-uniform
+                    const uint32_t bits = (uint32_t)__builtin_amdgcn_readfirstlane((int)sb[m].bits);
+                    if (bits & SMIS_SAME)
+                        sum_cos = smis_add_same(sum_cos, sb[m], bits, wi, wx, wy, wz);
+                    else
+                        sum_cos = smis_add(sum_cos, sb[m], bits, wi, wx, wy, wz);
+                }
                 }
             }
             const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);

@@ -1,7 +1,7 @@
 // Lambertian BSDF evaluation of world-mode neighbour records, shared by the filter kernels:
 // the reuse payload (NbrRec), check_wo_wi_valid and the diffuse pdf with the reference's FP32
 // decisions, and the split of pdf_m(wi) into a per-neighbour part (SmisRec) and a
-// per-direction part (smis_term) for the SMIS denominators.
+// per-direction part (smis_add) for the SMIS denominators.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -50,18 +50,23 @@ __device__ __forceinline__ float diffuse_eval(float3 ns, float3 ng, float3 wo, f
 }
 
 // What pdf_m(wi) of a reused neighbour m needs, apart from wi (diffuse_eval split in two): its
-// FP32 normals and cos_o for the reference's FP32 decisions, sgn(ng . ns) as +-1, and its FP64
-// shading normal for |cos_i|. A neighbour whose wo half of check_wo_wi_valid fails gets a zero
-// FP64 normal, so its term is 0 for every wi without a test per pair. Four 16-byte words, so a
-// staged record loads with four 128-bit LDS reads.
+// FP32 normals and cos_o for the reference's FP32 decisions, and three flags. Two 16-byte
+// words: the SMIS loops read one record per step for the whole wavefront from LDS, and the
+// bytes a record returns to 64 lanes are what bounds them. The FP64 shading normal for
+// |cos_i| is the exact widening of the FP32 one, formed in registers.
 struct __align__(16) SmisRec {
-    double nsx, nsy;
-    double nsz;
-    float co, fl;
-    float nsfx, nsfy, nsfz, ngx;
-    float ngy, ngz, pad0, pad1;
+    float nsx, nsy, nsz, co;
+    float ngx, ngy, ngz;
+    uint32_t bits;
 };
-static_assert(sizeof(SmisRec) == 64, "SmisRec layout");
+static_assert(sizeof(SmisRec) == 32, "SmisRec layout");
+// flags: ns and ng are the same bits (flat shading), and sgn(ng . ns) > 0 (so that
+// wi . ng = wi . ns, flipped = +1, and the wi half of check_wo_wi_valid always holds)
+constexpr uint32_t SMIS_SAME = 1u;
+// the wo half of check_wo_wi_valid holds (otherwise pdf_m(wi) = 0 for every wi)
+constexpr uint32_t SMIS_WO_OK = 2u;
+// flipped = sgn(ng . ns) is -1
+constexpr uint32_t SMIS_FLIP = 4u;
 
 __device__ __forceinline__ SmisRec smis_rec(const NbrRec& o) {
     const float3 ns = make_float3(o.ns_wox.x, o.ns_wox.y, o.ns_wox.z);
@@ -71,32 +76,44 @@ __device__ __forceinline__ SmisRec smis_rec(const NbrRec& o) {
     const float flipped = sgn(dot_rn(ng, ns));
     const float co = dot_rn(wo, ns);
     const bool wo_ok = sgn(flipped * co) * sgn(dot_rn(wo, ng)) > 0.0f;
+    // (flipped > 0 as well: a normal so short that ns . ns underflows keeps the general path)
+    const bool same = (__float_as_uint(ns.x) == __float_as_uint(ng.x)) &
+                      (__float_as_uint(ns.y) == __float_as_uint(ng.y)) &
+                      (__float_as_uint(ns.z) == __float_as_uint(ng.z)) & (flipped > 0.0f);
     SmisRec r;
-    r.nsx = wo_ok ? (double)ns.x : 0.0;
-    r.nsy = wo_ok ? (double)ns.y : 0.0;
-    r.nsz = wo_ok ? (double)ns.z : 0.0;
+    r.nsx = ns.x;
+    r.nsy = ns.y;
+    r.nsz = ns.z;
     r.co = co;
-    r.fl = flipped;
-    r.nsfx = ns.x;
-    r.nsfy = ns.y;
-    r.nsfz = ns.z;
     r.ngx = ng.x;
     r.ngy = ng.y;
     r.ngz = ng.z;
-    r.pad0 = r.pad1 = 0.0f;
+    r.bits = (same ? SMIS_SAME : 0u) | (wo_ok ? SMIS_WO_OK : 0u) | (flipped < 0.0f ? SMIS_FLIP : 0u);
     return r;
 }
-// |cos_i| of pdf_m(wi) (diffuse_eval without the 1/pi), 0 where pdf_m(wi) is 0: the same FP32
-// sign and hemisphere decisions, the same FP64 magnitude; branch-free
-__device__ __forceinline__ double smis_term(const SmisRec& r, float3 wi, double wx, double wy, double wz) {
-    const float ci = dot_rn(wi, make_float3(r.nsfx, r.nsfy, r.nsfz));
+// sum + |cos_i| of pdf_m(wi) (diffuse_eval without the 1/pi), the term being 0 where pdf_m(wi)
+// is 0: the same FP32 sign and hemisphere decisions, the same FP64 magnitude; branch-free.
+// `bits` is r.bits (passed apart so a caller can hold it in a scalar register). The term
+// enters as fma(|cd|, 1 or 0, sum), which rounds exactly like sum + term (|cd| * 1 and
+// |cd| * 0 are exact for the finite cd of unit vectors).
+__device__ __forceinline__ double smis_add(double sum, const SmisRec& r, uint32_t bits, float3 wi, double wx,
+                                           double wy, double wz) {
+    const float ci = dot_rn(wi, make_float3(r.nsx, r.nsy, r.nsz));
     const float cg = dot_rn(wi, make_float3(r.ngx, r.ngy, r.ngz));
-    // sgn(flipped * ci) * sgn(cg) > 0: both signs agree; & rather than && so nothing branches
-    const bool ok = ((r.fl * ci > 0.0f) == (cg > 0.0f)) & (r.co * ci >= 0.0f);
-    const double cd = wx * r.nsx + wy * r.nsy + wz * r.nsz;
-    // both arms use cd (0 * cd is a signed zero for the finite cd of unit vectors), so the
-    // select needs no branch around the FP64 dot
-    return ok ? fabs(cd) : 0.0 * cd;
+    // sgn(flipped * ci) * sgn(cg) > 0: both signs agree (flipped * ci is +-ci exactly); &
+    // rather than && so nothing branches
+    const bool ok = ((((bits & SMIS_FLIP) ? -ci : ci) > 0.0f) == (cg > 0.0f)) & (r.co * ci >= 0.0f) &
+                    ((bits & SMIS_WO_OK) != 0u);
+    const double cd = wx * (double)r.nsx + wy * (double)r.nsy + wz * (double)r.nsz;
+    return fma(fabs(cd), ok ? 1.0 : 0.0, sum);
+}
+// smis_add for a record with SMIS_SAME: the sign test reduces to the hemisphere test
+__device__ __forceinline__ double smis_add_same(double sum, const SmisRec& r, uint32_t bits, float3 wi, double wx,
+                                                double wy, double wz) {
+    const float ci = dot_rn(wi, make_float3(r.nsx, r.nsy, r.nsz));
+    const bool ok = (r.co * ci >= 0.0f) & ((bits & SMIS_WO_OK) != 0u);
+    const double cd = wx * (double)r.nsx + wy * (double)r.nsy + wz * (double)r.nsz;
+    return fma(fabs(cd), ok ? 1.0 : 0.0, sum);
 }
 
 }  // namespace nlpf
