@@ -383,12 +383,13 @@ def world_leg(h_unused, N, cpu=True):
         ms = 1e3 * float(np.median(times))
         host = hw.download_world(desc, bufs) if cpu else None
         # the O(K^2) SMIS denominators dominate: their pdf evaluations per second, and the VALU
-        # issue they imply at the filter's ~23.5 instructions per evaluation (inner loop of
-        # w_filter_wave_kernel), over the whole frame time
+        # issue they imply at 17 instructions per evaluation (inner loop of w_filter_wave_kernel
+        # for records with equal shading and geometric normals, as every record of this scene
+        # has; 25 otherwise), over the whole frame time
         out["radius_scale_2"] = {"ms_per_frame": ms, "value": pts / (ms / 1e3), "unit": "vertices/s",
                                  "neighbours_per_vertex": nb / max(pts, 1),
                                  "smis_pairs_per_frame": pairs, "smis_pairs_per_s": pairs / (ms / 1e3),
-                                 "smis_valu_issue_frac": pairs * 23.5 / (ms / 1e3) / VALU_ISSUE_PEAK}
+                                 "smis_valu_issue_frac": pairs * 17.0 / (ms / 1e3) / VALU_ISSUE_PEAK}
     if cpu:
         log("world mode, radius_scale 2: FP64 oracle on the same frame (CPU)")
         o = world_cpu(host, W, H, B, L, cfg2, nodes, roots, film, pts)

@@ -421,7 +421,9 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     // make the list written by all lanes visible to the whole wave
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    const uint32_t used = cnt < a.K ? cnt : a.K;
+    // wave-uniform by construction; readfirstlane lets the compiler keep the list loops' counters
+    // in scalar registers
+    const uint32_t used = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cnt < a.K ? cnt : a.K));
     const float inv_used = used ? 1.0f / (float)used : 0.0f;
     auto get = [&](uint32_t i) -> uint32_t { return i < WF_CAP ? lb[i] : gbuf[i]; };
     // With clustering, an uncapped list (cnt <= K) is every member of the vertex's cluster with
@@ -443,7 +445,8 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
             const double wx = wi.x, wy = wi.y, wz = wi.z;
             double sum_cos = 0.0;
             for (uint32_t m0 = 0; m0 < used; m0 += 64) {
-                const uint32_t mn = used - m0 < 64 ? used - m0 : 64;
+                const uint32_t mn =
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)(used - m0 < 64 ? used - m0 : 64));
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 __builtin_amdgcn_wave_barrier();
                 if ((uint32_t)lane < mn) sb[lane] = smis_rec(a.pk[get(m0 + (uint32_t)lane)]);
@@ -506,7 +509,8 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                 if (active && ind_ok) sum_cos = a.smis_sum[get(k)];
             } else {
                 for (uint32_t m0 = 0; m0 < used; m0 += 64) {
-                    const uint32_t mn = used - m0 < 64 ? used - m0 : 64;
+                    const uint32_t mn =
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(used - m0 < 64 ? used - m0 : 64));
                     // every lane has read the previous chunk before it is overwritten
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                     __builtin_amdgcn_wave_barrier();
