@@ -195,11 +195,9 @@ struct FilterArgs {
     double* smis_sum;  // [N] sum over the list of cos_m(wi_j), by reused vertex j
 };
 
-// One thread per vertex: for small neighbour sets (a few per vertex), where a wavefront per
-// vertex would leave most lanes idle.
-__global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
-    const uint32_t tl = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tl >= a.count) return;
+// Filter vertex tl of the launch on one thread; returns its candidate count and list length.
+__device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32_t tl, uint32_t& cnt_out,
+                                                     uint32_t& used_out) {
     const uint64_t N = a.N;
     const uint32_t c = a.pts[a.off + tl];
     const float3 p = make_float3(pl(a.rec, N, 0, c), pl(a.rec, N, 1, c), pl(a.rec, N, 2, c));
@@ -249,9 +247,9 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
             ++cnt;
         }
     }
-    atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)cnt);
     const uint32_t used = cnt < a.K ? cnt : a.K;
-    if (a.smis) atomicAdd(a.pair_count + (blockIdx.x & (NBR_SLOTS - 1)), (unsigned long long)used * used);
+    cnt_out = cnt;
+    used_out = used;
     const float inv_used = 1.0f / (float)used;
     // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
     // the exact sum (a plain running sum drifts by ~n ulp)
@@ -307,6 +305,25 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
     a.rad_out[3 * (uint64_t)c + 1] = L.y;
     a.rad_out[3 * (uint64_t)c + 2] = L.z;
     if (a.use_rng) a.rng[tl] = rng;
+}
+
+// One thread per vertex: for small neighbour sets (a few per vertex), where a wavefront per
+// vertex would leave most lanes idle. The neighbour and SMIS-pair counters are summed over the
+// wavefront first: one atomic per wavefront and counter.
+__global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
+    const uint32_t tl = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cnt = 0, used = 0;
+    if (tl < a.count) filter_vertex_thread(a, tl, cnt, used);
+    unsigned long long nc = cnt, np = a.smis ? (unsigned long long)used * used : 0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        nc += __shfl_xor(nc, off);
+        np += __shfl_xor(np, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (nc) atomicAdd(a.nbr_count + (blockIdx.x & (NBR_SLOTS - 1)), nc);
+        if (np) atomicAdd(a.pair_count + (blockIdx.x & (NBR_SLOTS - 1)), np);
+    }
 }
 
 // One wavefront per vertex (4 per workgroup).
