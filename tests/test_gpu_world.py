@@ -188,6 +188,23 @@ def test_world_large_clusters_share_smis_sums():
     check(film, rad, ofilm, orad, host, L, sens=sens)
 
 
+def test_world_clusters_over_batches():
+    """Clustering with the wavefront filter over several reservoir batches (vram 1 MB at K = 4
+    is 65536 points per batch): full clusters (K + 1 members) draw and sum their own SMIS
+    denominators, uncapped lists share theirs, and a list's first entry can sit in another
+    batch than its members."""
+    W, H, branch, L = 160, 128, 4, 2
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "cluster": True, "radius_scale": [3.0],
+           "reuse": [4], "vram": 1, "world_kernel": "wave"}
+    film, rad, host, st = run_world(cfg, W, H, branch, L)
+    assert st["points"][0] > 65536  # more than one batch
+    ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg)
+    assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    assert st["smis_pairs"][0] < 16 * st["points"][0]  # some lists shared
+    sens = sensitivity(host, W, H, branch, L, cfg, None, (), ofilm, orad)
+    check(film, rad, ofilm, orad, host, L, sens=sens)
+
+
 def test_world_single_level_matches_oracle():
     """max_depth 0: one level, no propagation, only the first hits are filtered."""
     W, H, branch, L = 40, 30, 3, 1
