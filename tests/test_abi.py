@@ -78,3 +78,42 @@ def test_null_arguments_rejected():
     assert lib.nlpf_create(b"{}", None) == 2
     assert lib.nlpf_destroy(None) == 2
     assert lib.nlpf_sync(None) == 2
+
+
+def rust_block():
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```rust\n(.*?)```", txt, flags=re.S)
+    return next(b for b in blocks if 'extern "C"' in b)
+
+
+def rust_struct_size(body):
+    """Size of a #[repr(C)] struct from its field list (C layout: each field at its alignment,
+    the total padded to the largest alignment)."""
+    prim = {"f32": 4, "i32": 4, "u32": 4, "f64": 8, "u64": 8, "usize": 8}
+    off = align_max = 0
+    for ty in re.findall(r"pub\s+\w+\s*:\s*(\[[^\]]+\]|\w+)", body):
+        m = re.match(r"\[\s*(\w+)\s*;\s*(\d+)\s*\]", ty)
+        size, align = (prim[m.group(1)] * int(m.group(2)), prim[m.group(1)]) if m else (prim[ty], prim[ty])
+        off = (off + align - 1) // align * align + size
+        align_max = max(align_max, align)
+    return (off + align_max - 1) // align_max * align_max
+
+
+def test_documented_rust_binding_matches_header(tmp_path):
+    """The Rust -sys declarations a caller would copy declare every entry point of the header,
+    and each documented #[repr(C)] struct has the header struct's size."""
+    block = rust_block()
+    declared = set(re.findall(r"pub fn (nlpf_[a-z0-9_]+)\s*\(", block))
+    assert declared == set(header_functions())
+    structs = dict(re.findall(r"pub struct (nlpf_\w+)\s*\{(.*?)\}", block, flags=re.S))
+    structs.pop("nlpf_handle", None)
+    assert len(structs) >= 7
+    names = sorted(structs)
+    src = tmp_path / "rs.c"
+    src.write_text('#include "nlpf.h"\n#include <stdio.h>\nint main(void){' +
+                   "".join(f'printf("%zu\\n", sizeof({n}));' for n in names) + "return 0;}\n")
+    exe = tmp_path / "rs"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    for n, c_size in zip(names, sizes):
+        assert rust_struct_size(structs[n]) == c_size, n
