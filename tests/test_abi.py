@@ -117,3 +117,25 @@ def test_documented_rust_binding_matches_header(tmp_path):
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     for n, c_size in zip(names, sizes):
         assert rust_struct_size(structs[n]) == c_size, n
+
+
+def test_python_mirror_structs_match_header(tmp_path):
+    """Every ctypes struct of the Python mirror has its header struct's size and field offsets."""
+    mirror = {"nlpf_screen_desc": N._ScreenDesc, "nlpf_gen_desc": N._GenDesc, "nlpf_stats": N._Stats,
+              "nlpf_world_desc": N._WorldDesc, "nlpf_mlp_run_desc": N._MlpRunDesc,
+              "nlpf_mlp_run_stats": N._MlpRunStats}
+    lines = []
+    for cname, cls in mirror.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+    src = tmp_path / "py.c"
+    src.write_text('#include "nlpf.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){' + "".join(lines) +
+                   "return 0;}\n")
+    exe = tmp_path / "py"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    vals = iter(int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split())
+    for cname, cls in mirror.items():
+        assert next(vals) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert next(vals) == getattr(cls, fname).offset, (cname, fname)
