@@ -139,3 +139,39 @@ def test_python_mirror_structs_match_header(tmp_path):
         assert next(vals) == C.sizeof(cls), cname
         for fname, _ in cls._fields_:
             assert next(vals) == getattr(cls, fname).offset, (cname, fname)
+
+
+def header_prototypes():
+    """{name: (return type, [parameter types])} from include/nlpf.h (comments and names
+    stripped; pointer parameters reduced to "ptr")."""
+    txt = open(os.path.join(ROOT, "include", "nlpf.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    protos = {}
+    for ret, name, params in re.findall(r"([A-Za-z_][\w \*]*?)\s*\b(nlpf_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", txt):
+        kinds = []
+        for p in [q.strip() for q in params.split(",") if q.strip() and q.strip() != "void"]:
+            if "*" in p:
+                kinds.append("ptr")
+            else:
+                kinds.append(re.sub(r"\s+\w+$", "", p).replace("const ", "").strip())
+        protos[name] = ("ptr" if "*" in ret else ret.strip(), kinds)
+    return protos
+
+
+def test_python_mirror_signatures_match_header():
+    """Each ctypes signature of the mirror has the header's parameter count, and integer and
+    float parameters of the same width."""
+    width = {C.c_int: ("int", 4), C.c_uint32: ("uint32_t", 4), C.c_int32: ("int32_t", 4),
+             C.c_uint64: ("uint64_t", 8), C.c_size_t: ("size_t", 8), C.c_float: ("float", 4),
+             C.c_double: ("double", 8), C.c_uint16: ("uint16_t", 2), C.c_uint8: ("uint8_t", 1)}
+    c_width = {"int": 4, "uint32_t": 4, "int32_t": 4, "uint64_t": 8, "size_t": 8, "float": 4, "double": 8,
+               "uint16_t": 2, "uint8_t": 1}
+    protos = header_prototypes()
+    for name, res, args in N._SIGS:
+        ret, params = protos[name]
+        assert len(args) == len(params), name
+        for i, (a, p) in enumerate(zip(args, params)):
+            if p == "ptr":
+                assert a in (C.c_void_p, C.c_char_p) or hasattr(a, "_type_") and a not in width, (name, i, a)
+            else:
+                assert a in width and width[a][1] == c_width[p], (name, i, a, p)
