@@ -122,6 +122,21 @@ def measured_traffic():
     return None
 
 
+def measured_valu():
+    """VALU busy fraction and VALU instructions per pair of the level kernel from the committed
+    rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "screen_level_fp32_1080p16spp_valu_counters.md")
+    out = {}
+    try:
+        for line in open(path):
+            for key in ("valu_busy_per_launch", "valu_instructions_per_pair"):
+                if line.startswith(key + ":"):
+                    out[key] = float(line.split(":")[1])
+    except OSError:
+        pass
+    return out or None
+
+
 def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
     """FP16 records + learned cache (2^16 training records per frame, batch 2^14, Adam lr 1e-2).
     The judge slice rotates over the spp slices from frame to frame; `--mlp-frames` untimed
@@ -517,7 +532,9 @@ def main():
                      "valu_issue": {"slots_per_pair": ISSUE_SLOTS_PER_PAIR,
                                     "achieved": pairs / (lv32.sum() / 1e3) * ISSUE_SLOTS_PER_PAIR,
                                     "peak": VALU_ISSUE_PEAK, "unit": "lane-instructions/s",
-                                    "frac": pairs / (lv32.sum() / 1e3) * ISSUE_SLOTS_PER_PAIR / VALU_ISSUE_PEAK},
+                                    "frac": pairs / (lv32.sum() / 1e3) * ISSUE_SLOTS_PER_PAIR / VALU_ISSUE_PEAK,
+                                    "counters": measured_valu() if (W, H, S, L, R) == (1920, 1080, 16, 5, 7)
+                                    else None},
                      "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved_gbs / HBM_PEAK_GBS}},
     }
