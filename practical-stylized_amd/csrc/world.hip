@@ -90,34 +90,31 @@ __global__ void w_keys_kernel(const float* rec, uint64_t N, const uint32_t* pts,
     keys[t] = cell_bucket(c[0], c[1], c[2], g.table);
 }
 
+// Bucket ranges of the sorted keys: the first entry of each bucket walks to the bucket's end and
+// writes [start, end) as one 8-byte table entry (one load per bucket for the readers); it also
+// counts the occupied buckets and the largest bucket.
 __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint32_t n, const float* rec, uint64_t N,
-                                uint32_t* bstart, uint32_t* bend, float4* pos_sorted, uint32_t* occupied) {
+                                uint2* brange, float4* pos_sorted, uint32_t* occupied, uint32_t* max_bucket) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t head = 0;
+    uint32_t head = 0, size = 0;
     if (e < n) {
         const uint32_t k = keys[e];
         if (e == 0 || keys[e - 1] != k) {
-            bstart[k] = e;
+            uint32_t f = e + 1;
+            while (f < n && keys[f] == k) ++f;
+            brange[k] = make_uint2(e, f);
             head = 1;
+            size = f - e;
         }
-        if (e == n - 1 || keys[e + 1] != k) bend[k] = e + 1;
         const uint32_t i = ids[e];
         pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
     }
     head = wave_sum(head);
-    if ((threadIdx.x & 63) == 0 && head) atomicAdd(occupied, head);
-}
-
-// largest bucket (entries from one bucket start to its end), after w_ranges_kernel
-__global__ void w_bucket_max_kernel(const uint32_t* keys, uint32_t n, const uint32_t* bend, uint32_t* out_max) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t size = 0;
-    if (e < n) {
-        const uint32_t k = keys[e];
-        if (e == 0 || keys[e - 1] != k) size = bend[k] - e;
-    }
     size = wave_max(size);
-    if ((threadIdx.x & 63) == 0 && size) atomicMax(out_max, size);
+    if ((threadIdx.x & 63) == 0 && head) {
+        atomicAdd(occupied, head);
+        atomicMax(max_bucket, size);
+    }
 }
 
 // largest cluster (seed + joiners) of the depth: with clustering, a bound on every vertex's
@@ -134,11 +131,10 @@ __global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint
     if ((threadIdx.x & 63) == 0 && members) atomicMax(out_max, members);
 }
 
-__global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint32_t* bstart, uint32_t* bend) {
+__global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint2* brange) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    bstart[keys[e]] = 0;
-    bend[keys[e]] = 0;
+    brange[keys[e]] = make_uint2(0u, 0u);
 }
 
 // per-path payload of level d (indirect record from the previous propagation)
@@ -172,8 +168,7 @@ struct FilterArgs {
     const uint32_t* cluster_id;  // nullptr when clustering is off
     const NbrRec* pk;
     const float4* pos_sorted;
-    const uint32_t* bstart;
-    const uint32_t* bend;
+    const uint2* brange;  // [start, end) of each bucket's entries
     Grid g;
     uint32_t K;
     uint32_t* nbuf;
@@ -224,8 +219,9 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
         r0[c] = r1[c] = 0;
         if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
             const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
-            r0[c] = a.bstart[b];
-            r1[c] = a.bend[b];
+            const uint2 br = a.brange[b];
+            r0[c] = br.x;
+            r1[c] = br.y;
         }
     }
 #pragma unroll
@@ -377,8 +373,9 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                   cz = (int)cell[2] + lane / 9 - 1;
         if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
             const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
-            c_e0 = a.bstart[b];
-            c_n = a.bend[b] - c_e0;
+            const uint2 br = a.brange[b];
+            c_e0 = br.x;
+            c_n = br.y - br.x;
         }
     }
     // inclusive prefix of the cell sizes over the lanes: the walk's candidates, cell by cell
@@ -605,7 +602,7 @@ __global__ void w_propagate_kernel(const float* rec_child, const float* rec_pare
 // upper bound of a point's neighbour count: the entries of the buckets of its 27-cell walk
 // (around its cluster seed when clustering), i.e. the candidates the filter tests
 __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
-                                  const uint32_t* cluster_id, const uint32_t* bstart, const uint32_t* bend,
+                                  const uint32_t* cluster_id, const uint2* brange,
                                   uint32_t* out_max) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t total = 0;
@@ -622,7 +619,8 @@ __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* 
                         cz >= (int)g.res[2])
                         continue;
                     const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
-                    total += bend[b] - bstart[b];
+                    const uint2 br = brange[b];
+                    total += br.y - br.x;
                 }
     }
     total = wave_max(total);
@@ -744,7 +742,7 @@ __global__ void w_seed_scatter_kernel(const uint32_t* flags, const uint32_t* see
 // with the smallest vertex ids first. Only seeds can take a proposal, so the walk visits a
 // bucket's seed entries (seed_entry[seed_rank[start] .. seed_rank[end]), in bucket order).
 __global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
-                                         const float4* pos_sorted, const uint32_t* bstart, const uint32_t* bend,
+                                         const float4* pos_sorted, const uint2* brange,
                                          const uint32_t* seed_rank, const uint32_t* seed_entry,
                                          const uint32_t* cluster_size, uint32_t cap, const uint32_t* cluster_id,
                                          unsigned long long* prop, uint32_t* n_prop, uint32_t* n_unclustered) {
@@ -763,8 +761,9 @@ __global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uin
                     if (x < 0 || y < 0 || z < 0 || x >= (int)g.res[0] || y >= (int)g.res[1] || z >= (int)g.res[2])
                         continue;
                     const uint32_t b = cell_bucket((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table);
-                    const uint32_t s1 = seed_rank[bend[b]];
-                    for (uint32_t si = seed_rank[bstart[b]]; si < s1; ++si) {
+                    const uint2 br = brange[b];
+                    const uint32_t s1 = seed_rank[br.y];
+                    for (uint32_t si = seed_rank[br.x]; si < s1; ++si) {
                         const float4 qe = pos_sorted[seed_entry[si]];
                         if (!(dist3(qe.x - px, qe.y - py, qe.z - pz) <= g.radius)) continue;
                         const uint32_t j = __float_as_uint(qe.w);
@@ -835,7 +834,8 @@ struct WorldWork {
     uint64_t N = 0, table = 0, nbuf_elems = 0, rng_n = 0, cub_bytes = 0;
     uint8_t* flag = nullptr;
     uint32_t *pts = nullptr, *keys = nullptr, *keys_sorted = nullptr, *ids_sorted = nullptr;
-    uint32_t *bstart = nullptr, *bend = nullptr, *bucket_seed = nullptr;
+    uint2* brange = nullptr;
+    uint32_t* bucket_seed = nullptr;
     uint32_t *cluster_id = nullptr, *cluster_size = nullptr, *counters = nullptr, *nbuf = nullptr;
     unsigned long long *prop = nullptr, *prop_sorted = nullptr, *nbr = nullptr;
     float4* pos_sorted = nullptr;
@@ -848,7 +848,7 @@ struct WorldWork {
 };
 
 static void world_free(WorldWork& w) {
-    void* ps[] = {w.flag, w.pts, w.keys, w.keys_sorted, w.ids_sorted, w.bstart, w.bend, w.bucket_seed,
+    void* ps[] = {w.flag, w.pts, w.keys, w.keys_sorted, w.ids_sorted, w.brange, w.bucket_seed,
                   w.cluster_id, w.cluster_size, w.counters, w.nbuf, w.prop, w.prop_sorted, w.nbr,
                   w.pos_sorted, w.pk, w.ind, w.rad, w.rng, w.seed_flags, w.seed_rank, w.seed_entry,
                   w.smis_sum, w.cub};
@@ -895,8 +895,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.keys, N);
     rc |= dalloc(h, &w.keys_sorted, N);
     rc |= dalloc(h, &w.ids_sorted, N);
-    rc |= dalloc(h, &w.bstart, table);
-    rc |= dalloc(h, &w.bend, table);
+    rc |= dalloc(h, &w.brange, table);
     rc |= dalloc(h, &w.bucket_seed, table);
     rc |= dalloc(h, &w.cluster_id, N);
     rc |= dalloc(h, &w.cluster_size, N);
@@ -926,8 +925,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     w.cub_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
     NLPF_HIP(h, hipMalloc(&w.cub, w.cub_bytes));
     // empty bucket ranges; the filter of every depth restores the entries it set
-    NLPF_HIP(h, hipMemsetAsync(w.bstart, 0, sizeof(uint32_t) * table, h->stream));
-    NLPF_HIP(h, hipMemsetAsync(w.bend, 0, sizeof(uint32_t) * table, h->stream));
+    NLPF_HIP(h, hipMemsetAsync(w.brange, 0, sizeof(uint2) * table, h->stream));
     NLPF_HIP(h, hipMemsetAsync(w.bucket_seed, 0xFF, sizeof(uint32_t) * table, h->stream));
     w.N = N;
     w.table = table;
@@ -1030,9 +1028,10 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             tb = w.cub_bytes;
             NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.keys_sorted, w.pts, w.ids_sorted,
                                                            (int)n, 0, (int)table_bits, st));
-            NLPF_HIP(h, hipMemsetAsync(w.counters + 4, 0, sizeof(uint32_t), st));
+            // counters 4..6: occupied buckets, candidate bound, largest bucket (or cluster)
+            NLPF_HIP(h, hipMemsetAsync(w.counters + 4, 0, 3 * sizeof(uint32_t), st));
             hipLaunchKernelGGL(w_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, w.ids_sorted, n, recd, N,
-                               w.bstart, w.bend, w.pos_sorted, w.counters + 4);
+                               w.brange, w.pos_sorted, w.counters + 4, w.counters + 6);
             hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind,
                                new_flags + (uint64_t)d * N, w.pk);
             if ((rc = check_launch(h, "world grid"))) return rc;
@@ -1060,7 +1059,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     for (int inner = 0;; ++inner) {
                         NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * 4, st));
                         hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                           w.pos_sorted, w.bstart, w.bend, w.seed_rank, w.seed_entry, w.cluster_size,
+                                           w.pos_sorted, w.brange, w.seed_rank, w.seed_entry, w.cluster_size,
                                            K, w.cluster_id, w.prop, w.counters + 1, w.counters + 2);
                         if ((rc = check_launch(h, "world cluster"))) return rc;
                         uint32_t hc[4];
@@ -1089,14 +1088,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             // point's candidate count can exceed K there are no draws, and the whole depth runs
             // as one launch with list slots sized by that count: first bounded by 27 x the
             // largest bucket (by the largest cluster when clustering), else counted exactly.
-            NLPF_HIP(h, hipMemsetAsync(w.counters + 5, 0, 2 * sizeof(uint32_t), st));
-            if (cfg.cluster)
+            if (cfg.cluster) {
+                NLPF_HIP(h, hipMemsetAsync(w.counters + 6, 0, sizeof(uint32_t), st));
                 hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, w.pts, n, w.cluster_id,
                                    w.cluster_size, w.counters + 6);
-            else
-                hipLaunchKernelGGL(w_bucket_max_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bend,
-                                   w.counters + 6);
-            if ((rc = check_launch(h, "world bucket size"))) return rc;
+                if ((rc = check_launch(h, "world cluster size"))) return rc;
+            }
             uint32_t hc[3];  // occupied buckets, (candidate count), largest bucket or cluster
             NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + 4, sizeof(hc), hipMemcpyDeviceToHost, st));
             NLPF_HIP(h, hipStreamSynchronize(st));
@@ -1112,7 +1109,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             uint64_t max_cand = bound;
             if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
                 hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                   cfg.cluster ? w.cluster_id : nullptr, w.bstart, w.bend, w.counters + 5);
+                                   cfg.cluster ? w.cluster_id : nullptr, w.brange, w.counters + 5);
                 if ((rc = check_launch(h, "world candidates"))) return rc;
                 max_cand = read_u32(h, w.counters + 5, rc);
                 if (rc) return rc;
@@ -1137,8 +1134,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     fa.cluster_id = cfg.cluster ? w.cluster_id : nullptr;
                     fa.pk = w.pk;
                     fa.pos_sorted = w.pos_sorted;
-                    fa.bstart = w.bstart;
-                    fa.bend = w.bend;
+                    fa.brange = w.brange;
                     fa.g = g;
                     fa.K = K;
                     fa.nbuf = w.nbuf;
@@ -1164,7 +1160,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 }
             }
             if ((rc = check_launch(h, "world filter"))) return rc;
-            hipLaunchKernelGGL(w_clear_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.bstart, w.bend);
+            hipLaunchKernelGGL(w_clear_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.brange);
         }
         if (d > 0) {
             hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
