@@ -76,6 +76,12 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Counters that every wavefront of a launch updates (neighbour counts, occupied buckets, bucket
+// and candidate maxima) are spread over NBR_SLOTS addresses, slot blockIdx % NBR_SLOTS: atomics
+// on one address serialize (about 13 ns each, tens of microseconds per launch); the host sums
+// or maxes the slots in the readback it makes anyway.
+constexpr uint32_t NBR_SLOTS = 256;
+
 __global__ void w_flag_kernel(const int32_t* path_depth, uint64_t N, int d, uint8_t* flag) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) flag[i] = path_depth[i] >= d ? 1 : 0;
@@ -90,35 +96,53 @@ __global__ void w_keys_kernel(const float* rec, uint64_t N, const uint32_t* pts,
     keys[t] = cell_bucket(c[0], c[1], c[2], g.table);
 }
 
-// Bucket ranges of the sorted keys: the first entry of each bucket walks to the bucket's end and
-// writes [start, end) as one 8-byte table entry (one load per bucket for the readers); it also
-// counts the occupied buckets and the largest bucket.
+// Bucket ranges of the sorted keys as one 8-byte table entry [start, end) per bucket (one load
+// per bucket for the readers): the first entry of a bucket writes the start, the last one the
+// end. The last entry also finds the start by a galloping search back over the sorted keys
+// (O(log size) loads, so large buckets cost no serial walk) for the largest-bucket bound, and
+// the first entries count the occupied buckets (both over NBR_SLOTS slots).
 __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint32_t n, const float* rec, uint64_t N,
                                 uint2* brange, float4* pos_sorted, uint32_t* occupied, uint32_t* max_bucket) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t head = 0, size = 0;
     if (e < n) {
         const uint32_t k = keys[e];
-        if (e == 0 || keys[e - 1] != k) {
-            uint32_t f = e + 1;
-            while (f < n && keys[f] == k) ++f;
-            brange[k] = make_uint2(e, f);
-            head = 1;
-            size = f - e;
+        uint32_t* half = reinterpret_cast<uint32_t*>(brange + k);
+        head = (e == 0 || keys[e - 1] != k) ? 1u : 0u;
+        if (head) half[0] = e;
+        if (e + 1 == n || keys[e + 1] != k) {
+            half[1] = e + 1;
+            uint32_t s = e;
+            if (!head) {
+                // keys[hi] == k throughout; the loop ends with keys[hi - step] != k or hi < step
+                uint32_t hi = e, step = 1;
+                while (hi >= step && keys[hi - step] == k) {
+                    hi -= step;
+                    step <<= 1;
+                }
+                uint32_t lo = hi >= step ? hi - step + 1 : 0;
+                while (lo < hi) {  // first index in [lo, hi] with key k
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (keys[mid] == k) hi = mid; else lo = mid + 1;
+                }
+                s = lo;
+            }
+            size = e + 1 - s;
         }
         const uint32_t i = ids[e];
         pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
     }
     head = wave_sum(head);
     size = wave_max(size);
-    if ((threadIdx.x & 63) == 0 && head) {
-        atomicAdd(occupied, head);
-        atomicMax(max_bucket, size);
+    if ((threadIdx.x & 63) == 0) {  // a wavefront may hold a bucket's last entry but no first one
+        const uint32_t slot = blockIdx.x & (NBR_SLOTS - 1);
+        if (head) atomicAdd(occupied + slot, head);
+        if (size) atomicMax(max_bucket + slot, size);
     }
 }
 
-// largest cluster (seed + joiners) of the depth: with clustering, a bound on every vertex's
-// candidate count (its candidates are members of its own cluster)
+// largest cluster (seed + joiners) of the depth, max over out_max[NBR_SLOTS]: with clustering, a
+// bound on every vertex's candidate count (its candidates are members of its own cluster)
 __global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint32_t* cluster_id,
                                      const uint32_t* cluster_size, uint32_t* out_max) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -128,7 +152,7 @@ __global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint
         if (cluster_id[i] == i) members = cluster_size[i] + 1u;
     }
     members = wave_max(members);
-    if ((threadIdx.x & 63) == 0 && members) atomicMax(out_max, members);
+    if ((threadIdx.x & 63) == 0 && members) atomicMax(out_max + (blockIdx.x & (NBR_SLOTS - 1)), members);
 }
 
 __global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint2* brange) {
@@ -154,9 +178,6 @@ __global__ void w_pack_kernel(const float* rec, uint64_t N, const uint32_t* pts,
     pk[i] = r;
 }
 
-// per-depth neighbour counters: a kernel adds to slot blockIdx % NBR_SLOTS, so hundreds of
-// thousands of per-vertex additions do not serialize on one address; the host sums the slots
-constexpr uint32_t NBR_SLOTS = 256;
 
 struct FilterArgs {
     const float* rec;
@@ -599,8 +620,9 @@ __global__ void w_propagate_kernel(const float* rec_child, const float* rec_pare
     ind[7 * N + i] = pl(rec_parent, N, 27, i);
 }
 
-// upper bound of a point's neighbour count: the entries of the buckets of its 27-cell walk
-// (around its cluster seed when clustering), i.e. the candidates the filter tests
+// upper bound of a point's neighbour count, max over out_max[NBR_SLOTS]: the entries of the
+// buckets of its 27-cell walk (around its cluster seed when clustering), i.e. the candidates the
+// filter tests
 __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
                                   const uint32_t* cluster_id, const uint2* brange,
                                   uint32_t* out_max) {
@@ -624,7 +646,7 @@ __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* 
                 }
     }
     total = wave_max(total);
-    if ((threadIdx.x & 63) == 0 && total) atomicMax(out_max, total);
+    if ((threadIdx.x & 63) == 0 && total) atomicMax(out_max + (blockIdx.x & (NBR_SLOTS - 1)), total);
 }
 
 __global__ void w_fill_kernel(float* p, uint64_t n, float v) {
@@ -741,11 +763,12 @@ __global__ void w_seed_scatter_kernel(const uint32_t* flags, const uint32_t* see
 // radius that still has room; key = seed << 32 | vertex so a sort groups proposals by seed
 // with the smallest vertex ids first. Only seeds can take a proposal, so the walk visits a
 // bucket's seed entries (seed_entry[seed_rank[start] .. seed_rank[end]), in bucket order).
-__global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+__global__ void __launch_bounds__(256) w_cluster_propose_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
                                          const float4* pos_sorted, const uint2* brange,
                                          const uint32_t* seed_rank, const uint32_t* seed_entry,
                                          const uint32_t* cluster_size, uint32_t cap, const uint32_t* cluster_id,
                                          unsigned long long* prop, uint32_t* n_prop, uint32_t* n_unclustered) {
+    // launched with 256-thread workgroups (4 wavefronts); n_unclustered has NBR_SLOTS slots
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = t < n ? pts[t] : 0u;
     const bool open = t < n && cluster_id[i] == 0xFFFFFFFFu;
@@ -774,16 +797,24 @@ __global__ void w_cluster_propose_kernel(const float* rec, uint64_t N, const uin
                     }
                 }
     }
-    // one counter update per wavefront; proposal slots from a ballot prefix count (the
-    // proposals are sorted afterwards, so their order here does not matter)
-    const uint32_t lane = threadIdx.x & 63;
+    // the open count goes to slot blockIdx % NBR_SLOTS, one update per wavefront; proposal slots
+    // come from one atomic per workgroup on the shared allocator, split by ballot prefix counts
+    // (the proposals are sorted afterwards, so their order here does not matter)
+    __shared__ uint32_t wave_count[4], block_base;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t opened = wave_sum(open ? 1u : 0u);
-    if (lane == 0 && opened) atomicAdd(n_unclustered, opened);
+    if (lane == 0 && opened) atomicAdd(n_unclustered + (blockIdx.x & (NBR_SLOTS - 1)), opened);
     const bool has = seed != 0xFFFFFFFFu;
     const uint64_t mask = __ballot(has);
-    uint32_t base = 0;
-    if (lane == 0 && mask) base = atomicAdd(n_prop, (uint32_t)__popcll(mask));
-    base = (uint32_t)__shfl((int)base, 0);
+    if (lane == 0) wave_count[wv] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = wave_count[0] + wave_count[1] + wave_count[2] + wave_count[3];
+        block_base = total ? atomicAdd(n_prop, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = block_base;
+    for (uint32_t k = 0; k < wv; ++k) base += wave_count[k];
     if (has) prop[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = ((unsigned long long)seed << 32) | i;
 }
 // first index of the run of proposals to `seed` that holds index e (binary search: the keys
@@ -829,6 +860,16 @@ __global__ void w_cluster_size_kernel(const unsigned long long* prop_sorted, uin
 // host driver
 // ---------------------------------------------------------------------------------------
 namespace nlpf {
+
+// Slot arrays after the 8 scalar counters, NBR_SLOTS each: open (unclustered) vertices and
+// occupied buckets (summed), candidate bound and largest bucket or cluster (maxed).
+enum GridSlots : uint32_t {
+    SLOT_OPEN = 8,
+    SLOT_OCCUPIED = 8 + NBR_SLOTS,
+    SLOT_CAND = 8 + 2 * NBR_SLOTS,
+    SLOT_LARGEST = 8 + 3 * NBR_SLOTS,
+    SLOT_END = 8 + 4 * NBR_SLOTS
+};
 
 struct WorldWork {
     uint64_t N = 0, table = 0, nbuf_elems = 0, rng_n = 0, cub_bytes = 0;
@@ -899,7 +940,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.bucket_seed, table);
     rc |= dalloc(h, &w.cluster_id, N);
     rc |= dalloc(h, &w.cluster_size, N);
-    rc |= dalloc(h, &w.counters, 8);
+    rc |= dalloc(h, &w.counters, SLOT_END);  // 8 scalars, then the GridSlots arrays
     rc |= dalloc(h, &w.nbuf, nbuf_elems);
     rc |= dalloc(h, &w.prop, N);
     rc |= dalloc(h, &w.prop_sorted, N);
@@ -932,6 +973,18 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     w.nbuf_elems = nbuf_elems;
     w.rng_n = rng_n;
     return NLPF_OK;
+}
+
+
+static uint32_t slot_sum(const uint32_t* v) {
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < NBR_SLOTS; ++i) s += v[i];
+    return s;
+}
+static uint32_t slot_max(const uint32_t* v) {
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < NBR_SLOTS; ++i) m = std::max(m, v[i]);
+    return m;
 }
 
 static uint32_t read_u32(nlpf_handle* h, const uint32_t* d, int& rc) {
@@ -1028,10 +1081,10 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             tb = w.cub_bytes;
             NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.keys_sorted, w.pts, w.ids_sorted,
                                                            (int)n, 0, (int)table_bits, st));
-            // counters 4..6: occupied buckets, candidate bound, largest bucket (or cluster)
-            NLPF_HIP(h, hipMemsetAsync(w.counters + 4, 0, 3 * sizeof(uint32_t), st));
+            // slot arrays: occupied buckets, candidate bound, largest bucket (or cluster)
+            NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_OCCUPIED, 0, 3 * NBR_SLOTS * sizeof(uint32_t), st));
             hipLaunchKernelGGL(w_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, w.ids_sorted, n, recd, N,
-                               w.brange, w.pos_sorted, w.counters + 4, w.counters + 6);
+                               w.brange, w.pos_sorted, w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST);
             hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind,
                                new_flags + (uint64_t)d * N, w.pk);
             if ((rc = check_launch(h, "world grid"))) return rc;
@@ -1057,16 +1110,16 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                        w.seed_entry);
                     uint32_t n_uncl = 0;
                     for (int inner = 0;; ++inner) {
-                        NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * 4, st));
+                        NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * (SLOT_OPEN + NBR_SLOTS), st));
                         hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
                                            w.pos_sorted, w.brange, w.seed_rank, w.seed_entry, w.cluster_size,
-                                           K, w.cluster_id, w.prop, w.counters + 1, w.counters + 2);
+                                           K, w.cluster_id, w.prop, w.counters + 1, w.counters + SLOT_OPEN);
                         if ((rc = check_launch(h, "world cluster"))) return rc;
-                        uint32_t hc[4];
+                        uint32_t hc[SLOT_OPEN + NBR_SLOTS];  // proposal count at 1, open-vertex slots
                         NLPF_HIP(h, hipMemcpyAsync(hc, w.counters, sizeof(hc), hipMemcpyDeviceToHost, st));
                         NLPF_HIP(h, hipStreamSynchronize(st));
                         const uint32_t n_prop = hc[1];
-                        n_uncl = hc[2];
+                        n_uncl = slot_sum(hc + SLOT_OPEN);
                         if (n_prop == 0) break;
                         tb = w.cub_bytes;
                         NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(w.cub, tb, w.prop, w.prop_sorted, (int)n_prop, 0,
@@ -1089,16 +1142,16 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             // as one launch with list slots sized by that count: first bounded by 27 x the
             // largest bucket (by the largest cluster when clustering), else counted exactly.
             if (cfg.cluster) {
-                NLPF_HIP(h, hipMemsetAsync(w.counters + 6, 0, sizeof(uint32_t), st));
+                NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_LARGEST, 0, NBR_SLOTS * sizeof(uint32_t), st));
                 hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, w.pts, n, w.cluster_id,
-                                   w.cluster_size, w.counters + 6);
+                                   w.cluster_size, w.counters + SLOT_LARGEST);
                 if ((rc = check_launch(h, "world cluster size"))) return rc;
             }
-            uint32_t hc[3];  // occupied buckets, (candidate count), largest bucket or cluster
-            NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + 4, sizeof(hc), hipMemcpyDeviceToHost, st));
+            uint32_t hc[3 * NBR_SLOTS];  // occupied buckets, (candidate bound), largest bucket or cluster
+            NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(hc), hipMemcpyDeviceToHost, st));
             NLPF_HIP(h, hipStreamSynchronize(st));
-            const uint32_t occupied = hc[0];
-            const uint64_t bound = (cfg.cluster ? 1ull : 27ull) * hc[2];
+            const uint32_t occupied = slot_sum(hc);
+            const uint64_t bound = (cfg.cluster ? 1ull : 27ull) * slot_max(hc + 2 * NBR_SLOTS);
             // kernel choice: records lie on surfaces, so about 9 of the 27 visited cells are
             // occupied and a vertex meets ~9 n / (occupied buckets) candidates; a wavefront per
             // vertex pays off from ~16 of them (with clusters, the cluster cap bounds the set)
@@ -1109,10 +1162,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             uint64_t max_cand = bound;
             if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
                 hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                   cfg.cluster ? w.cluster_id : nullptr, w.brange, w.counters + 5);
+                                   cfg.cluster ? w.cluster_id : nullptr, w.brange, w.counters + SLOT_CAND);
                 if ((rc = check_launch(h, "world candidates"))) return rc;
-                max_cand = read_u32(h, w.counters + 5, rc);
-                if (rc) return rc;
+                NLPF_HIP(h, hipMemcpyAsync(hc + NBR_SLOTS, w.counters + SLOT_CAND, NBR_SLOTS * sizeof(uint32_t),
+                                           hipMemcpyDeviceToHost, st));
+                NLPF_HIP(h, hipStreamSynchronize(st));
+                max_cand = slot_max(hc + NBR_SLOTS);
             }
             const bool single = max_cand <= K && (uint64_t)n * std::max<uint64_t>(max_cand, 1) <= w.nbuf_elems;
             const uint64_t batch =
