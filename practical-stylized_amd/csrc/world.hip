@@ -27,7 +27,9 @@ struct Grid {
     float pmin[3], pmax[3];
     uint32_t res[3];
     float radius;
-    uint64_t table;
+    uint64_t table;      // bucket space of the cell hash
+    uint4* ctab;         // compact table of the occupied buckets, cmask + 1 slots
+    uint32_t cmask, cshift;
 };
 
 // hash (lm_path_filtering.rs:20-23), u64 arithmetic
@@ -35,6 +37,30 @@ __device__ __forceinline__ uint32_t cell_bucket(uint32_t x, uint32_t y, uint32_t
     const uint64_t hsh = ((uint64_t)x * 73856093ull) ^ ((uint64_t)y * 19349663ull) ^ ((uint64_t)z * 83492791ull);
     return (uint32_t)(hsh % table);
 }
+// Compact bucket table. The cell hash keeps the reference's bucket space (hash_table_size
+// buckets, so colliding cells still share a bucket and its entries), but only the depth's
+// occupied buckets are stored: one 16-byte slot {bucket, start, end, seed} each, open addressing
+// with linear probing from a Fibonacci hash of the bucket id, at most a quarter full. Lookups
+// then touch a table of a few MB instead of 8 bytes x 64 Mi buckets.
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;  // bucket ids are below 2^30
+__device__ __forceinline__ uint32_t ctab_home(const Grid& g, uint32_t b) { return (b * 2654435769u) >> g.cshift; }
+// [start, end) of bucket b's entries, probing from slot s; (0, 0) when b holds none
+__device__ __forceinline__ uint2 bucket_range_from(const Grid& g, uint32_t b, uint32_t s) {
+    for (;;) {  // ends: the table always has empty slots
+        const uint4 v = g.ctab[s];
+        if (v.x == b) return make_uint2(v.y, v.z);
+        if (v.x == kEmptyKey) return make_uint2(0u, 0u);
+        s = (s + 1) & g.cmask;
+    }
+}
+__device__ __forceinline__ uint2 bucket_range(const Grid& g, uint32_t b) { return bucket_range_from(g, b, ctab_home(g, b)); }
+// slot of bucket b, which holds at least one entry
+__device__ __forceinline__ uint32_t bucket_slot(const Grid& g, uint32_t b) {
+    uint32_t s = ctab_home(g, b);
+    while (g.ctab[s].x != b) s = (s + 1) & g.cmask;
+    return s;
+}
+
 // HashGridVar::get_cell (:46-53)
 __device__ __forceinline__ void get_cell(const Grid& g, float px, float py, float pz, uint32_t c[3]) {
     const float p[3] = {px, py, pz};
@@ -96,22 +122,20 @@ __global__ void w_keys_kernel(const float* rec, uint64_t N, const uint32_t* pts,
     keys[t] = cell_bucket(c[0], c[1], c[2], g.table);
 }
 
-// Bucket ranges of the sorted keys as one 8-byte table entry [start, end) per bucket (one load
-// per bucket for the readers): the first entry of a bucket writes the start, the last one the
-// end. The last entry also finds the start by a galloping search back over the sorted keys
-// (O(log size) loads, so large buckets cost no serial walk) for the largest-bucket bound, and
-// the first entries count the occupied buckets (both over NBR_SLOTS slots).
+// Bucket ranges of the sorted keys into the compact table (cleared beforehand): the last entry
+// of each bucket finds the bucket's start by a galloping search back over the sorted keys
+// (O(log size) loads, so large buckets cost no serial walk), claims a slot and writes [start,
+// end); the same entries bound the largest bucket and the first entries count the occupied
+// buckets (both over NBR_SLOTS slots).
 __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint32_t n, const float* rec, uint64_t N,
-                                uint2* brange, float4* pos_sorted, uint32_t* occupied, uint32_t* max_bucket) {
+                                const uint8_t* new_flags, Grid g, float4* pos_sorted, uint8_t* flags_sorted,
+                                uint32_t* occupied, uint32_t* max_bucket) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t head = 0, size = 0;
     if (e < n) {
         const uint32_t k = keys[e];
-        uint32_t* half = reinterpret_cast<uint32_t*>(brange + k);
         head = (e == 0 || keys[e - 1] != k) ? 1u : 0u;
-        if (head) half[0] = e;
         if (e + 1 == n || keys[e + 1] != k) {
-            half[1] = e + 1;
             uint32_t s = e;
             if (!head) {
                 // keys[hi] == k throughout; the loop ends with keys[hi - step] != k or hi < step
@@ -128,9 +152,14 @@ __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint3
                 s = lo;
             }
             size = e + 1 - s;
+            uint32_t slot = ctab_home(g, k);  // one last entry per bucket: k is not in the table yet
+            while (atomicCAS(&g.ctab[slot].x, kEmptyKey, k) != kEmptyKey) slot = (slot + 1) & g.cmask;
+            g.ctab[slot].y = s;
+            g.ctab[slot].z = e + 1;
         }
         const uint32_t i = ids[e];
         pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
+        flags_sorted[e] = new_flags[i];
     }
     head = wave_sum(head);
     size = wave_max(size);
@@ -155,10 +184,10 @@ __global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint
     if ((threadIdx.x & 63) == 0 && members) atomicMax(out_max + (blockIdx.x & (NBR_SLOTS - 1)), members);
 }
 
-__global__ void w_clear_ranges_kernel(const uint32_t* keys, uint32_t n, uint2* brange) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    brange[keys[e]] = make_uint2(0u, 0u);
+// empty compact table: no bucket, no entries, no seed
+__global__ void w_ctab_clear_kernel(uint4* ctab, uint32_t slots) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < slots) ctab[s] = make_uint4(kEmptyKey, 0u, 0u, kEmptyKey);
 }
 
 // per-path payload of level d (indirect record from the previous propagation)
@@ -189,7 +218,7 @@ struct FilterArgs {
     const uint32_t* cluster_id;  // nullptr when clustering is off
     const NbrRec* pk;
     const float4* pos_sorted;
-    const uint2* brange;  // [start, end) of each bucket's entries
+    const uint8_t* flags_sorted;  // new_flags of each entry's path, in entry order
     Grid g;
     uint32_t K;
     uint32_t* nbuf;
@@ -232,36 +261,70 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
     uint32_t cell[3];
     get_cell(a.g, q.x, q.y, q.z, cell);
     uint32_t cnt = 0;
-    // the 27 bucket ranges are loaded up front (one round trip), then walked in order
-    uint32_t r0[27], r1[27];
-#pragma unroll
-    for (int c = 0; c < 27; ++c) {
-        const int cx = (int)cell[0] + c % 3 - 1, cy = (int)cell[1] + (c / 3) % 3 - 1, cz = (int)cell[2] + c / 9 - 1;
-        r0[c] = r1[c] = 0;
-        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
-            const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
-            const uint2 br = a.brange[b];
-            r0[c] = br.x;
-            r1[c] = br.y;
+    // the first two list entries stay in registers (at small radii most lists hold one or two)
+    uint32_t b0 = 0, b1 = 0;
+    auto put = [&](uint32_t i, uint32_t j) {
+        if (i == 0) b0 = j;
+        else if (i == 1) b1 = j;
+        else buf[i] = j;
+    };
+    auto visit = [&](const float4 qe, uint32_t fl) {
+        if (!(dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius)) return;
+        const uint32_t j = __float_as_uint(qe.w);
+        if (a.cluster_id && a.cluster_id[j] != mycluster) return;
+        if (a.flag_check && fl != myflags) return;
+        // reservoir over the reuse cap (:813-821)
+        if (cnt < a.K) {
+            put(cnt, j);
+        } else {
+            const uint32_t r = pcg_next(rng) % (cnt + 1);
+            if (r < a.K) put(r, j);
         }
-    }
+        ++cnt;
+    };
+    // nine cells at a time: their bucket ranges in one round of loads, then their first entries
+    // (most buckets hold one or two at small radii) in one more, then the walk in cell order
 #pragma unroll
-    for (int c = 0; c < 27; ++c) {
-        const uint32_t e1 = r1[c];
-        for (uint32_t e = r0[c]; e < e1; ++e) {
-            const float4 qe = a.pos_sorted[e];
-            if (!(dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius)) continue;
-            const uint32_t j = __float_as_uint(qe.w);
-            if (a.cluster_id && a.cluster_id[j] != mycluster) continue;
-            if (a.flag_check && (uint32_t)a.new_flags[j] != myflags) continue;
-            // reservoir over the reuse cap (:813-821)
-            if (cnt < a.K) {
-                buf[cnt] = j;
-            } else {
-                const uint32_t r = pcg_next(rng) % (cnt + 1);
-                if (r < a.K) buf[r] = j;
+    for (int g0 = 0; g0 < 27; g0 += 9) {
+        uint32_t r0[9], r1[9], bk[9];
+        uint4 v[9];
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {  // first probes of the nine lookups
+            const int c = g0 + u;
+            const int cx = (int)cell[0] + c % 3 - 1, cy = (int)cell[1] + (c / 3) % 3 - 1, cz = (int)cell[2] + c / 9 - 1;
+            bk[u] = kEmptyKey;  // a cell outside the grid matches the empty slot below: no entries
+            v[u] = make_uint4(kEmptyKey, 0u, 0u, 0u);
+            if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
+                bk[u] = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
+                v[u] = a.g.ctab[ctab_home(a.g, bk[u])];
             }
-            ++cnt;
+        }
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            uint2 br = make_uint2(v[u].y, v[u].z);
+            if (v[u].x != bk[u])
+                br = v[u].x == kEmptyKey ? make_uint2(0u, 0u)
+                                         : bucket_range_from(a.g, bk[u], (ctab_home(a.g, bk[u]) + 1) & a.g.cmask);
+            r0[u] = br.x;
+            r1[u] = br.y;
+        }
+        float4 qf[9];
+        uint32_t ff[9];
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            qf[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ff[u] = 0;
+            if (r0[u] < r1[u]) {
+                qf[u] = a.pos_sorted[r0[u]];
+                ff[u] = a.flags_sorted[r0[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            if (r0[u] < r1[u]) {
+                visit(qf[u], ff[u]);
+                for (uint32_t e = r0[u] + 1; e < r1[u]; ++e) visit(a.pos_sorted[e], a.flags_sorted[e]);
+            }
         }
     }
     const uint32_t used = cnt < a.K ? cnt : a.K;
@@ -271,8 +334,9 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
     // hundreds of reused terms per vertex: Kahan-compensated sums keep FP32 within ~1 ulp of
     // the exact sum (a plain running sum drifts by ~n ulp)
     float dsum[3] = {0, 0, 0}, dc[3] = {0, 0, 0}, isum[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
+    auto get = [&](uint32_t i) { return i == 0 ? b0 : i == 1 ? b1 : buf[i]; };
     for (uint32_t k = 0; k < used; ++k) {
-        const uint32_t j = buf[k];
+        const uint32_t j = get(k);
         const NbrRec& r = a.pk[j];
         const float4 dw = r.dwi_pdf;
         if (dw.w > 0.0f) {
@@ -297,7 +361,7 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
             const double wx = wi.x, wy = wi.y, wz = wi.z;
             double sum_cos = 0.0;
             for (uint32_t m = 0; m < used; ++m) {
-                const SmisRec r = smis_rec(a.pk[buf[m]]);
+                const SmisRec r = smis_rec(a.pk[get(m)]);
                 sum_cos = smis_add(sum_cos, r, r.bits, wi, wx, wy, wz);
             }
             const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);
@@ -394,7 +458,7 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                   cz = (int)cell[2] + lane / 9 - 1;
         if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
             const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
-            const uint2 br = a.brange[b];
+            const uint2 br = bucket_range(a.g, b);
             c_e0 = br.x;
             c_n = br.y - br.x;
         }
@@ -425,10 +489,11 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                 start = ce;
             }
             const float4 qe = a.pos_sorted[e];
+            const uint32_t fl = a.flags_sorted[e];
             j = __float_as_uint(qe.w);
             ok = dist3(qe.x - q.x, qe.y - q.y, qe.z - q.z) <= a.g.radius;
             if (ok && a.cluster_id && a.cluster_id[j] != mycluster) ok = false;
-            if (ok && a.flag_check && (uint32_t)a.new_flags[j] != myflags) ok = false;
+            if (ok && a.flag_check && fl != myflags) ok = false;
         }
         const uint64_t mask = __ballot(ok);
         const uint32_t total = (uint32_t)__popcll(mask);
@@ -624,7 +689,7 @@ __global__ void w_propagate_kernel(const float* rec_child, const float* rec_pare
 // buckets of its 27-cell walk (around its cluster seed when clustering), i.e. the candidates the
 // filter tests
 __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
-                                  const uint32_t* cluster_id, const uint2* brange,
+                                  const uint32_t* cluster_id,
                                   uint32_t* out_max) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t total = 0;
@@ -641,7 +706,7 @@ __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* 
                         cz >= (int)g.res[2])
                         continue;
                     const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
-                    const uint2 br = brange[b];
+                    const uint2 br = bucket_range(g, b);
                     total += br.y - br.x;
                 }
     }
@@ -718,33 +783,32 @@ __global__ void w_rng_init_kernel(Pcg32* rng, uint32_t n, uint64_t key) {
 // ---- deterministic clustering (replaces the CAS seeding / joining of :532-623) ----
 // seed per bucket = smallest unclustered path id in it
 __global__ void w_cluster_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
-                                      const uint32_t* cluster_id, uint32_t* bucket_seed) {
+                                      const uint32_t* cluster_id) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = pts[t];
     if (cluster_id[i] != 0xFFFFFFFFu) return;
     uint32_t c[3];
     get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
-    atomicMin(&bucket_seed[cell_bucket(c[0], c[1], c[2], g.table)], i);
+    atomicMin(&g.ctab[bucket_slot(g, cell_bucket(c[0], c[1], c[2], g.table))].w, i);
 }
 __global__ void w_cluster_mark_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
-                                           uint32_t* cluster_id, const uint32_t* bucket_seed) {
+                                           uint32_t* cluster_id) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = pts[t];
     if (cluster_id[i] != 0xFFFFFFFFu) return;
     uint32_t c[3];
     get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
-    if (bucket_seed[cell_bucket(c[0], c[1], c[2], g.table)] == i) cluster_id[i] = i;
+    if (g.ctab[bucket_slot(g, cell_bucket(c[0], c[1], c[2], g.table))].w == i) cluster_id[i] = i;
 }
-__global__ void w_cluster_reset_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
-                                            uint32_t* bucket_seed) {
+__global__ void w_cluster_reset_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = pts[t];
     uint32_t c[3];
     get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
-    bucket_seed[cell_bucket(c[0], c[1], c[2], g.table)] = 0xFFFFFFFFu;
+    g.ctab[bucket_slot(g, cell_bucket(c[0], c[1], c[2], g.table))].w = kEmptyKey;
 }
 // seed index of one seeding round over the bucket-sorted entries: seed_rank[e] = number of
 // seed entries before e (exclusive scan of these flags), seed_entry[seed_rank[e]] = e
@@ -764,7 +828,7 @@ __global__ void w_seed_scatter_kernel(const uint32_t* flags, const uint32_t* see
 // with the smallest vertex ids first. Only seeds can take a proposal, so the walk visits a
 // bucket's seed entries (seed_entry[seed_rank[start] .. seed_rank[end]), in bucket order).
 __global__ void __launch_bounds__(256) w_cluster_propose_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
-                                         const float4* pos_sorted, const uint2* brange,
+                                         const float4* pos_sorted,
                                          const uint32_t* seed_rank, const uint32_t* seed_entry,
                                          const uint32_t* cluster_size, uint32_t cap, const uint32_t* cluster_id,
                                          unsigned long long* prop, uint32_t* n_prop, uint32_t* n_unclustered) {
@@ -784,7 +848,7 @@ __global__ void __launch_bounds__(256) w_cluster_propose_kernel(const float* rec
                     if (x < 0 || y < 0 || z < 0 || x >= (int)g.res[0] || y >= (int)g.res[1] || z >= (int)g.res[2])
                         continue;
                     const uint32_t b = cell_bucket((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table);
-                    const uint2 br = brange[b];
+                    const uint2 br = bucket_range(g, b);
                     const uint32_t s1 = seed_rank[br.y];
                     for (uint32_t si = seed_rank[br.x]; si < s1; ++si) {
                         const float4 qe = pos_sorted[seed_entry[si]];
@@ -872,14 +936,15 @@ enum GridSlots : uint32_t {
 };
 
 struct WorldWork {
-    uint64_t N = 0, table = 0, nbuf_elems = 0, rng_n = 0, cub_bytes = 0;
+    uint64_t N = 0, nbuf_elems = 0, rng_n = 0, cub_bytes = 0;
     uint8_t* flag = nullptr;
     uint32_t *pts = nullptr, *keys = nullptr, *keys_sorted = nullptr, *ids_sorted = nullptr;
-    uint2* brange = nullptr;
-    uint32_t* bucket_seed = nullptr;
+    uint4* ctab = nullptr;  // compact bucket table, ctab_slots slots
+    uint64_t ctab_slots = 0;
     uint32_t *cluster_id = nullptr, *cluster_size = nullptr, *counters = nullptr, *nbuf = nullptr;
     unsigned long long *prop = nullptr, *prop_sorted = nullptr, *nbr = nullptr;
     float4* pos_sorted = nullptr;
+    uint8_t* flags_sorted = nullptr;
     NbrRec* pk = nullptr;
     float *ind = nullptr, *rad = nullptr;
     Pcg32* rng = nullptr;
@@ -889,9 +954,9 @@ struct WorldWork {
 };
 
 static void world_free(WorldWork& w) {
-    void* ps[] = {w.flag, w.pts, w.keys, w.keys_sorted, w.ids_sorted, w.brange, w.bucket_seed,
+    void* ps[] = {w.flag, w.pts, w.keys, w.keys_sorted, w.ids_sorted, w.ctab,
                   w.cluster_id, w.cluster_size, w.counters, w.nbuf, w.prop, w.prop_sorted, w.nbr,
-                  w.pos_sorted, w.pk, w.ind, w.rad, w.rng, w.seed_flags, w.seed_rank, w.seed_entry,
+                  w.pos_sorted, w.flags_sorted, w.pk, w.ind, w.rad, w.rng, w.seed_flags, w.seed_rank, w.seed_entry,
                   w.smis_sum, w.cub};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -925,9 +990,8 @@ static int dalloc(nlpf_handle* h, T** p, uint64_t n) {
     return NLPF_OK;
 }
 
-static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t table, uint64_t nbuf_elems,
-                         uint64_t rng_n) {
-    if (w.N >= N && w.table == table && w.nbuf_elems >= nbuf_elems && w.rng_n >= rng_n) return NLPF_OK;
+static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t nbuf_elems, uint64_t rng_n) {
+    if (w.N >= N && w.nbuf_elems >= nbuf_elems && w.rng_n >= rng_n) return NLPF_OK;
     NLPF_HIP(h, hipStreamSynchronize(h->stream));
     world_free(w);
     int rc = 0;
@@ -936,8 +1000,10 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.keys, N);
     rc |= dalloc(h, &w.keys_sorted, N);
     rc |= dalloc(h, &w.ids_sorted, N);
-    rc |= dalloc(h, &w.brange, table);
-    rc |= dalloc(h, &w.bucket_seed, table);
+    // compact bucket table: at most N occupied buckets in 4 N or more slots (a power of two)
+    w.ctab_slots = 64;
+    while (w.ctab_slots < 4 * N) w.ctab_slots <<= 1;
+    rc |= dalloc(h, &w.ctab, w.ctab_slots);
     rc |= dalloc(h, &w.cluster_id, N);
     rc |= dalloc(h, &w.cluster_size, N);
     rc |= dalloc(h, &w.counters, SLOT_END);  // 8 scalars, then the GridSlots arrays
@@ -946,6 +1012,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     rc |= dalloc(h, &w.prop_sorted, N);
     rc |= dalloc(h, &w.nbr, 64 * NBR_SLOTS);  // neighbours, then SMIS pairs, per depth
     rc |= dalloc(h, &w.pos_sorted, N);
+    rc |= dalloc(h, &w.flags_sorted, N);
     rc |= dalloc(h, &w.pk, N);
     rc |= dalloc(h, &w.ind, 8 * N);
     rc |= dalloc(h, &w.rad, 3 * N);
@@ -965,11 +1032,7 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t tabl
     NLPF_HIP(h, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, w.seed_flags, w.seed_rank, (int)(N + 1), h->stream));
     w.cub_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
     NLPF_HIP(h, hipMalloc(&w.cub, w.cub_bytes));
-    // empty bucket ranges; the filter of every depth restores the entries it set
-    NLPF_HIP(h, hipMemsetAsync(w.brange, 0, sizeof(uint2) * table, h->stream));
-    NLPF_HIP(h, hipMemsetAsync(w.bucket_seed, 0xFF, sizeof(uint32_t) * table, h->stream));
     w.N = N;
-    w.table = table;
     w.nbuf_elems = nbuf_elems;
     w.rng_n = rng_n;
     return NLPF_OK;
@@ -1035,7 +1098,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     const uint64_t nbuf_elems = std::max<uint64_t>(1, (uint64_t)cfg.vram * 1024 * 1024 / 4 + 1);
     nlpf_world_state* ws = world_state(h);
     WorldWork& w = ws->w;
-    int rc = world_reserve(h, w, N, cfg.hash_table_size, std::max<uint64_t>(nbuf_elems, kmax), batch_max);
+    int rc = world_reserve(h, w, N, std::max<uint64_t>(nbuf_elems, kmax), batch_max);
     if (rc) return rc;
     const uint32_t table_bits = std::max(1, (int)std::ceil(std::log2((double)cfg.hash_table_size)));
     hipStream_t st = h->stream;
@@ -1076,6 +1139,14 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             }
             g.radius = radius;
             g.table = cfg.hash_table_size;
+            // this depth's compact table: 4 n slots or more, a power of two
+            uint32_t cbits = 6;
+            while ((1ull << cbits) < 4ull * n) ++cbits;
+            g.ctab = w.ctab;
+            g.cmask = (uint32_t)((1ull << cbits) - 1);
+            g.cshift = 32u - cbits;
+            hipLaunchKernelGGL(w_ctab_clear_kernel, grid1(1ull << cbits), dim3(256), 0, st, w.ctab,
+                               (uint32_t)(1ull << cbits));
             hipLaunchKernelGGL(w_keys_kernel, grid1(n), dim3(256), 0, st, rec + (uint64_t)d * WP * N, N, w.pts, n, g,
                                w.keys);
             tb = w.cub_bytes;
@@ -1084,7 +1155,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             // slot arrays: occupied buckets, candidate bound, largest bucket (or cluster)
             NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_OCCUPIED, 0, 3 * NBR_SLOTS * sizeof(uint32_t), st));
             hipLaunchKernelGGL(w_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, w.ids_sorted, n, recd, N,
-                               w.brange, w.pos_sorted, w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST);
+                               new_flags + (uint64_t)d * N, g, w.pos_sorted, w.flags_sorted,
+                               w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST);
             hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind,
                                new_flags + (uint64_t)d * N, w.pk);
             if ((rc = check_launch(h, "world grid"))) return rc;
@@ -1094,12 +1166,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 NLPF_HIP(h, hipMemsetAsync(w.cluster_size, 0, sizeof(uint32_t) * N, st));
                 for (int outer = 0;; ++outer) {
                     if (outer > 4096) NLPF_FAIL(h, NLPF_E_NUMERIC, "clustering did not converge");
+                    // per-bucket seeds live in the compact table's fourth word
                     hipLaunchKernelGGL(w_cluster_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                       w.cluster_id, w.bucket_seed);
+                                       w.cluster_id);
                     hipLaunchKernelGGL(w_cluster_mark_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                       w.cluster_id, w.bucket_seed);
-                    hipLaunchKernelGGL(w_cluster_reset_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                       w.bucket_seed);
+                                       w.cluster_id);
+                    hipLaunchKernelGGL(w_cluster_reset_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g);
                     // seed entries of every bucket, for the propose walks of this round
                     hipLaunchKernelGGL(w_seed_flags_kernel, grid1((uint64_t)n + 1), dim3(256), 0, st, w.ids_sorted, n,
                                        w.cluster_id, w.seed_flags);
@@ -1112,7 +1184,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     for (int inner = 0;; ++inner) {
                         NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * (SLOT_OPEN + NBR_SLOTS), st));
                         hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                           w.pos_sorted, w.brange, w.seed_rank, w.seed_entry, w.cluster_size,
+                                           w.pos_sorted, w.seed_rank, w.seed_entry, w.cluster_size,
                                            K, w.cluster_id, w.prop, w.counters + 1, w.counters + SLOT_OPEN);
                         if ((rc = check_launch(h, "world cluster"))) return rc;
                         uint32_t hc[SLOT_OPEN + NBR_SLOTS];  // proposal count at 1, open-vertex slots
@@ -1162,7 +1234,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             uint64_t max_cand = bound;
             if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
                 hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
-                                   cfg.cluster ? w.cluster_id : nullptr, w.brange, w.counters + SLOT_CAND);
+                                   cfg.cluster ? w.cluster_id : nullptr, w.counters + SLOT_CAND);
                 if ((rc = check_launch(h, "world candidates"))) return rc;
                 NLPF_HIP(h, hipMemcpyAsync(hc + NBR_SLOTS, w.counters + SLOT_CAND, NBR_SLOTS * sizeof(uint32_t),
                                            hipMemcpyDeviceToHost, st));
@@ -1189,7 +1261,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     fa.cluster_id = cfg.cluster ? w.cluster_id : nullptr;
                     fa.pk = w.pk;
                     fa.pos_sorted = w.pos_sorted;
-                    fa.brange = w.brange;
+                    fa.flags_sorted = w.flags_sorted;
                     fa.g = g;
                     fa.K = K;
                     fa.nbuf = w.nbuf;
@@ -1215,7 +1287,6 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 }
             }
             if ((rc = check_launch(h, "world filter"))) return rc;
-            hipLaunchKernelGGL(w_clear_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, n, w.brange);
         }
         if (d > 0) {
             hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
