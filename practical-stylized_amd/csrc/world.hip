@@ -4,7 +4,8 @@
 // Per depth d = D..0 (PathFiltering::render, lm_path_filtering.rs:1673-1774):
 //   gather   paths with a depth-d vertex, compacted in path order      (kernel_gather_points :507-531)
 //   grid     Teschner-hashed cell buckets, built by a stable radix sort of (bucket, path) pairs
-//            instead of the CAS push-front lists (HashGrid::build_kernel :230-255); a bucket's
+//            instead of the CAS push-front lists (HashGrid::build_kernel :230-255), with the
+//            occupied buckets' entry ranges in a compact open-addressing table; a bucket's
 //            entries are visited in ascending path order, every bucket of the 27-cell
 //            neighbourhood is walked (duplicates from colliding cells included, :60-123)
 //   cluster  (optional) deterministic seeding / joining                 (:532-623, 1675-1710)
