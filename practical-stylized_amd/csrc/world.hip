@@ -842,25 +842,48 @@ __global__ void __launch_bounds__(256) w_cluster_propose_kernel(const float* rec
         const float px = pl(rec, N, 0, i), py = pl(rec, N, 1, i), pz = pl(rec, N, 2, i);
         uint32_t cell[3];
         get_cell(g, px, py, pz, cell);
-        for (int cz = -1; cz <= 1 && seed == 0xFFFFFFFFu; ++cz)
-            for (int cy = -1; cy <= 1 && seed == 0xFFFFFFFFu; ++cy)
-                for (int cx = -1; cx <= 1 && seed == 0xFFFFFFFFu; ++cx) {
-                    const int x = (int)cell[0] + cx, y = (int)cell[1] + cy, z = (int)cell[2] + cz;
-                    if (x < 0 || y < 0 || z < 0 || x >= (int)g.res[0] || y >= (int)g.res[1] || z >= (int)g.res[2])
-                        continue;
-                    const uint32_t b = cell_bucket((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table);
-                    const uint2 br = bucket_range(g, b);
-                    const uint32_t s1 = seed_rank[br.y];
-                    for (uint32_t si = seed_rank[br.x]; si < s1; ++si) {
-                        const float4 qe = pos_sorted[seed_entry[si]];
-                        if (!(dist3(qe.x - px, qe.y - py, qe.z - pz) <= g.radius)) continue;
-                        const uint32_t j = __float_as_uint(qe.w);
-                        if (cluster_size[j] < cap) {
-                            seed = j;
-                            break;
-                        }
+        // the 27-cell walk (dz outer, dx inner) nine cells at a time: their bucket lookups in one
+        // round of loads, their seed-rank bounds in a second, then the seeds in walk order
+        for (int g0 = 0; g0 < 27 && seed == 0xFFFFFFFFu; g0 += 9) {
+            uint32_t bk[9], s0[9], s1[9];
+            uint4 v[9];
+#pragma unroll
+            for (int u = 0; u < 9; ++u) {
+                const int c = g0 + u;
+                const int x = (int)cell[0] + c % 3 - 1, y = (int)cell[1] + (c / 3) % 3 - 1, z = (int)cell[2] + c / 9 - 1;
+                bk[u] = kEmptyKey;  // a cell outside the grid matches the empty slot: no entries
+                v[u] = make_uint4(kEmptyKey, 0u, 0u, 0u);
+                if (x >= 0 && y >= 0 && z >= 0 && x < (int)g.res[0] && y < (int)g.res[1] && z < (int)g.res[2]) {
+                    bk[u] = cell_bucket((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table);
+                    v[u] = g.ctab[ctab_home(g, bk[u])];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 9; ++u) {
+                uint2 br = make_uint2(v[u].y, v[u].z);
+                if (v[u].x != bk[u])
+                    br = v[u].x == kEmptyKey ? make_uint2(0u, 0u)
+                                             : bucket_range_from(g, bk[u], (ctab_home(g, bk[u]) + 1) & g.cmask);
+                s0[u] = s1[u] = 0;
+                if (br.x < br.y) {
+                    s0[u] = seed_rank[br.x];
+                    s1[u] = seed_rank[br.y];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 9; ++u) {
+                if (seed != 0xFFFFFFFFu) break;
+                for (uint32_t si = s0[u]; si < s1[u]; ++si) {
+                    const float4 qe = pos_sorted[seed_entry[si]];
+                    if (!(dist3(qe.x - px, qe.y - py, qe.z - pz) <= g.radius)) continue;
+                    const uint32_t j = __float_as_uint(qe.w);
+                    if (cluster_size[j] < cap) {
+                        seed = j;
+                        break;
                     }
                 }
+            }
+        }
     }
     // the open count goes to slot blockIdx % NBR_SLOTS, one update per wavefront; proposal slots
     // come from one atomic per workgroup on the shared allocator, split by ballot prefix counts
