@@ -73,6 +73,24 @@ __device__ __forceinline__ void get_cell(const Grid& g, float px, float py, floa
     }
 }
 
+// Lookups of the 27-cell walk around `cell` (cell c = 9 (dz+1) + 3 (dy+1) + (dx+1), dx inner),
+// split so that a thread can issue the first probes of several cells as one round of loads:
+// first_probe loads the home slot of cell c's bucket (bk = its bucket, kEmptyKey outside the
+// grid, which then matches the empty slot it returns); probe_range finishes the lookup.
+__device__ __forceinline__ uint4 first_probe(const Grid& g, const uint32_t cell[3], int c, uint32_t& bk) {
+    const int cx = (int)cell[0] + c % 3 - 1, cy = (int)cell[1] + (c / 3) % 3 - 1, cz = (int)cell[2] + c / 9 - 1;
+    bk = kEmptyKey;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] || cz >= (int)g.res[2])
+        return make_uint4(kEmptyKey, 0u, 0u, 0u);
+    bk = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
+    return g.ctab[ctab_home(g, bk)];
+}
+__device__ __forceinline__ uint2 probe_range(const Grid& g, const uint4 v, uint32_t bk) {
+    if (v.x == bk) return make_uint2(v.y, v.z);
+    if (v.x == kEmptyKey) return make_uint2(0u, 0u);
+    return bucket_range_from(g, bk, (ctab_home(g, bk) + 1) & g.cmask);
+}
+
 // |d| without fused multiply-adds, so the radius test rounds exactly like plain IEEE code
 __device__ __forceinline__ float dist3(float x, float y, float z) {
 #pragma clang fp contract(off)
@@ -290,22 +308,10 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
         uint32_t r0[9], r1[9], bk[9];
         uint4 v[9];
 #pragma unroll
-        for (int u = 0; u < 9; ++u) {  // first probes of the nine lookups
-            const int c = g0 + u;
-            const int cx = (int)cell[0] + c % 3 - 1, cy = (int)cell[1] + (c / 3) % 3 - 1, cz = (int)cell[2] + c / 9 - 1;
-            bk[u] = kEmptyKey;  // a cell outside the grid matches the empty slot below: no entries
-            v[u] = make_uint4(kEmptyKey, 0u, 0u, 0u);
-            if (cx >= 0 && cy >= 0 && cz >= 0 && cx < (int)a.g.res[0] && cy < (int)a.g.res[1] && cz < (int)a.g.res[2]) {
-                bk[u] = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, a.g.table);
-                v[u] = a.g.ctab[ctab_home(a.g, bk[u])];
-            }
-        }
+        for (int u = 0; u < 9; ++u) v[u] = first_probe(a.g, cell, g0 + u, bk[u]);
 #pragma unroll
         for (int u = 0; u < 9; ++u) {
-            uint2 br = make_uint2(v[u].y, v[u].z);
-            if (v[u].x != bk[u])
-                br = v[u].x == kEmptyKey ? make_uint2(0u, 0u)
-                                         : bucket_range_from(a.g, bk[u], (ctab_home(a.g, bk[u]) + 1) & a.g.cmask);
+            const uint2 br = probe_range(a.g, v[u], bk[u]);
             r0[u] = br.x;
             r1[u] = br.y;
         }
@@ -699,17 +705,18 @@ __global__ void w_cand_max_kernel(const float* rec, uint64_t N, const uint32_t* 
         const uint32_t qc = cluster_id ? cluster_id[c] : c;
         uint32_t cell[3];
         get_cell(g, pl(rec, N, 0, qc), pl(rec, N, 1, qc), pl(rec, N, 2, qc), cell);
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int cx = (int)cell[0] + dx, cy = (int)cell[1] + dy, cz = (int)cell[2] + dz;
-                    if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.res[0] || cy >= (int)g.res[1] ||
-                        cz >= (int)g.res[2])
-                        continue;
-                    const uint32_t b = cell_bucket((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, g.table);
-                    const uint2 br = bucket_range(g, b);
-                    total += br.y - br.x;
-                }
+#pragma unroll
+        for (int g0 = 0; g0 < 27; g0 += 9) {  // nine lookups per round of loads
+            uint32_t bk[9];
+            uint4 v[9];
+#pragma unroll
+            for (int u = 0; u < 9; ++u) v[u] = first_probe(g, cell, g0 + u, bk[u]);
+#pragma unroll
+            for (int u = 0; u < 9; ++u) {
+                const uint2 br = probe_range(g, v[u], bk[u]);
+                total += br.y - br.x;
+            }
+        }
     }
     total = wave_max(total);
     if ((threadIdx.x & 63) == 0 && total) atomicMax(out_max + (blockIdx.x & (NBR_SLOTS - 1)), total);
@@ -803,21 +810,18 @@ __global__ void w_cluster_mark_seed_kernel(const float* rec, uint64_t N, const u
     get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
     if (g.ctab[bucket_slot(g, cell_bucket(c[0], c[1], c[2], g.table))].w == i) cluster_id[i] = i;
 }
-__global__ void w_cluster_reset_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t i = pts[t];
-    uint32_t c[3];
-    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
-    g.ctab[bucket_slot(g, cell_bucket(c[0], c[1], c[2], g.table))].w = kEmptyKey;
-}
 // seed index of one seeding round over the bucket-sorted entries: seed_rank[e] = number of
 // seed entries before e (exclusive scan of these flags), seed_entry[seed_rank[e]] = e
-__global__ void w_seed_flags_kernel(const uint32_t* ids_sorted, uint32_t n, const uint32_t* cluster_id,
-                                    uint32_t* flags) {
+// The same pass clears the round's per-bucket seed words (the mark kernel has read them): the
+// first entry of each bucket resets its slot, so every occupied bucket starts the next round
+// at kEmptyKey.
+__global__ void w_seed_flags_kernel(const uint32_t* keys_sorted, const uint32_t* ids_sorted, uint32_t n, Grid g,
+                                    const uint32_t* cluster_id, uint32_t* flags) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e > n) return;
     flags[e] = e < n && cluster_id[ids_sorted[e]] == ids_sorted[e] ? 1u : 0u;
+    if (e < n && (e == 0 || keys_sorted[e - 1] != keys_sorted[e]))
+        g.ctab[bucket_slot(g, keys_sorted[e])].w = kEmptyKey;
 }
 __global__ void w_seed_scatter_kernel(const uint32_t* flags, const uint32_t* seed_rank, uint32_t n,
                                       uint32_t* seed_entry) {
@@ -848,22 +852,10 @@ __global__ void __launch_bounds__(256) w_cluster_propose_kernel(const float* rec
             uint32_t bk[9], s0[9], s1[9];
             uint4 v[9];
 #pragma unroll
-            for (int u = 0; u < 9; ++u) {
-                const int c = g0 + u;
-                const int x = (int)cell[0] + c % 3 - 1, y = (int)cell[1] + (c / 3) % 3 - 1, z = (int)cell[2] + c / 9 - 1;
-                bk[u] = kEmptyKey;  // a cell outside the grid matches the empty slot: no entries
-                v[u] = make_uint4(kEmptyKey, 0u, 0u, 0u);
-                if (x >= 0 && y >= 0 && z >= 0 && x < (int)g.res[0] && y < (int)g.res[1] && z < (int)g.res[2]) {
-                    bk[u] = cell_bucket((uint32_t)x, (uint32_t)y, (uint32_t)z, g.table);
-                    v[u] = g.ctab[ctab_home(g, bk[u])];
-                }
-            }
+            for (int u = 0; u < 9; ++u) v[u] = first_probe(g, cell, g0 + u, bk[u]);
 #pragma unroll
             for (int u = 0; u < 9; ++u) {
-                uint2 br = make_uint2(v[u].y, v[u].z);
-                if (v[u].x != bk[u])
-                    br = v[u].x == kEmptyKey ? make_uint2(0u, 0u)
-                                             : bucket_range_from(g, bk[u], (ctab_home(g, bk[u]) + 1) & g.cmask);
+                const uint2 br = probe_range(g, v[u], bk[u]);
                 s0[u] = s1[u] = 0;
                 if (br.x < br.y) {
                     s0[u] = seed_rank[br.x];
@@ -1195,10 +1187,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                        w.cluster_id);
                     hipLaunchKernelGGL(w_cluster_mark_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
                                        w.cluster_id);
-                    hipLaunchKernelGGL(w_cluster_reset_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g);
                     // seed entries of every bucket, for the propose walks of this round
-                    hipLaunchKernelGGL(w_seed_flags_kernel, grid1((uint64_t)n + 1), dim3(256), 0, st, w.ids_sorted, n,
-                                       w.cluster_id, w.seed_flags);
+                    hipLaunchKernelGGL(w_seed_flags_kernel, grid1((uint64_t)n + 1), dim3(256), 0, st, w.keys_sorted,
+                                       w.ids_sorted, n, g, w.cluster_id, w.seed_flags);
                     tb = w.cub_bytes;
                     NLPF_HIP(h, hipcub::DeviceScan::ExclusiveSum(w.cub, tb, w.seed_flags, w.seed_rank, (int)(n + 1),
                                                                  st));
