@@ -1094,7 +1094,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     if (desc->branch != cfg.branch) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: branch must equal config branch");
     const uint64_t npix = (uint64_t)desc->width * desc->height;
     const uint64_t N = npix * desc->branch;
-    if (N >= (1ull << 31)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: too many paths (< 2^31)");
+    // the compact bucket table hashes into at most 2^32 slots (4 per path or more)
+    if (N >= (1ull << 30)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: too many paths (< 2^30)");
     const uint32_t L = desc->levels;
     float bmin[3], bmax[3];
     for (int i = 0; i < 3; ++i) {
