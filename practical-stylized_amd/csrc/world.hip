@@ -189,8 +189,10 @@ __global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint3
     }
 }
 
-// largest cluster (seed + joiners) of the depth, max over out_max[NBR_SLOTS]: with clustering, a
-// bound on every vertex's candidate count (its candidates are members of its own cluster)
+// largest cluster (seed + joiners) of the depth, max over out_max[NBR_SLOTS]. With clustering a
+// vertex's candidates are members of its own cluster, but a bucket is walked once per visited
+// cell that hashes to it, so its members can be counted up to 27 times: 27 x this value bounds
+// every candidate count
 __global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint32_t* cluster_id,
                                      const uint32_t* cluster_size, uint32_t* out_max) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -783,9 +785,11 @@ __global__ void w_film_kernel(const float* rad0, const float* rec0, const int32_
     film[3 * px + 2] = wsum > 0.0f ? az / wsum : 0.0f;
 }
 
-__global__ void w_rng_init_kernel(Pcg32* rng, uint32_t n, uint64_t key) {
+// reservoir streams of the batch slots, seeded as the reference's
+// init_pcg32_buffer_with_seed (sampler/mod.rs:157-174)
+__global__ void w_rng_init_kernel(Pcg32* rng, uint32_t n, StdKey key) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) rng[t] = pcg_seq_offset(t, key);
+    if (t < n) rng[t] = reservoir_stream(key, t);
 }
 
 // ---- deterministic clustering (replaces the CAS seeding / joining of :532-623) ----
@@ -1094,8 +1098,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     if (desc->branch != cfg.branch) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: branch must equal config branch");
     const uint64_t npix = (uint64_t)desc->width * desc->height;
     const uint64_t N = npix * desc->branch;
-    // the compact bucket table hashes into at most 2^32 slots (4 per path or more)
-    if (N >= (1ull << 30)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: too many paths (< 2^30)");
+    // the compact bucket table holds 4 slots per path or more, a power of two counted in 32-bit
+    // slot indices: at most 2^29 paths keep it within 2^31 slots
+    if (N > (1ull << 29)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: too many paths (at most 2^29)");
     const uint32_t L = desc->levels;
     float bmin[3], bmax[3];
     for (int i = 0; i < 3; ++i) {
@@ -1122,7 +1127,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
 
     NLPF_HIP(h, hipMemsetAsync(w.nbr, 0, sizeof(unsigned long long) * 64 * NBR_SLOTS, st));
     hipLaunchKernelGGL(w_rng_init_kernel, grid1(batch_max), dim3(256), 0, st, w.rng, (uint32_t)batch_max,
-                       splitmix64(cfg.seed));
+                       std_seed_from_u64(cfg.seed));
     hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
     if (radiance_out)
         hipLaunchKernelGGL(w_fill_kernel, grid1(3 * N * L), dim3(256), 0, st, radiance_out, 3 * N * L, 0.0f);
@@ -1221,14 +1226,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     if (n_uncl == 0) break;
                 }
             }
-            // kernel choice: records lie on surfaces, so about 9 of the 27 visited cells are occupied
-            // and a vertex meets ~9 n / (occupied buckets) candidates; a wavefront per vertex pays
-            // off from ~16 of them (with clusters, the cluster cap bounds the set)
             // The reference filters in batches of vram / (4 K) points (:1714-1744); batching is
             // observable only through the reservoir streams, which are per batch slot. When no
             // point's candidate count can exceed K there are no draws, and the whole depth runs
             // as one launch with list slots sized by that count: first bounded by 27 x the
-            // largest bucket (by the largest cluster when clustering), else counted exactly.
+            // largest bucket (27 x the largest cluster when clustering: colliding cells walk one
+            // bucket, and so its members, more than once), else counted exactly.
             if (cfg.cluster) {
                 NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_LARGEST, 0, NBR_SLOTS * sizeof(uint32_t), st));
                 hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, w.pts, n, w.cluster_id,
@@ -1239,7 +1242,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(hc), hipMemcpyDeviceToHost, st));
             NLPF_HIP(h, hipStreamSynchronize(st));
             const uint32_t occupied = slot_sum(hc);
-            const uint64_t bound = (cfg.cluster ? 1ull : 27ull) * slot_max(hc + 2 * NBR_SLOTS);
+            const uint64_t bound = 27ull * slot_max(hc + 2 * NBR_SLOTS);
             // kernel choice: records lie on surfaces, so about 9 of the 27 visited cells are
             // occupied and a vertex meets ~9 n / (occupied buckets) candidates; a wavefront per
             // vertex pays off from ~16 of them (with clusters, the cluster cap bounds the set)
@@ -1342,5 +1345,23 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
         stt.vertices += stt.points[d];
     }
     h->stats = stt;
+    return NLPF_OK;
+}
+
+extern "C" int nlpf_pcg32_fill(uint64_t seq, uint64_t offset, uint32_t n, uint32_t* host_out) {
+    if (n && !host_out) return NLPF_E_INVALID_ARG;
+    Pcg32 g = pcg_seq_offset(seq, offset);
+    for (uint32_t i = 0; i < n; ++i) host_out[i] = pcg_next(g);
+    return NLPF_OK;
+}
+
+extern "C" int nlpf_reservoir_streams(uint64_t seed, uint64_t first, uint64_t count, uint32_t draws,
+                                      uint32_t* host_out) {
+    if (count && draws && !host_out) return NLPF_E_INVALID_ARG;
+    const StdKey key = std_seed_from_u64(seed);
+    for (uint64_t t = 0; t < count; ++t) {
+        Pcg32 g = reservoir_stream(key, first + t);
+        for (uint32_t k = 0; k < draws; ++k) host_out[t * draws + k] = pcg_next(g);
+    }
     return NLPF_OK;
 }

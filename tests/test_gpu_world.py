@@ -159,14 +159,44 @@ def test_world_cluster_mode_matches_oracle(kernel, styled):
     f2, r2, _, _ = run_world(cfg, W, H, branch, L, nodes, roots)
     assert np.array_equal(f1, f2) and np.array_equal(r1, r2)
     assert np.isfinite(f1).all()
-    # clusters cap the neighbour sets: at most reuse + 1 members (seed + K joiners)
+    # clusters cap the neighbour sets at reuse + 1 members (seed + K joiners), each visited at
+    # most once per walked cell that hashes to its bucket
     for d in range(L):
         if st["points"][d]:
-            assert st["neighbors"][d] <= st["points"][d] * 17
+            assert st["neighbors"][d] <= st["points"][d] * 17 * 27
     ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots)
     assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
     sens = sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad)
     check(f1, r1, ofilm, orad, host, L, sens=sens)
+
+
+@pytest.mark.parametrize("kernel", ["thread", "wave"])
+def test_world_clusters_with_bucket_collisions(kernel):
+    """cluster = true on a 13-bucket table: nearly every walked cell collides with another, so
+    a bucket (and the cluster members in it) is visited several times per walk and candidate
+    counts exceed the cluster size. The reuse cap sits above the largest cluster but below
+    27 times it, so only the exact candidate count may decide whether a depth can run without
+    reservoir draws."""
+    W, H, branch, L = 24, 24, 2, 3
+    K = 24
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "cluster": True, "radius_scale": [2.5],
+           "reuse": [K], "hash_table_size": 13, "world_kernel": kernel}
+    nodes, roots = styles()
+    film, rad, host, st = run_world(cfg, W, H, branch, L, nodes, roots, seed=9)
+    ofilm, orad, onbr, cid = O.world_run(host["records"], host["flags"], host["new_flags"], host["material"],
+                                         host["path_depth"], W, H, branch, L, AMIN, AMAX, cfg, nodes=nodes,
+                                         roots=roots, want_radiance=True, want_clusters=True)
+    assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    for d in range(L):
+        m = host["path_depth"] >= d
+        if m.any():
+            sizes = np.bincount(cid[d][m]).astype(np.int64)
+            assert sizes.max() <= K < 27 * sizes.max()
+            if d == 0:
+                # without duplicate visits a vertex meets at most its own cluster's members
+                assert st["neighbors"][0] > int(sizes @ sizes)
+    sens = sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad)
+    check(film, rad, ofilm, orad, host, L, sens=sens)
 
 
 def test_world_large_clusters_share_smis_sums():
