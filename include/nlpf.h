@@ -265,7 +265,12 @@ typedef struct nlpf_gen_desc {
  * the image.  Per pixel the pass's samples are added in branch order.  radiance_out (optional)
  * [L][N][3] = every vertex's filtered, stylized radiance (v.radiance). Config fields used:
  * max_depth, branch, radius_scale, reuse, vram (batching), smis, cluster,
- * no_first_hit_stylization, disable_flag_check, scene_bounds, seed, hash_table_size. */
+ * no_first_hit_stylization, disable_flag_check, scene_bounds, seed, hash_table_size.
+ * Reservoir draws (more candidates than reuse) of the t-th point of a batch come from stream t,
+ * Pcg32 new_seq_offset(t, t-th u64 of StdRng::seed_from_u64(seed)), as
+ * init_pcg32_buffer_with_seed (sampler/mod.rs:157-174) seeds the reference's rng_buf (with seed
+ * 0, :1349-1353); streams advance across depths.  The reference's tracer draws from the same
+ * buffer before filtering, so its filter starts from streams its trace has advanced. */
 #define NLPF_WORLD_PLANES 28
 typedef struct nlpf_world_desc {
     uint32_t width, height, branch, levels;
@@ -348,6 +353,15 @@ int nlpf_write_image_host(const float* rgba, uint32_t width, uint32_t height, co
 int nlpf_write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
                            const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* material_roots,
                            uint32_t n_materials, uint32_t sphere_segments);
+
+/* ---- random streams (host side, the same code the kernels run) ----
+ * nlpf_pcg32_fill: n draws of Pcg32 new_seq_offset(seq, offset).gen_u32() (sampler/mod.rs:91-117).
+ * nlpf_reservoir_streams: the first `draws` gen_u32 outputs of world-mode reservoir streams
+ * first..first+count-1 for config seed `seed` ([count][draws] into host_out).
+ * No reference entry point: the reference's sampler is internal to its kernels; these let a
+ * caller reproduce the library's draws. */
+int nlpf_pcg32_fill(uint64_t seq, uint64_t offset, uint32_t n, uint32_t* host_out);
+int nlpf_reservoir_streams(uint64_t seed, uint64_t first, uint64_t count, uint32_t draws, uint32_t* host_out);
 
 /* ---- learned inner-expectation cache (radiance-cache MLP) ----
  * 84 inputs (clamped position, sin/cos of 2^k pi p for k = 0..11 per coordinate, normal,

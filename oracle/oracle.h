@@ -60,6 +60,15 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                  const int32_t* roots, uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out,
                  uint32_t* cluster_out);
 
+/* Random streams (oracle_world.c): Pcg32 new_seq_offset(seq, offset) draws; the ChaCha block
+ * function; rand_core's seed_from_u64 key expansion; rand_chacha's ChaCha12 u64 stream; the
+ * first `draws` outputs of reservoir streams first..first+count-1 as the reference seeds them. */
+int or_pcg32_fill(uint64_t seq, uint64_t offset, uint32_t n, uint32_t* out);
+void or_chacha_block(const uint32_t in[16], int rounds, uint32_t out[16]);
+void or_std_seed_from_u64(uint64_t state, uint32_t key[8]);
+void or_chacha12_u64(const uint32_t key[8], uint64_t first, uint64_t n, uint64_t* out);
+int or_reservoir_streams(uint64_t seed, uint64_t first, uint64_t count, uint32_t draws, uint32_t* out);
+
 /* FP64 reference gradient of the screen-mode NL-PF; returns the number of parameters
  * (3 + 16 * ramp nodes) written to grad, or -1. */
 int or_screen_grad_f64(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, const double sig[3],

@@ -106,6 +106,42 @@ def screen_run(records, W, H, S, L, radius, sigmas, nodes=None, root=-1, precisi
     return (film, lv) if want_levels else film
 
 
+def pcg32(seq, offset, n):
+    """n draws of Pcg32 new_seq_offset(seq, offset)."""
+    out = np.empty(n, np.uint32)
+    load().or_pcg32_fill(C.c_uint64(seq), C.c_uint64(offset), C.c_uint32(n), _p(out))
+    return out
+
+
+def chacha_block(words, rounds):
+    inp = np.ascontiguousarray(words, np.uint32)
+    out = np.empty(16, np.uint32)
+    load().or_chacha_block(_p(inp), C.c_int(rounds), _p(out))
+    return out
+
+
+def std_seed_from_u64(state):
+    key = np.empty(8, np.uint32)
+    load().or_std_seed_from_u64(C.c_uint64(state), _p(key))
+    return key
+
+
+def chacha12_u64(key, first, n):
+    key = np.ascontiguousarray(key, np.uint32)
+    out = np.empty(n, np.uint64)
+    load().or_chacha12_u64(_p(key), C.c_uint64(first), C.c_uint64(n), _p(out))
+    return out
+
+
+def reservoir_streams(seed, first, count, draws):
+    out = np.empty((count, draws), np.uint32)
+    rc = load().or_reservoir_streams(C.c_uint64(seed), C.c_uint64(first), C.c_uint64(count), C.c_uint32(draws),
+                                     _p(out))
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return out
+
+
 def world_neighbor_counts(pos, pmin, pmax, radius, table):
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
     out = np.empty(len(pos), np.uint32)

@@ -14,10 +14,14 @@
  * with DiffuseBsdf (svm/surface/diffuse.rs:22-41) behind SurfaceClosure::evaluate
  * (svm/surface/mod.rs:1701-1744) and Pcg32 (sampler/mod.rs:79-137).
  *
- * Determinism choices shared with the library (the reference's order is scheduling-dependent):
- * grid entries of one bucket are visited in ascending path id; the reservoir RNG of the
- * thread handling the t-th point of a batch is Pcg32 new_seq_offset(t, splitmix64(seed)),
- * carried across depths.
+ * Reservoir streams as the reference seeds them (init_pcg32_buffer_with_seed,
+ * sampler/mod.rs:157-174): the t-th point of a batch draws from Pcg32 new_seq_offset(t, s_t),
+ * where s_t is the t-th u64 of rand 0.8.5's StdRng::seed_from_u64(seed) (rand_chacha 0.3.1
+ * ChaCha12 keyed by rand_core 0.6.4's PCG32 seed expansion; those crates are not vendored, so
+ * their published algorithms are restated below), carried across depths.
+ *
+ * Determinism choice shared with the library (the reference's order is scheduling-dependent):
+ * grid entries of one bucket are visited in ascending path id.
  *
  * Clustering (cluster = true) is the library's deterministic replacement of the reference's
  * racy CAS seeding / joining (:532-623, 1675-1710), restated sequentially:
@@ -55,11 +59,99 @@ static pcg_t pcg_seq(uint64_t seq, uint64_t off) {
     pcg_next(&g);
     return g;
 }
-static uint64_t splitmix(uint64_t x) {
-    x += 0x9e3779b97f4a7c15ull;
-    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-    return x ^ (x >> 31);
+
+/* ChaCha block function (Bernstein; RFC 7539 2.3 for the layout of constants, key, counter and
+ * nonce): `rounds` rounds (rand_chacha's ChaCha12 uses 12) over the 16 input words, then the
+ * input added back. */
+#define OR_ROTL(v, c) (((v) << (c)) | ((v) >> (32 - (c))))
+#define OR_QR(a, b, c, d)                                                                          \
+    do {                                                                                           \
+        a += b; d ^= a; d = OR_ROTL(d, 16);                                                        \
+        c += d; b ^= c; b = OR_ROTL(b, 12);                                                        \
+        a += b; d ^= a; d = OR_ROTL(d, 8);                                                         \
+        c += d; b ^= c; b = OR_ROTL(b, 7);                                                         \
+    } while (0)
+void or_chacha_block(const uint32_t in[16], int rounds, uint32_t out[16]) {
+    uint32_t x[16];
+    memcpy(x, in, sizeof(x));
+    for (int r = 0; r < rounds; r += 2) {
+        OR_QR(x[0], x[4], x[8], x[12]);
+        OR_QR(x[1], x[5], x[9], x[13]);
+        OR_QR(x[2], x[6], x[10], x[14]);
+        OR_QR(x[3], x[7], x[11], x[15]);
+        OR_QR(x[0], x[5], x[10], x[15]);
+        OR_QR(x[1], x[6], x[11], x[12]);
+        OR_QR(x[2], x[7], x[8], x[13]);
+        OR_QR(x[3], x[4], x[9], x[14]);
+    }
+    for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
+}
+
+/* rand_core 0.6.4 SeedableRng::seed_from_u64: the 32-byte seed is eight outputs of a PCG32
+ * (XSH-RR) that advances its state before each output, stored little-endian. */
+void or_std_seed_from_u64(uint64_t state, uint32_t key[8]) {
+    for (int i = 0; i < 8; ++i) {
+        state = state * 6364136223846793005ull + 11634580027462260723ull;
+        const uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
+        const uint32_t rot = (uint32_t)(state >> 59);
+        key[i] = (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+}
+
+/* rand_chacha 0.3.1 ChaCha12Rng from a 32-byte key (stream 0): out[t] = the t-th next_u64, i.e.
+ * keystream words 2t (low half) and 2t+1; block b holds words 16b..16b+15 at 64-bit counter b
+ * (state words 12-13), stream words 14-15 zero. */
+void or_chacha12_u64(const uint32_t key[8], uint64_t first, uint64_t n, uint64_t* out) {
+    uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+    uint32_t blk[16];
+    uint64_t cur = ~0ull;
+    memcpy(in + 4, key, 32);
+    for (uint64_t t = first; t < first + n; ++t) {
+        const uint64_t b = t >> 3;
+        if (b != cur) {
+            in[12] = (uint32_t)b;
+            in[13] = (uint32_t)(b >> 32);
+            in[14] = in[15] = 0;
+            or_chacha_block(in, 12, blk);
+            cur = b;
+        }
+        const int w = (int)(t & 7) * 2;
+        out[t - first] = (uint64_t)blk[w] | ((uint64_t)blk[w + 1] << 32);
+    }
+}
+
+/* the reference's reservoir streams, init_pcg32_buffer_with_seed(count, seed): stream t is
+ * Pcg32 new_seq_offset(t, t-th u64 of StdRng::seed_from_u64(seed)) */
+static int std_pcg_streams(uint64_t seed, uint64_t count, pcg_t* out) {
+    uint32_t key[8];
+    uint64_t* s = (uint64_t*)malloc(sizeof(uint64_t) * (count ? count : 1));
+    if (!s) return -1;
+    or_std_seed_from_u64(seed, key);
+    or_chacha12_u64(key, 0, count, s);
+    for (uint64_t t = 0; t < count; ++t) out[t] = pcg_seq(t, s[t]);
+    free(s);
+    return 0;
+}
+
+/* n draws of Pcg32 new_seq_offset(seq, offset).gen_u32() (sampler/mod.rs:91-117) */
+int or_pcg32_fill(uint64_t seq, uint64_t offset, uint32_t n, uint32_t* out) {
+    pcg_t g = pcg_seq(seq, offset);
+    for (uint32_t i = 0; i < n; ++i) out[i] = pcg_next(&g);
+    return 0;
+}
+
+/* the first gen_u32 draws of the reference's reservoir streams first..first+count-1 */
+int or_reservoir_streams(uint64_t seed, uint64_t first, uint64_t count, uint32_t draws, uint32_t* out) {
+    pcg_t* g = (pcg_t*)malloc(sizeof(pcg_t) * (first + count ? first + count : 1));
+    if (!g) return -1;
+    if (std_pcg_streams(seed, first + count, g)) {
+        free(g);
+        return -1;
+    }
+    for (uint64_t t = 0; t < count; ++t)
+        for (uint32_t k = 0; k < draws; ++k) out[t * draws + k] = pcg_next(&g[first + t]);
+    free(g);
+    return 0;
 }
 
 typedef struct {
@@ -287,8 +379,10 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
     if (batch_max > N) batch_max = N;
     pcg_t* rng = (pcg_t*)malloc(sizeof(pcg_t) * batch_max);
     if (!rng) return -1;
-    const uint64_t key = splitmix(seed);
-    for (uint64_t t = 0; t < batch_max; ++t) rng[t] = pcg_seq(t, key);
+    if (std_pcg_streams(seed, batch_max, rng)) {
+        free(rng);
+        return -1;
+    }
     float ext[3];
     for (int i = 0; i < 3; ++i) ext[i] = aabb_max[i] - aabb_min[i];
     const float avg = (ext[0] + ext[1] + ext[2]) / 3.0f;

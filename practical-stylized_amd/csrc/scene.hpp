@@ -26,6 +26,62 @@ NLPF_HD Pcg32 pcg_seq_offset(uint64_t seq, uint64_t offset) {
     pcg_next(g);
     return g;
 }
+// ChaCha block function (Bernstein; RFC 7539 2.3 layout): `rounds` rounds over 16 words, then
+// the input added back
+NLPF_HD uint32_t rotl32(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+NLPF_HD void chacha_qr(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
+    a += b; d = rotl32(d ^ a, 16);
+    c += d; b = rotl32(b ^ c, 12);
+    a += b; d = rotl32(d ^ a, 8);
+    c += d; b = rotl32(b ^ c, 7);
+}
+NLPF_HD void chacha_block(const uint32_t in[16], int rounds, uint32_t out[16]) {
+    uint32_t x[16];
+    for (int i = 0; i < 16; ++i) x[i] = in[i];
+    for (int r = 0; r < rounds; r += 2) {
+        chacha_qr(x[0], x[4], x[8], x[12]);
+        chacha_qr(x[1], x[5], x[9], x[13]);
+        chacha_qr(x[2], x[6], x[10], x[14]);
+        chacha_qr(x[3], x[7], x[11], x[15]);
+        chacha_qr(x[0], x[5], x[10], x[15]);
+        chacha_qr(x[1], x[6], x[11], x[12]);
+        chacha_qr(x[2], x[7], x[8], x[13]);
+        chacha_qr(x[3], x[4], x[9], x[14]);
+    }
+    for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
+}
+// 32-byte key of rand 0.8.5's StdRng::seed_from_u64 (rand_core 0.6.4: eight outputs of a
+// PCG32 that advances before each output, little-endian)
+struct StdKey {
+    uint32_t w[8];
+};
+NLPF_HD StdKey std_seed_from_u64(uint64_t state) {
+    StdKey k;
+    for (int i = 0; i < 8; ++i) {
+        state = state * 6364136223846793005ull + 11634580027462260723ull;
+        const uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
+        const uint32_t rot = (uint32_t)(state >> 59);
+        k.w[i] = (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    return k;
+}
+// t-th next_u64 of StdRng (rand_chacha 0.3.1 ChaCha12Rng, stream 0): keystream words 2t and
+// 2t+1 of block t/8, the block counter in state words 12-13
+NLPF_HD uint64_t std_rng_u64(const StdKey& k, uint64_t t) {
+    uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+    for (int i = 0; i < 8; ++i) in[4 + i] = k.w[i];
+    const uint64_t b = t >> 3;
+    in[12] = (uint32_t)b;
+    in[13] = (uint32_t)(b >> 32);
+    in[14] = in[15] = 0u;
+    uint32_t out[16];
+    chacha_block(in, 12, out);
+    const int w = (int)(t & 7) * 2;
+    return (uint64_t)out[w] | ((uint64_t)out[w + 1] << 32);
+}
+// reservoir stream t as the reference seeds it (init_pcg32_buffer_with_seed,
+// sampler/mod.rs:157-174): new_seq_offset(t, t-th u64 of StdRng::seed_from_u64(seed))
+NLPF_HD Pcg32 reservoir_stream(const StdKey& k, uint64_t t) { return pcg_seq_offset(t, std_rng_u64(k, t)); }
 NLPF_HD uint64_t splitmix64(uint64_t x) {
     x += 0x9e3779b97f4a7c15ull;
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;

@@ -159,6 +159,8 @@ _SIGS = [
                                         C.POINTER(C.c_uint32), C.c_void_p]),
     ("nlpf_write_world_scene", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_GenDesc), C.c_void_p,
                                          C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("nlpf_pcg32_fill", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p]),
+    ("nlpf_reservoir_streams", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p]),
 ]
 
 
@@ -193,6 +195,27 @@ def write_image_host(rgba, path):
     rc = lib.nlpf_write_image_host(rgba.ctypes.data_as(C.c_void_p), W, H, os.fsencode(path))
     if rc != NLPF_OK:
         raise NlpfError(rc, lib.nlpf_last_error(None).decode())
+
+
+def pcg32(seq, offset, n):
+    """n draws of Pcg32 new_seq_offset(seq, offset) (crates/akari_render/src/sampler/mod.rs:91-117),
+    computed on the host by the library's own generator code. Needs no GPU."""
+    out = np.empty(n, np.uint32)
+    rc = load().nlpf_pcg32_fill(seq, offset, n, out.ctypes.data_as(C.c_void_p))
+    if rc != NLPF_OK:
+        raise NlpfError(rc, "nlpf_pcg32_fill")
+    return out
+
+
+def reservoir_streams(seed, first, count, draws):
+    """[count][draws]: the first draws of world-mode reservoir streams first..first+count-1 for
+    config seed `seed`, seeded as init_pcg32_buffer_with_seed (sampler/mod.rs:157-174) seeds
+    the reference's. Host side; needs no GPU."""
+    out = np.empty((count, draws), np.uint32)
+    rc = load().nlpf_reservoir_streams(seed, first, count, draws, out.ctypes.data_as(C.c_void_p))
+    if rc != NLPF_OK:
+        raise NlpfError(rc, "nlpf_reservoir_streams")
+    return out
 
 
 def style_from_scene(scene, materials=None, capacity=1024):

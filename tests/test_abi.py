@@ -175,3 +175,14 @@ def test_python_mirror_signatures_match_header():
                 assert a in (C.c_void_p, C.c_char_p) or hasattr(a, "_type_") and a not in width, (name, i, a)
             else:
                 assert a in width and width[a][1] == c_width[p], (name, i, a, p)
+
+
+def test_library_random_streams_known_answers(oracle):
+    """The library's own generator code (host copy of the kernels' functions): the pcg32-demo
+    outputs, and reservoir streams equal to the oracle's (which the published ChaCha12 /
+    StdRng vectors pin)."""
+    from test_oracle_world import PCG_DEMO
+    assert [int(v) for v in N.pcg32(54, 42, 6)] == PCG_DEMO
+    for seed in (0, 7):
+        assert np.array_equal(N.reservoir_streams(seed, 0, 70, 3), oracle.reservoir_streams(seed, 0, 70, 3))
+        assert np.array_equal(N.reservoir_streams(seed, 1000, 9, 2), oracle.reservoir_streams(seed, 1000, 9, 2))

@@ -55,6 +55,23 @@ def test_forward_matches_oracle(h):
     assert np.median(err) < 1e-6 and (err < 1e-4).mean() > 0.999 and err.max() < 1e-2, (np.median(err), err.max())
 
 
+def test_forward_closed_form_network(h):
+    """The hand-built |x| network (every value exact in bf16): the MFMA forward must give
+    (|x|, x, 2 relu(x) + 0.5) bit for bit, reading x from any non-padding input column."""
+    from known_nets import X_VALUES, abs_net, abs_net_expected
+    rng = np.random.default_rng(5)
+    n = 256
+    x = np.resize(X_VALUES, n)
+    for col in (int(np.flatnonzero(~M.PAD)[0]), int(np.flatnonzero(~M.PAD)[-1])):
+        h.mlp_set_params(abs_net(col))
+        feats = random_feats(rng, n)
+        feats[:, col] = x
+        out = h.alloc(n * 12)
+        h.mlp_infer(None, None, None, 0, None, h.to_device(feats), n, out)
+        y = out.download(np.float32, 3 * n).reshape(n, 3).astype(np.float64)
+        assert np.array_equal(y, abs_net_expected(x)), (col, np.abs(y - abs_net_expected(x)).max())
+
+
 def test_gradient_and_adam_match_oracle(h):
     rng = np.random.default_rng(1)
     params = random_params(rng)

@@ -69,3 +69,49 @@ def test_encoder_lane_groups_share_code():
         trig = [c for c in cols if c[0] in ("sin", "cos")]
         assert len(cols) == 24 and len(trig) == 18
         assert {c[1] for c in trig} == {3 * g, 3 * g + 1, 3 * g + 2}
+
+
+def test_forward_closed_form_network(monkeypatch):
+    """The hand-built |x| network: the oracle's forward must give (|x|, x, 2 relu(x) + 0.5)
+    exactly, whatever the other input columns hold."""
+    from known_nets import X_VALUES, abs_net, abs_net_expected
+    rng = np.random.default_rng(3)
+    for col in (0, int(np.flatnonzero(~M.PAD)[-1])):
+        feats = rng.uniform(-1, 1, (len(X_VALUES), M.IN))
+        feats[:, col] = X_VALUES
+        y, _ = M.forward(abs_net(col), feats)
+        assert np.array_equal(y, abs_net_expected(X_VALUES)), (col, y)
+
+
+def test_forward_closed_form_catches_planted_mistakes(monkeypatch):
+    """Mistakes planted in the oracle (a transposed hidden weight, a hidden layer without its
+    ReLU) each change the closed-form network's output."""
+    from known_nets import X_VALUES, abs_net, abs_net_expected
+    feats = np.zeros((len(X_VALUES), M.IN))
+    feats[:, 0] = X_VALUES
+    want = abs_net_expected(X_VALUES)
+    p = abs_net(0)
+    for l in range(1, 5):
+        q = p.copy()
+        wo, _ = M.OFFS[l]
+        q[wo:wo + 64 * 64] = q[wo:wo + 64 * 64].reshape(64, 64).T.ravel()
+        assert not np.array_equal(M.forward(q, feats)[0], want), l
+    monkeypatch.setattr(M, "LAYERS", M.LAYERS - 1)  # layer 4 without its ReLU
+    assert not np.array_equal(M.forward(p, feats)[0], want)
+
+
+def test_adam_first_steps_closed_form():
+    """Step 1 of Adam moves every parameter by -lr g / (|g| + eps) (the bias-corrected moments
+    are g and g^2); step 2 from g1 = 2, g2 = -1 at lr 0.01: m2 = 0.9*0.2 - 0.1 = 0.08,
+    v2 = 0.999*0.004 + 0.001 = 0.004996, m2/(1 - 0.81) = 0.42105263157894735,
+    v2/(1 - 0.998001) = 2.4992496248124062, update -0.01*0.42105263157894735/1.5809015227...
+    = -0.0026633."""
+    g = np.array([2.0, -0.5, 1e-3, 0.0, -7.0])
+    p0 = np.array([0.1, 0.2, -0.3, 0.4, 0.0])
+    lr = 0.01
+    p1, m, v = M.adam(p0, g, np.zeros(5), np.zeros(5), 1, lr)
+    np.testing.assert_allclose(p1 - p0, -lr * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-18)
+    assert np.allclose(np.abs(p1 - p0)[g != 0], lr, rtol=1e-4)  # the sign step of Adam's first update
+    p2, _, _ = M.adam(p1[:1], np.array([-1.0]), m[:1], v[:1], 2, lr)
+    assert abs((p2[0] - p1[0]) - (-0.01 * 0.42105263157894735 / (np.sqrt(2.4992496248124062) + 1e-8))) < 1e-15
+    assert abs((p2[0] - p1[0]) - (-0.0026633)) < 1e-7

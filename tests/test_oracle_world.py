@@ -389,3 +389,51 @@ def test_result_independent_of_thread_count(oracle, tmp_path):
     subprocess.run([sys.executable, "-c", script, str(tmp_path / "rec.npy"), str(tmp_path / "rad1.npy")],
                    check=True, env=env)
     assert np.array_equal(np.load(tmp_path / "rad1.npy"), rad)
+
+
+# ---- random streams: published known answers ----
+# PCG32 (O'Neill's pcg32-demo: pcg32_srandom_r(42, 54), the seeding of new_seq_offset(54, 42))
+PCG_DEMO = [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+CHACHA_CONST = [0x61707865, 0x3320646e, 0x79622d32, 0x6b206574]
+# RFC 7539 2.3.2: ChaCha20 block of key 00..1f, counter 1, nonce 00000009 0000004a 00000000
+RFC7539_232 = [0xe4e7f110, 0x15593bd1, 0x1fdd0f50, 0xc47120a3, 0xc7f4d1c7, 0x0368c033, 0x9aaa2204, 0x4e6cd4c3,
+               0x466482d2, 0x09aa9f07, 0x05d7c214, 0xa2028bd9, 0xd19c12b5, 0xb94e16de, 0xe883d0cb, 0x4e3c50a2]
+# RFC 7539 A.1 test vector 1: all-zero key, nonce and counter
+RFC7539_A1 = bytes.fromhex("76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7"
+                           "da41597c5157488d7724e03fb8d84a376a43b8f41518a11cc387b669b2ee6586")
+
+
+def test_pcg32_known_answer(oracle):
+    assert [int(v) for v in oracle.pcg32(54, 42, 6)] == PCG_DEMO
+
+
+def test_chacha_block_known_answers(oracle):
+    key = [int.from_bytes(bytes(range(4 * i, 4 * i + 4)), "little") for i in range(8)]
+    got = oracle.chacha_block(CHACHA_CONST + key + [1, 0x09000000, 0x4a000000, 0], 20)
+    assert [int(v) for v in got] == RFC7539_232
+    assert oracle.chacha_block(CHACHA_CONST + [0] * 12, 20).astype("<u4").tobytes() == RFC7539_A1
+
+
+def test_std_rng_known_answers(oracle):
+    """rand 0.8.5 StdRng (ChaCha12) value-stability test (rngs/std.rs test_stdrng_construction):
+    from_seed([1,0,0,0, 23,0,0,0, 200,1,0,0, 210,30,0,0, 0...]) gives next_u64
+    10719222850664546238; from_rng of that generator (its next 32 bytes as the key) gives
+    14064965282130556830.  rand_core 0.6.4 seed_from_u64(0) into an 8-byte seed reads
+    5029875928683246316 (its value-breakage test)."""
+    x = oracle.chacha12_u64([1, 23, 456, 7890, 0, 0, 0, 0], 0, 5)
+    assert int(x[0]) == 10719222850664546238
+    key = [w for v in x[1:5] for w in (int(v) & 0xFFFFFFFF, int(v) >> 32)]
+    assert int(oracle.chacha12_u64(key, 0, 1)[0]) == 14064965282130556830
+    k = oracle.std_seed_from_u64(0)
+    assert int(k[0]) | (int(k[1]) << 32) == 5029875928683246316
+    # block boundaries: the t-th u64 does not depend on where the stream is entered
+    full = oracle.chacha12_u64(k, 0, 40)
+    assert np.array_equal(oracle.chacha12_u64(k, 13, 27), full[13:])
+
+
+def test_reservoir_streams_follow_reference_seeding(oracle):
+    """stream t = Pcg32 new_seq_offset(t, t-th u64 of StdRng::seed_from_u64(seed))"""
+    seeds = oracle.chacha12_u64(oracle.std_seed_from_u64(0), 0, 20)
+    got = oracle.reservoir_streams(0, 3, 17, 4)
+    for t in range(3, 20):
+        assert np.array_equal(got[t - 3], oracle.pcg32(t, int(seeds[t]), 4))
