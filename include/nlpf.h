@@ -205,6 +205,21 @@ typedef struct nlpf_screen_desc {
 int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
                     const float* anchors, void* film_rgb, void* level_out);
 
+/* nlpf_screen_run without level_out, allocating nothing: no hipMalloc / hipFree, no event, no
+ * synchronisation (film_rgb is ready once h's stream is; nlpf_sync waits for it).  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  The level outputs live in a static device array of
+ * NLPF_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the process ends;
+ * slices run in groups of as many as fit (two 3 x H x W images of float, double for FP64, per
+ * slice; one for a single level), so one slice must fit: FP32/FP16 up to 11.1 M pixels (4K),
+ * FP64 up to 5.5 M.  Larger images return NLPF_E_INVALID_ARG.  film_rgb equals
+ * nlpf_screen_run's bit for bit; it holds partial sums until the pass has finished.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (several handles or
+ * streams) race.  Timings in nlpf_get_stats stay 0. */
+#define NLPF_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int nlpf_screen_run_noalloc(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                            const float* anchors, void* film_rgb);
+
 /* Adjoint of nlpf_screen_run (FP32 records) with respect to the bandwidths and every colour-ramp
  * node's stop values, for a loss whose gradient with respect to the film is dL_dfilm
  * ([H][W][3], device).  level_out must hold the forward pass's level outputs (same records,

@@ -24,9 +24,9 @@ static int screen_variant() {
     return v;
 }
 hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
-                            float* film, hipStream_t st);
+                            float* film, hipStream_t st, uint32_t s_total = 0, int accumulate = 0, int finalize = 1);
 hipError_t screen_film_fp64(const double* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
-                            double* film, hipStream_t st);
+                            double* film, hipStream_t st, uint32_t s_total = 0, int accumulate = 0, int finalize = 1);
 hipError_t launch_gen_screen(float* out, uint32_t W, uint32_t H, uint32_t L, uint32_t s_begin, uint32_t s_count,
                              const nlpf_gen_desc& gd, hipStream_t st);
 hipError_t launch_pack_fp16(const float* in, __half* out, float* anchors, uint32_t W, uint32_t H, uint32_t nslices,
@@ -104,6 +104,28 @@ using namespace nlpf;
 static std::mutex g_create_err_mu;
 static std::string g_create_err;
 
+// Scratch of nlpf_screen_run_noalloc: every GPU the library runs on holds its own copy until
+// the process ends.  In host code `scratch` names a host-side stand-in, not device memory:
+// host code uses only the per-device address that hipGetSymbolAddress returns.
+static __device__ __align__(256) unsigned char scratch[NLPF_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_noalloc_mu;
+static std::vector<void*> g_noalloc_ptr;  // by device ordinal
+
+// address of this device's copy of `scratch` (the current device), resolved once per device
+static hipError_t noalloc_scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((size_t)device >= g_noalloc_ptr.size()) g_noalloc_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[device] = p;
+    }
+    *out = g_noalloc_ptr[device];
+    return hipSuccess;
+}
+
 namespace nlpf {
 void set_global_error(const std::string& msg) {
     std::lock_guard<std::mutex> lk(g_create_err_mu);
@@ -136,6 +158,12 @@ int nlpf_create(const char* config_json, nlpf_handle** out) {
         g_create_err = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
         delete h;
         return NLPF_E_HIP;
+    }
+    // the no-allocation entry point's static scratch on this device; a failure here only
+    // disables that entry point (it reports the missing address)
+    if (noalloc_scratch_address(h->device, &h->noalloc_scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        h->noalloc_scratch = nullptr;
     }
     *out = h;
     return NLPF_OK;
@@ -293,6 +321,96 @@ static int validate_screen_desc(nlpf_handle* h, const nlpf_screen_desc* d) {
     return NLPF_OK;
 }
 
+// one level launch of the screen pass over slices [s0, s0 + S) of desc's records, writing
+// slot `slot` of out ([S][out_levels][3][H][W]) and reading the child level from slot `cslot`
+static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                               const float* anchors, void* out, uint32_t out_levels, uint32_t s0, uint32_t S,
+                               int d) {
+    const uint64_t plane = (uint64_t)desc->width * desc->height;
+    const uint32_t L = desc->levels;
+    const uint32_t slot = out_levels == L ? (uint32_t)d : (uint32_t)(d & 1);
+    const uint32_t cslot = out_levels == L ? (uint32_t)(d + 1) : (uint32_t)((d + 1) & 1);
+    const int root = h->h_roots.empty() ? -1 : h->h_roots[0];
+    const int need_pos = root >= 0 && tree_has_type(h->h_nodes, root, NLPF_STY_HALFTONE);
+    const double log2e = 1.4426950408889634;
+    const double kp = sqrt(log2e / (2.0 * (double)h->cfg.sigma_pos * h->cfg.sigma_pos));
+    const double kn = sqrt(log2e / (2.0 * (double)h->cfg.sigma_normal * h->cfg.sigma_normal));
+    const double kl = sqrt(log2e / (2.0 * (double)h->cfg.sigma_lum * h->cfg.sigma_lum));
+    // records [s][L][15][H][W]; FP16 anchors [s][L][ceil(H/8)][ceil(W/8)][3]
+    const uint32_t abx = (desc->width + 7) / 8, aby = (desc->height + 7) / 8;
+    const char* rec = (const char*)records + (size_t)s0 * L * NLPF_SCREEN_PLANES * plane * elem_size(desc->precision);
+    hipError_t e;
+    if (desc->precision == NLPF_FP64) {
+        ScreenArgs<double> a{};
+        a.rec = rec;
+        a.anchors = nullptr;
+        a.out = (double*)out;
+        a.nodes = h->d_nodes;
+        a.root = root;
+        a.need_pos = need_pos;
+        a.W = desc->width;
+        a.H = desc->height;
+        a.S = S;
+        a.L = L;
+        a.d = (uint32_t)d;
+        a.out_levels = out_levels;
+        a.out_slot = slot;
+        a.child_slot = cslot;
+        a.has_child = d + 1 < (int)L;
+        a.plane = plane;
+        a.kp = kp;
+        a.kn = kn;
+        a.kl = kl;
+        a.rscale = 1.0;
+        e = screen_level_fp64(a, desc->radius, 0, h->stream);
+    } else {
+        ScreenArgs<float> a{};
+        a.rec = rec;
+        a.anchors = desc->precision == NLPF_FP16 ? anchors + (size_t)s0 * L * abx * aby * 3 : nullptr;
+        a.out = (float*)out;
+        a.nodes = h->d_nodes;
+        a.root = root;
+        a.need_pos = need_pos;
+        a.W = desc->width;
+        a.H = desc->height;
+        a.S = S;
+        a.L = L;
+        a.d = (uint32_t)d;
+        a.out_levels = out_levels;
+        a.out_slot = slot;
+        a.child_slot = cslot;
+        a.has_child = d + 1 < (int)L;
+        a.plane = plane;
+        a.abx = abx;
+        a.aby = aby;
+        a.kp = (float)kp;
+        a.kn = (float)kn;
+        a.kl = (float)kl;
+        a.rscale = desc->precision == NLPF_FP16 ? NLPF_FP16_RADIANCE_SCALE : 1.0f;
+        e = desc->precision == NLPF_FP16 ? screen_level_fp16(a, desc->radius, screen_variant(), h->stream)
+                                         : screen_level_fp32(a, desc->radius, screen_variant(), h->stream);
+    }
+    if (e != hipSuccess) NLPF_FAIL(h, NLPF_E_HIP, std::string("screen level launch: ") + hipGetErrorString(e));
+    return NLPF_OK;
+}
+
+// algorithmic figures of a screen pass (timings left to the caller)
+static nlpf_stats screen_pass_stats(const nlpf_screen_desc* desc) {
+    nlpf_stats st{};
+    const uint64_t plane = (uint64_t)desc->width * desc->height;
+    const uint32_t L = desc->levels, S = desc->spp;
+    const double out_b = 3.0 * (desc->precision == NLPF_FP64 ? 8 : 4);
+    st.vertices = plane * S * L;
+    const double rec_b = (double)elem_size(desc->precision) * NLPF_SCREEN_PLANES;
+    // every record read once, every level output written once and (below the top) read once
+    st.bytes_algorithmic = (uint64_t)((double)st.vertices * (rec_b + out_b) + (double)plane * S * (L - 1) * out_b);
+    // per neighbour pair: |dp|^2 8 (3 sub, mul, 2 FMA), normal term 8 (4 FMA), luminance 3
+    // (sub, FMA), exp 1, accumulate 7 (weight-sum add, 3 FMA)
+    const double K = 2.0 * desc->radius + 1.0;
+    st.flops_algorithmic = (double)st.vertices * K * K * 27.0;
+    return st;
+}
+
 int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                     void* film, void* level_out) {
     if (!h) return NLPF_E_INVALID_ARG;
@@ -313,70 +431,11 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
         if (rc) return rc;
         outbuf = h->scratch;
     }
-    const int root = h->h_roots.empty() ? -1 : h->h_roots[0];
-    const int need_pos = root >= 0 && tree_has_type(h->h_nodes, root, NLPF_STY_HALFTONE);
-    const double log2e = 1.4426950408889634;
-    const double kp = sqrt(log2e / (2.0 * (double)h->cfg.sigma_pos * h->cfg.sigma_pos));
-    const double kn = sqrt(log2e / (2.0 * (double)h->cfg.sigma_normal * h->cfg.sigma_normal));
-    const double kl = sqrt(log2e / (2.0 * (double)h->cfg.sigma_lum * h->cfg.sigma_lum));
-
     rc = record_event(h, 0);
     if (rc) return rc;
     for (int d = (int)L - 1; d >= 0; --d) {
-        const uint32_t slot = out_levels == L ? (uint32_t)d : (uint32_t)(d & 1);
-        const uint32_t cslot = out_levels == L ? (uint32_t)(d + 1) : (uint32_t)((d + 1) & 1);
-        hipError_t e;
-        if (f64) {
-            ScreenArgs<double> a{};
-            a.rec = records;
-            a.anchors = nullptr;
-            a.out = (double*)outbuf;
-            a.nodes = h->d_nodes;
-            a.root = root;
-            a.need_pos = need_pos;
-            a.W = desc->width;
-            a.H = desc->height;
-            a.S = S;
-            a.L = L;
-            a.d = (uint32_t)d;
-            a.out_levels = out_levels;
-            a.out_slot = slot;
-            a.child_slot = cslot;
-            a.has_child = d + 1 < (int)L;
-            a.plane = plane;
-            a.kp = kp;
-            a.kn = kn;
-            a.kl = kl;
-            a.rscale = 1.0;
-            e = screen_level_fp64(a, desc->radius, 0, h->stream);
-        } else {
-            ScreenArgs<float> a{};
-            a.rec = records;
-            a.anchors = desc->precision == NLPF_FP16 ? anchors : nullptr;
-            a.out = (float*)outbuf;
-            a.nodes = h->d_nodes;
-            a.root = root;
-            a.need_pos = need_pos;
-            a.W = desc->width;
-            a.H = desc->height;
-            a.S = S;
-            a.L = L;
-            a.d = (uint32_t)d;
-            a.out_levels = out_levels;
-            a.out_slot = slot;
-            a.child_slot = cslot;
-            a.has_child = d + 1 < (int)L;
-            a.plane = plane;
-            a.abx = (desc->width + 7) / 8;
-            a.aby = (desc->height + 7) / 8;
-            a.kp = (float)kp;
-            a.kn = (float)kn;
-            a.kl = (float)kl;
-            a.rscale = desc->precision == NLPF_FP16 ? NLPF_FP16_RADIANCE_SCALE : 1.0f;
-            e = desc->precision == NLPF_FP16 ? screen_level_fp16(a, desc->radius, screen_variant(), h->stream)
-                                             : screen_level_fp32(a, desc->radius, screen_variant(), h->stream);
-        }
-        if (e != hipSuccess) NLPF_FAIL(h, NLPF_E_HIP, std::string("screen level launch: ") + hipGetErrorString(e));
+        rc = screen_level_launch(h, desc, records, anchors, outbuf, out_levels, 0, S, d);
+        if (rc) return rc;
         rc = record_event(h, 1 + (L - 1 - d));
         if (rc) return rc;
     }
@@ -387,23 +446,54 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
     if (rc) return rc;
     NLPF_HIP(h, hipStreamSynchronize(h->stream));
 
-    nlpf_stats st{};
+    nlpf_stats st = screen_pass_stats(desc);
     st.total_ms = event_ms(h, 0, 1 + L);
     for (uint32_t i = 0; i < L && i < 32; ++i) {
         const uint32_t d = L - 1 - i;
         st.level_ms[d] = event_ms(h, i, i + 1);
     }
     st.film_ms = event_ms(h, L, L + 1);
-    st.vertices = plane * S * L;
-    const double rec_b = (double)elem_size(desc->precision) * NLPF_SCREEN_PLANES;
-    const double out_b = 3.0 * asz;
-    // every record read once, every level output written once and (below the top) read once
-    st.bytes_algorithmic = (uint64_t)((double)st.vertices * (rec_b + out_b) + (double)plane * S * (L - 1) * out_b);
-    // per neighbour pair: |dp|^2 8 (3 sub, mul, 2 FMA), normal term 8 (4 FMA), luminance 3
-    // (sub, FMA), exp 1, accumulate 7 (weight-sum add, 3 FMA)
-    const double K = 2.0 * desc->radius + 1.0;
-    st.flops_algorithmic = (double)st.vertices * K * K * 27.0;
     h->stats = st;
+    return NLPF_OK;
+}
+
+int nlpf_screen_run_noalloc(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                            const float* anchors, void* film) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    int rc = validate_screen_desc(h, desc);
+    if (rc) return rc;
+    if (!records || !film) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run_noalloc: NULL records or film");
+    if (desc->precision == NLPF_FP16 && !anchors)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run_noalloc: FP16 records need position anchors");
+    if (!h->noalloc_scratch)
+        NLPF_FAIL(h, NLPF_E_HIP, "nlpf_screen_run_noalloc: the static scratch of this device is not resolved");
+    const bool f64 = desc->precision == NLPF_FP64;
+    const uint64_t plane = (uint64_t)desc->width * desc->height;
+    const uint32_t L = desc->levels, S = desc->spp;
+    const uint32_t out_levels = L > 1 ? 2 : 1;
+    // two level images per slice (one for a single level); as many slices per group as fit
+    const uint64_t per_slice = (uint64_t)out_levels * 3 * plane * (f64 ? 8 : 4);
+    const uint64_t group = NLPF_NOALLOC_SCRATCH_BYTES / per_slice;
+    if (group == 0)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG,
+                  "nlpf_screen_run_noalloc: one slice needs " + std::to_string(per_slice) +
+                      " bytes of scratch, more than the static " + std::to_string(NLPF_NOALLOC_SCRATCH_BYTES) +
+                      " (use nlpf_screen_run)");
+    for (uint32_t s0 = 0; s0 < S; s0 += (uint32_t)group) {
+        const uint32_t cnt = (uint32_t)std::min<uint64_t>(group, S - s0);
+        for (int d = (int)L - 1; d >= 0; --d) {
+            rc = screen_level_launch(h, desc, records, anchors, h->noalloc_scratch, out_levels, s0, cnt, d);
+            if (rc) return rc;
+        }
+        const int acc = s0 > 0, fin = s0 + cnt == S;
+        hipError_t e =
+            f64 ? screen_film_fp64((const double*)h->noalloc_scratch, cnt, out_levels, 0, plane, (double*)film,
+                                   h->stream, S, acc, fin)
+                : screen_film_fp32((const float*)h->noalloc_scratch, cnt, out_levels, 0, plane, (float*)film,
+                                   h->stream, S, acc, fin);
+        if (e != hipSuccess) NLPF_FAIL(h, NLPF_E_HIP, std::string("screen film launch: ") + hipGetErrorString(e));
+    }
+    h->stats = screen_pass_stats(desc);
     return NLPF_OK;
 }
 

@@ -71,6 +71,8 @@ struct nlpf_handle {
     // reusable device scratch
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // this device's static scratch of nlpf_screen_run_noalloc (not owned: process lifetime)
+    void* noalloc_scratch = nullptr;
     // learned-cache and world-mode state, created on first use (owned by the handle, so
     // different handles never share state)
     nlpf_mlp_state* mlp = nullptr;

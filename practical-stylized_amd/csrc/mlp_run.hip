@@ -24,7 +24,7 @@ namespace nlpf {
 hipError_t screen_level_fp32(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st);
 hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st);
 hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
-                            float* film, hipStream_t st);
+                            float* film, hipStream_t st, uint32_t s_total = 0, int accumulate = 0, int finalize = 1);
 int mlp_infer_planar(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                      uint32_t depth, uint64_t sample0, uint64_t n, float* planar_out, uint32_t out_levels,
                      uint32_t out_slot, uint32_t out_slice0);

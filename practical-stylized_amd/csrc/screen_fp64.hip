@@ -29,11 +29,11 @@ hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, int varian
 }
 
 hipError_t screen_film_fp64(const double* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
-                          double* film, hipStream_t st) {
+                          double* film, hipStream_t st, uint32_t s_total, int accumulate, int finalize) {
     const uint32_t bs = 256;
     const uint64_t nb = (plane + bs - 1) / bs;
     hipLaunchKernelGGL(screen_film_kernel<double>, dim3((uint32_t)nb), dim3(bs), 0, st, out, S, out_levels, slot0,
-                       plane, film);
+                       plane, film, s_total ? s_total : S, accumulate, finalize);
     return hipGetLastError();
 }
 

@@ -324,20 +324,28 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     }
 }
 
-// film[y][x][c] = mean over slices of out_0 (deterministic slice order)
+// film[y][x][c] = mean over slices of out_0 (deterministic slice order).  The slices can come
+// in consecutive groups: `accumulate` continues the running sums the previous group left in
+// film, `finalize` scales them by 1 / s_total; the sums are then the ones a single launch over
+// all slices forms.
 template <typename A>
 __global__ void screen_film_kernel(const A* out, uint32_t S, uint32_t out_levels, uint32_t slot0,
-                                   uint64_t plane, A* film) {
+                                   uint64_t plane, A* film, uint32_t s_total, int accumulate, int finalize) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= plane) return;
     A acc[3] = {(A)0, (A)0, (A)0};
+    if (accumulate) {
+        acc[0] = film[3 * i];
+        acc[1] = film[3 * i + 1];
+        acc[2] = film[3 * i + 2];
+    }
     for (uint32_t s = 0; s < S; ++s) {
         const A* o = out + ((size_t)s * out_levels + slot0) * 3 * plane;
         acc[0] += o[i];
         acc[1] += o[plane + i];
         acc[2] += o[2 * plane + i];
     }
-    const A inv = (A)1 / (A)S;
+    const A inv = finalize ? (A)1 / (A)s_total : (A)1;
     film[3 * i] = acc[0] * inv;
     film[3 * i + 1] = acc[1] * inv;
     film[3 * i + 2] = acc[2] * inv;

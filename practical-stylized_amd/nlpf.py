@@ -22,6 +22,7 @@ FP32, FP16, FP64 = 0, 1, 2
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "fp64": FP64}
 SCREEN_PLANES = 15
 FP16_RADIANCE_SCALE = 64.0
+NOALLOC_SCRATCH_BYTES = 256 << 20  # NLPF_NOALLOC_SCRATCH_BYTES
 
 STY = dict(BASE=0, TOON=1, PALETTE=2, SATURATION=3, REX=4, EIGHT_BIT=5, HALFTONE=6,
            HUE_SHIFT=7, RAMP=8, ENABLE_IF=9, SET_IDX=10)
@@ -119,6 +120,8 @@ _SIGS = [
                                C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_screen_run", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
+    ("nlpf_screen_run_noalloc", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
     ("nlpf_get_stats", C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
     ("nlpf_generate_screen_records", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc),
                                                C.POINTER(_GenDesc), C.c_void_p, C.c_void_p]),
@@ -355,6 +358,9 @@ class Handle:
     def copy(self, dst, src, nbytes):
         self._check(self.lib.nlpf_copy(self.h, _ptr(dst), _ptr(src), nbytes))
 
+    def memset(self, dst, value, nbytes):
+        self._check(self.lib.nlpf_memset(self.h, _ptr(dst), value, nbytes))
+
     def sync(self):
         self._check(self.lib.nlpf_sync(self.h))
 
@@ -395,6 +401,12 @@ class Handle:
     def screen_run(self, desc, records, anchors=None, film=None, level_out=None):
         self._check(self.lib.nlpf_screen_run(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
                                              _ptr(film), _ptr(level_out)))
+
+    def screen_run_noalloc(self, desc, records, anchors, film):
+        """nlpf_screen_run without level outputs and without any device allocation; asynchronous
+        (call sync() before reading film)."""
+        self._check(self.lib.nlpf_screen_run_noalloc(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
+                                                     _ptr(film)))
 
     def screen_backward(self, desc, records, level_out, dfilm, grad):
         """Adjoint pass; grad receives [dsigma_pos, dsigma_normal, dsigma_lum, ramp stops...]."""

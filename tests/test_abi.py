@@ -186,3 +186,10 @@ def test_library_random_streams_known_answers(oracle):
     for seed in (0, 7):
         assert np.array_equal(N.reservoir_streams(seed, 0, 70, 3), oracle.reservoir_streams(seed, 0, 70, 3))
         assert np.array_equal(N.reservoir_streams(seed, 1000, 9, 2), oracle.reservoir_streams(seed, 1000, 9, 2))
+
+
+def test_mirror_constants_match_header():
+    txt = open(os.path.join(ROOT, "include", "nlpf.h")).read()
+    m = re.search(r"#define NLPF_NOALLOC_SCRATCH_BYTES \((\d+)ull << (\d+)\)", txt)
+    assert m and int(m.group(1)) << int(m.group(2)) == N.NOALLOC_SCRATCH_BYTES
+    assert re.search(r"#define NLPF_SCREEN_PLANES (\d+)", txt).group(1) == str(N.SCREEN_PLANES)
