@@ -205,6 +205,17 @@ typedef struct nlpf_screen_desc {
 int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
                     const float* anchors, void* film_rgb, void* level_out);
 
+/* One chunk of a frame too large for HBM at once: desc->spp slices that are spp slices
+ * [slice0, slice0 + desc->spp) of a total_spp-slice frame (records, anchors and level_out
+ * hold just this chunk).  Chunks are passed in slice order, slice0 = 0 first; film_rgb holds
+ * the running per-pixel sums of out_0 between calls and, after the chunk that ends at
+ * total_spp, the frame's film, equal bit for bit to one nlpf_screen_run over all total_spp
+ * slices.  No reference function: it stands for the memory-limited batch loop of
+ * PathFiltering::render (lm_path_filtering.rs:1711-1755) in screen mode. */
+int nlpf_screen_run_chunk(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                          const float* anchors, void* film_rgb, void* level_out, uint32_t slice0,
+                          uint32_t total_spp);
+
 /* nlpf_screen_run without level_out, allocating nothing: no hipMalloc / hipFree, no event, no
  * synchronisation (film_rgb is ready once h's stream is; nlpf_sync waits for it).  No
  * counterpart in the reference: it is for callers that cannot allocate device memory (code under

@@ -411,14 +411,17 @@ static nlpf_stats screen_pass_stats(const nlpf_screen_desc* desc) {
     return st;
 }
 
-int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
-                    void* film, void* level_out) {
-    if (!h) return NLPF_E_INVALID_ARG;
+// screen pass over desc->spp slices; with s_total > desc->spp the slices are spp slices
+// [slice0, slice0 + desc->spp) of an s_total-slice frame and film carries the frame's running
+// sums between calls (accumulate: continue them; finalize: divide by s_total)
+static int screen_run_impl(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                           void* film, void* level_out, uint32_t s_total, int accumulate, int finalize,
+                           const char* name) {
     int rc = validate_screen_desc(h, desc);
     if (rc) return rc;
-    if (!records || !film) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run: NULL records or film");
+    if (!records || !film) NLPF_FAIL(h, NLPF_E_INVALID_ARG, std::string(name) + ": NULL records or film");
     if (desc->precision == NLPF_FP16 && !anchors)
-        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run: FP16 records need position anchors");
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, std::string(name) + ": FP16 records need position anchors");
     const bool f64 = desc->precision == NLPF_FP64;
     const size_t asz = f64 ? 8 : 4;
     const uint64_t plane = (uint64_t)desc->width * desc->height;
@@ -439,8 +442,10 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
         rc = record_event(h, 1 + (L - 1 - d));
         if (rc) return rc;
     }
-    hipError_t e = f64 ? screen_film_fp64((const double*)outbuf, S, out_levels, 0, plane, (double*)film, h->stream)
-                       : screen_film_fp32((const float*)outbuf, S, out_levels, 0, plane, (float*)film, h->stream);
+    hipError_t e = f64 ? screen_film_fp64((const double*)outbuf, S, out_levels, 0, plane, (double*)film, h->stream,
+                                          s_total, accumulate, finalize)
+                       : screen_film_fp32((const float*)outbuf, S, out_levels, 0, plane, (float*)film, h->stream,
+                                          s_total, accumulate, finalize);
     if (e != hipSuccess) NLPF_FAIL(h, NLPF_E_HIP, std::string("screen film launch: ") + hipGetErrorString(e));
     rc = record_event(h, 1 + L);
     if (rc) return rc;
@@ -455,6 +460,23 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
     st.film_ms = event_ms(h, L, L + 1);
     h->stats = st;
     return NLPF_OK;
+}
+
+int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                    void* film, void* level_out) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    return screen_run_impl(h, desc, records, anchors, film, level_out, desc ? desc->spp : 0, 0, 1,
+                           "nlpf_screen_run");
+}
+
+int nlpf_screen_run_chunk(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
+                          void* film, void* level_out, uint32_t slice0, uint32_t total_spp) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    if (!desc) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "screen desc is NULL");
+    if (desc->spp == 0 || (uint64_t)slice0 + desc->spp > total_spp)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run_chunk: slices [slice0, slice0 + spp) must lie in total_spp");
+    return screen_run_impl(h, desc, records, anchors, film, level_out, total_spp, slice0 > 0,
+                           slice0 + desc->spp == total_spp, "nlpf_screen_run_chunk");
 }
 
 int nlpf_screen_run_noalloc(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,

@@ -120,6 +120,8 @@ _SIGS = [
                                C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_screen_run", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
+    ("nlpf_screen_run_chunk", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("nlpf_screen_run_noalloc", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
     ("nlpf_get_stats", C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
@@ -401,6 +403,12 @@ class Handle:
     def screen_run(self, desc, records, anchors=None, film=None, level_out=None):
         self._check(self.lib.nlpf_screen_run(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
                                              _ptr(film), _ptr(level_out)))
+
+    def screen_run_chunk(self, desc, records, anchors, film, slice0, total_spp, level_out=None):
+        """Slices [slice0, slice0 + desc.spp) of a total_spp-slice frame; film carries the running
+        sums between chunks and holds the frame's film after the last one."""
+        self._check(self.lib.nlpf_screen_run_chunk(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
+                                                   _ptr(film), _ptr(level_out), slice0, total_spp))
 
     def screen_run_noalloc(self, desc, records, anchors, film):
         """nlpf_screen_run without level outputs and without any device allocation; asynchronous

@@ -58,16 +58,20 @@ def test_adjoint_matches_fp64_reference_gradient(R):
         host = host_records_fp32(rec, W, H, S, L)
         ref = O.screen_grad(host, W, H, S, L, R, SIG, table, 1, 2 * (f - target))
         scale = np.abs(ref).max()
-        assert np.abs(g - ref).max() <= 2e-3 * scale, (g, ref)
-        for k in range(3):
-            assert abs(g[k] - ref[k]) <= 2e-3 * max(abs(ref[k]), 1e-3 * scale), (k, g[k], ref[k])
+        # every component on its own: 1e-4 relative, plus 1e-6 of the largest component for
+        # components that are sums with heavy cancellation (FP32 partial sums of O(1e5) terms)
+        rel = np.abs(g - ref) / np.maximum(np.abs(ref), 1e-30)
+        print(f"adjoint r={R}: per-component relative error vs FP64 reference {np.array2string(rel, precision=2)}")
+        assert (np.abs(g - ref) <= 1e-4 * np.abs(ref) + 1e-6 * scale).all(), (rel, g, ref)
+        assert ((ref == 0) == (g == 0)).all()  # untouched stops get exactly zero
 
 
-def test_adjoint_matches_fp64_gpu_finite_differences():
-    """C3 shape, scaled down: 4 levels, 15x15 window, ramp(16) + hue shift."""
+@pytest.mark.parametrize("R", [3, 7])
+def test_adjoint_matches_fp64_gpu_finite_differences(R):
+    """C3 shape, scaled down: 4 levels, 7x7 (config 3's window) and 15x15, ramp(16) + hue shift."""
     with N.Handle({"type": "pf", "mode": "screen"}) as h:
-        W, H, S, L, R = 240, 136, 2, 4, 7
-        desc, rec, table, target, film = problem(h, W, H, S, L, R, seed=77)
+        W, H, S, L = 240, 136, 2, 4
+        desc, rec, table, target, film = problem(h, W, H, S, L, R, seed=77 + R)
         g, _ = gpu_grad(h, desc, rec, film, target)
         host = host_records_fp32(rec, W, H, S, L)
         rec64 = h.to_device(host.astype(np.float64))
@@ -101,11 +105,14 @@ def test_adjoint_matches_fp64_gpu_finite_differences():
         h.set_style(table, [1])
         h.set_sigmas(*SIG)
         scale = max(abs(c[0]) for c in fd)
+        errs = np.array([min(abs(c - g[q]) for c in fd[q]) / max(abs(fd[q][0]), 1e-30) for q in range(19)])
+        print(f"adjoint r={R}: per-component relative error vs FP64 finite differences "
+              f"{np.array2string(errs, precision=2)}")
         for q in range(19):
             err = min(abs(c - g[q]) for c in fd[q])
-            # h = 1e-3 (the step the north-star check names): many vertices straddle kinks at
-            # this size, so the agreement is at the 1% level of the largest component
-            assert err <= 1e-2 * max(abs(fd[q][0]), 0.05 * scale), (q, fd[q], g[q])
+            # h = 1e-3 (the step the north-star check names): vertices straddling a kink of the
+            # ramp within the step make the differences themselves inexact at ~1e-3
+            assert err <= 3e-3 * abs(fd[q][0]) + 1e-4 * scale, (q, fd[q], g[q])
 
 
 def test_gradient_array_bounds_one_ramp():
