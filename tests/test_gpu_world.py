@@ -262,3 +262,68 @@ def test_world_pass_without_hits():
         st = h.stats()
     assert (film == 0).all() and (rad == 0).all()
     assert st["vertices"] == 0 and all(v == 0 for v in st["neighbors"][:L])
+
+
+@pytest.mark.parametrize("rs", [None, 2.0])
+def test_world_config1_full_frame(rs):
+    """Config 1 at its full size: 256x256 pixels x 4 paths, max_depth 3 (4 levels), the 4-level
+    toon at depth 1, SMIS, no clustering and a reuse cap above every neighbour count, at the
+    reference's default radius_scale [0.1, 0.05] and at 2.0; every vertex against the FP64
+    oracle.  Every vertex and film pixel is held to 1e-5 relative (measured: below 4e-7), except
+    toon band flips: at depth 1 only where the quantiser input lies within 1e-5 of a band edge,
+    at depth 0 only on the flipped paths and their neighbours (within the grid radius)."""
+    from scipy.spatial import cKDTree
+    W, H, branch, L = 256, 256, 4, 4
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "smis": True, "cluster": False,
+           "reuse": [1 << 16]}
+    if rs is not None:
+        cfg["radius_scale"] = [rs]
+    nodes = N.style_table([N.toon(0.25, depth_mask=N.depth_mask(1))])
+    off = N.style_table([N.toon(0.25, depth_mask=0)])
+    roots = [0, 0, 0, 0, 0, 0, 0, -1]
+    film, rad, host, st = run_world(cfg, W, H, branch, L, nodes, roots, seed=1)
+    ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots)
+    assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    _, prad, _ = oracle_world(host, W, H, branch, L, cfg, off, roots)
+    pd = host["path_depth"]
+    n = W * H * branch
+    rec = host["records"].reshape(L, 28, n)
+    mat = host["material"].reshape(L, n)
+    report = []
+    flipped = np.zeros(n, bool)
+    for d in (3, 2, 1, 0):
+        m = pd >= d
+        err = pixel_rel_err(rad[d], orad[d])
+        bad = m & ~(err <= 1e-5)
+        if d == 1:
+            near = toon_edge_world(prad[1]) & (mat[1] != 7)
+            assert not (bad & ~near).any(), (d, err[bad & ~near].max())
+            flipped = bad
+        elif d == 0:
+            allowed = flipped.copy()
+            if flipped.any():
+                radius = float(np.mean(np.array(AMAX) - np.array(AMIN))) * 0.01 * (0.1 if rs is None else rs)
+                tree = cKDTree(rec[0, 0:3].T[m])
+                idx = np.flatnonzero(m)
+                for j in tree.query_ball_point(rec[0, 0:3].T[flipped & m], 1.0001 * radius):
+                    allowed[idx[j]] = True
+            assert not (bad & ~allowed).any(), (d, err[bad & ~allowed].max())
+        else:
+            assert not bad.any(), (d, err[bad].max())
+        report.append((d, float(err[m].max()), float((err[m] > 1e-5).mean()), int(bad.sum())))
+    ef = pixel_rel_err(film.reshape(-1, 3), ofilm.reshape(-1, 3))
+    fb = ~(ef <= 1e-5)
+    pix_allowed = np.zeros(W * H, bool)
+    pix_allowed[np.flatnonzero(allowed) % (W * H)] = True
+    assert not (fb & ~pix_allowed).any(), ef[fb & ~pix_allowed].max()
+    print(f"config 1 world frame radius_scale {rs or 'default'}: per depth (max err, fraction > 1e-5, "
+          f"outside bound) {report}; film max {ef.max():.2e}, fraction > 1e-5 {(ef > 1e-5).mean():.2e}, "
+          f"toon flips {int(flipped.sum())}")
+    assert flipped.mean() < 1e-3
+
+
+def toon_edge_world(pre, w=0.25, tol=1e-5):
+    lum = np.clip(pre, 0, 1e5) @ np.array([0.2126, 0.7152, 0.0722])
+    enc = np.where(lum > 1, lum / w, np.where(lum <= 0.0031308, lum * 12.92,
+                                               1.055 * np.power(np.maximum(lum, 0), 1 / 2.4) - 0.055) / w)
+    return np.abs(enc - np.round(enc)) < tol * np.maximum(1, np.abs(enc))
