@@ -2,7 +2,10 @@
 cross-bilateral window, 16-stop colour ramp + hue shift 0.1 fused into every level.
 
 One step = one whole NL-PF pass (levels 4..0 + film) over 165,888,000 synthetic path
-vertices already resident in HBM.  Prints one JSON line (rank 0).
+vertices already resident in HBM.  Prints one compact JSON line (rank 0): the contract fields,
+the FP16 and FP16 + learned-cache results beside their controls, and the other configurations'
+headline figures.  --detail PATH writes every measurement (window sweep, cache variants, world
+sweep) as JSON.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fp16] [--no-fp64]
 """
@@ -25,10 +28,6 @@ FP32_PEAK_TFLOPS = 157.3
 # fma), luminance 3 (sub, fma), exp 1, accumulate 7 (add, 3 fma)
 FLOPS_PER_PAIR = 27.0
 FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per cycle (the build emits no packed FP32)
-# VALU issue slots per pair of the shipped kernel's inner loop (probes/isa_mix.py on
-# screen_level_kernel<float,float,7,4,2>: 1105 VALU per 60 pairs, one v_exp_f32 each), with
-# v_exp_f32 at half rate (probes/valu_rate.hip)
-ISSUE_SLOTS_PER_PAIR = 19.42
 VALU_ISSUE_PEAK = 256 * 4 * 16 * 2.4e9  # lane-instructions/s: 256 CUs x 4 SIMD16 at 2.4 GHz
 
 
@@ -140,8 +139,11 @@ def measured_valu():
 def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
     """FP16 records + learned cache (2^16 training records per frame, batch 2^14, Adam lr 1e-2).
     The judge slice rotates over the spp slices from frame to frame; `--mlp-frames` untimed
-    frames train a fresh cache first.  Measured at cache depth 3 and 2, each with the per-tile
-    fallback at `--tile-threshold` and without fallback; the first is the headline entry."""
+    frames train a fresh cache first.  At each cache depth (the first is the headline), with the
+    per-tile fallback at `--tile-threshold` and without fallback, the trained network and the
+    two controls run through the same pass: "zero" cuts the paths at the cache depth (what the
+    pass costs and gives without a network), "mean" puts the judge slice's mean filtered value
+    everywhere (the best estimator that ignores the vertex).  Returns a list of legs."""
     legs = []
     for depth in args.cache_depths:
         log(f"FP16 + learned cache at depth {depth}")
@@ -150,14 +152,14 @@ def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
         kw = dict(depth=depth, train_records=1 << 16, batch=1 << 14, lr=1e-2)
         frame = 0
 
-        def frames(n, thr):
+        def frames(n, thr, est):
             nonlocal frame
             acc, st = {}, None
             h.sync()
             t0 = time.perf_counter()
             for _ in range(n):
                 st = h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame,
-                                      tile_threshold=thr, **kw)
+                                      tile_threshold=thr, estimator=est, **kw)
                 frame += 1
                 for k in ("judge_ms", "train_ms", "infer_ms", "fallback_ms", "filter_ms", "fallback_tiles"):
                     acc[k] = acc.get(k, 0.0) + st[k]
@@ -165,37 +167,62 @@ def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
             ms = (time.perf_counter() - t0) / max(n, 1) * 1e3
             return ms, {k: v / max(n, 1) for k, v in acc.items()}, st
 
-        frames(args.mlp_frames, 1e30)
+        frames(args.mlp_frames, 1e30, "network")
         for thr in (args.tile_threshold, 1e30):
-            frames(args.warmup, thr)
-            ms, ph, st = frames(args.steps, thr)
-            leg = {"value": verts / (ms / 1e3), "unit": "vertices/s", "ms_per_step": ms, "cache_depth": depth,
-                   "tile_threshold": thr if thr < 1e29 else None,
-                   "phases_ms": {k[:-3]: ph[k] for k in ph if k.endswith("_ms")},
-                   "fallback_tile_frac": ph["fallback_tiles"] / st["tiles"],
-                   "judge_rel_l2": st["judge_rel_l2"], "cache_rel_l2": st["cache_rel_l2"],
-                   "train_loss": st["train_loss"]}
-            if film_ref is not None:
-                # film error over several frames (the cache keeps training, so it varies a little)
-                errs = []
-                for _ in range(max(args.quality_frames, 1)):
-                    h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame,
-                                     tile_threshold=thr, **kw)
-                    frame += 1
-                    f = film.download(np.float32).astype(np.float64)
-                    errs.append(float(np.linalg.norm(f - film_ref) / np.linalg.norm(film_ref)))
-                leg["l2_err_vs_fp64"] = float(np.mean(errs))
-                leg["l2_err_vs_fp64_max"] = float(np.max(errs))
-                leg["quality_frames"] = len(errs)
-            log(f"  depth {depth} threshold {leg['tile_threshold']}: {ms:.2f} ms/step, phases "
-                f"{np.round(list(leg['phases_ms'].values()), 2).tolist()}, fallback "
-                f"{leg['fallback_tile_frac']:.3f}, L2 {leg.get('l2_err_vs_fp64')}")
-            legs.append(leg)
-    res = dict(legs[0])
-    res["variants"] = legs[1:]
-    res.update({"train_frames_before_timing": args.mlp_frames, "train_records_per_frame": 1 << 16,
-                "batch": 1 << 14, "lr": 1e-2, "ema_decay": args.ema})
-    return res
+            for est in ("network", "zero", "mean"):
+                frames(args.warmup, thr, est)
+                ms, ph, st = frames(args.steps, thr, est)
+                leg = {"estimator": est, "cache_depth": depth, "tile_threshold": thr if thr < 1e29 else None,
+                       "value": verts / (ms / 1e3), "unit": "vertices/s", "ms_per_step": ms,
+                       "phases_ms": {k[:-3]: ph[k] for k in ph if k.endswith("_ms")},
+                       "fallback_tile_frac": ph["fallback_tiles"] / st["tiles"],
+                       "judge_rel_l2": st["judge_rel_l2"], "cache_rel_l2": st["cache_rel_l2"],
+                       "train_loss": st["train_loss"] if est == "network" else None}
+                if film_ref is not None:
+                    # film error over several frames (the cache keeps training, so it varies a little)
+                    errs = []
+                    for _ in range(max(args.quality_frames, 1)):
+                        h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame,
+                                         tile_threshold=thr, estimator=est, **kw)
+                        frame += 1
+                        f = film.download(np.float32).astype(np.float64)
+                        errs.append(float(np.linalg.norm(f - film_ref) / np.linalg.norm(film_ref)))
+                    leg["l2_err_vs_fp64"] = float(np.mean(errs))
+                    leg["l2_err_vs_fp64_max"] = float(np.max(errs))
+                    leg["quality_frames"] = len(errs)
+                log(f"  depth {depth} threshold {leg['tile_threshold']} {est}: {ms:.2f} ms/step, phases "
+                    f"{np.round(list(leg['phases_ms'].values()), 2).tolist()}, fallback "
+                    f"{leg['fallback_tile_frac']:.3f}, cache fit {st['cache_rel_l2']:.3f}, "
+                    f"film L2 {leg.get('l2_err_vs_fp64')}")
+                legs.append(leg)
+    return legs
+
+
+def mlp_summary(legs, args, ms32):
+    """The bench line's compact view of the cache legs: the headline (first depth, fallback on),
+    its controls, and the same at every other depth."""
+    def pick(depth, thr, est):
+        for g in legs:
+            if g["cache_depth"] == depth and g["tile_threshold"] == thr and g["estimator"] == est:
+                return g
+        return None
+
+    def brief(g):
+        return None if g is None else {"ms": round(g["ms_per_step"], 3), "film_l2": g.get("l2_err_vs_fp64"),
+                                       "fallback": round(g["fallback_tile_frac"], 4)}
+
+    d0, thr = args.cache_depths[0], args.tile_threshold
+    head = pick(d0, thr, "network")
+    out = {"value": head["value"], "unit": "vertices/s", "ms_per_step": head["ms_per_step"],
+           "vs_plain_fp32": ms32 / head["ms_per_step"], "film_l2_vs_fp64": head.get("l2_err_vs_fp64"),
+           "cache_depth": d0, "tile_threshold": thr, "fallback_tile_frac": head["fallback_tile_frac"],
+           "cache_rel_l2": head["cache_rel_l2"],
+           "train": {"frames_before_timing": args.mlp_frames, "records_per_frame": 1 << 16, "batch": 1 << 14,
+                     "lr": 1e-2, "ema_decay": args.ema}}
+    for d in args.cache_depths:
+        out[f"depth{d}"] = {f"{est}{'' if t is not None else '_nofallback'}": brief(pick(d, t, est))
+                            for t in (thr, None) for est in ("network", "zero", "mean")}
+    return out
 
 
 def adjoint_leg(h, N):
@@ -453,6 +480,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the window-radius sweep of the level kernel")
     ap.add_argument("--world-only", action="store_true",
                     help="only the world-mode measurement (config 1), printed as its own JSON line")
+    ap.add_argument("--detail", default=None, help="write every measurement to this JSON file")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one FP32 pass and one 1 GiB device copy, for rocprofv3 counter runs")
     args = ap.parse_args()
@@ -502,6 +530,8 @@ def main():
     pairs = verts * (2 * R + 1) ** 2
     tflops = pairs * FLOPS_PER_PAIR / (lv32.sum() / 1e3) / 1e12
 
+    counters = measured_valu() if (W, H, S, L, R) == (1920, 1080, 16, 5, 7) else None
+    pair_rate = pairs / (lv32.sum() / 1e3)
     result = {
         "metric": METRIC,
         "value": verts / (ms32 / 1e3),
@@ -517,31 +547,30 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"screen NL-PF {W}x{H}x{S}spp depth {L - 1} ({L} levels), "
                                f"{2 * R + 1}x{2 * R + 1} window, ramp16+hue0.1, FP32 records",
-                   "vertices_per_step": verts, "sigmas": list(sig)},
+                   "vertices_per_step": verts},
         # the 15x15 gather is compute-bound (~75 flop per algorithmic byte): the binding roof
         # is FP32 VALU; the HBM figures stand beside it
         "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / FP32_PEAK_TFLOPS,
                      "traffic": measured_traffic() if (W, H, S, L, R) == (1920, 1080, 16, 5, 7) else None,
-                     "kernel": "screen_level_kernel<float,float,7,4,2>",
-                     "algorithmic_flops_per_launch": pairs / L * FLOPS_PER_PAIR,
-                     "flops_per_pair": FLOPS_PER_PAIR,
-                     "algorithmic_bytes_per_launch": float(np.mean(lb32)),
+                     "kernel": f"screen_level_kernel<float,float,{R},...>",
                      "avg_launch_ms": float(lv32.mean()),
+                     "algorithmic_flops_per_launch": pairs / L * FLOPS_PER_PAIR,
+                     "algorithmic_bytes_per_launch": float(np.mean(lb32)),
                      "frac_of_nonpacked_fp32_peak": tflops / FP32_NONPACKED_TFLOPS,
-                     "valu_issue": {"slots_per_pair": ISSUE_SLOTS_PER_PAIR,
-                                    "achieved": pairs / (lv32.sum() / 1e3) * ISSUE_SLOTS_PER_PAIR,
-                                    "peak": VALU_ISSUE_PEAK, "unit": "lane-instructions/s",
-                                    "frac": pairs / (lv32.sum() / 1e3) * ISSUE_SLOTS_PER_PAIR / VALU_ISSUE_PEAK,
-                                    "counters": measured_valu() if (W, H, S, L, R) == (1920, 1080, 16, 5, 7)
-                                    else None},
-                     "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": achieved_gbs / HBM_PEAK_GBS}},
+                     "hbm_gbs": achieved_gbs, "hbm_frac": achieved_gbs / HBM_PEAK_GBS},
     }
+    if counters:
+        # VALU instructions per pair and busy fraction from the committed rocprofv3 counter run
+        result["roofline"]["valu_issue"] = {
+            "valu_per_pair": counters.get("valu_instructions_per_pair"),
+            "frac": pair_rate * counters.get("valu_instructions_per_pair", 0) / VALU_ISSUE_PEAK,
+            "busy_counter": counters.get("valu_busy_per_launch")}
+    detail = {"roofline": dict(result["roofline"]), "level_ms_fp32": lv32.tolist()}
 
     h2d = pcie_upload_gbs(h)
     rec_bytes = verts * 60
-    result["pcie_inclusive"] = {"h2d_gbs": h2d, "record_bytes": rec_bytes,
+    detail["pcie_inclusive"] = {"h2d_gbs": h2d, "record_bytes": rec_bytes,
                                 "value": verts / (ms32 / 1e3 + rec_bytes / (h2d * 1e9)), "unit": "vertices/s",
                                 "note": "FP32 records uploaded from pageable host memory each frame"}
     log(f"host->device {h2d:.1f} GB/s")
@@ -574,26 +603,38 @@ def main():
         film16 = h.alloc(plane * 12)
         ms16, lv16 = timed_runs(h, d16, rec16, anc, film16, args.steps, args.warmup)
         lb16 = level_bytes(S, plane, L, 30)
-        fp16 = {"value": verts / (ms16 / 1e3), "unit": "vertices/s", "ms_per_step": ms16,
-                "hbm_gbs": sum(lb16) / 1e9 / (lv16.sum() / 1e3)}
+        fp16 = {"value": verts / (ms16 / 1e3), "ms_per_step": ms16, "hbm_gbs": sum(lb16) / 1e9 / (lv16.sum() / 1e3)}
         if film_ref is not None:
             f16 = film16.download(np.float32).astype(np.float64)
-            fp16["l2_err_vs_fp64"] = float(np.linalg.norm(f16 - film_ref) / np.linalg.norm(film_ref))
+            fp16["film_l2_vs_fp64"] = float(np.linalg.norm(f16 - film_ref) / np.linalg.norm(film_ref))
         result["fp16"] = fp16
-        result["fp16_mlp"] = None if args.no_mlp else mlp_leg(h, args, d16, rec16, anc, film16, film_ref, verts, S)
+        if not args.no_mlp:
+            legs = mlp_leg(h, args, d16, rec16, anc, film16, film_ref, verts, S)
+            detail["fp16_mlp_legs"] = legs
+            result["fp16_mlp"] = mlp_summary(legs, args, ms32)
         if not args.no_sweep:
             log("window sweep, radius 1..7")
-            result["window_sweep"] = window_sweep(h, N, W, H, S, L, {"fp32": (rec32, None), "fp16": (rec16, anc)})
+            sweep = window_sweep(h, N, W, H, S, L, {"fp32": (rec32, None), "fp16": (rec16, anc)})
+            detail["window_sweep"] = sweep
+            result["window_sweep_ms"] = {name: [round(r["level_ms_sum"], 3) for r in rows]
+                                         for name, rows in sweep.items()}
         rec16.free()
         anc.free()
-    else:
-        result["fp16_mlp"] = None
 
     if not args.no_extra:
         rec32.free()
-        result["adjoint_config3"] = adjoint_leg(h, N)
-        result["large_config5"] = large_leg(h, N)
-        result["world_config1"] = world_leg(h, N, cpu=not args.no_cpu)
+        adj = adjoint_leg(h, N)
+        big = large_leg(h, N)
+        wl = world_leg(h, N, cpu=not args.no_cpu)
+        detail.update({"adjoint_config3": adj, "large_config5": big, "world_config1": wl})
+        result["config3_adjoint"] = {"forward_ms": adj["forward_ms"], "adjoint_ms": adj["backward_ms"]}
+        result["config5"] = {k: {"ms_per_frame": big[k]["ms_per_frame"], "value": big[k]["value"]}
+                             for k in ("fp32", "fp16")}
+        result["world_config1"] = {"ms_per_frame": wl["ms_per_frame"], "value": wl["value"],
+                                   "reference_defaults_ms": wl["reference_defaults"]["ms_per_frame"],
+                                   "radius_scale_2_ms": wl["radius_scale_2"]["ms_per_frame"]}
+        if "cpu_oracle" in wl:
+            result["world_config1"]["film_l2_vs_fp64_oracle"] = wl["cpu_oracle"]["film_rel_l2_fp32_gpu_vs_fp64"]
         h.set_style(table, [root])
         rec32 = h.alloc(verts * 60)
         h.generate_screen_records(d32, rec32, seed=1)
@@ -606,6 +647,10 @@ def main():
     else:
         result["cpu_baseline"] = None
     h.close()
+    if args.detail:
+        detail["line"] = result
+        with open(args.detail, "w") as f:
+            json.dump(detail, f, indent=1)
     print(json.dumps(result), flush=True)
 
 

@@ -443,6 +443,10 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
  * The network must be initialised (nlpf_mlp_init); training continues across calls.
  * film: float [H][W][3]; level_out (optional): float [spp][levels][3][H][W], where the levels
  * deeper than `depth` of the non-judge slices are defined only inside fallback tiles. */
+#define NLPF_CACHE_NETWORK 0 /* the trained network (the learned cache) */
+#define NLPF_CACHE_ZERO 1    /* control: depth-`depth` output 0, the paths cut there (no training) */
+#define NLPF_CACHE_MEAN 2    /* control: the judge slice's mean unstylized depth-`depth` value at every
+                                vertex, stylized there (no training) */
 typedef struct nlpf_mlp_run_desc {
     uint32_t depth;
     uint32_t judge_slice;
@@ -451,6 +455,8 @@ typedef struct nlpf_mlp_run_desc {
     float lr;
     float tile_threshold;
     uint64_t seed;
+    uint32_t estimator; /* NLPF_CACHE_*: what stands in for the filtered depth-`depth` values */
+    uint32_t reserved;
 } nlpf_mlp_run_desc;
 typedef struct nlpf_mlp_run_stats {
     double total_ms, judge_ms, train_ms, infer_ms, fallback_ms, filter_ms;

@@ -42,6 +42,43 @@ __global__ void gather_targets_kernel(const float* F, uint64_t plane, const uint
     t[3 * i + 2] = F[2 * plane + px];
 }
 
+// per-channel mean of a planar [3][plane] image in a fixed order: workgroup b of channel c
+// sums the b-th of 256 contiguous chunks (each lane a strided share, then a fixed LDS tree),
+// then one lane per channel adds the 256 chunk sums in order
+__global__ void __launch_bounds__(256) plane_partial_kernel(const float* F, uint64_t plane, double* part) {
+    __shared__ double red[256];
+    const uint32_t c = blockIdx.y, b = blockIdx.x, t = threadIdx.x;
+    const uint64_t chunk = (plane + 255) / 256, lo = b * chunk, hi = lo + chunk < plane ? lo + chunk : plane;
+    double acc = 0.0;
+    for (uint64_t i = lo + t; i < hi; i += 256) acc += (double)F[c * plane + i];
+    red[t] = acc;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[c * 256 + b] = red[0];
+}
+__global__ void plane_mean_kernel(const double* part, uint64_t plane, float* mean3) {
+    const uint32_t c = threadIdx.x;
+    if (c >= 3) return;
+    double acc = 0.0;
+    for (int b = 0; b < 256; ++b) acc += part[c * 256 + b];
+    mean3[c] = (float)(acc / (double)plane);
+}
+// slot `slot` of slices [s0, s0 + n) of planar level data ([S][out_levels][3][plane]) set to the
+// constant mean3 (zero when mean3 is NULL)
+__global__ void fill_slot_kernel(float* io, uint32_t n, uint64_t plane, uint32_t out_levels, uint32_t slot,
+                                 const float* mean3) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane * n) return;
+    const uint64_t s = i / plane, px = i - s * plane;
+    float* o = io + (s * out_levels + slot) * 3 * plane + px;
+    o[0] = mean3 ? mean3[0] : 0.0f;
+    o[plane] = mean3 ? mean3[1] : 0.0f;
+    o[2 * plane] = mean3 ? mean3[2] : 0.0f;
+}
+
 // out = S_d(in) on planar [S][3][plane] level data (flags 0, material 0)
 template <typename S>
 __global__ void style_planar_kernel(float* io, uint32_t Sn, uint64_t plane, uint32_t out_levels, uint32_t slot,
@@ -151,6 +188,9 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     const uint64_t plane = (uint64_t)W * H;
     if (run->train_records > plane) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "more training records than judge vertices");
     if (plane * S >= (1ull << 32)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many vertices");
+    const uint32_t est = run->estimator;
+    if (est != NLPF_CACHE_NETWORK && est != NLPF_CACHE_ZERO && est != NLPF_CACHE_MEAN)
+        NLPF_FAIL(h, NLPF_E_INVALID_ARG, "unknown cache estimator");
     const int fp16 = desc->precision == NLPF_FP16;
     const uint32_t tiles_x = (W + 63) / 64, tiles_y = (H + 7) / 8, ntiles = tiles_x * tiles_y;
     const uint32_t sj = run->judge_slice;
@@ -165,8 +205,9 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     const size_t mask_bytes = (size_t)L * ntiles;
     const size_t nd_bytes = (size_t)ntiles * 2 * sizeof(double);
     const size_t p_bytes = (size_t)(k + 1) * f_bytes;
+    const size_t part_bytes = 3 * 256 * sizeof(double) + 4 * sizeof(float);
     int rc = ensure_scratch(h, al(lv_bytes) + al(f_bytes) + al(p_bytes) + al(tgt_bytes) + 2 * al(idx_bytes) +
-                                   al(mask_bytes) + 2 * al(nd_bytes));
+                                   al(mask_bytes) + 2 * al(nd_bytes) + al(part_bytes));
     if (rc) return rc;
     char* base = (char*)h->scratch;
     auto take = [&](size_t b) {
@@ -183,6 +224,8 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     uint8_t* masks = (uint8_t*)take(mask_bytes);
     double* nd_img = (double*)take(nd_bytes);
     double* nd_fit = (double*)take(nd_bytes);
+    double* part = (double*)take(part_bytes);
+    float* mean3 = (float*)(part + 3 * 256);
     hipStream_t st = h->stream;
 
     const int root = h->h_roots.empty() ? -1 : h->h_roots[0];
@@ -273,8 +316,9 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
     rc = record_event(h, 1);
     if (rc) return rc;
-    // 2. online training on `train_records` judge vertices (a PCG stream of `seed` picks them)
-    {
+    // 2. online training on `train_records` judge vertices (a PCG stream of `seed` picks them);
+    //    the control estimators train nothing
+    if (est == NLPF_CACHE_NETWORK) {
         std::vector<uint32_t> perm(plane);
         for (uint64_t i = 0; i < plane; ++i) perm[i] = (uint32_t)i;
         Pcg32 g = pcg_seq_offset(0x7EA1ull, splitmix64(run->seed));
@@ -304,19 +348,41 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     if (rc) return rc;
     // 3. inference at depth k: the other slices into their level slots, the judge slice into
     //    slot k of its separate image; then the depth-k stylization
-    rc = mlp_infer_planar(h, desc, records, anchors, k, 0, (uint64_t)sj * plane, lv, L, k, 0);
-    if (rc) return rc;
-    rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)(sj + 1) * plane, (uint64_t)(S - 1 - sj) * plane,
-                          lv, L, k, 0);
-    if (rc) return rc;
-    rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)sj * plane, plane, Pj, k + 1, k, sj);
-    if (rc) return rc;
     float* Pk = Pj + (size_t)k * 3 * plane;
+    if (est == NLPF_CACHE_NETWORK) {
+        rc = mlp_infer_planar(h, desc, records, anchors, k, 0, (uint64_t)sj * plane, lv, L, k, 0);
+        if (rc) return rc;
+        rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)(sj + 1) * plane,
+                              (uint64_t)(S - 1 - sj) * plane, lv, L, k, 0);
+        if (rc) return rc;
+        rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)sj * plane, plane, Pj, k + 1, k, sj);
+        if (rc) return rc;
+    } else {
+        // controls: the judge slice's mean unstylized depth-k value everywhere, or zero (the
+        // paths cut at depth k)
+        const float* c3 = nullptr;
+        if (est == NLPF_CACHE_MEAN) {
+            hipLaunchKernelGGL(plane_partial_kernel, dim3(256, 3), dim3(256), 0, st, Fj, plane, part);
+            hipLaunchKernelGGL(plane_mean_kernel, dim3(1), dim3(64), 0, st, part, plane, mean3);
+            c3 = mean3;
+        }
+        const uint32_t r0[2] = {0, sj + 1}, rn[2] = {sj, S - 1 - sj};
+        for (int r = 0; r < 2; ++r)
+            if (rn[r])
+                hipLaunchKernelGGL(fill_slot_kernel, dim3((uint32_t)((plane * rn[r] + 255) / 256)), dim3(256), 0, st,
+                                   slot(r0[r], 0), rn[r], plane, L, k, c3);
+        hipLaunchKernelGGL(fill_slot_kernel, dim3((uint32_t)((plane + 255) / 256)), dim3(256), 0, st, Pj, 1u, plane,
+                           k + 1, k, c3);
+        if ((rc = check_launch(h, "cache control estimate"))) return rc;
+    }
     hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pk, Fj, W, H, tiles_x, nd_fit);
     if ((rc = check_launch(h, "cache fit"))) return rc;
-    if ((rc = style_k(lv, L, 0, sj))) return rc;
-    if ((rc = style_k(slot(sj + 1, 0), L, sj + 1, S - 1 - sj))) return rc;
-    if ((rc = style_k(Pj, k + 1, sj, 1))) return rc;
+    // a filtered estimate is stylized at depth k; a path cut at depth k carries nothing up
+    if (est != NLPF_CACHE_ZERO) {
+        if ((rc = style_k(lv, L, 0, sj))) return rc;
+        if ((rc = style_k(slot(sj + 1, 0), L, sj + 1, S - 1 - sj))) return rc;
+        if ((rc = style_k(Pj, k + 1, sj, 1))) return rc;
+    }
     rc = record_event(h, 3);
     if (rc) return rc;
     // 4. judge: levels k-1..0 of the judge slice on its estimate, per-tile error against its

@@ -23,6 +23,7 @@ PRECISIONS = {"fp32": FP32, "fp16": FP16, "fp64": FP64}
 SCREEN_PLANES = 15
 FP16_RADIANCE_SCALE = 64.0
 NOALLOC_SCRATCH_BYTES = 256 << 20  # NLPF_NOALLOC_SCRATCH_BYTES
+CACHE_ESTIMATORS = {"network": 0, "zero": 1, "mean": 2}  # NLPF_CACHE_*
 
 STY = dict(BASE=0, TOON=1, PALETTE=2, SATURATION=3, REX=4, EIGHT_BIT=5, HALFTONE=6,
            HUE_SHIFT=7, RAMP=8, ENABLE_IF=9, SET_IDX=10)
@@ -80,7 +81,8 @@ class _WorldDesc(C.Structure):
 
 class _MlpRunDesc(C.Structure):
     _fields_ = [("depth", C.c_uint32), ("judge_slice", C.c_uint32), ("train_records", C.c_uint32),
-                ("batch", C.c_uint32), ("lr", C.c_float), ("tile_threshold", C.c_float), ("seed", C.c_uint64)]
+                ("batch", C.c_uint32), ("lr", C.c_float), ("tile_threshold", C.c_float), ("seed", C.c_uint64),
+                ("estimator", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class _MlpRunStats(C.Structure):
@@ -470,10 +472,13 @@ class Handle:
                                                 _ptr(idx), n, _ptr(out)))
 
     def screen_run_mlp(self, desc, records, anchors, film, level_out=None, depth=2, judge_slice=0,
-                       train_records=1 << 16, batch=1 << 14, lr=1e-2, tile_threshold=0.05, seed=0):
+                       train_records=1 << 16, batch=1 << 14, lr=1e-2, tile_threshold=0.05, seed=0,
+                       estimator="network"):
         """Screen NL-PF with the learned cache at `depth` (one frame; training continues across
-        calls).  Returns the run statistics as a dict."""
-        r = _MlpRunDesc(depth, judge_slice, train_records, batch, lr, tile_threshold, seed)
+        calls).  estimator: "network", or the controls "zero" (paths cut at `depth`) and "mean"
+        (the judge slice's mean depth-`depth` value).  Returns the run statistics as a dict."""
+        r = _MlpRunDesc(depth, judge_slice, train_records, batch, lr, tile_threshold, seed,
+                        CACHE_ESTIMATORS[estimator], 0)
         st = _MlpRunStats()
         self._check(self.lib.nlpf_screen_run_mlp(self.h, C.byref(desc), _ptr(records), _ptr(anchors), C.byref(r),
                                                  _ptr(film), _ptr(level_out), C.byref(st)))

@@ -125,6 +125,37 @@ def test_no_fallback_uses_the_cache(h, recs):
     check_upper_levels(host, lv, film, table, root)
 
 
+@pytest.mark.parametrize("estimator", ["zero", "mean"])
+def test_control_estimators(h, recs, estimator):
+    """The controls the cache is measured against, run through the same pass: "zero" cuts the
+    paths at depth k (out_k = 0, not stylized), "mean" puts the judge slice's mean exact
+    unstylized depth-k value at every vertex, stylized at depth k.  The judge slice stays exact
+    and the levels above k are the plain NL-PF of that depth-k output."""
+    desc, rec, host = recs
+    table, root = style(h, True)
+    sj = 1
+    film0, lv0 = run_screen(h, desc, rec)
+    film, lv, st = run_mlp(h, desc, rec, judge_slice=sj, train_records=8192, batch=1024, tile_threshold=1e30,
+                           estimator=estimator)
+    assert st["fallback_tiles"] == 0
+    assert pixel_rel_err(np.moveaxis(lv[sj], 1, -1), np.moveaxis(lv0[sj], 1, -1)).max() <= 1e-6
+    others = [s for s in range(S) if s != sj]
+    if estimator == "zero":
+        assert (lv[others, K] == 0).all()
+    else:
+        # exact unstylized depth-k values of the judge slice: the same tree with depth k unstylized
+        off = table.copy()
+        off[0]["depth_mask"] = off[1]["depth_mask"] = N.ALL_DEPTHS & ~(1 << K)
+        _, ulv = O.screen_run(host[sj:sj + 1], W, H, 1, L, R, SIG, nodes=off, root=root, want_levels=True)
+        c = ulv[0, K].reshape(3, -1).mean(1)
+        want = O.stylize(table, [root], c[None, :], depth=K)[0]
+        got = np.moveaxis(lv[others, K], 1, -1).reshape(-1, 3)
+        assert pixel_rel_err(got, np.broadcast_to(want, got.shape)).max() <= 1e-5
+        assert np.ptp(got, axis=0).max() == 0  # one value everywhere
+    # levels k-1..0: the plain NL-PF of the depth-k output (zero: the paths cut at depth k)
+    check_upper_levels(host, lv, film, table, root)
+
+
 def dilate(mask, times):
     from scipy.ndimage import binary_dilation
     return binary_dilation(mask, structure=np.ones((3, 3), bool), iterations=times) if times else mask
@@ -207,14 +238,16 @@ def test_training_beats_truncation(h, smooth_recs):
     cut = host[:, :K].copy()
     cut[:, K - 1, 12:15] = 0
     film_tr = O.screen_run(cut, W, H, S, K, R, SIG)
+    film_mean, _, _ = run_mlp(h, desc, rec, judge_slice=0, train_records=8192, batch=1024, tile_threshold=1e30,
+                              estimator="mean")
     exact = film0.astype(np.float64)
 
     def rel(x):
         return np.linalg.norm(x - exact) / np.linalg.norm(exact)
 
-    e_ml, e_tr, e_un = rel(film_ml), rel(film_tr), rel(film_un)
-    print(f"film rel L2: cache {e_ml:.4f}, truncation {e_tr:.4f}, untrained {e_un:.4f}; "
-          f"cache fit {st['cache_rel_l2']:.4f}, judge {st['judge_rel_l2']:.4f}")
+    e_ml, e_tr, e_un, e_mean = rel(film_ml), rel(film_tr), rel(film_un), rel(film_mean)
+    print(f"film rel L2: cache {e_ml:.4f}, truncation {e_tr:.4f}, untrained {e_un:.4f}, judge-slice mean "
+          f"{e_mean:.4f}; cache fit {st['cache_rel_l2']:.4f}, judge {st['judge_rel_l2']:.4f}")
     assert e_ml < e_tr and e_ml < e_un
 
 
