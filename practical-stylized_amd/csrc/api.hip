@@ -347,6 +347,7 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
         a.out = (double*)out;
         a.nodes = h->d_nodes;
         a.root = root;
+        a.chain = make_chain(h->h_nodes.data(), root, 0u);
         a.need_pos = need_pos;
         a.W = desc->width;
         a.H = desc->height;
@@ -370,6 +371,7 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
         a.out = (float*)out;
         a.nodes = h->d_nodes;
         a.root = root;
+        a.chain = make_chain(h->h_nodes.data(), root, 0u);
         a.need_pos = need_pos;
         a.W = desc->width;
         a.H = desc->height;

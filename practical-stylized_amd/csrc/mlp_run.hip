@@ -237,6 +237,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     a0.out = lv;
     a0.nodes = h->d_nodes;
     a0.root = root;
+    a0.chain = make_chain(h->h_nodes.data(), root, 0u);
     a0.need_pos = need_pos;
     a0.W = W;
     a0.H = H;
@@ -309,6 +310,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     {
         ScreenArgs<float> af = aj;
         af.root = -1;
+        af.chain = StyleChain{};
         NLPF_HIP(h, level(af, k));
     }
     NLPF_HIP(h, hipMemcpyAsync(Fj, slot(sj, k), f_bytes, hipMemcpyDeviceToDevice, st));

@@ -33,6 +33,7 @@ struct ScreenArgs {
     A kp, kn, kl;
     A rscale;  // radiance decode factor (FP16 records store radiance / NLPF_FP16_RADIANCE_SCALE)
     const uint8_t* tile_mask;  // optional [ceil(H/8)][ceil(W/64)]: 0 = skip the 64x8 tile (all slices)
+    StyleChain chain;          // root's chain for flags 0 by value (not valid: walk nodes from root)
 };
 
 __device__ __forceinline__ float ld_rec(const float* p, size_t i) { return p[i]; }
@@ -86,6 +87,7 @@ struct alignas(2 * sizeof(A)) V2 {
 template <typename S, typename A, int RAD, int NW, int R>
 __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) {
     static_assert(R <= 2 * RAD + 1, "rows per lane must not exceed the window height");
+    static_assert(64 * NW >= NLPF_CHAIN_MAX * 32, "one lane per ramp-table entry");
     constexpr int TW = 64, TH = NW * R, HT = TH + 2 * RAD, NT = 64 * NW;
     constexpr int K = 2 * RAD + 1;
     // staged columns start at an even x (pairs of records load as one aligned 2-element
@@ -97,6 +99,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     __shared__ V4<A> shA[HT * WS + 2];
     __shared__ V4<A> shB[HT * WS + 2];
     __shared__ V2<A> shC[HT * WS + 2];
+    __shared__ float ramp_tabs[NLPF_CHAIN_MAX * 32];  // the chain's ramp stops, for per-lane lookups
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
@@ -109,6 +112,11 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     const A* child = a.has_child ? a.out + ((size_t)s * a.out_levels + a.child_slot) * 3 * plane : nullptr;
     const float* anc = a.anchors ? a.anchors + (size_t)((size_t)s * a.L + a.d) * a.abx * a.aby * 3 : nullptr;
     const bool even_w = (W & 1) == 0;
+    if (threadIdx.x < NLPF_CHAIN_MAX * 32) {  // visible to the epilogue after the staging barrier
+        const int i = threadIdx.x >> 5;
+        if (a.chain.valid && i < a.chain.n && a.chain.node[i].type == NLPF_STY_RAMP)
+            ramp_tabs[threadIdx.x] = a.chain.node[i].p[threadIdx.x & 31];
+    }
 
     // ---- stage tile + halo in LDS, two horizontally adjacent records per thread ----
     // put(): guide features and level input of one loaded record pair into LDS slots
@@ -220,7 +228,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     // outputs k with 0 <= r - k <= 2 RAD.  The edge rows (r < R-1 or r > 2 RAD) run in fully
     // unrolled loops where that test is a compile-time constant; the middle rows feed every
     // output, so no branch sits between the R independent dependency chains.  Row partials
-    // are plain FP32 sums over up to two rows, folded into Kahan-compensated totals.
+    // are plain FP32 sums of about 30 terms, folded into Kahan-compensated totals.
     const int lc = threadIdx.x & 63;
     const int r0 = (threadIdx.x >> 6) * R;
     const A kn = a.kn;
@@ -281,16 +289,21 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
             rw[k] = rx[k] = ry[k] = rz[k] = (A)0;
         }
     };
+    // middle rows feed every output; they are summed plainly in groups of G rows (about 30
+    // terms, the two 15-term rows of the 15x15 window) between compensated additions
+    constexpr int G = 30 / K > 2 ? 30 / K : 2;
+    constexpr int MID = 2 * RAD + 2 - R, NG = MID / G, REM = MID % G;
 #pragma unroll
     for (int r = 0; r < R - 1; ++r) row(r, 0, r);
     if (R > 1) flush();
-    for (int r = R - 1; r + 1 <= 2 * RAD; r += 2) {
-        row(r, 0, R - 1);
-        row(r + 1, 0, R - 1);
+    for (int gi = 0; gi < NG; ++gi) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) row(R - 1 + gi * G + g, 0, R - 1);
         flush();
     }
-    if ((2 * RAD + 2 - R) % 2) {
-        row(2 * RAD, 0, R - 1);
+    if (REM) {
+#pragma unroll
+        for (int g = 0; g < REM; ++g) row(R - 1 + NG * G + g, 0, R - 1);
         flush();
     }
 #pragma unroll
@@ -298,13 +311,24 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     if (R > 1) flush();
 
     // ---- epilogue: normalise, stylize, store ----
+    // The outputs are normalised first; one rolled loop then stylizes and stores them, taking
+    // output k by selects, so the stylization code exists once and no register array is
+    // indexed dynamically.
     A* outp = a.out + ((size_t)s * a.out_levels + a.out_slot) * 3 * plane;
+    v3<A> Fo[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
+        const A inv = m_div((A)1, Wt[k]);  // Wt includes the centre's own weight, 1 for unit normals
+        Fo[k] = v3<A>{Sx[k] * inv, Sy[k] * inv, Sz[k] * inv};
+    }
+#pragma unroll 1
+    for (int k = 0; k < R; ++k) {
         const int ox = tx0 + lc, oy = ty0 + r0 + k;
+        v3<A> F = Fo[0];
+#pragma unroll
+        for (int j = 1; j < R; ++j)
+            if (k == j) F = Fo[j];
         if (ox < W && oy < H) {
-            const A inv = (A)1 / Wt[k];
-            v3<A> F{Sx[k] * inv, Sy[k] * inv, Sz[k] * inv};
             const size_t o = (size_t)oy * W + ox;
             v3<A> p{(A)0, (A)0, (A)0};
             if (a.need_pos) {
@@ -316,7 +340,10 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
                     p.z += (A)q[2];
                 }
             }
-            if (a.root >= 0) F = apply_style<A>(a.nodes, a.root, a.d, 0u, F, p);
+            if (a.chain.valid)
+                F = apply_chain<A>(a.chain, a.d, F, p, ramp_tabs);
+            else if (a.root >= 0)
+                F = apply_style<A>(a.nodes, a.root, a.d, 0u, F, p);
             outp[o] = F.x;
             outp[plane + o] = F.y;
             outp[2 * plane + o] = F.z;

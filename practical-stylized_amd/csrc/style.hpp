@@ -34,6 +34,19 @@ NLPF_HD float m_fmod(float a, float b) { return fmodf(a, b); }
 NLPF_HD double m_fmod(double a, double b) { return fmod(a, b); }
 NLPF_HD float m_sqrt(float a) { return sqrtf(a); }
 NLPF_HD double m_sqrt(double a) { return sqrt(a); }
+// Quotients inside the stylization nodes.  On the GPU an FP32 quotient is a * rcp(b) (v_rcp_f32,
+// within 1 ulp; the product within 2 ulp of the correctly rounded quotient, against an FP32
+// tolerance of 1e-5): a correctly rounded division takes about ten instructions.  Every
+// divisor here is a luminance, a colour channel difference, a ramp segment or a constant, far
+// from the denormal range.  FP64 (the error oracle) and host code divide exactly.
+template <typename T> NLPF_HD T m_div(T a, T b) { return a / b; }
+NLPF_HD float m_div(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * __builtin_amdgcn_rcpf(b);
+#else
+    return a / b;
+#endif
+}
 template <typename T> NLPF_HD T m_min(T a, T b) { return a < b ? a : b; }
 template <typename T> NLPF_HD T m_max(T a, T b) { return a > b ? a : b; }
 template <typename T> NLPF_HD T m_clamp(T v, T lo, T hi) { return m_min(m_max(v, lo), hi); }
@@ -42,8 +55,15 @@ template <typename T> NLPF_HD v3<T> clamp3(v3<T> v, T lo, T hi) {
 }
 template <typename T> NLPF_HD T hmax(v3<T> v) { return m_max(m_max(v.x, v.y), v.z); }
 template <typename T> NLPF_HD T hmin(v3<T> v) { return m_min(m_min(v.x, v.y), v.z); }
+// f64::rem_euclid / f32::rem_euclid for m > 0.  fmod(a, m) is exact, and for a in (-m, 2m) it
+// is a or a - m (exact by Sterbenz), so that range skips the general remainder sequence with
+// the same result bit for bit; the HSV conversions only ever pass values in it.
 template <typename T> NLPF_HD T rem_euclid(T a, T m) {
-    T r = m_fmod(a, m);
+    T r;
+    if (a > -m && a < m + m)
+        r = a >= m ? a - m : a;
+    else
+        r = m_fmod(a, m);
     return r < T(0) ? r + m : r;
 }
 
@@ -65,14 +85,17 @@ template <typename T> NLPF_HD v3<T> linear_to_srgb(v3<T> c) {
     return {linear_to_srgb1(c.x), linear_to_srgb1(c.y), linear_to_srgb1(c.z)};
 }
 // HSV conversions (color.rs:551-589)
+// The three hue sectors differ only in the numerator and the offset: they are selected before
+// the one division, so every lane runs the same instructions (same arithmetic as branching).
 template <typename T> NLPF_HD v3<T> rgb_to_hsv(v3<T> c) {
-    T mx = hmax(c), mn = hmin(c), diff = mx - mn, h;
+    const T mx = hmax(c), mn = hmin(c), diff = mx - mn;
+    const bool rx = mx == c.x, ry = !rx && mx == c.y;
+    const T num = rx ? c.y - c.z : (ry ? c.z - c.x : c.x - c.y);
+    const T q = m_div(num, diff);
+    T h = rx ? rem_euclid(q, T(6)) : q + (ry ? T(2) : T(4));
     if (mx == mn) h = T(0);
-    else if (mx == c.x) h = rem_euclid((c.y - c.z) / diff, T(6));
-    else if (mx == c.y) h = (c.z - c.x) / diff + T(2);
-    else h = (c.x - c.y) / diff + T(4);
-    h = h / T(6);
-    T s = mx == T(0) ? T(0) : diff / mx;
+    h = m_div(h, T(6));
+    const T s = mx == T(0) ? T(0) : m_div(diff, mx);
     return {h, s, mx};
 }
 template <typename T> NLPF_HD T hsv_channel(T n, v3<T> hsv) {
@@ -109,22 +132,32 @@ template <typename T> NLPF_HD v3<T> toon(v3<T> li, T w) {
     return hue * nl;
 }
 
-// colour ramp (no reference node): hue-preserving piecewise-linear luminance remap
+// colour ramp (no reference node): hue-preserving piecewise-linear luminance remap.
+// Segment k is the first stop with !(pos[k] <= x), n when every stop is <= x.  The stops
+// ascend (nlpf_set_style rejects other tables), so k is the number of stops <= x, found by a
+// branch-free binary search in power-of-two steps (the same k as the linear search, NaN
+// included: every comparison is false and k = 0).
+#define NLPF_RAMP_STOPS 16
 template <typename T> NLPF_HD T ramp_eval(const float* pos, const float* val, uint32_t n, T x) {
     if (n == 0) return x;
+    if (n > NLPF_RAMP_STOPS) n = NLPF_RAMP_STOPS;
     uint32_t k = 0;
-    while (k < n && T(pos[k]) <= x) ++k;
+#pragma unroll
+    for (uint32_t step = NLPF_RAMP_STOPS; step > 0; step >>= 1) {
+        const uint32_t j = k + step;
+        if (j <= n && T(pos[j - 1]) <= x) k = j;
+    }
     if (k == 0) return T(val[0]);
     if (k == n) return T(val[n - 1]);
-    T x0 = T(pos[k - 1]), x1 = T(pos[k]);
-    T t = (x - x0) / (x1 - x0);
+    const T x0 = T(pos[k - 1]), x1 = T(pos[k]);
+    const T t = m_div(x - x0, x1 - x0);
     return T(val[k - 1]) + t * (T(val[k]) - T(val[k - 1]));
 }
 template <typename T> NLPF_HD v3<T> ramp(v3<T> li, const float* pos, const float* val, uint32_t n) {
     li = clamp3(li, T(0), T(100000));
     T l = lum709(li);
     if (l <= T(0)) return li;
-    return (li / l) * ramp_eval(pos, val, n, l);
+    return li * m_div(ramp_eval(pos, val, n, l), l);
 }
 
 // sty_hue_shift: hue rotation in the reference's HSV (no clamping, so HDR-safe)
@@ -236,8 +269,11 @@ NLPF_HD bool depth_active(uint32_t mask, uint32_t depth) {
     return (mask >> (depth < 31u ? depth : 31u)) & 1u;
 }
 
+// ramp_tab (optional): a copy of the node's 32 ramp parameters to read instead of nd.p (a
+// kernel keeps it in LDS, where the stop lookups of the binary search are cheap)
 template <typename T>
-NLPF_HD v3<T> apply_node(const nlpf_style_node& nd, uint32_t depth, v3<T> li, v3<T> p) {
+NLPF_HD v3<T> apply_node(const nlpf_style_node& nd, uint32_t depth, v3<T> li, v3<T> p,
+                         const float* ramp_tab = nullptr) {
     if (!depth_active(nd.depth_mask, depth)) return li;
     switch (nd.type) {
         case NLPF_STY_TOON: return toon(li, T(nd.p[0]));
@@ -247,20 +283,20 @@ NLPF_HD v3<T> apply_node(const nlpf_style_node& nd, uint32_t depth, v3<T> li, v3
         case NLPF_STY_EIGHT_BIT: return eight_bit(li, depth, T(nd.p[0]));
         case NLPF_STY_HALFTONE: return halftone(li, depth, nd, p);
         case NLPF_STY_HUE_SHIFT: return hue_shift(li, T(nd.p[0]));
-        case NLPF_STY_RAMP: return ramp(li, nd.p, nd.p + 16, nd.count);
+        case NLPF_STY_RAMP: {
+            const float* q = ramp_tab ? ramp_tab : nd.p;
+            return ramp(li, q, q + 16, nd.count);
+        }
         default: return li;
     }
 }
 
 #define NLPF_STYLE_STACK 16
-// SurfaceClosure::apply_stylization (svm/surface/mod.rs:1824-1842) over a node tree:
-// descend from the root picking EnableIf branches by the flags' style index, then apply
-// the collected wrapper transforms innermost first.
-template <typename T>
-NLPF_HD v3<T> apply_style(const nlpf_style_node* nodes, int root, uint32_t depth, uint32_t flags,
-                          v3<T> li, v3<T> p) {
-    if (flags & NLPF_FLAG_DISABLED) return li;
-    int stack[NLPF_STYLE_STACK];
+// The wrapper nodes SurfaceClosure::apply_stylization (svm/surface/mod.rs:1824-1842) runs for
+// one material and flags: descend from the root picking EnableIf branches by the flags' style
+// index; seq[0..n) receives the wrapper transforms outermost first (they apply innermost first).
+NLPF_HD int resolve_style(const nlpf_style_node* nodes, int root, uint32_t flags, int* seq) {
+    if (flags & NLPF_FLAG_DISABLED) return 0;
     int sp = 0;
     int cur = root;
     const uint32_t sidx = style_index(flags);
@@ -271,11 +307,44 @@ NLPF_HD v3<T> apply_style(const nlpf_style_node* nodes, int root, uint32_t depth
         } else if (nd.type == NLPF_STY_SET_IDX || nd.type == NLPF_STY_BASE) {
             cur = nd.inner;
         } else {
-            stack[sp++] = cur;
+            seq[sp++] = cur;
             cur = nd.inner;
         }
     }
+    return sp;
+}
+template <typename T>
+NLPF_HD v3<T> apply_style(const nlpf_style_node* nodes, int root, uint32_t depth, uint32_t flags,
+                          v3<T> li, v3<T> p) {
+    int stack[NLPF_STYLE_STACK];
+    int sp = resolve_style(nodes, root, flags, stack);
     while (sp > 0) li = apply_node<T>(nodes[stack[--sp]], depth, li, p);
+    return li;
+}
+
+// A resolved chain held by value (kernel arguments): node[0] applies first.  Kernels whose
+// vertices all share one material and flags read every parameter as a uniform scalar load.
+#define NLPF_CHAIN_MAX 4
+struct StyleChain {
+    int32_t valid;  // 0 (zero-initialised): not resolved, walk the node table from the root
+    int32_t n;      // nodes in use
+    nlpf_style_node node[NLPF_CHAIN_MAX];
+};
+inline StyleChain make_chain(const nlpf_style_node* nodes, int root, uint32_t flags) {
+    StyleChain c{};
+    int seq[NLPF_STYLE_STACK];
+    const int n = root < 0 ? 0 : resolve_style(nodes, root, flags, seq);
+    if (n > NLPF_CHAIN_MAX) return c;
+    c.valid = 1;
+    c.n = n;
+    for (int i = 0; i < n; ++i) c.node[i] = nodes[seq[n - 1 - i]];
+    return c;
+}
+// tabs (optional): [NLPF_CHAIN_MAX][32] copies of the ramp nodes' parameters
+template <typename T>
+NLPF_HD v3<T> apply_chain(const StyleChain& c, uint32_t depth, v3<T> li, v3<T> p, const float* tabs = nullptr) {
+    for (int i = 0; i < NLPF_CHAIN_MAX; ++i)
+        if (i < c.n) li = apply_node<T>(c.node[i], depth, li, p, tabs ? tabs + 32 * i : nullptr);
     return li;
 }
 
