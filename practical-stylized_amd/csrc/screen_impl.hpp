@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     // outputs k with 0 <= r - k <= 2 RAD.  The edge rows (r < R-1 or r > 2 RAD) run in fully
     // unrolled loops where that test is a compile-time constant; the middle rows feed every
     // output, so no branch sits between the R independent dependency chains.  Row partials
-    // are plain FP32 sums of about 30 terms, folded into Kahan-compensated totals.
+    // are plain FP32 sums over up to two rows, folded into Kahan-compensated totals.
     const int lc = threadIdx.x & 63;
     const int r0 = (threadIdx.x >> 6) * R;
     const A kn = a.kn;
@@ -289,21 +289,16 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
             rw[k] = rx[k] = ry[k] = rz[k] = (A)0;
         }
     };
-    // middle rows feed every output; they are summed plainly in groups of G rows (about 30
-    // terms, the two 15-term rows of the 15x15 window) between compensated additions
-    constexpr int G = 30 / K > 2 ? 30 / K : 2;
-    constexpr int MID = 2 * RAD + 2 - R, NG = MID / G, REM = MID % G;
 #pragma unroll
     for (int r = 0; r < R - 1; ++r) row(r, 0, r);
     if (R > 1) flush();
-    for (int gi = 0; gi < NG; ++gi) {
-#pragma unroll
-        for (int g = 0; g < G; ++g) row(R - 1 + gi * G + g, 0, R - 1);
+    for (int r = R - 1; r + 1 <= 2 * RAD; r += 2) {
+        row(r, 0, R - 1);
+        row(r + 1, 0, R - 1);
         flush();
     }
-    if (REM) {
-#pragma unroll
-        for (int g = 0; g < REM; ++g) row(R - 1 + NG * G + g, 0, R - 1);
+    if ((2 * RAD + 2 - R) % 2) {
+        row(2 * RAD, 0, R - 1);
         flush();
     }
 #pragma unroll
