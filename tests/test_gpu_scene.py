@@ -74,3 +74,124 @@ def test_generator_traces_the_written_scene(tmp_path, W, H):
             else:
                 want = np.array(nodes["albedo"]["value"], np.float32)
                 assert np.abs(alb - want).max() <= 2 * np.spacing(np.float32(1.0)), m
+
+
+def _first_sphere_hit(o, d, tmax):
+    """True where the segment o + t d, 0 < t < tmax, meets one of the two spheres."""
+    hit = np.zeros(len(o), bool)
+    for c, r in CENTERS.values():
+        oc = o - np.array(c)
+        b = (oc * d).sum(1)
+        cc = (oc * oc).sum(1) - r * r
+        disc = b * b - cc
+        sq = np.sqrt(np.maximum(disc, 0))
+        t0, t1 = -b - sq, -b + sq
+        hit |= (disc > 0) & (((t0 > 1e-4) & (t0 < tmax)) | ((t1 > 1e-4) & (t1 < tmax)))
+    return hit
+
+
+def test_generator_light_transport_closed_forms():
+    """The tracer's light transport, from the stored geometry and the reference's formulas
+    (lm_path_filtering.rs:1062-1218 kernel_mk_trace, :1260-1316 sample_light, :388-426
+    handle_surface_light, light/area.rs:50-131 AreaLight):
+      direct.wi      unit vector towards a point of the light quad, not blocked by a sphere;
+      direct.pdf     dist^2 / (light area x |cos| at the light);
+      direct.irr.    the light's emission (15 on its lower face, 0 behind it) stylized by the
+                     light material's tree at depth d+1 with the vertex's new flags;
+      direct.mis     1 when a sample is stored (all direct planes 0 otherwise);
+      le, mis        15 on light hits from below, 0 elsewhere; the balance heuristic of the
+                     previous bsdf pdf against the light pdf for light hits at depth > 0, else 1;
+      wo             towards the previous vertex; bsdf_pdf |cos(ns, wi_next)| / pi of the
+                     direction to the next vertex."""
+    W, H, branch, L, seed = 64, 48, 2, 4, 11
+    nodes = N.style_table([
+        N.toon(0.4, depth_mask=N.depth_mask(2)),  # 0: light (15 -> 14.8), stylized at depth 2 only
+        N.saturation(0.5, inner=-1),       # 1
+        N.enable_if(2, inner=0, else_=-1), # 2: light root: toon only for style index 2
+        N.set_idx(2, inner=1),             # 3: sphere0 root sets style index 2
+    ])
+    roots = [1, 1, 1, 1, 1, 3, 1, 2]
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch}
+    with N.Handle(cfg) as h:
+        h.set_style(nodes, roots)
+        desc = h.world_desc(W, H, branch, L)
+        bufs = h.alloc_world(desc)
+        h.generate_world_records(desc, bufs, seed=seed)
+        host = h.download_world(desc, bufs)
+    check_light_transport(host, nodes, roots, W, H, branch, L)
+
+
+def _ray_origin(rec, d, sel, target):
+    """The tracer's ray origin off vertex d towards target: 1e-4 along the geometric normal
+    turned to the target's side (offset_ray_origin / face_forward in the reference)."""
+    p, ng = rec[d, 0:3].T[sel], rec[d, 6:9].T[sel]
+    s = np.where((ng * (target - p)).sum(1) >= 0, 1.0, -1.0)
+    return p + ng * s[:, None] * 1e-4
+
+
+def check_light_transport(host, nodes, roots, W, H, branch, L):
+    import oracle as O
+    n = W * H * branch
+    rec = host["records"].reshape(L, 28, n).astype(np.float64)
+    mat = host["material"].reshape(L, n)
+    nfl = host["new_flags"].reshape(L, n)
+    pd = host["path_depth"]
+    LY, LH, AREA = 0.499, 0.12, (2 * 0.12) ** 2
+    n_direct = n_occl = 0
+    for d in range(L):
+        on = pd >= d
+        p = rec[d, 0:3].T
+        dirp = rec[d, 19:28].T
+        # ---- next-event estimation records ----
+        cand = on & (d + 1 < L) & (mat[d] != 7)
+        stored = cand & (dirp[:, 7] == 1)
+        empty = cand & (dirp[:, 7] == 0)
+        assert (dirp[empty] == 0)[:, :8].all()
+        assert (dirp[on & ~cand][:, :8] == 0).all()
+        assert set(np.unique(dirp[cand, 7])) <= {0.0, 1.0}
+        n_direct += int(stored.sum())
+        n_occl += int(empty.sum())
+        wi = dirp[stored, 0:3]
+        ps = p[stored]
+        assert (np.abs(np.linalg.norm(wi, axis=1) - 1) < 1e-5).all()
+        t = (LY - ps[:, 1]) / wi[:, 1]
+        q = ps + t[:, None] * wi
+        assert (t > 0).all()
+        assert (np.abs(q[:, 0]) <= LH + 1e-4).all() and (np.abs(q[:, 2]) <= LH + 1e-4).all()
+        pdf = t * t / (AREA * np.abs(wi[:, 1]))
+        steep = np.abs(wi[:, 1]) > 0.05  # the plane intersection amplifies FP32 rounding at grazing angles
+        assert np.allclose(dirp[stored, 6][steep], pdf[steep], rtol=1e-4), np.abs(dirp[stored, 6] / pdf - 1).max()
+        assert not _first_sphere_hit(ps + 1e-4 * wi, wi, 0.99 * t).any()
+        e = np.where(wi[:, 1] > 0, 15.0, 0.0)
+        want = O.stylize(nodes, roots, np.stack([e, e, e], 1), depth=d + 1, flags=nfl[d][stored],
+                         material=np.full(int(stored.sum()), 7), pos=q)
+        got = dirp[stored, 3:6]
+        assert (np.abs(got - want) <= 1e-5 * max(1.0, np.abs(want).max(initial=0.0))).all()
+        # ---- emission and its MIS weight ----
+        wo = rec[d, 9:12].T
+        le, mis = rec[d, 15:18].T, rec[d, 18]
+        light = on & (mat[d] == 7)
+        assert (le[on & ~light] == 0).all() and (mis[on & ~light] == 1).all()
+        front = light & (wo[:, 1] < 0)
+        assert (le[front] == 15).all() and (le[light & ~front] == 0).all()
+        if d == 0:
+            assert (mis[light] == 1).all()
+        elif light.any():
+            # the light pdf from the previous ray's origin
+            o = _ray_origin(rec, d - 1, light, p[light])
+            dist2 = ((p[light] - o) ** 2).sum(1)
+            lpdf = dist2 / (AREA * np.maximum(np.abs(wo[light, 1]), 1e-6))
+            bp = rec[d - 1, 27][light]
+            assert np.allclose(mis[light], bp / (bp + lpdf), rtol=1e-3, atol=1e-6)
+        # ---- wo and the bsdf pdf of the continuation ----
+        nxt = on & (pd >= d + 1)
+        if nxt.any():
+            tgt = rec[d + 1, 0:3].T[nxt]
+            fwd = tgt - _ray_origin(rec, d, nxt, tgt)
+            fwd /= np.linalg.norm(fwd, axis=1)[:, None]
+            cosi = np.abs((fwd * rec[d, 3:6].T[nxt]).sum(1))
+            ok = cosi > 0.02
+            assert np.allclose(rec[d, 27][nxt][ok], cosi[ok] / np.pi, rtol=1e-3)
+            assert np.abs(rec[d + 1, 9:12].T[nxt] + fwd).max() < 1e-3  # wo at d+1
+    # both kinds occur, and the style table acts on the light (toon under style index 2)
+    assert n_direct > 1000 and n_occl > 100
