@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# NLPF_ORACLE_LIB selects another build of the same sources (the host-sanitizer build)
+LIB_PATH = os.environ.get("NLPF_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 _lib = None
 
 

@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnlpf.so")
+# NLPF_LIB selects another build of the same sources (the host-sanitizer build)
+LIB_PATH = os.environ.get("NLPF_LIB") or os.path.join(HERE, "libnlpf.so")
 
 NLPF_OK = 0
 FP32, FP16, FP64 = 0, 1, 2
