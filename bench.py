@@ -122,13 +122,13 @@ def measured_traffic():
 
 
 def measured_valu():
-    """VALU busy fraction and VALU instructions per pair of the level kernel from the committed
+    """VALU instructions and FP32 flops per neighbour pair of the level kernel from the committed
     rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "screen_level_fp32_1080p16spp_valu_counters.md")
     out = {}
     try:
         for line in open(path):
-            for key in ("valu_busy_per_launch", "valu_instructions_per_pair"):
+            for key in ("valu_instructions_per_pair", "fp32_flops_per_pair"):
                 if line.startswith(key + ":"):
                     out[key] = float(line.split(":")[1])
     except OSError:
@@ -561,11 +561,10 @@ def main():
                      "hbm_gbs": achieved_gbs, "hbm_frac": achieved_gbs / HBM_PEAK_GBS},
     }
     if counters:
-        # VALU instructions per pair and busy fraction from the committed rocprofv3 counter run
+        # VALU instructions per pair from the committed rocprofv3 counter run, at the measured pair rate
         result["roofline"]["valu_issue"] = {
             "valu_per_pair": counters.get("valu_instructions_per_pair"),
-            "frac": pair_rate * counters.get("valu_instructions_per_pair", 0) / VALU_ISSUE_PEAK,
-            "busy_counter": counters.get("valu_busy_per_launch")}
+            "frac": pair_rate * counters.get("valu_instructions_per_pair", 0) / VALU_ISSUE_PEAK}
     detail = {"roofline": dict(result["roofline"]), "level_ms_fp32": lv32.tolist()}
 
     h2d = pcie_upload_gbs(h)

@@ -79,11 +79,17 @@ __global__ void fill_slot_kernel(float* io, uint32_t n, uint64_t plane, uint32_t
     o[2 * plane] = mean3 ? mean3[2] : 0.0f;
 }
 
-// out = S_d(in) on planar [S][3][plane] level data (flags 0, material 0)
+// out = S_d(in) on planar [S][3][plane] level data (flags 0, material 0); the chain comes by
+// value, its ramp stops through LDS (as in the level kernel)
 template <typename S>
-__global__ void style_planar_kernel(float* io, uint32_t Sn, uint64_t plane, uint32_t out_levels, uint32_t slot,
-                                    const nlpf_style_node* nodes, int root, uint32_t d, const S* rec,
-                                    const float* anchors, uint32_t W, uint32_t H, uint32_t L, int need_pos) {
+__global__ void __launch_bounds__(256) style_planar_kernel(float* io, uint32_t Sn, uint64_t plane,
+                                                           uint32_t out_levels, uint32_t slot, StyleChain chain,
+                                                           uint32_t d, const S* rec, const float* anchors, uint32_t W,
+                                                           uint32_t H, uint32_t L, int need_pos) {
+    __shared__ float tabs[NLPF_CHAIN_MAX * 32];
+    if (threadIdx.x < NLPF_CHAIN_MAX * 32 && (int)(threadIdx.x >> 5) < chain.n)
+        tabs[threadIdx.x] = chain.node[threadIdx.x >> 5].p[threadIdx.x & 31];
+    __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= plane * Sn) return;
     const uint64_t s = i / plane, px = i - s * plane;
@@ -101,7 +107,7 @@ __global__ void style_planar_kernel(float* io, uint32_t Sn, uint64_t plane, uint
         }
     }
     v3<float> v{o[0], o[plane], o[2 * plane]};
-    v = apply_style<float>(nodes, root, d, 0u, v, p);
+    v = apply_chain<float>(chain, d, v, p, tabs);
     o[0] = v.x;
     o[plane] = v.y;
     o[2 * plane] = v.z;
@@ -273,17 +279,20 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     aj.out = slot(sj, 0);
     // stylize slot k of `n` slices of planar level data io ([n][out_levels][3][W*H]) whose
     // records start at slice s0
+    const StyleChain chain_k = make_chain(h->h_nodes.data(), root, 0u);
+    if (root >= 0 && !chain_k.valid)
+        NLPF_FAIL(h, NLPF_E_UNSUPPORTED, "the learned cache stylizes chains of at most 4 nodes");
     auto style_k = [&](float* io, uint32_t out_levels, uint32_t s0, uint32_t n) -> int {
         if (root < 0 || n == 0) return NLPF_OK;
         const uint64_t cnt = plane * n;
         if (fp16)
             hipLaunchKernelGGL(style_planar_kernel<__half>, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, io,
-                               n, plane, out_levels, k, h->d_nodes, root, k, (const __half*)slice_rec(s0),
-                               slice_anc(s0), W, H, L, need_pos);
+                               n, plane, out_levels, k, chain_k, k, (const __half*)slice_rec(s0), slice_anc(s0), W,
+                               H, L, need_pos);
         else
             hipLaunchKernelGGL(style_planar_kernel<float>, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, io,
-                               n, plane, out_levels, k, h->d_nodes, root, k, (const float*)slice_rec(s0), nullptr, W,
-                               H, L, need_pos);
+                               n, plane, out_levels, k, chain_k, k, (const float*)slice_rec(s0), nullptr, W, H, L,
+                               need_pos);
         return check_launch(h, "style depth k");
     };
     // one level over the slices other than the judge slice (two contiguous ranges)

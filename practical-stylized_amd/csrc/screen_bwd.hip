@@ -22,6 +22,14 @@ namespace nlpf {
 #define ADJ_NP (3 + 16 * ADJ_MAX_RAMPS)
 #define ADJ_MAX_CHAIN 8
 
+// the stylization chain of material 0 with flags 0, resolved on the host and passed by value
+// (node[0] applies first); slot[k]: ramp ordinal of node k, or -1
+struct AdjChain {
+    int32_t n;
+    int32_t slot[ADJ_MAX_CHAIN];
+    nlpf_style_node node[ADJ_MAX_CHAIN];
+};
+
 struct BwdArgs {
     const float* rec;       // FP32 record planes [S][L][15][H][W]
     const float* level_out; // forward outputs [S][L][3][H][W]
@@ -29,73 +37,51 @@ struct BwdArgs {
     float* g_out;           // dL/dout_{d+1} [S][3][H][W] (pass B)
     float* u;               // h / W        [S][3][H][W]
     float* partial;         // [nblocks][ADJ_NP] accumulated over levels
-    const nlpf_style_node* nodes;
-    const int32_t* ramp_slot;  // per node: ramp ordinal or -1
-    int root;
     uint32_t W, H, S, L, d;
     uint64_t plane;
     uint32_t tiles_x, tiles_y, nblocks;
     float kp, kn, kl;
     float sig[3];
     int has_child;
+    AdjChain chain;
 };
 
-// chain of nodes applied to F at depth d (innermost first), as apply_style walks it with flags 0
-__device__ __forceinline__ int style_chain(const nlpf_style_node* nodes, int root, int* seq) {
-    int tmp[NLPF_STYLE_STACK];
-    int n = 0, cur = root;
-    while (cur >= 0 && n < NLPF_STYLE_STACK) {
-        const nlpf_style_node& nd = nodes[cur];
-        if (nd.type == NLPF_STY_ENABLE_IF) {
-            cur = nd.idx == 0 ? nd.inner : nd.else_;
-        } else if (nd.type == NLPF_STY_SET_IDX || nd.type == NLPF_STY_BASE) {
-            cur = nd.inner;
-        } else {
-            tmp[n++] = cur;
-            cur = nd.inner;
-        }
-    }
-    const int m = n < ADJ_MAX_CHAIN ? n : ADJ_MAX_CHAIN;
-    for (int k = 0; k < m; ++k) seq[k] = tmp[n - 1 - k];
-    return m;
-}
-
-// h = J_S(F)^T g; ramp stop gradients accumulated into pg[3 + 16 * slot + k]
-__device__ __forceinline__ void style_vjp(const BwdArgs& a, v3<float> F, v3<float> g, float* pg, v3<float>& h) {
-    int seq[ADJ_MAX_CHAIN];
-    const int n = a.root >= 0 ? style_chain(a.nodes, a.root, seq) : 0;
+// h = J_S(F)^T g; ramp stop gradients accumulated into pg[3 + 16 * slot + k].  tabs: the chain
+// nodes' 32 parameters in LDS ([ADJ_MAX_CHAIN][32]) for the per-lane ramp lookups.
+__device__ __forceinline__ void style_vjp(const BwdArgs& a, v3<float> F, v3<float> g, float* pg, v3<float>& h,
+                                          const float* tabs) {
+    const int n = a.chain.n;
     v3<float> xs[ADJ_MAX_CHAIN];
     v3<float> x = F;
     const v3<float> p0{0.0f, 0.0f, 0.0f};
-#pragma unroll
     for (int k = 0; k < ADJ_MAX_CHAIN; ++k) {
         if (k < n) {
             xs[k] = x;
-            x = apply_node<float>(a.nodes[seq[k]], a.d, x, p0);
+            x = apply_node<float>(a.chain.node[k], a.d, x, p0, tabs + 32 * k);
         }
     }
-#pragma unroll
     for (int k = ADJ_MAX_CHAIN - 1; k >= 0; --k) {
         if (k < n) {
-            const nlpf_style_node& nd = a.nodes[seq[k]];
+            const nlpf_style_node& nd = a.chain.node[k];
+            const float* tab = tabs + 32 * k;
             const v3<float> xi = xs[k];
             if (nd.type == NLPF_STY_RAMP && depth_active(nd.depth_mask, a.d)) {
-                const int slot = a.ramp_slot[seq[k]];
+                const int slot = a.chain.slot[k];
                 const v3<float> c = clamp3(xi, 0.0f, 100000.0f);
                 const float l = lum709(c);
                 if (slot >= 0 && l > 0.0f) {
                     const float q = (g.x * c.x + g.y * c.y + g.z * c.z) / l;
-                    uint32_t kk = 0;
-                    while (kk < nd.count && nd.p[kk] <= l) ++kk;
+                    const uint32_t cnt = nd.count < NLPF_RAMP_STOPS ? nd.count : NLPF_RAMP_STOPS;
+                    const uint32_t kk = ramp_segment(tab, cnt, l);  // the forward's segment
                     float w0 = 0.0f, w1 = 0.0f;
                     int i0 = 0, i1 = 0;
                     if (kk == 0) {
                         w0 = 1.0f;
-                    } else if (kk == nd.count) {
-                        i0 = (int)nd.count - 1;
+                    } else if (kk == cnt) {
+                        i0 = (int)cnt - 1;
                         w0 = 1.0f;
                     } else {
-                        const float t = (l - nd.p[kk - 1]) / (nd.p[kk] - nd.p[kk - 1]);
+                        const float t = (l - tab[kk - 1]) / (tab[kk] - tab[kk - 1]);
                         i0 = (int)kk - 1;
                         i1 = (int)kk;
                         w0 = 1.0f - t;
@@ -113,7 +99,7 @@ __device__ __forceinline__ void style_vjp(const BwdArgs& a, v3<float> F, v3<floa
             }
             const v3<Dual3> xd{Dual3::var(xi.x, 0), Dual3::var(xi.y, 1), Dual3::var(xi.z, 2)};
             const v3<Dual3> pd{Dual3(0.0f), Dual3(0.0f), Dual3(0.0f)};
-            const v3<Dual3> yd = apply_node<Dual3>(nd, a.d, xd, pd);
+            const v3<Dual3> yd = apply_node<Dual3>(nd, a.d, xd, pd, tab);
             v3<float> gn;
             gn.x = g.x * yd.x.d[0] + g.y * yd.y.d[0] + g.z * yd.z.d[0];
             gn.y = g.x * yd.x.d[1] + g.y * yd.y.d[1] + g.z * yd.z.d[1];
@@ -220,6 +206,10 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
     __shared__ V4<float> shB[HT * WS];
     __shared__ V2<float> shC[HT * WS];
     __shared__ float red[NW][ADJ_NP];
+    __shared__ float tabs[ADJ_MAX_CHAIN * 32];  // chain parameters for per-lane ramp lookups
+    static_assert(64 * NW >= ADJ_MAX_CHAIN * 32, "one lane per table entry");
+    if (threadIdx.x < ADJ_MAX_CHAIN * 32 && (int)(threadIdx.x >> 5) < a.chain.n)
+        tabs[threadIdx.x] = a.chain.node[threadIdx.x >> 5].p[threadIdx.x & 31];  // read after the staging barrier
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
@@ -338,7 +328,7 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
             const v3<float> F{Sx[k] * inv, Sy[k] * inv, Sz[k] * inv};
             const v3<float> g{gin[o], gin[plane + o], gin[2 * plane + o]};
             v3<float> hh;
-            style_vjp(a, F, g, pg, hh);
+            style_vjp(a, F, g, pg, hh, tabs);
             const float Fv[3] = {F.x, F.y, F.z};
             const float hv[3] = {hh.x, hh.y, hh.z};
 #pragma unroll
@@ -559,9 +549,9 @@ extern "C" int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc
     const uint64_t plane = (uint64_t)desc->width * desc->height;
     const uint32_t S = desc->spp, L = desc->levels;
     const uint32_t nb = screen_bwd_blocks(desc->width, desc->height, S);
-    // scratch: g (2 slots), u, partials, ramp slots
+    // scratch: g (2 slots), u, partials
     const size_t gbytes = (size_t)S * 3 * plane * sizeof(float);
-    const size_t need = 3 * gbytes + (size_t)nb * ADJ_NP * sizeof(float) + 64 * sizeof(int32_t) + 256;
+    const size_t need = 3 * gbytes + (size_t)nb * ADJ_NP * sizeof(float) + 256;
     int rc = ensure_scratch(h, need);
     if (rc) return rc;
     char* base = (char*)h->scratch;
@@ -569,10 +559,7 @@ extern "C" int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc
     float* g1 = (float*)(base + gbytes);
     float* u = (float*)(base + 2 * gbytes);
     float* partial = (float*)(base + 3 * gbytes);
-    int32_t* d_slots = (int32_t*)(base + 3 * gbytes + (size_t)nb * ADJ_NP * sizeof(float));
     hipStream_t st = h->stream;
-    if (!slots.empty())
-        NLPF_HIP(h, hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     NLPF_HIP(h, hipMemsetAsync(partial, 0, (size_t)nb * ADJ_NP * sizeof(float), st));
     hipLaunchKernelGGL(bwd_seed_kernel, dim3((uint32_t)((plane * S + 255) / 256)), dim3(256), 0, st, dL_dfilm, g0, S,
                        plane);
@@ -582,9 +569,6 @@ extern "C" int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc
     a.level_out = (const float*)level_out;
     a.partial = partial;
     a.u = u;
-    a.nodes = h->d_nodes;
-    a.ramp_slot = d_slots;
-    a.root = root;
     a.W = desc->width;
     a.H = desc->height;
     a.S = S;
@@ -596,6 +580,15 @@ extern "C" int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc
     a.sig[0] = h->cfg.sigma_pos;
     a.sig[1] = h->cfg.sigma_normal;
     a.sig[2] = h->cfg.sigma_lum;
+    {
+        int seq[NLPF_STYLE_STACK];
+        const int n = root >= 0 ? resolve_style(h->h_nodes.data(), root, 0u, seq) : 0;
+        a.chain.n = n;  // n <= ADJ_MAX_CHAIN, checked above
+        for (int k = 0; k < n; ++k) {
+            a.chain.node[k] = h->h_nodes[seq[n - 1 - k]];
+            a.chain.slot[k] = slots[seq[n - 1 - k]];
+        }
+    }
     rc = record_event(h, 0);
     if (rc) return rc;
     float* gin = g0;

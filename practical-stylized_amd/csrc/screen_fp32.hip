@@ -8,9 +8,9 @@ namespace nlpf {
 // Variants: 4 waves per workgroup with 2 (variant 0, 64x8 tiles), 4 (1) or 3 (2) rows per
 // lane. Default per radius: the fastest at 1920x1080x16 spp x 5 levels on MI355X, FP32 and
 // FP16 records alike (also timed and never fastest: 8 or 2 waves with 2 rows, 2 waves with 4).
-// The 2-row tiles reach 256 VGPRs and spill to AGPRs at radius 2, 5 and 6 (one wave per
-// SIMD); those radii take 3 or 4 rows per lane.
-static constexpr int kDefaultVariant[8] = {0, 2, 2, 2, 0, 1, 2, 0};
+// The 2-row tiles run out of VGPRs at radius 5 and 6 (one wave per SIMD); those radii take 4
+// or 3 rows per lane.
+static constexpr int kDefaultVariant[8] = {0, 2, 0, 0, 0, 1, 2, 0};
 
 template <int RAD>
 static hipError_t dispatch_fp32(const ScreenArgs<float>& a, int variant, hipStream_t st) {

@@ -138,15 +138,19 @@ template <typename T> NLPF_HD v3<T> toon(v3<T> li, T w) {
 // branch-free binary search in power-of-two steps (the same k as the linear search, NaN
 // included: every comparison is false and k = 0).
 #define NLPF_RAMP_STOPS 16
-template <typename T> NLPF_HD T ramp_eval(const float* pos, const float* val, uint32_t n, T x) {
-    if (n == 0) return x;
-    if (n > NLPF_RAMP_STOPS) n = NLPF_RAMP_STOPS;
+template <typename T> NLPF_HD uint32_t ramp_segment(const float* pos, uint32_t n, T x) {
     uint32_t k = 0;
 #pragma unroll
     for (uint32_t step = NLPF_RAMP_STOPS; step > 0; step >>= 1) {
         const uint32_t j = k + step;
         if (j <= n && T(pos[j - 1]) <= x) k = j;
     }
+    return k;
+}
+template <typename T> NLPF_HD T ramp_eval(const float* pos, const float* val, uint32_t n, T x) {
+    if (n == 0) return x;
+    if (n > NLPF_RAMP_STOPS) n = NLPF_RAMP_STOPS;
+    const uint32_t k = ramp_segment(pos, n, x);
     if (k == 0) return T(val[0]);
     if (k == n) return T(val[n - 1]);
     const T x0 = T(pos[k - 1]), x1 = T(pos[k]);
