@@ -127,6 +127,21 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // or maxes the slots in the readback it makes anyway.
 constexpr uint32_t NBR_SLOTS = 256;
 
+// paths per last depth: hist[k] = #{t : path_depth[t] == k}, so a depth's point count is a
+// suffix sum (one readback for the whole pass)
+__global__ void __launch_bounds__(256) w_depth_hist_kernel(const int32_t* path_depth, uint64_t N, uint32_t* hist) {
+    __shared__ uint32_t lh[32];
+    if (threadIdx.x < 32) lh[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) {
+        const int32_t k = path_depth[t];
+        if (k >= 0 && k < 32) atomicAdd(lh + k, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 32 && lh[threadIdx.x]) atomicAdd(hist + threadIdx.x, lh[threadIdx.x]);
+}
+
 __global__ void w_flag_kernel(const int32_t* path_depth, uint64_t N, int d, uint8_t* flag) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) flag[i] = path_depth[i] >= d ? 1 : 0;
@@ -952,7 +967,8 @@ enum GridSlots : uint32_t {
     SLOT_OCCUPIED = 8 + NBR_SLOTS,
     SLOT_CAND = 8 + 2 * NBR_SLOTS,
     SLOT_LARGEST = 8 + 3 * NBR_SLOTS,
-    SLOT_END = 8 + 4 * NBR_SLOTS
+    SLOT_DEPTHS = 8 + 4 * NBR_SLOTS,  // paths per last depth 0..31
+    SLOT_END = 8 + 4 * NBR_SLOTS + 32
 };
 
 struct WorldWork {
@@ -971,6 +987,10 @@ struct WorldWork {
     uint32_t *seed_flags = nullptr, *seed_rank = nullptr, *seed_entry = nullptr;
     double* smis_sum = nullptr;
     void* cub = nullptr;
+    // pinned host memory for the readbacks (slot counters, depth counts, per-depth statistics):
+    // copies into it need no staging blit
+    uint32_t* host = nullptr;
+    unsigned long long* host_nbr = nullptr;
 };
 
 static void world_free(WorldWork& w) {
@@ -980,6 +1000,8 @@ static void world_free(WorldWork& w) {
                   w.smis_sum, w.cub};
     for (void* p : ps)
         if (p) (void)hipFree(p);
+    if (w.host) (void)hipHostFree(w.host);
+    if (w.host_nbr) (void)hipHostFree(w.host_nbr);
     w = WorldWork{};
 }
 
@@ -1042,6 +1064,9 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t nbuf
     rc |= dalloc(h, &w.seed_entry, N);
     rc |= dalloc(h, &w.smis_sum, N);
     if (rc) return rc;
+    NLPF_HIP(h, hipHostMalloc((void**)&w.host, sizeof(uint32_t) * SLOT_END, hipHostMallocDefault));
+    NLPF_HIP(h, hipHostMalloc((void**)&w.host_nbr, sizeof(unsigned long long) * 64 * NBR_SLOTS,
+                              hipHostMallocDefault));
     // temp storage for the hipcub primitives at the largest size
     size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
     hipcub::CountingInputIterator<uint32_t> it(0);
@@ -1070,15 +1095,6 @@ static uint32_t slot_max(const uint32_t* v) {
     return m;
 }
 
-static uint32_t read_u32(nlpf_handle* h, const uint32_t* d, int& rc) {
-    uint32_t v = 0;
-    if (hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess) {
-        h->err = "device readback failed";
-        rc = NLPF_E_HIP;
-    }
-    return v;
-}
 
 }  // namespace nlpf
 
@@ -1134,6 +1150,15 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     if ((rc = check_launch(h, "world setup"))) return rc;
     rc = record_event(h, 0);
     if (rc) return rc;
+    // points of every depth in one readback: #{t : path_depth[t] >= d}
+    NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_DEPTHS, 0, 32 * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(w_depth_hist_kernel, grid1(N), dim3(256), 0, st, path_depth, N, w.counters + SLOT_DEPTHS);
+    if ((rc = check_launch(h, "world depth counts"))) return rc;
+    NLPF_HIP(h, hipMemcpyAsync(w.host + SLOT_DEPTHS, w.counters + SLOT_DEPTHS, 32 * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, st));
+    NLPF_HIP(h, hipStreamSynchronize(st));
+    uint32_t points_at[33] = {};
+    for (int k = 31; k >= 0; --k) points_at[k] = points_at[k + 1] + w.host[SLOT_DEPTHS + k];
 
     nlpf_stats stt{};
     const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
@@ -1145,8 +1170,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
         hipcub::CountingInputIterator<uint32_t> it(0);
         size_t tb = w.cub_bytes;
         NLPF_HIP(h, hipcub::DeviceSelect::Flagged(w.cub, tb, it, w.flag, w.pts, w.counters, (int)N, st));
-        const uint32_t n = read_u32(h, w.counters, rc);
-        if (rc) return rc;
+        const uint32_t n = points_at[d];  // the count the selection writes to w.counters[0]
         stt.points[d] = n;
         if (n > 0) {
             Grid g;
@@ -1208,8 +1232,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                            w.pos_sorted, w.seed_rank, w.seed_entry, w.cluster_size,
                                            K, w.cluster_id, w.prop, w.counters + 1, w.counters + SLOT_OPEN);
                         if ((rc = check_launch(h, "world cluster"))) return rc;
-                        uint32_t hc[SLOT_OPEN + NBR_SLOTS];  // proposal count at 1, open-vertex slots
-                        NLPF_HIP(h, hipMemcpyAsync(hc, w.counters, sizeof(hc), hipMemcpyDeviceToHost, st));
+                        uint32_t* hc = w.host;  // proposal count at 1, open-vertex slots
+                        NLPF_HIP(h, hipMemcpyAsync(hc, w.counters, sizeof(uint32_t) * (SLOT_OPEN + NBR_SLOTS),
+                                                   hipMemcpyDeviceToHost, st));
                         NLPF_HIP(h, hipStreamSynchronize(st));
                         const uint32_t n_prop = hc[1];
                         n_uncl = slot_sum(hc + SLOT_OPEN);
@@ -1238,8 +1263,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                    w.cluster_size, w.counters + SLOT_LARGEST);
                 if ((rc = check_launch(h, "world cluster size"))) return rc;
             }
-            uint32_t hc[3 * NBR_SLOTS];  // occupied buckets, (candidate bound), largest bucket or cluster
-            NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(hc), hipMemcpyDeviceToHost, st));
+            uint32_t* hc = w.host + SLOT_OCCUPIED;  // occupied buckets, (candidate bound), largest bucket or cluster
+            NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(uint32_t) * 3 * NBR_SLOTS,
+                                       hipMemcpyDeviceToHost, st));
             NLPF_HIP(h, hipStreamSynchronize(st));
             const uint32_t occupied = slot_sum(hc);
             const uint64_t bound = 27ull * slot_max(hc + 2 * NBR_SLOTS);
@@ -1326,8 +1352,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     if ((rc = check_launch(h, "world film"))) return rc;
     rc = record_event(h, 1 + L);
     if (rc) return rc;
-    std::vector<unsigned long long> slots(64 * NBR_SLOTS);
-    NLPF_HIP(h, hipMemcpyAsync(slots.data(), w.nbr, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost,
+    const unsigned long long* slots = w.host_nbr;
+    NLPF_HIP(h, hipMemcpyAsync(w.host_nbr, w.nbr, sizeof(unsigned long long) * 64 * NBR_SLOTS, hipMemcpyDeviceToHost,
                                st));
     NLPF_HIP(h, hipStreamSynchronize(st));
     unsigned long long nb[32] = {}, np[32] = {};
