@@ -138,11 +138,11 @@ def measured_valu():
 
 def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
     """FP16 records + learned cache (2^16 training records per frame, batch 2^14, Adam lr 1e-2).
-    The judge slice rotates over the spp slices from frame to frame; `--mlp-frames` untimed
+    The audit slice rotates over the spp slices from frame to frame; `--mlp-frames` untimed
     frames train a fresh cache first.  At each cache depth (the first is the headline), with the
     per-tile fallback at `--tile-threshold` and without fallback, the trained network and the
     two controls run through the same pass: "zero" cuts the paths at the cache depth (what the
-    pass costs and gives without a network), "mean" puts the judge slice's mean filtered value
+    pass costs and gives without a network), "mean" puts the audit slice's mean filtered value
     everywhere (the best estimator that ignores the vertex).  Returns a list of legs."""
     legs = []
     for depth in args.cache_depths:
@@ -158,10 +158,10 @@ def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
             h.sync()
             t0 = time.perf_counter()
             for _ in range(n):
-                st = h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame,
+                st = h.screen_run_mlp(desc, rec, anc, film, None, audit_slice=frame % S, seed=frame,
                                       tile_threshold=thr, estimator=est, **kw)
                 frame += 1
-                for k in ("judge_ms", "train_ms", "infer_ms", "fallback_ms", "filter_ms", "fallback_tiles"):
+                for k in ("audit_ms", "train_ms", "infer_ms", "fallback_ms", "filter_ms", "fallback_tiles"):
                     acc[k] = acc.get(k, 0.0) + st[k]
             h.sync()
             ms = (time.perf_counter() - t0) / max(n, 1) * 1e3
@@ -176,13 +176,13 @@ def mlp_leg(h, args, desc, rec, anc, film, film_ref, verts, S):
                        "value": verts / (ms / 1e3), "unit": "vertices/s", "ms_per_step": ms,
                        "phases_ms": {k[:-3]: ph[k] for k in ph if k.endswith("_ms")},
                        "fallback_tile_frac": ph["fallback_tiles"] / st["tiles"],
-                       "judge_rel_l2": st["judge_rel_l2"], "cache_rel_l2": st["cache_rel_l2"],
+                       "audit_rel_l2": st["audit_rel_l2"], "cache_rel_l2": st["cache_rel_l2"],
                        "train_loss": st["train_loss"] if est == "network" else None}
                 if film_ref is not None:
                     # film error over several frames (the cache keeps training, so it varies a little)
                     errs = []
                     for _ in range(max(args.quality_frames, 1)):
-                        h.screen_run_mlp(desc, rec, anc, film, None, judge_slice=frame % S, seed=frame,
+                        h.screen_run_mlp(desc, rec, anc, film, None, audit_slice=frame % S, seed=frame,
                                          tile_threshold=thr, estimator=est, **kw)
                         frame += 1
                         f = film.download(np.float32).astype(np.float64)

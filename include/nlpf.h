@@ -198,7 +198,7 @@ typedef struct nlpf_screen_desc {
     int32_t radius;
 } nlpf_screen_desc;
 
-/* Screen-mode pass: no reference function (the north-star workload defines it); it takes the
+/* Screen-mode pass: no reference function (the screen-space filtering workload defines it); it takes the
  * place of the depth loop PathFiltering::render runs (lm_path_filtering.rs:1673-1774:
  * kernel_filter :999-1060 per depth, kernel_propagate :973-998 to the parent, kernel_splat +
  * kernel_update_film :428-506) with an image-space window instead of the hash grid. */
@@ -430,26 +430,26 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
                        double* out_rgb);
 
 /* Screen-mode NL-PF with the learned cache standing in for the filtered estimate at depth
- * `depth` (records deeper than `depth` are read only for the judge slice and fallback tiles):
- *   judge slice  spp slice `judge_slice` filtered exactly at every level: training targets
+ * `depth` (records deeper than `depth` are read only for the audit slice and fallback tiles):
+ *   audit slice  spp slice `audit_slice` filtered exactly at every level: training targets
  *                (its unstylized depth-`depth` values at `train_records` vertices, Adam steps
  *                of `batch`, rate `lr`) and its exact level-0 image;
  *   inference    the network's estimate at depth `depth` for every slice, stylized there;
- *   judge        levels depth-1..0 of the judge slice rerun on the estimate; 64x8 tiles whose
+ *   audit        levels depth-1..0 of the audit slice rerun on the estimate; 64x8 tiles whose
  *                relative L2 error against the exact level-0 image exceeds `tile_threshold`
  *                are filtered exactly at depths levels-1..depth in every slice (with the
  *                dependency halo: the level-0 tile mask dilated by one tile per level);
  *   levels depth-1..0 and the film as nlpf_screen_run.
  * The network must be initialised (nlpf_mlp_init); training continues across calls.
  * film: float [H][W][3]; level_out (optional): float [spp][levels][3][H][W], where the levels
- * deeper than `depth` of the non-judge slices are defined only inside fallback tiles. */
+ * deeper than `depth` of the non-audit slices are defined only inside fallback tiles. */
 #define NLPF_CACHE_NETWORK 0 /* the trained network (the learned cache) */
 #define NLPF_CACHE_ZERO 1    /* control: depth-`depth` output 0, the paths cut there (no training) */
-#define NLPF_CACHE_MEAN 2    /* control: the judge slice's mean unstylized depth-`depth` value at every
+#define NLPF_CACHE_MEAN 2    /* control: the audit slice's mean unstylized depth-`depth` value at every
                                 vertex, stylized there (no training) */
 typedef struct nlpf_mlp_run_desc {
     uint32_t depth;
-    uint32_t judge_slice;
+    uint32_t audit_slice;
     uint32_t train_records;
     uint32_t batch;
     float lr;
@@ -459,13 +459,13 @@ typedef struct nlpf_mlp_run_desc {
     uint32_t reserved;
 } nlpf_mlp_run_desc;
 typedef struct nlpf_mlp_run_stats {
-    double total_ms, judge_ms, train_ms, infer_ms, fallback_ms, filter_ms;
+    double total_ms, audit_ms, train_ms, infer_ms, fallback_ms, filter_ms;
     uint64_t tiles;                   /* 64x8 tiles per image */
     uint64_t fallback_tiles;          /* level-0 tiles over the threshold */
     uint64_t fallback_tiles_filtered; /* tile-levels refiltered exactly per slice (halo included) */
     float train_loss;                 /* last training batch's loss */
-    float cache_rel_l2;               /* estimate vs exact unstylized depth-`depth` values, judge slice */
-    float judge_rel_l2;               /* level-0 image from the estimate vs exact, judge slice */
+    float cache_rel_l2;               /* estimate vs exact unstylized depth-`depth` values, audit slice */
+    float audit_rel_l2;               /* level-0 image from the estimate vs exact, audit slice */
 } nlpf_mlp_run_stats;
 int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
                         const float* anchors, const nlpf_mlp_run_desc* run, float* film,

@@ -1,13 +1,13 @@
 // Screen-mode NL-PF with the learned inner-expectation cache at depth k.
 //
 // Per frame (one call):
-//   1. judge slice  levels D..k of one spp slice filtered exactly (plain NL-PF); the unstylized
+//   1. audit slice  levels D..k of one spp slice filtered exactly (plain NL-PF); the unstylized
 //                   depth-k values F_k are the training targets; levels k-1..0 from the exact
 //                   depth-k output give the slice's exact images (kept as its final result);
-//   2. train        `train_records` judge vertices, Adam steps of `batch` on BF16 MFMA;
+//   2. train        `train_records` audit vertices, Adam steps of `batch` on BF16 MFMA;
 //   3. infer        the network estimates F_k for every slice (levels > k are not read),
 //                   stylized at depth k like a filtered value;
-//   4. judge        levels k-1..0 of the judge slice rerun on its estimate (in a separate
+//   4. audit        levels k-1..0 of the audit slice rerun on its estimate (in a separate
 //                   image) give P; per 64x8 tile, the relative L2 of P against the exact
 //                   level-0 image E is the error the cache puts into the image;
 //   5. fallback     tiles above the threshold are filtered exactly at depths D..k in every
@@ -113,9 +113,9 @@ __global__ void __launch_bounds__(256) style_planar_kernel(float* io, uint32_t S
     o[2 * plane] = v.z;
 }
 
-// squared error and squared norm of the prediction per 64x8 tile of the judge slice
+// squared error and squared norm of the prediction per 64x8 tile of the audit slice
 // (one workgroup per tile, fixed-order reduction)
-__global__ void tile_error_kernel(const float* pred, const float* judge, uint32_t W, uint32_t H, uint32_t tiles_x,
+__global__ void tile_error_kernel(const float* pred, const float* audit, uint32_t W, uint32_t H, uint32_t tiles_x,
                                   double* num_den) {
     __shared__ double sn[256], sd[256];
     const uint32_t tile = blockIdx.x;
@@ -127,7 +127,7 @@ __global__ void tile_error_kernel(const float* pred, const float* judge, uint32_
         if (x < W && y < H) {
             const uint64_t o = (uint64_t)y * W + x;
             for (int c = 0; c < 3; ++c) {
-                const double a = pred[c * plane + o], b = judge[c * plane + o];
+                const double a = pred[c * plane + o], b = audit[c * plane + o];
                 num += (a - b) * (a - b);
                 den += b * b;
             }
@@ -188,18 +188,18 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     if (W == 0 || H == 0 || S == 0 || L == 0 || L > 31 || desc->radius < 1 || desc->radius > 7)
         NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run_mlp: bad screen desc");
     if (k < 1 || k >= L) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "cache depth must be in [1, levels - 1]");
-    if (run->judge_slice >= S) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "judge slice out of range");
+    if (run->audit_slice >= S) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "audit slice out of range");
     if (run->batch == 0 || run->batch % 128 != 0 || run->train_records % run->batch != 0)
         NLPF_FAIL(h, NLPF_E_INVALID_ARG, "batch must be a multiple of 128 dividing train_records");
     const uint64_t plane = (uint64_t)W * H;
-    if (run->train_records > plane) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "more training records than judge vertices");
+    if (run->train_records > plane) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "more training records than audit vertices");
     if (plane * S >= (1ull << 32)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many vertices");
     const uint32_t est = run->estimator;
     if (est != NLPF_CACHE_NETWORK && est != NLPF_CACHE_ZERO && est != NLPF_CACHE_MEAN)
         NLPF_FAIL(h, NLPF_E_INVALID_ARG, "unknown cache estimator");
     const int fp16 = desc->precision == NLPF_FP16;
     const uint32_t tiles_x = (W + 63) / 64, tiles_y = (H + 7) / 8, ntiles = tiles_x * tiles_y;
-    const uint32_t sj = run->judge_slice;
+    const uint32_t sj = run->audit_slice;
     const size_t esz = fp16 ? 2 : 4;
     const uint32_t abx = (W + 7) / 8, aby = (H + 7) / 8;
 
@@ -222,10 +222,10 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
         return p;
     };
     float* lv = level_out ? level_out : (float*)take(lv_bytes);
-    float* Fj = (float*)take(f_bytes);  // judge slice: exact unstylized F_k (training targets)
-    float* Pj = (float*)take(p_bytes);  // judge slice on its estimate: levels 0..k [k+1][3][W*H]
+    float* Fj = (float*)take(f_bytes);  // audit slice: exact unstylized F_k (training targets)
+    float* Pj = (float*)take(p_bytes);  // audit slice on its estimate: levels 0..k [k+1][3][W*H]
     float* tgt = (float*)take(tgt_bytes);
-    uint32_t* lidx = (uint32_t*)take(idx_bytes);  // pixel ids within the judge slice
+    uint32_t* lidx = (uint32_t*)take(idx_bytes);  // pixel ids within the audit slice
     uint32_t* gidx = (uint32_t*)take(idx_bytes);  // sample ids s*W*H + pixel
     uint8_t* masks = (uint8_t*)take(mask_bytes);
     double* nd_img = (double*)take(nd_bytes);
@@ -272,7 +272,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     };
     auto slice_anc = [&](uint32_t s) { return fp16 ? anchors + (size_t)s * L * abx * aby * 3 : (const float*)nullptr; };
     auto slot = [&](uint32_t s, uint32_t d) { return lv + ((size_t)s * L + d) * 3 * plane; };
-    ScreenArgs<float> aj = a0;  // the judge slice alone
+    ScreenArgs<float> aj = a0;  // the audit slice alone
     aj.S = 1;
     aj.rec = slice_rec(sj);
     aj.anchors = slice_anc(sj);
@@ -295,7 +295,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
                                need_pos);
         return check_launch(h, "style depth k");
     };
-    // one level over the slices other than the judge slice (two contiguous ranges)
+    // one level over the slices other than the audit slice (two contiguous ranges)
     auto level_others = [&](ScreenArgs<float> a, uint32_t d) -> hipError_t {
         const uint32_t r0[2] = {0, sj + 1}, rn[2] = {sj, S - 1 - sj};
         for (int r = 0; r < 2; ++r) {
@@ -314,7 +314,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     nlpf_mlp_run_stats rs{};
     rc = record_event(h, 0);
     if (rc) return rc;
-    // 1. judge slice, exact: levels D..k+1, unstylized F_k (targets), stylized depth k, levels k-1..0
+    // 1. audit slice, exact: levels D..k+1, unstylized F_k (targets), stylized depth k, levels k-1..0
     for (int d = (int)L - 1; d > (int)k; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
     {
         ScreenArgs<float> af = aj;
@@ -327,7 +327,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
     rc = record_event(h, 1);
     if (rc) return rc;
-    // 2. online training on `train_records` judge vertices (a PCG stream of `seed` picks them);
+    // 2. online training on `train_records` audit vertices (a PCG stream of `seed` picks them);
     //    the control estimators train nothing
     if (est == NLPF_CACHE_NETWORK) {
         std::vector<uint32_t> perm(plane);
@@ -357,7 +357,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     }
     rc = record_event(h, 2);
     if (rc) return rc;
-    // 3. inference at depth k: the other slices into their level slots, the judge slice into
+    // 3. inference at depth k: the other slices into their level slots, the audit slice into
     //    slot k of its separate image; then the depth-k stylization
     float* Pk = Pj + (size_t)k * 3 * plane;
     if (est == NLPF_CACHE_NETWORK) {
@@ -369,7 +369,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
         rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)sj * plane, plane, Pj, k + 1, k, sj);
         if (rc) return rc;
     } else {
-        // controls: the judge slice's mean unstylized depth-k value everywhere, or zero (the
+        // controls: the audit slice's mean unstylized depth-k value everywhere, or zero (the
         // paths cut at depth k)
         const float* c3 = nullptr;
         if (est == NLPF_CACHE_MEAN) {
@@ -396,7 +396,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     }
     rc = record_event(h, 3);
     if (rc) return rc;
-    // 4. judge: levels k-1..0 of the judge slice on its estimate, per-tile error against its
+    // 4. audit: levels k-1..0 of the audit slice on its estimate, per-tile error against its
     //    exact level-0 image
     {
         ScreenArgs<float> ap = aj;
@@ -413,7 +413,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
         }
     }
     hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pj, slot(sj, 0), W, H, tiles_x, nd_img);
-    if ((rc = check_launch(h, "tile judge"))) return rc;
+    if ((rc = check_launch(h, "tile audit"))) return rc;
     std::vector<double> hi(2 * (size_t)ntiles), hf(2 * (size_t)ntiles);
     NLPF_HIP(h, hipMemcpyAsync(hi.data(), nd_img, nd_bytes, hipMemcpyDeviceToHost, st));
     NLPF_HIP(h, hipMemcpyAsync(hf.data(), nd_fit, nd_bytes, hipMemcpyDeviceToHost, st));
@@ -432,11 +432,11 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
             ++nfall;
         }
     }
-    rs.judge_rel_l2 = id_ > 0.0 ? (float)std::sqrt(in_ / id_) : 0.0f;
+    rs.audit_rel_l2 = id_ > 0.0 ? (float)std::sqrt(in_ / id_) : 0.0f;
     rs.cache_rel_l2 = fd_ > 0.0 ? (float)std::sqrt(fn_ / fd_) : 0.0f;
     rs.tiles = ntiles;
     rs.fallback_tiles = nfall;
-    // 5. exact filtering of the fallback tiles in the other slices (the judge slice is exact)
+    // 5. exact filtering of the fallback tiles in the other slices (the audit slice is exact)
     if (nfall && S > 1) {
         std::vector<uint8_t> hm((size_t)L * ntiles, 0);
         for (uint32_t d = 1; d < L; ++d) {
@@ -461,7 +461,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     rc = record_event(h, 5);
     if (rc) return rc;
     NLPF_HIP(h, hipStreamSynchronize(st));
-    rs.judge_ms = event_ms(h, 0, 1);
+    rs.audit_ms = event_ms(h, 0, 1);
     rs.train_ms = event_ms(h, 1, 2);
     rs.infer_ms = event_ms(h, 2, 3);
     rs.fallback_ms = event_ms(h, 3, 4);

@@ -1,7 +1,7 @@
 """Python host mirror of libnlpf's C ABI (ctypes).
 
 This is the binding a Python caller (tests, bench.py) uses; a Rust caller binds the same
-symbols through the `extern "C"` block shown in INTEGRATION.md.  It keeps the reference
+symbols through an `extern "C"` block over include/nlpf.h.  It keeps the reference
 integrator's configuration surface: `Config` carries the fields and defaults of
 `lm_path_filtering::Config` (crates/akari_integrator/src/lm_path_filtering.rs:264-298).
 
@@ -81,16 +81,16 @@ class _WorldDesc(C.Structure):
 
 
 class _MlpRunDesc(C.Structure):
-    _fields_ = [("depth", C.c_uint32), ("judge_slice", C.c_uint32), ("train_records", C.c_uint32),
+    _fields_ = [("depth", C.c_uint32), ("audit_slice", C.c_uint32), ("train_records", C.c_uint32),
                 ("batch", C.c_uint32), ("lr", C.c_float), ("tile_threshold", C.c_float), ("seed", C.c_uint64),
                 ("estimator", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class _MlpRunStats(C.Structure):
-    _fields_ = [("total_ms", C.c_double), ("judge_ms", C.c_double), ("train_ms", C.c_double),
+    _fields_ = [("total_ms", C.c_double), ("audit_ms", C.c_double), ("train_ms", C.c_double),
                 ("infer_ms", C.c_double), ("fallback_ms", C.c_double), ("filter_ms", C.c_double),
                 ("tiles", C.c_uint64), ("fallback_tiles", C.c_uint64), ("fallback_tiles_filtered", C.c_uint64),
-                ("train_loss", C.c_float), ("cache_rel_l2", C.c_float), ("judge_rel_l2", C.c_float)]
+                ("train_loss", C.c_float), ("cache_rel_l2", C.c_float), ("audit_rel_l2", C.c_float)]
 
 
 WORLD_PLANES = 28
@@ -472,13 +472,13 @@ class Handle:
         self._check(self.lib.nlpf_mlp_reference(self.h, C.byref(desc), _ptr(records), _ptr(anchors), depth,
                                                 _ptr(idx), n, _ptr(out)))
 
-    def screen_run_mlp(self, desc, records, anchors, film, level_out=None, depth=2, judge_slice=0,
+    def screen_run_mlp(self, desc, records, anchors, film, level_out=None, depth=2, audit_slice=0,
                        train_records=1 << 16, batch=1 << 14, lr=1e-2, tile_threshold=0.05, seed=0,
                        estimator="network"):
         """Screen NL-PF with the learned cache at `depth` (one frame; training continues across
         calls).  estimator: "network", or the controls "zero" (paths cut at `depth`) and "mean"
-        (the judge slice's mean depth-`depth` value).  Returns the run statistics as a dict."""
-        r = _MlpRunDesc(depth, judge_slice, train_records, batch, lr, tile_threshold, seed,
+        (the audit slice's mean depth-`depth` value).  Returns the run statistics as a dict."""
+        r = _MlpRunDesc(depth, audit_slice, train_records, batch, lr, tile_threshold, seed,
                         CACHE_ESTIMATORS[estimator], 0)
         st = _MlpRunStats()
         self._check(self.lib.nlpf_screen_run_mlp(self.h, C.byref(desc), _ptr(records), _ptr(anchors), C.byref(r),

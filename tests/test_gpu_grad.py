@@ -110,7 +110,7 @@ def test_adjoint_matches_fp64_gpu_finite_differences(R):
               f"{np.array2string(errs, precision=2)}")
         for q in range(19):
             err = min(abs(c - g[q]) for c in fd[q])
-            # h = 1e-3 (the step the north-star check names): vertices straddling a kink of the
+            # h = 1e-3 (the stated finite-difference step): vertices straddling a kink of the
             # ramp within the step make the differences themselves inexact at ~1e-3
             assert err <= 3e-3 * abs(fd[q][0]) + 1e-4 * scale, (q, fd[q], g[q])
 

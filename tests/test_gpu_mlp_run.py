@@ -2,13 +2,13 @@
 
 What the pass must do, checked piece by piece:
   * every tile falling back (threshold < 0) gives the plain NL-PF result;
-  * with no fallback, the judge slice is the plain result, the other slices hold the stylized
+  * with no fallback, the audit slice is the plain result, the other slices hold the stylized
     network estimate at the cache depth, and the levels above it are the plain NL-PF applied to
     that mixed depth-k output (FP64 oracle on records whose depth k-1 radiance carries the
     GPU's depth-k values: in_{k-1} = radiance + throughput * out_k);
   * with a threshold between tile errors, exactly the tiles whose relative L2 error on the
-    judge slice exceeds it are filtered exactly (halo included), the rest keep the estimate;
-  * training on the judge slice makes the estimate better than truncating the paths at depth k;
+    audit slice exceeds it are filtered exactly (halo included), the rest keep the estimate;
+  * training on the audit slice makes the estimate better than truncating the paths at depth k;
   * the pass is deterministic.
 FP32 records, tolerance 1e-5 relative per pixel against the FP64 oracle."""
 import numpy as np
@@ -100,7 +100,7 @@ def test_all_fallback_equals_plain(h, recs):
     desc, rec, _ = recs
     style(h, True)
     film0, lv0 = run_screen(h, desc, rec)
-    film, lv, st = run_mlp(h, desc, rec, judge_slice=1, train_records=8192, batch=1024, tile_threshold=-1.0)
+    film, lv, st = run_mlp(h, desc, rec, audit_slice=1, train_records=8192, batch=1024, tile_threshold=-1.0)
     assert st["fallback_tiles"] == st["tiles"] == ((W + 63) // 64) * ((H + 7) // 8)
     assert pixel_rel_err(np.moveaxis(lv[:, :K + 1], 2, -1), np.moveaxis(lv0[:, :K + 1], 2, -1)).max() <= 1e-6
     assert pixel_rel_err(film, film0).max() <= 1e-6
@@ -112,9 +112,9 @@ def test_no_fallback_uses_the_cache(h, recs):
     h.mlp_init(5)
     film0, lv0 = run_screen(h, desc, rec)
     sj = 2
-    film, lv, st = run_mlp(h, desc, rec, judge_slice=sj, train_records=8192, batch=1024, tile_threshold=1e30)
-    assert st["fallback_tiles"] == 0 and np.isfinite(st["train_loss"]) and np.isfinite(st["judge_rel_l2"])
-    # the judge slice is filtered exactly at every level
+    film, lv, st = run_mlp(h, desc, rec, audit_slice=sj, train_records=8192, batch=1024, tile_threshold=1e30)
+    assert st["fallback_tiles"] == 0 and np.isfinite(st["train_loss"]) and np.isfinite(st["audit_rel_l2"])
+    # the audit slice is filtered exactly at every level
     assert pixel_rel_err(np.moveaxis(lv[sj], 1, -1), np.moveaxis(lv0[sj], 1, -1)).max() <= 1e-6
     # the other slices hold the stylized estimate of the trained network
     others = [s for s in range(S) if s != sj]
@@ -128,14 +128,14 @@ def test_no_fallback_uses_the_cache(h, recs):
 @pytest.mark.parametrize("estimator", ["zero", "mean"])
 def test_control_estimators(h, recs, estimator):
     """The controls the cache is measured against, run through the same pass: "zero" cuts the
-    paths at depth k (out_k = 0, not stylized), "mean" puts the judge slice's mean exact
-    unstylized depth-k value at every vertex, stylized at depth k.  The judge slice stays exact
+    paths at depth k (out_k = 0, not stylized), "mean" puts the audit slice's mean exact
+    unstylized depth-k value at every vertex, stylized at depth k.  The audit slice stays exact
     and the levels above k are the plain NL-PF of that depth-k output."""
     desc, rec, host = recs
     table, root = style(h, True)
     sj = 1
     film0, lv0 = run_screen(h, desc, rec)
-    film, lv, st = run_mlp(h, desc, rec, judge_slice=sj, train_records=8192, batch=1024, tile_threshold=1e30,
+    film, lv, st = run_mlp(h, desc, rec, audit_slice=sj, train_records=8192, batch=1024, tile_threshold=1e30,
                            estimator=estimator)
     assert st["fallback_tiles"] == 0
     assert pixel_rel_err(np.moveaxis(lv[sj], 1, -1), np.moveaxis(lv0[sj], 1, -1)).max() <= 1e-6
@@ -143,7 +143,7 @@ def test_control_estimators(h, recs, estimator):
     if estimator == "zero":
         assert (lv[others, K] == 0).all()
     else:
-        # exact unstylized depth-k values of the judge slice: the same tree with depth k unstylized
+        # exact unstylized depth-k values of the audit slice: the same tree with depth k unstylized
         off = table.copy()
         off[0]["depth_mask"] = off[1]["depth_mask"] = N.ALL_DEPTHS & ~(1 << K)
         _, ulv = O.screen_run(host[sj:sj + 1], W, H, 1, L, R, SIG, nodes=off, root=root, want_levels=True)
@@ -171,8 +171,8 @@ def test_partial_fallback_selects_tiles(h, recs, radius):
     h.mlp_init(11)
     sj = 0
     film0, lv0 = run_screen(h, desc, rec)
-    # lr 0 keeps the weights, so the judge's image from the estimate is known before the call:
-    # levels k-1..0 of the judge slice on records carrying the estimate at depth k
+    # lr 0 keeps the weights, so the audit's image from the estimate is known before the call:
+    # levels k-1..0 of the audit slice on records carrying the estimate at depth k
     pred = predict(h, desc, rec, list(range(S)))
     mod = host[sj:sj + 1, :K].copy()
     thr_k = mod[0, K - 1, 12:15].astype(np.float64)
@@ -189,11 +189,11 @@ def test_partial_fallback_selects_tiles(h, recs, radius):
         thr = 0.5 * (e[i] + e[i + 1])
         assert gaps[i] > 1e-3 * thr, (e[lo:hi + 1], thr)
         mask0 = err > thr
-        film, lv, st = run_mlp(h, desc, rec, judge_slice=sj, train_records=4096, batch=1024, lr=0.0,
+        film, lv, st = run_mlp(h, desc, rec, audit_slice=sj, train_records=4096, batch=1024, lr=0.0,
                                tile_threshold=float(thr))
         assert st["fallback_tiles"] == mask0.sum() and 0 < mask0.sum() < mask0.size
         jrel = np.linalg.norm(plv[0, 0] - lv0[sj, 0]) / np.linalg.norm(lv0[sj, 0])
-        assert abs(st["judge_rel_l2"] - jrel) < 1e-4 * jrel
+        assert abs(st["audit_rel_l2"] - jrel) < 1e-4 * jrel
         mk = dilate(mask0, K)  # depth-k tiles a level-0 fallback tile depends on
         pm = np.repeat(np.repeat(mk, 8, 0), 64, 1)[:H, :W]
         for s in range(S):
@@ -229,16 +229,16 @@ def test_training_beats_truncation(h, smooth_recs):
     style(h, False)
     film0, _ = run_screen(h, desc, rec)
     h.mlp_init(3)
-    film_un, _, _ = run_mlp(h, desc, rec, judge_slice=0, train_records=8192, batch=1024, lr=0.0,
+    film_un, _, _ = run_mlp(h, desc, rec, audit_slice=0, train_records=8192, batch=1024, lr=0.0,
                             tile_threshold=1e30)
     st = None
     for frame in range(4):  # training continues across frames
-        film_ml, _, st = run_mlp(h, desc, rec, judge_slice=0, train_records=8192, batch=1024, tile_threshold=1e30,
+        film_ml, _, st = run_mlp(h, desc, rec, audit_slice=0, train_records=8192, batch=1024, tile_threshold=1e30,
                                  seed=frame)
     cut = host[:, :K].copy()
     cut[:, K - 1, 12:15] = 0
     film_tr = O.screen_run(cut, W, H, S, K, R, SIG)
-    film_mean, _, _ = run_mlp(h, desc, rec, judge_slice=0, train_records=8192, batch=1024, tile_threshold=1e30,
+    film_mean, _, _ = run_mlp(h, desc, rec, audit_slice=0, train_records=8192, batch=1024, tile_threshold=1e30,
                               estimator="mean")
     exact = film0.astype(np.float64)
 
@@ -246,8 +246,8 @@ def test_training_beats_truncation(h, smooth_recs):
         return np.linalg.norm(x - exact) / np.linalg.norm(exact)
 
     e_ml, e_tr, e_un, e_mean = rel(film_ml), rel(film_tr), rel(film_un), rel(film_mean)
-    print(f"film rel L2: cache {e_ml:.4f}, truncation {e_tr:.4f}, untrained {e_un:.4f}, judge-slice mean "
-          f"{e_mean:.4f}; cache fit {st['cache_rel_l2']:.4f}, judge {st['judge_rel_l2']:.4f}")
+    print(f"film rel L2: cache {e_ml:.4f}, truncation {e_tr:.4f}, untrained {e_un:.4f}, audit-slice mean "
+          f"{e_mean:.4f}; cache fit {st['cache_rel_l2']:.4f}, audit {st['audit_rel_l2']:.4f}")
     assert e_ml < e_tr and e_ml < e_un
 
 
@@ -257,7 +257,7 @@ def test_deterministic(h, recs):
     outs = []
     for _ in range(2):
         h.mlp_init(9)
-        f, lv, st = run_mlp(h, desc, rec, judge_slice=1, train_records=8192, batch=1024, tile_threshold=0.5)
+        f, lv, st = run_mlp(h, desc, rec, audit_slice=1, train_records=8192, batch=1024, tile_threshold=0.5)
         outs.append((f, lv[:, :K + 1], st["fallback_tiles"], st["train_loss"]))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     assert outs[0][2:] == outs[1][2:]
@@ -266,9 +266,9 @@ def test_deterministic(h, recs):
 def test_invalid_arguments(h, recs):
     desc, rec, _ = recs
     film = h.alloc(PLANE * 12)
-    for kw in ({"depth": 0}, {"depth": L}, {"judge_slice": S}, {"batch": 100}, {"train_records": 3000, "batch": 1024},
+    for kw in ({"depth": 0}, {"depth": L}, {"audit_slice": S}, {"batch": 100}, {"train_records": 3000, "batch": 1024},
                {"train_records": 2 * PLANE, "batch": 1024}):
-        args = dict(depth=K, judge_slice=0, train_records=4096, batch=1024)
+        args = dict(depth=K, audit_slice=0, train_records=4096, batch=1024)
         args.update(kw)
         with pytest.raises(N.NlpfError):
             h.screen_run_mlp(desc, rec, None, film, None, **args)
