@@ -9,7 +9,8 @@
  * Conventions
  *  - every entry point returns an int status (NLPF_OK == 0); on failure
  *    nlpf_last_error(h) (or nlpf_last_error(NULL) for the calls without a handle:
- *    nlpf_create, nlpf_style_from_scene, nlpf_write_world_scene, nlpf_write_image_host)
+ *    nlpf_create, nlpf_style_from_scene, nlpf_bsdfs_from_scene, nlpf_write_world_scene,
+ *    nlpf_write_image_host)
  *    holds a message;
  *  - buffers are device pointers owned by the caller, or allocated through
  *    nlpf_alloc (hipMalloc on the handle's device);
@@ -127,7 +128,8 @@ typedef struct nlpf_handle nlpf_handle;
 int nlpf_create(const char* config_json, nlpf_handle** out);
 int nlpf_destroy(nlpf_handle* h);
 /* Message of the last failing call on h; h == NULL gives the last failure of a call without a
- * handle (nlpf_create, nlpf_style_from_scene, nlpf_write_world_scene, nlpf_write_image_host). */
+ * handle (nlpf_create, nlpf_style_from_scene, nlpf_bsdfs_from_scene, nlpf_write_world_scene,
+ * nlpf_write_image_host). */
 const char* nlpf_last_error(const nlpf_handle* h);
 /* Parsed configuration echoed back as JSON (for host-side mirrors and tests). */
 int nlpf_get_config_json(const nlpf_handle* h, char* buf, size_t cap);
@@ -271,13 +273,14 @@ typedef struct nlpf_gen_desc {
     float albedo_lo, albedo_hi;
 } nlpf_gen_desc;
 
-/* ---- world mode: the reference's hash-grid NL-PF for Lambertian records ----
+/* ---- world mode: the reference's hash-grid NL-PF ----
  * Replaces PathFiltering::render's filter loop (lm_path_filtering.rs:1673-1788) for records
  * traced elsewhere (kernel_mk_trace :1062-1218 or nlpf_generate_world_records).
  * N = width*height*branch paths, path t = b*width*height + pixel.  Float planes per level d,
  * layout records[(d*NLPF_WORLD_PLANES + c)*N + t]:
  *   0-2 p, 3-5 ns (shading normal), 6-8 ng (geometric normal), 9-11 wo, 12-14 albedo R
- *   (diffuse reflectance, f = R/pi |cos|), 15-17 le, 18 mis (emission MIS weight),
+ *   (base colour: a diffuse material has f = R/pi |cos|; see nlpf_set_bsdfs), 15-17 le,
+ *   18 mis (emission MIS weight),
  *   19-21 direct.wi, 22-24 direct.irradiance, 25 direct.pdf, 26 direct.mis, 27 bsdf_pdf.
  * flags / new_flags: StylizationFlags bytes [L][N]; material: stylization material [L][N];
  * path_depth[t]: last depth with a vertex, -1 for a path whose camera ray missed.
@@ -306,6 +309,49 @@ typedef struct nlpf_world_desc {
 int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const float* records,
                    const uint8_t* flags, const uint8_t* new_flags, const uint16_t* material,
                    const int32_t* path_depth, float* film, float* radiance_out);
+
+/* ---- world mode: each material's BSDF ----
+ * The world filter evaluates every centre vertex's BSDF for its neighbours' directions and, for
+ * SMIS, every reused neighbour's own BSDF (Svm::dispatch_surface, svm/eval.rs:589-624, through
+ * SurfaceClosure::evaluate, svm/surface/mod.rs:1701-1744).  Entry m of the table is material
+ * m's BSDF; a material without an entry is diffuse.  The base colour is the record's albedo
+ * plane in every kind.
+ *   NLPF_BSDF_DIFFUSE     DiffuseBsdf, reflectance albedo / pi (svm/surface/diffuse.rs:22-41,
+ *                         101-121).
+ *   NLPF_BSDF_PRINCIPLED  the principled BSDF (svm/surface/principled.rs:120-330) where it needs
+ *                         no precomputed albedo table: coat_weight == 0 and either
+ *                         metallic >= 1 - 1e-4 (the metal lobe; specular_ior_level and
+ *                         transmission are then never evaluated) or specular_ior_level == 0 and
+ *                         transmission == 0 (the specular layer's weight is then 0).  It is then
+ *                         Mix(diffuse, metal, metallic) (BsdfMixture, mod.rs:1560-1593), the metal
+ *                         lobe a Trowbridge-Reitz reflection (roughness^2, at least 1e-4, visible
+ *                         normal pdf; microfacet.rs) with the conductor Fresnel of
+ *                         artistic_to_conductor_fresnel(albedo, specular_tint) (mod.rs:2128-2157).
+ *                         Other principled parameter sets are rejected with NLPF_E_UNSUPPORTED.
+ *   NLPF_BSDF_EMISSION    an emission-only surface: no reflection (EmissiveSurface without an
+ *                         inner BSDF, mod.rs:1347-1372).
+ * No reference function takes this table: it stands for the SVM material data the reference's
+ * filter reads per hit. */
+#define NLPF_BSDF_DIFFUSE 0u
+#define NLPF_BSDF_PRINCIPLED 1u
+#define NLPF_BSDF_EMISSION 2u
+typedef struct nlpf_bsdf {
+    uint32_t kind;
+    float metallic, roughness;
+    float specular_tint[3];
+    float specular_ior_level, transmission, coat_weight;
+    uint32_t reserved; /* 0 */
+} nlpf_bsdf;            /* 40 bytes */
+int nlpf_set_bsdfs(nlpf_handle* h, const nlpf_bsdf* table, uint32_t n_materials);
+/* Read each named material's BSDF from a reference scene file (the graph below its output node,
+ * through the sty_* nodes' "surface" inputs, which is what their closures evaluate): "diffuse" ->
+ * NLPF_BSDF_DIFFUSE, "emission" -> NLPF_BSDF_EMISSION, "principled" -> NLPF_BSDF_PRINCIPLED with
+ * its constant metallic, roughness, specular_tint, specular_ior_level, transmission_weight and
+ * coat_weight inputs (the base colour may be anything: it comes from the records).  Any other
+ * surface, a non-constant input of those, a normal map, or a principled parameter set outside
+ * the subset above fails with a message naming the material. */
+int nlpf_bsdfs_from_scene(const char* scene_json, const char* const* materials, uint32_t n_materials,
+                          nlpf_bsdf* out);
 
 /* Trace the procedural scene (below) into world-mode records, one path per (pixel, branch),
  * as kernel_mk_trace does (lm_path_filtering.rs:1062-1218, area-light NEE light/area.rs:50-131): camera ray, NEE to the quad light with its radiance stylized at

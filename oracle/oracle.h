@@ -48,6 +48,24 @@ int or_screen_run_f32(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius
                       double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
                       float* film, float* level_out, int threads);
 
+/* World-mode BSDFs per material (oracle_bsdf.c); the same 40-byte layout as the library's
+ * nlpf_bsdf. */
+enum { OR_BSDF_DIFFUSE = 0, OR_BSDF_PRINCIPLED = 1, OR_BSDF_EMISSION = 2 };
+typedef struct or_bsdf {
+    uint32_t kind;
+    float metallic, roughness;
+    float specular_tint[3];
+    float specular_ior_level, transmission, coat_weight;
+    uint32_t reserved;
+} or_bsdf;
+double or_ggx_d(double cos_h, double alpha);
+double or_ggx_lambda(double cos_w, double alpha);
+double or_fr_complex(double cos_i, double n, double k);
+void or_conductor_nk(double r, double g, double* n, double* k);
+/* pdf of wi; fc (optional) = f(wo, wi) |cos_i| per channel, R = the record's albedo (base colour) */
+double or_bsdf_eval(const or_bsdf* b, const double R[3], const double ns[3], const double ng[3], const double wo[3],
+                    const double wi[3], double fc[3]);
+
 /* World-mode NL-PF, FP64; see oracle_world.c.  cluster != 0 runs the deterministic clustering
  * (cluster_out, optional, receives every level's cluster ids [L][N], 0xFFFFFFFF off-level). */
 int or_world_neighbor_counts(const float* pos, uint32_t n, const float pmin[3], const float pmax[3], float radius,
@@ -58,7 +76,7 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                  const uint32_t* reuse, uint32_t n_reuse, uint64_t vram_mb, int smis, int no_first_hit_stylization,
                  int disable_flag_check, int cluster, uint64_t seed, uint64_t table, const or_node* nodes,
                  const int32_t* roots, uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out,
-                 uint32_t* cluster_out);
+                 uint32_t* cluster_out, const or_bsdf* bsdfs, uint32_t n_bsdfs);
 
 /* Random streams (oracle_world.c): Pcg32 new_seq_offset(seq, offset) draws; the ChaCha block
  * function; rand_core's seed_from_u64 key expansion; rand_chacha's ChaCha12 u64 stream; the

@@ -40,7 +40,16 @@ def load():
     lib.or_world_neighbor_counts.argtypes = [vp, u32, vp, vp, C.c_float, u64, vp]
     lib.or_world_neighbor_counts.restype = i32
     lib.or_world_run.argtypes = [u32, u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, u32, u64, i32, i32,
-                                 i32, i32, u64, u64, vp, vp, u32, vp, vp, vp, vp]
+                                 i32, i32, u64, u64, vp, vp, u32, vp, vp, vp, vp, vp, u32]
+    lib.or_ggx_d.argtypes = [dbl, dbl]
+    lib.or_ggx_d.restype = dbl
+    lib.or_ggx_lambda.argtypes = [dbl, dbl]
+    lib.or_ggx_lambda.restype = dbl
+    lib.or_fr_complex.argtypes = [dbl, dbl, dbl]
+    lib.or_fr_complex.restype = dbl
+    lib.or_conductor_nk.argtypes = [dbl, dbl, vp, vp]
+    lib.or_bsdf_eval.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.or_bsdf_eval.restype = dbl
     lib.or_world_run.restype = i32
     lib.or_screen_grad_f64.argtypes = [u32, u32, u32, u32, i32, vp, vp, vp, u32, i32, vp, vp, vp]
     lib.or_screen_grad_f64.restype = i32
@@ -50,6 +59,49 @@ def load():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# world-mode BSDF table entry (the library's nlpf_bsdf layout); kinds
+BSDF_DTYPE = np.dtype([("kind", "<u4"), ("metallic", "<f4"), ("roughness", "<f4"), ("specular_tint", "<f4", 3),
+                       ("specular_ior_level", "<f4"), ("transmission", "<f4"), ("coat_weight", "<f4"),
+                       ("reserved", "<u4")])
+BSDF_DIFFUSE, BSDF_PRINCIPLED, BSDF_EMISSION = 0, 1, 2
+
+
+def bsdf(kind=BSDF_PRINCIPLED, metallic=1.0, roughness=0.5, specular_tint=(1.0, 1.0, 1.0), specular_ior_level=0.0,
+         transmission=0.0, coat_weight=0.0):
+    e = np.zeros(1, BSDF_DTYPE)
+    e["kind"], e["metallic"], e["roughness"] = kind, metallic, roughness
+    e["specular_tint"] = specular_tint
+    e["specular_ior_level"], e["transmission"], e["coat_weight"] = specular_ior_level, transmission, coat_weight
+    return e
+
+
+def ggx_d(cos_h, alpha):
+    return load().or_ggx_d(float(cos_h), float(alpha))
+
+
+def ggx_lambda(cos_w, alpha):
+    return load().or_ggx_lambda(float(cos_w), float(alpha))
+
+
+def fr_complex(cos_i, n, k):
+    return load().or_fr_complex(float(cos_i), float(n), float(k))
+
+
+def conductor_nk(r, g):
+    n, k = np.zeros(1), np.zeros(1)
+    load().or_conductor_nk(float(r), float(g), _p(n), _p(k))
+    return float(n[0]), float(k[0])
+
+
+def bsdf_eval(entry, R, ns, ng, wo, wi):
+    """(pdf, f |cos_i| per channel) of one table entry (None: diffuse) for world-space vectors."""
+    fc = np.zeros(3)
+    e = None if entry is None else np.ascontiguousarray(entry, BSDF_DTYPE)
+    v = [np.ascontiguousarray(x, np.float64) for x in (R, ns, ng, wo, wi)]
+    pdf = load().or_bsdf_eval(_p(e), *[_p(x) for x in v], _p(fc))
+    return pdf, fc
 
 
 def stylize(nodes, roots, rgb, depth=0, flags=None, material=None, pos=None):
@@ -154,10 +206,12 @@ def world_neighbor_counts(pos, pmin, pmax, radius, table):
 
 
 def world_run(rec, flags, new_flags, material, path_depth, W, H, branch, L, aabb_min, aabb_max, cfg,
-              nodes=None, roots=(), want_radiance=False, want_clusters=False):
+              nodes=None, roots=(), want_radiance=False, want_clusters=False, bsdfs=None):
     """cfg: dict with the "pf" fields (radius_scale, reuse, vram, smis, no_first_hit_stylization,
     disable_flag_check, cluster, seed, hash_table_size).  want_clusters adds the cluster ids
-    [L][N] (0xFFFFFFFF off-level) to the returned tuple."""
+    [L][N] (0xFFFFFFFF off-level) to the returned tuple.  bsdfs: BSDF_DTYPE table by material
+    (materials without an entry are diffuse)."""
+    bs = None if bsdfs is None or len(bsdfs) == 0 else np.ascontiguousarray(bsdfs, BSDF_DTYPE)
     lib = load()
     N = W * H * branch
     rec = np.ascontiguousarray(rec, np.float32)
@@ -183,7 +237,8 @@ def world_run(rec, flags, new_flags, material, path_depth, W, H, branch, L, aabb
                           int(cfg.get("no_first_hit_stylization", False)), int(cfg.get("disable_flag_check", False)),
                           int(cfg.get("cluster", True)), int(cfg.get("seed", 0)),
                           int(cfg.get("hash_table_size", 64 * 1024 * 1024)),
-                          _p(nodes_arr), _p(roots), len(roots), _p(film), _p(rad), _p(nbr), _p(cl))
+                          _p(nodes_arr), _p(roots), len(roots), _p(film), _p(rad), _p(nbr), _p(cl),
+                          _p(bs), 0 if bs is None else len(bs))
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     out = (film, rad, nbr) if want_radiance else (film,)

@@ -11,8 +11,9 @@
  *   kernel_filter / kernel_propagate          :999-1060, :973-998
  *   depth loop, radius / resolution           :1563-1598, :1673-1774
  *   kernel_splat / kernel_update_film         :479-506, :428-477 (+ film.rs:196-229)
- * with DiffuseBsdf (svm/surface/diffuse.rs:22-41) behind SurfaceClosure::evaluate
- * (svm/surface/mod.rs:1701-1744) and Pcg32 (sampler/mod.rs:79-137).
+ * with each material's BSDF behind SurfaceClosure::evaluate (svm/surface/mod.rs:1701-1744;
+ * DiffuseBsdf, or the principled subset of oracle_bsdf.c) and Pcg32 (sampler/mod.rs:79-137).
+ * The SMIS denominators evaluate each reused neighbour's own BSDF, normals and wo (:886-935).
  *
  * Reservoir streams as the reference seeds them (init_pcg32_buffer_with_seed,
  * sampler/mod.rs:157-174): the t-th point of a batch draws from Pcg32 new_seq_offset(t, s_t),
@@ -40,7 +41,6 @@
 #include "oracle.h"
 
 #define NP 28
-#define INV_PI 0.318309886183790671538
 
 typedef struct {
     uint64_t state, inc;
@@ -161,26 +161,10 @@ typedef struct {
 static double F(lvl_t l, int c, uint64_t i) { return l.rec[(uint64_t)c * l.N + i]; }
 
 /* Hemisphere and leak tests are decided in FP32, as the reference computes them (its
- * closures run in f32); the estimator arithmetic itself stays FP64. */
-static float dotf(const double a[3], const double b[3]) {
-    float p0 = (float)a[0] * (float)b[0], p1 = (float)a[1] * (float)b[1], p2 = (float)a[2] * (float)b[2];
-    float s = p0 + p1;
-    return s + p2;
-}
-static int valid_dir(const double ns[3], const double ng[3], const double wo[3], const double wi[3]) {
-#define SG(x) ((x) > 0 ? 1.0f : -1.0f)
-    float fl = SG(dotf(ng, ns));
-    return SG(fl * dotf(wo, ns)) * SG(dotf(wo, ng)) > 0 && SG(fl * dotf(wi, ns)) * SG(dotf(wi, ng)) > 0;
-}
-/* returns pdf; *fcos = |cos_i| / pi when the pair is valid and same-hemisphere */
-static double diffuse(const double ns[3], const double ng[3], const double wo[3], const double wi[3], double* fcos) {
-    *fcos = 0;
-    if (!valid_dir(ns, ng, wo, wi)) return 0;
-    float cof = dotf(wo, ns), cif = dotf(wi, ns);
-    if (!(cof * cif >= 0.0f)) return 0;
-    double ci = ns[0] * wi[0] + ns[1] * wi[1] + ns[2] * wi[2];
-    *fcos = fabs(ci) * INV_PI;
-    return *fcos;
+ * closures run in f32), inside or_bsdf_eval; the estimator arithmetic itself stays FP64. */
+/* BSDF of material m: its table entry, diffuse without one */
+static const or_bsdf* bsdf_of(const or_bsdf* bsdfs, uint32_t n_bsdfs, uint16_t m) {
+    return bsdfs && m < n_bsdfs ? &bsdfs[m] : NULL;
 }
 
 static uint32_t bucket_of(const float pmin[3], const float pmax[3], const uint32_t res[3], uint64_t table,
@@ -352,7 +336,7 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                  const uint32_t* reuse, uint32_t n_reuse, uint64_t vram_mb, int smis, int no_first_hit_stylization,
                  int disable_flag_check, int cluster, uint64_t seed, uint64_t table, const or_node* nodes,
                  const int32_t* roots, uint32_t n_materials, double* film, double* radiance_out, uint64_t* nbr_out,
-                 uint32_t* cluster_out) {
+                 uint32_t* cluster_out, const or_bsdf* bsdfs, uint32_t n_bsdfs) {
     const uint64_t npix = (uint64_t)W * H, N = npix * branch;
     double* ind = (double*)calloc(8 * N, sizeof(double));
     double* rad = (double*)calloc(3 * N * L, sizeof(double));
@@ -426,6 +410,7 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                     const double ng[3] = {F(lv, 6, c), F(lv, 7, c), F(lv, 8, c)};
                     const double wo[3] = {F(lv, 9, c), F(lv, 10, c), F(lv, 11, c)};
                     const double R[3] = {F(lv, 12, c), F(lv, 13, c), F(lv, 14, c)};
+                    const or_bsdf* bc = bsdf_of(bsdfs, n_bsdfs, material[(uint64_t)d * N + c]);
                     const uint8_t myf = new_flags[(uint64_t)d * N + c];
                     int32_t cell[3];
                     cell_of(aabb_min, aabb_max, res, qx, qy, qz, cell);
@@ -466,26 +451,28 @@ int or_world_run(uint32_t W, uint32_t H, uint32_t branch, uint32_t L, const floa
                         const uint32_t j = buf[k];
                         const double dpdf = F(lv, 25, j);
                         if (dpdf > 0) {
-                            double wi[3] = {F(lv, 19, j), F(lv, 20, j), F(lv, 21, j)}, fc;
-                            double bp = diffuse(ns, ng, wo, wi, &fc);
+                            double wi[3] = {F(lv, 19, j), F(lv, 20, j), F(lv, 21, j)}, fc[3];
+                            double bp = or_bsdf_eval(bc, R, ns, ng, wo, wi, fc);
                             double w = dpdf / (dpdf + bp);
                             for (int ch = 0; ch < 3; ++ch)
-                                direct[ch] += R[ch] * fc * F(lv, 22 + ch, j) * F(lv, 26, j) * w / dpdf / (double)used;
+                                direct[ch] += fc[ch] * F(lv, 22 + ch, j) * F(lv, 26, j) * w / dpdf / (double)used;
                         }
                         double wi[3] = {ind[j], ind[N + j], ind[2 * N + j]};
                         if (sqrt(wi[0] * wi[0] + wi[1] * wi[1] + wi[2] * wi[2]) == 0) continue;
-                        double fc;
-                        double bp = diffuse(ns, ng, wo, wi, &fc);
+                        double fc[3];
+                        double bp = or_bsdf_eval(bc, R, ns, ng, wo, wi, fc);
                         double reused[3];
-                        for (int ch = 0; ch < 3; ++ch) reused[ch] = R[ch] * fc * ind[(3 + ch) * N + j] * ind[6 * N + j];
+                        for (int ch = 0; ch < 3; ++ch) reused[ch] = fc[ch] * ind[(3 + ch) * N + j] * ind[6 * N + j];
                         if (smis) {
                             double sum = 0;
                             for (uint32_t m = 0; m < used; ++m) {
                                 uint32_t o = buf[m];
                                 double ons[3] = {F(lv, 3, o), F(lv, 4, o), F(lv, 5, o)};
                                 double ong[3] = {F(lv, 6, o), F(lv, 7, o), F(lv, 8, o)};
-                                double owo[3] = {F(lv, 9, o), F(lv, 10, o), F(lv, 11, o)}, f2;
-                                sum += diffuse(ons, ong, owo, wi, &f2);
+                                double owo[3] = {F(lv, 9, o), F(lv, 10, o), F(lv, 11, o)};
+                                double oR[3] = {F(lv, 12, o), F(lv, 13, o), F(lv, 14, o)};
+                                sum += or_bsdf_eval(bsdf_of(bsdfs, n_bsdfs, material[(uint64_t)d * N + o]), oR, ons, ong,
+                                                    owo, wi, NULL);
                             }
                             for (int ch = 0; ch < 3; ++ch) indirect[ch] += reused[ch] / sum;
                         } else if (bp > 0) {

@@ -176,6 +176,7 @@ int nlpf_destroy(nlpf_handle* h) {
     mlp_release(h);
     if (h->d_nodes) (void)hipFree(h->d_nodes);
     if (h->d_roots) (void)hipFree(h->d_roots);
+    if (h->d_bsdf) (void)hipFree(h->d_bsdf);
     if (h->scratch) (void)hipFree(h->scratch);
     for (hipEvent_t ev : h->events) (void)hipEventDestroy(ev);
     if (h->stream) (void)hipStreamDestroy(h->stream);

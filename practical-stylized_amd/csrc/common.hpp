@@ -65,6 +65,10 @@ struct nlpf_handle {
     int32_t* d_roots = nullptr;
     std::vector<nlpf_style_node> h_nodes;
     std::vector<int32_t> h_roots;
+    // world-mode BSDF table (nlpf_set_bsdfs): the validated entries and their device form
+    std::vector<nlpf_bsdf> h_bsdf;
+    void* d_bsdf = nullptr;
+    int bsdf_nondiffuse = 0;  // some entry is not NLPF_BSDF_DIFFUSE
     // timing
     std::vector<hipEvent_t> events;
     nlpf_stats stats{};

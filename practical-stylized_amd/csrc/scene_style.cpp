@@ -371,3 +371,100 @@ extern "C" int nlpf_style_from_scene(const char* scene_json, const char* const* 
         return NLPF_E_OOM;
     }
 }
+
+// BSDF of each named material: the base surface below the output node, reached through the sty_*
+// nodes' "surface" inputs (what their closures' evaluate delegates to, svm/surface/mod.rs:451-880)
+static int bsdfs_from_scene(const char* scene_json, const char* const* materials, uint32_t n_materials,
+                            nlpf_bsdf* out) {
+    if (!scene_json || (n_materials && (!materials || !out))) {
+        set_global_error("nlpf_bsdfs_from_scene: NULL argument");
+        return NLPF_E_INVALID_ARG;
+    }
+    JParser p{scene_json, ""};
+    JVal root;
+    if (!p.parse(root)) {
+        set_global_error("scene JSON: " + p.err);
+        return NLPF_E_INVALID_ARG;
+    }
+    const JVal* mats = jget(root, "materials");
+    if (!mats || mats->kind != JVal::Obj) {
+        set_global_error("scene JSON: no \"materials\" object");
+        return NLPF_E_INVALID_ARG;
+    }
+    for (uint32_t m = 0; m < n_materials; ++m) {
+        if (!materials[m]) {
+            set_global_error("nlpf_bsdfs_from_scene: NULL material name");
+            return NLPF_E_INVALID_ARG;
+        }
+        const JVal* mat = jget(*mats, materials[m]);
+        const JVal* shader = mat ? jget(*mat, "shader") : nullptr;
+        const JVal* gnodes = shader ? jget(*shader, "nodes") : nullptr;
+        const JVal* outn = shader ? jget(*shader, "output") : nullptr;
+        const JVal* out_id = outn ? jget(*outn, "id") : nullptr;
+        if (!gnodes || gnodes->kind != JVal::Obj || !out_id || out_id->kind != JVal::Str) {
+            set_global_error(std::string("scene JSON: material '") + materials[m] + "' has no shader graph");
+            return NLPF_E_INVALID_ARG;
+        }
+        Lowering lw;
+        lw.nodes = gnodes;
+        lw.material = materials[m];
+        auto fail = [&](const std::string& msg, int code) {
+            lw.fail(msg);
+            set_global_error(lw.err);
+            return code;
+        };
+        std::string id = out_id->str;
+        const JVal* n = lw.node(id);
+        for (int hop = 0; n; ++hop) {
+            const char* t = Lowering::type_of(*n);
+            const char* field = !strcmp(t, "output") ? "node" : (!strncmp(t, "sty_", 4) ? "surface" : nullptr);
+            if (!field) break;
+            if (hop > 64) return fail("surface chain deeper than 64 nodes (or cyclic)", NLPF_E_INVALID_ARG);
+            if (!lw.ref_id(*n, field, id)) return fail("", NLPF_E_INVALID_ARG);
+            n = lw.node(id);
+        }
+        if (!n) return fail("unknown node '" + id + "'", NLPF_E_INVALID_ARG);
+        nlpf_bsdf b;
+        memset(&b, 0, sizeof(b));
+        const std::string t = Lowering::type_of(*n);
+        if (t == "diffuse") {
+            b.kind = NLPF_BSDF_DIFFUSE;
+        } else if (t == "emission") {
+            b.kind = NLPF_BSDF_EMISSION;
+        } else if (t == "principled") {
+            b.kind = NLPF_BSDF_PRINCIPLED;
+            if (!lw.scalar(*n, "metallic", b.metallic) || !lw.scalar(*n, "roughness", b.roughness) ||
+                !lw.color(*n, "specular_tint", b.specular_tint) ||
+                !lw.scalar(*n, "specular_ior_level", b.specular_ior_level) ||
+                !lw.scalar(*n, "transmission_weight", b.transmission) ||
+                !lw.scalar(*n, "coat_weight", b.coat_weight))
+                return fail("", NLPF_E_UNSUPPORTED);
+            if (b.coat_weight != 0.0f)
+                return fail("principled coat (coat_weight != 0) needs the reference's precomputed albedo table",
+                            NLPF_E_UNSUPPORTED);
+            if (b.metallic < 1.0f - 1e-4f && (b.specular_ior_level != 0.0f || b.transmission != 0.0f))
+                return fail("principled with metallic < 1 needs specular_ior_level 0 and transmission 0",
+                            NLPF_E_UNSUPPORTED);
+            // normal_map with a zero normal keeps the shading frame (principled.rs:313-321,
+            // mod.rs:2469-2506); anything else is a normal map
+            float nrm[3];
+            if (!lw.color(*n, "normal", nrm)) return fail("", NLPF_E_UNSUPPORTED);
+            if (nrm[0] != 0.0f || nrm[1] != 0.0f || nrm[2] != 0.0f)
+                return fail("principled normal map is not supported", NLPF_E_UNSUPPORTED);
+        } else {
+            return fail("surface '" + t + "' is not supported by the world filter", NLPF_E_UNSUPPORTED);
+        }
+        out[m] = b;
+    }
+    return NLPF_OK;
+}
+
+extern "C" int nlpf_bsdfs_from_scene(const char* scene_json, const char* const* materials, uint32_t n_materials,
+                                     nlpf_bsdf* out) {
+    try {
+        return bsdfs_from_scene(scene_json, materials, n_materials, out);
+    } catch (const std::bad_alloc&) {
+        set_global_error("nlpf_bsdfs_from_scene: out of host memory");
+        return NLPF_E_OOM;
+    }
+}

@@ -13,6 +13,7 @@
 //   propagate the stylized radiance to the parent's indirect record     (kernel_propagate :973-998)
 // then kernel_splat + kernel_update_film (:428-506) as one per-pixel deterministic kernel.
 #include <hipcub/hipcub.hpp>
+#include <cmath>
 #include "common.hpp"
 #include "scene.hpp"
 
@@ -228,7 +229,7 @@ __global__ void w_ctab_clear_kernel(uint4* ctab, uint32_t slots) {
 
 // per-path payload of level d (indirect record from the previous propagation)
 __global__ void w_pack_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, const float* ind,
-                              const uint8_t* new_flags, NbrRec* pk) {
+                              const uint8_t* new_flags, const uint16_t* material, NbrRec* pk) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = pts[t];
@@ -239,7 +240,7 @@ __global__ void w_pack_kernel(const float* rec, uint64_t N, const uint32_t* pts,
     r.iirr = make_float4(ind[3 * N + i], ind[4 * N + i], ind[5 * N + i], __uint_as_float((uint32_t)new_flags[i]));
     r.ns_wox = make_float4(pl(rec, N, 3, i), pl(rec, N, 4, i), pl(rec, N, 5, i), pl(rec, N, 9, i));
     r.ng_woy = make_float4(pl(rec, N, 6, i), pl(rec, N, 7, i), pl(rec, N, 8, i), pl(rec, N, 10, i));
-    r.woz = make_float4(pl(rec, N, 11, i), 0.0f, 0.0f, 0.0f);
+    r.woz = make_float4(pl(rec, N, 11, i), __uint_as_float((uint32_t)material[i]), 0.0f, 0.0f);
     pk[i] = r;
 }
 
@@ -264,6 +265,8 @@ struct FilterArgs {
     const nlpf_style_node* nodes;
     const int32_t* roots;
     uint32_t n_materials;
+    const WBsdf* bsdf;  // BSDF by material (n_bsdf entries; diffuse beyond); read by the GLOSSY kernels
+    uint32_t n_bsdf;
     int depth;
     int smis, flag_check, stylize;
     float* rad_out;     // [N][3] level radiance (v.radiance)
@@ -276,7 +279,65 @@ struct FilterArgs {
     double* smis_sum;  // [N] sum over the list of cos_m(wi_j), by reused vertex j
 };
 
+// The centre vertex's BSDF for a neighbour's direction: the pdf, and f |cos_i| per channel with
+// the base colour R included. Without GLOSSY every material is diffuse.
+template <bool GLOSSY>
+struct CentreBsdf {
+    float3 R, ns, ng, wo;
+    WBsdf b;
+    GlossCentre gc;
+    __device__ __forceinline__ CentreBsdf(const FilterArgs& a, uint32_t c, float3 R_, float3 ns_, float3 ng_,
+                                          float3 wo_)
+        : R(R_), ns(ns_), ng(ng_), wo(wo_) {
+        b.kind = NLPF_BSDF_DIFFUSE;
+        if (GLOSSY) {
+            const uint32_t m = a.material[c];
+            if (m < a.n_bsdf) b = a.bsdf[m];
+            if (b.kind == NLPF_BSDF_PRINCIPLED) gc = gloss_centre(b, R);
+        }
+    }
+    __device__ __forceinline__ float eval(float3 wi, float3& fc) const {
+        if (!GLOSSY || b.kind == NLPF_BSDF_DIFFUSE) {
+            float fcos;
+            const float pdf = diffuse_eval(ns, ng, wo, wi, fcos);
+            fc = make_float3(R.x * fcos, R.y * fcos, R.z * fcos);
+            return pdf;
+        }
+        if (b.kind == NLPF_BSDF_PRINCIPLED) return principled_eval(b, gc, R, ns, ng, wo, wi, fc);
+        fc = make_float3(0.0f, 0.0f, 0.0f);  // emission only
+        return 0.0f;
+    }
+};
+
+// A reused neighbour's SMIS record (and, for a principled one, its SmisGloss part): an
+// emission-only neighbour has pdf_m = 0 for every direction (its wo bit cleared)
+template <bool GLOSSY>
+__device__ __forceinline__ SmisRec smis_rec_of(const FilterArgs& a, const NbrRec& o, SmisGloss& g) {
+    SmisRec r = smis_rec(o);
+    if (GLOSSY) {
+        const uint32_t m = __float_as_uint(o.woz.y);
+        if (m < a.n_bsdf) {
+            const WBsdf b = a.bsdf[m];
+            if (b.kind == NLPF_BSDF_PRINCIPLED) {
+                g = smis_gloss(o, b, r.co);
+                r.bits |= SMIS_GLOSSY;
+            } else if (b.kind == NLPF_BSDF_EMISSION) {
+                r.bits &= ~SMIS_WO_OK;
+            }
+        }
+    }
+    return r;
+}
+template <bool GLOSSY>
+__device__ __forceinline__ double smis_term(double sum, const SmisRec& r, const SmisGloss& g, uint32_t bits, float3 wi,
+                                            double wx, double wy, double wz) {
+    if (GLOSSY && (bits & SMIS_GLOSSY)) return smis_add_gloss(sum, r, g, bits, wi, wx, wy, wz);
+    if (bits & SMIS_SAME) return smis_add_same(sum, r, bits, wi, wx, wy, wz);
+    return smis_add(sum, r, bits, wi, wx, wy, wz);
+}
+
 // Filter vertex tl of the launch on one thread; returns its candidate count and list length.
+template <bool GLOSSY>
 __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32_t tl, uint32_t& cnt_out,
                                                      uint32_t& used_out) {
     const uint64_t N = a.N;
@@ -291,6 +352,7 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
     const uint32_t mycluster = a.cluster_id ? a.cluster_id[c] : 0u;
     Pcg32 rng = a.use_rng ? a.rng[tl] : Pcg32{0ull, 1ull};
     uint32_t* buf = a.nbuf + (uint64_t)tl * a.stride;
+    const CentreBsdf<GLOSSY> cb(a, c, R, ns, ng, wo);
 
     float3 q = p;
     if (a.cluster_id) q = make_float3(pl(a.rec, N, 0, mycluster), pl(a.rec, N, 1, mycluster), pl(a.rec, N, 2, mycluster));
@@ -364,29 +426,31 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
         const NbrRec& r = a.pk[j];
         const float4 dw = r.dwi_pdf;
         if (dw.w > 0.0f) {
-            float fcos;
-            const float bpdf = diffuse_eval(ns, ng, wo, make_float3(dw.x, dw.y, dw.z), fcos);
+            float3 fc;
+            const float bpdf = cb.eval(make_float3(dw.x, dw.y, dw.z), fc);
             const float4 di = r.dirr_mis;
             const float w = dw.w / (dw.w + bpdf);  // mis_weight, balance heuristic (:305-316)
-            const float sc = fcos * di.w * w / dw.w * inv_used;
-            kahan_add(dsum[0], dc[0], R.x * sc * di.x);
-            kahan_add(dsum[1], dc[1], R.y * sc * di.y);
-            kahan_add(dsum[2], dc[2], R.z * sc * di.z);
+            const float sc = di.w * w / dw.w * inv_used;
+            kahan_add(dsum[0], dc[0], fc.x * sc * di.x);
+            kahan_add(dsum[1], dc[1], fc.y * sc * di.y);
+            kahan_add(dsum[2], dc[2], fc.z * sc * di.z);
         }
         const float4 iw = r.iwi_mis;
         const float3 wi = make_float3(iw.x, iw.y, iw.z);
         if (sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z) == 0.0f) continue;
-        float fcos;
-        const float bpdf = diffuse_eval(ns, ng, wo, wi, fcos);
+        float3 fc;
+        const float bpdf = cb.eval(wi, fc);
         const float4 ir = r.iirr;
-        const float3 reused = make_float3(R.x * fcos * ir.x * iw.w, R.y * fcos * ir.y * iw.w, R.z * fcos * ir.z * iw.w);
+        const float3 reused = make_float3(fc.x * ir.x * iw.w, fc.y * ir.y * iw.w, fc.z * ir.z * iw.w);
         if (a.smis) {
-            // sum_m pdf_m(wi) over every reused neighbour m, in order, in FP64
+            // sum_m pdf_m(wi) over every reused neighbour m, in order, in FP64 (in units of
+            // pi * pdf: a diffuse term is |cos_i|)
             const double wx = wi.x, wy = wi.y, wz = wi.z;
             double sum_cos = 0.0;
             for (uint32_t m = 0; m < used; ++m) {
-                const SmisRec r = smis_rec(a.pk[get(m)]);
-                sum_cos = smis_add(sum_cos, r, r.bits, wi, wx, wy, wz);
+                SmisGloss g;
+                const SmisRec r = smis_rec_of<GLOSSY>(a, a.pk[get(m)], g);
+                sum_cos = smis_term<GLOSSY>(sum_cos, r, g, r.bits, wi, wx, wy, wz);
             }
             const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);
             kahan_add(isum[0], ic[0], reused.x / sum_pdf);
@@ -415,10 +479,11 @@ __device__ __forceinline__ void filter_vertex_thread(const FilterArgs& a, uint32
 // One thread per vertex: for small neighbour sets (a few per vertex), where a wavefront per
 // vertex would leave most lanes idle. The neighbour and SMIS-pair counters are summed over the
 // wavefront first: one atomic per wavefront and counter.
+template <bool GLOSSY>
 __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
     const uint32_t tl = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t cnt = 0, used = 0;
-    if (tl < a.count) filter_vertex_thread(a, tl, cnt, used);
+    if (tl < a.count) filter_vertex_thread<GLOSSY>(a, tl, cnt, used);
     unsigned long long nc = cnt, np = a.smis ? (unsigned long long)used * used : 0ull;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -442,9 +507,11 @@ __global__ void __launch_bounds__(256) w_filter_thread_kernel(FilterArgs a) {
 //            pdf_m(wi_k) over them in order, in FP64, with a wave-uniform index m;
 //   reduce   per-lane Kahan sums, then a fixed shuffle tree (deterministic).
 constexpr uint32_t WF_CAP = 1024;
+template <bool GLOSSY>
 __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     __shared__ uint32_t lbuf[4][WF_CAP];
     __shared__ SmisRec sbuf[4][64];
+    __shared__ SmisGloss gsbuf[GLOSSY ? 4 : 1][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t tl = blockIdx.x * 4 + wv;
     if (tl >= a.count) return;  // uniform per wave; no workgroup barrier below
@@ -462,6 +529,8 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     uint32_t* gbuf = a.nbuf + (uint64_t)tl * a.stride;
     uint32_t* lb = lbuf[wv];
     SmisRec* sb = sbuf[wv];
+    SmisGloss* sg = gsbuf[GLOSSY ? wv : 0];
+    const CentreBsdf<GLOSSY> cb(a, c, R, ns, ng, wo);
     auto put = [&](uint32_t i, uint32_t j) {
         if (i < WF_CAP)
             lb[i] = j;
@@ -550,6 +619,24 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
     const uint32_t used = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cnt < a.K ? cnt : a.K));
     const float inv_used = used ? 1.0f / (float)used : 0.0f;
     auto get = [&](uint32_t i) -> uint32_t { return i < WF_CAP ? lb[i] : gbuf[i]; };
+    // stage the SMIS records of list entries m0 .. m0+mn-1 in LDS (lane m holds entry m0 + m)
+    auto stage = [&](uint32_t m0, uint32_t mn) {
+        if ((uint32_t)lane < mn) {
+            SmisGloss g;
+            sb[lane] = smis_rec_of<GLOSSY>(a, a.pk[get(m0 + (uint32_t)lane)], g);
+            if (GLOSSY) sg[lane] = g;
+        }
+    };
+    // sum + pi * pdf_m(wi) over the staged records 0 .. mn-1, in order; records are broadcast
+    // from LDS, so their flags are wave-uniform
+    auto smis_chunk = [&](double sum, uint32_t mn, float3 wi, double wx, double wy, double wz) {
+#pragma unroll 4
+        for (uint32_t m = 0; m < mn; ++m) {
+            const uint32_t bits = (uint32_t)__builtin_amdgcn_readfirstlane((int)sb[m].bits);
+            sum = smis_term<GLOSSY>(sum, sb[m], sg[GLOSSY ? m : 0], bits, wi, wx, wy, wz);
+        }
+        return sum;
+    };
     // With clustering, an uncapped list (cnt <= K) is every member of the vertex's cluster with
     // its flags, in the walk order around the cluster seed: each vertex j of the list has this
     // same list. Its first entry sums pdf_m(wi_j) over it once for every j (in the order used
@@ -573,18 +660,10 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(used - m0 < 64 ? used - m0 : 64));
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 __builtin_amdgcn_wave_barrier();
-                if ((uint32_t)lane < mn) sb[lane] = smis_rec(a.pk[get(m0 + (uint32_t)lane)]);
+                stage(m0, mn);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 __builtin_amdgcn_wave_barrier();
-#pragma unroll 4
-                for (uint32_t m = 0; m < mn; ++m) {
-                    // records are broadcast from LDS: their flags are wave-uniform
-                    const uint32_t bits = (uint32_t)__builtin_amdgcn_readfirstlane((int)sb[m].bits);
-                    if (bits & SMIS_SAME)
-                        sum_cos = smis_add_same(sum_cos, sb[m], bits, wi, wx, wy, wz);
-                    else
-                        sum_cos = smis_add(sum_cos, sb[m], bits, wi, wx, wy, wz);
-                }
+                sum_cos = smis_chunk(sum_cos, mn, wi, wx, wy, wz);
             }
             if (k < used) a.smis_sum[j] = sum_cos;
         }
@@ -606,23 +685,23 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
             const NbrRec& r = a.pk[j];
             const float4 dw = r.dwi_pdf;
             if (dw.w > 0.0f) {
-                float fcos;
-                const float dpdf = diffuse_eval(ns, ng, wo, make_float3(dw.x, dw.y, dw.z), fcos);
+                float3 fc;
+                const float dpdf = cb.eval(make_float3(dw.x, dw.y, dw.z), fc);
                 const float4 di = r.dirr_mis;
                 const float w = dw.w / (dw.w + dpdf);  // mis_weight, balance heuristic (:305-316)
-                const float sc = fcos * di.w * w / dw.w * inv_used;
-                kahan_add(dsum[0], dc[0], R.x * sc * di.x);
-                kahan_add(dsum[1], dc[1], R.y * sc * di.y);
-                kahan_add(dsum[2], dc[2], R.z * sc * di.z);
+                const float sc = di.w * w / dw.w * inv_used;
+                kahan_add(dsum[0], dc[0], fc.x * sc * di.x);
+                kahan_add(dsum[1], dc[1], fc.y * sc * di.y);
+                kahan_add(dsum[2], dc[2], fc.z * sc * di.z);
             }
             const float4 iw = r.iwi_mis;
             wi = make_float3(iw.x, iw.y, iw.z);
             if (sqrtf(wi.x * wi.x + wi.y * wi.y + wi.z * wi.z) != 0.0f) {
                 ind_ok = true;
-                float fcos;
-                bpdf = diffuse_eval(ns, ng, wo, wi, fcos);
+                float3 fc;
+                bpdf = cb.eval(wi, fc);
                 const float4 ir = r.iirr;
-                reused = make_float3(R.x * fcos * ir.x * iw.w, R.y * fcos * ir.y * iw.w, R.z * fcos * ir.z * iw.w);
+                reused = make_float3(fc.x * ir.x * iw.w, fc.y * ir.y * iw.w, fc.z * ir.z * iw.w);
             }
         }
         if (a.smis) {
@@ -638,18 +717,10 @@ __global__ void __launch_bounds__(256) w_filter_wave_kernel(FilterArgs a) {
                     // every lane has read the previous chunk before it is overwritten
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                     __builtin_amdgcn_wave_barrier();
-                    if ((uint32_t)lane < mn) sb[lane] = smis_rec(a.pk[get(m0 + (uint32_t)lane)]);
+                    stage(m0, mn);
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                     __builtin_amdgcn_wave_barrier();
-#pragma unroll 4
-                    for (uint32_t m = 0; m < mn; ++m) {
-                    // records are broadcast from LDS: their flags are wave-uniform
-                    const uint32_t bits = (uint32_t)__builtin_amdgcn_readfirstlane((int)sb[m].bits);
-                    if (bits & SMIS_SAME)
-                        sum_cos = smis_add_same(sum_cos, sb[m], bits, wi, wx, wy, wz);
-                    else
-                        sum_cos = smis_add(sum_cos, sb[m], bits, wi, wx, wy, wz);
-                }
+                    sum_cos = smis_chunk(sum_cos, mn, wi, wx, wy, wz);
                 }
             }
             const float sum_pdf = (float)(sum_cos * 0.318309886183790671538);
@@ -1100,6 +1171,59 @@ static uint32_t slot_max(const uint32_t* v) {
 
 using namespace nlpf;
 
+// Validate a material BSDF table (the principled subset that needs no albedo table) and keep
+// its device form: alpha = max(roughness^2, 1e-4) in FP32 as from_roughness forms it
+extern "C" int nlpf_set_bsdfs(nlpf_handle* h, const nlpf_bsdf* table, uint32_t n) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    if (n > NLPF_STY_MAX_MATERIALS) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "too many materials (max 65536)");
+    if (n && !table) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "NULL BSDF table");
+    std::vector<nlpf_bsdf> hv(table, table + n);
+    std::vector<WBsdf> dv(n);
+    int nondiffuse = 0;
+    for (uint32_t m = 0; m < n; ++m) {
+        const nlpf_bsdf& b = hv[m];
+        const std::string at = " (material " + std::to_string(m) + ")";
+        if (b.reserved != 0) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_bsdf.reserved must be 0" + at);
+        if (b.kind > NLPF_BSDF_EMISSION) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "unknown BSDF kind" + at);
+        WBsdf w{};
+        w.kind = b.kind;
+        if (b.kind == NLPF_BSDF_PRINCIPLED) {
+            const float v[] = {b.metallic, b.roughness, b.specular_tint[0], b.specular_tint[1], b.specular_tint[2],
+                               b.specular_ior_level, b.transmission, b.coat_weight};
+            for (float x : v)
+                if (!std::isfinite(x)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "non-finite principled parameter" + at);
+            if (!(b.metallic >= 0.0f && b.metallic <= 1.0f))
+                NLPF_FAIL(h, NLPF_E_INVALID_ARG, "metallic must lie in [0, 1]" + at);
+            if (!(b.roughness >= 0.0f)) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "roughness must be >= 0" + at);
+            if (b.coat_weight != 0.0f)
+                NLPF_FAIL(h, NLPF_E_UNSUPPORTED,
+                          "principled coat (coat_weight != 0) needs the reference's precomputed albedo table" + at);
+            if (b.metallic < 1.0f - MIX_EPS && (b.specular_ior_level != 0.0f || b.transmission != 0.0f))
+                NLPF_FAIL(h, NLPF_E_UNSUPPORTED,
+                          "principled with metallic < 1 needs specular_ior_level 0 and transmission 0 (its "
+                          "specular layer reads the reference's precomputed albedo table)" + at);
+            w.metallic = b.metallic;
+            const float r2 = b.roughness * b.roughness;
+            w.alpha = r2 > 1e-4f ? r2 : 1e-4f;
+            for (int c = 0; c < 3; ++c) w.tint[c] = b.specular_tint[c];
+        }
+        if (b.kind != NLPF_BSDF_DIFFUSE) nondiffuse = 1;
+        dv[m] = w;
+    }
+    NLPF_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->d_bsdf) NLPF_HIP(h, hipFree(h->d_bsdf));
+    h->d_bsdf = nullptr;
+    h->h_bsdf.clear();
+    h->bsdf_nondiffuse = 0;
+    if (n) {
+        NLPF_HIP(h, hipMalloc(&h->d_bsdf, sizeof(WBsdf) * n));
+        NLPF_HIP(h, hipMemcpy(h->d_bsdf, dv.data(), sizeof(WBsdf) * n, hipMemcpyHostToDevice));
+    }
+    h->h_bsdf = hv;
+    h->bsdf_nondiffuse = nondiffuse;
+    return NLPF_OK;
+}
+
 extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const float* rec, const uint8_t* flags,
                               const uint8_t* new_flags, const uint16_t* material, const int32_t* path_depth,
                               float* film, float* radiance_out) {
@@ -1204,7 +1328,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                new_flags + (uint64_t)d * N, g, w.pos_sorted, w.flags_sorted,
                                w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST);
             hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind,
-                               new_flags + (uint64_t)d * N, w.pk);
+                               new_flags + (uint64_t)d * N, material + (uint64_t)d * N, w.pk);
             if ((rc = check_launch(h, "world grid"))) return rc;
             const uint32_t K = cfg.reuse_at((uint32_t)d);
             if (cfg.cluster) {
@@ -1316,6 +1440,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     fa.nodes = h->d_nodes;
                     fa.roots = h->d_roots;
                     fa.n_materials = (uint32_t)h->h_roots.size();
+                    fa.bsdf = (const WBsdf*)h->d_bsdf;
+                    fa.n_bsdf = (uint32_t)h->h_bsdf.size();
                     fa.depth = d;
                     fa.smis = cfg.smis;
                     fa.flag_check = !cfg.disable_flag_check;
@@ -1325,10 +1451,17 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     fa.pair_count = w.nbr + (uint64_t)(32 + d) * NBR_SLOTS;
                     fa.smis_phase = phase;
                     fa.smis_sum = w.smis_sum;
-                    if (wave_kernel)
-                        hipLaunchKernelGGL(w_filter_wave_kernel, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
+                    // materials that are not all diffuse take the kernels with the principled and
+                    // emission-only BSDFs (more LDS and registers)
+                    const bool glossy = h->bsdf_nondiffuse != 0;
+                    if (wave_kernel && glossy)
+                        hipLaunchKernelGGL(w_filter_wave_kernel<true>, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
+                    else if (wave_kernel)
+                        hipLaunchKernelGGL(w_filter_wave_kernel<false>, dim3((cnt + 3) / 4), dim3(256), 0, st, fa);
+                    else if (glossy)
+                        hipLaunchKernelGGL(w_filter_thread_kernel<true>, grid1(cnt), dim3(256), 0, st, fa);
                     else
-                        hipLaunchKernelGGL(w_filter_thread_kernel, grid1(cnt), dim3(256), 0, st, fa);
+                        hipLaunchKernelGGL(w_filter_thread_kernel<false>, grid1(cnt), dim3(256), 0, st, fa);
                 }
             }
             if ((rc = check_launch(h, "world filter"))) return rc;

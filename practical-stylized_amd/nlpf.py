@@ -39,6 +39,24 @@ STYLE_NODE_DTYPE = np.dtype([
 ])
 assert STYLE_NODE_DTYPE.itemsize == 256
 
+# nlpf_bsdf, 40 bytes (include/nlpf.h): a material's BSDF in world mode
+BSDF_DTYPE = np.dtype([
+    ("kind", "<u4"), ("metallic", "<f4"), ("roughness", "<f4"), ("specular_tint", "<f4", (3,)),
+    ("specular_ior_level", "<f4"), ("transmission", "<f4"), ("coat_weight", "<f4"), ("reserved", "<u4"),
+])
+assert BSDF_DTYPE.itemsize == 40
+BSDF_DIFFUSE, BSDF_PRINCIPLED, BSDF_EMISSION = 0, 1, 2
+
+
+def bsdf_table(entries):
+    """BSDF table from dicts ({"kind": ..., "metallic": ..., ...}; missing fields 0, tint 1)."""
+    t = np.zeros(len(entries), BSDF_DTYPE)
+    t["specular_tint"] = 1.0
+    for i, e in enumerate(entries):
+        for k, v in e.items():
+            t[i][k] = v
+    return t
+
 # defaults of lm_path_filtering::Config (lm_path_filtering.rs:280-298) + screen-mode fields
 DEFAULT_CONFIG = {
     "max_depth": 7, "repeats": 1, "branch": 1, "radius_scale": [0.1, 0.05],
@@ -168,6 +186,8 @@ _SIGS = [
     ("nlpf_write_world_scene", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_GenDesc), C.c_void_p,
                                          C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("nlpf_pcg32_fill", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p]),
+    ("nlpf_set_bsdfs", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("nlpf_bsdfs_from_scene", C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_void_p]),
     ("nlpf_reservoir_streams", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p]),
 ]
 
@@ -248,6 +268,23 @@ def style_from_scene(scene, materials=None, capacity=1024):
     if rc != NLPF_OK:
         raise NlpfError(rc, lib.nlpf_last_error(None).decode())
     return nodes[:n.value].copy(), roots[:len(materials)].copy(), list(materials)
+
+
+def bsdfs_from_scene(scene, materials):
+    """BSDF table (BSDF_DTYPE) of the named materials of a reference scene file (path or JSON
+    text); raises NlpfError naming the first material outside the supported kinds. Needs no GPU."""
+    lib = load()
+    if isinstance(scene, (bytes, str)) and not str(scene).lstrip().startswith("{"):
+        with open(scene, "rb") as f:
+            text = f.read()
+    else:
+        text = scene.encode() if isinstance(scene, str) else bytes(scene)
+    names = (C.c_char_p * max(1, len(materials)))(*[m.encode() for m in materials])
+    out = np.zeros(max(1, len(materials)), BSDF_DTYPE)
+    rc = lib.nlpf_bsdfs_from_scene(text, names, len(materials), out.ctypes.data_as(C.c_void_p))
+    if rc != NLPF_OK:
+        raise NlpfError(rc, lib.nlpf_last_error(None).decode())
+    return out[:len(materials)].copy()
 
 
 SCENE_MATERIALS = ["floor", "ceiling", "back", "left", "right", "sphere0", "sphere1", "light"]
@@ -374,6 +411,12 @@ class Handle:
         roots = np.ascontiguousarray(roots, dtype=np.int32)
         self._check(self.lib.nlpf_set_style(self.h, nodes.ctypes.data_as(C.c_void_p), len(nodes),
                                             roots.ctypes.data_as(C.c_void_p), len(roots)))
+
+    def set_bsdfs(self, table):
+        """World-mode BSDF of each material (BSDF_DTYPE entries; materials beyond are diffuse)."""
+        table = np.ascontiguousarray(table, dtype=BSDF_DTYPE)
+        self._check(self.lib.nlpf_set_bsdfs(self.h, table.ctypes.data_as(C.c_void_p) if len(table) else None,
+                                            len(table)))
 
     def stylize(self, rgb, depth=0, flags=None, material=None, pos=None):
         """Stylize host rgb[n,3] on the GPU; depth is an int or an array of n depths."""

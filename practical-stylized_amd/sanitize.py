@@ -58,7 +58,7 @@ def build_library():
 
 def build_oracle():
     src = [os.path.join(ROOT, "oracle", f) for f in ("oracle_style.c", "oracle_screen.c", "oracle_world.c",
-                                                      "oracle_grad.c")]
+                                                      "oracle_grad.c", "oracle_bsdf.c")]
     lib = os.path.join(OUT, "liboracle.so")
     subprocess.run([CLANG, "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
                     "-fno-sanitize-recover=undefined", "-shared-libasan", "-fopenmp", "-fPIC", "-std=c11",

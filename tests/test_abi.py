@@ -141,6 +141,22 @@ def test_python_mirror_structs_match_header(tmp_path):
             assert next(vals) == getattr(cls, fname).offset, (cname, fname)
 
 
+def test_bsdf_table_layouts_match_header(tmp_path):
+    """The Python mirror's BSDF_DTYPE and the oracle's or_bsdf have nlpf_bsdf's size and offsets."""
+    import oracle as O
+    fields = list(N.BSDF_DTYPE.names)
+    src = tmp_path / "bsdf.c"
+    src.write_text('#include "nlpf.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){' +
+                   'printf("%zu\\n", sizeof(nlpf_bsdf));' +
+                   "".join(f'printf("%zu\\n", offsetof(nlpf_bsdf, {f}));' for f in fields) + "return 0;}\n")
+    exe = tmp_path / "bsdf"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert vals[0] == N.BSDF_DTYPE.itemsize == O.BSDF_DTYPE.itemsize
+    for f, off in zip(fields, vals[1:]):
+        assert N.BSDF_DTYPE.fields[f][1] == off == O.BSDF_DTYPE.fields[f][1], f
+
+
 def header_prototypes():
     """{name: (return type, [parameter types])} from include/nlpf.h (comments and names
     stripped; pointer parameters reduced to "ptr")."""

@@ -18,10 +18,12 @@ AMIN = [-0.5, -0.5, -0.5]
 AMAX = [0.5, 0.5, 0.5]
 
 
-def run_world(cfg, W, H, branch, L, nodes=None, roots=(), seed=3):
+def run_world(cfg, W, H, branch, L, nodes=None, roots=(), seed=3, bsdfs=None):
     with N.Handle(cfg) as h:
         if nodes is not None:
             h.set_style(nodes, roots)
+        if bsdfs is not None:
+            h.set_bsdfs(bsdfs)
         desc = h.world_desc(W, H, branch, L, AMIN, AMAX)
         recs = h.alloc_world(desc)
         h.generate_world_records(desc, recs, seed=seed, albedo_lo=0.2, albedo_hi=0.8)
@@ -31,9 +33,9 @@ def run_world(cfg, W, H, branch, L, nodes=None, roots=(), seed=3):
     return film, rad, host, stats
 
 
-def oracle_world(host, W, H, branch, L, cfg, nodes=None, roots=()):
+def oracle_world(host, W, H, branch, L, cfg, nodes=None, roots=(), bsdfs=None):
     return O.world_run(host["records"], host["flags"], host["new_flags"], host["material"], host["path_depth"],
-                       W, H, branch, L, AMIN, AMAX, cfg, nodes=nodes, roots=roots, want_radiance=True)
+                       W, H, branch, L, AMIN, AMAX, cfg, nodes=nodes, roots=roots, want_radiance=True, bsdfs=bsdfs)
 
 
 def styles():
@@ -49,7 +51,7 @@ def styles():
     return nodes, roots
 
 
-def sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad, trials=3, seed=0):
+def sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad, trials=3, seed=0, bsdfs=None):
     """Per-vertex and per-pixel relative change of the FP64 oracle under 1-ulp input noise
     (planes 3..27: normals, directions, albedo, emission, direct light, pdfs, MIS weights)."""
     rng = np.random.default_rng(seed)
@@ -61,7 +63,7 @@ def sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad, trials=3,
         r = rec0.astype(np.float64)
         r[:, 3:] *= 1 + rng.uniform(-1, 1, r[:, 3:].shape) * 2.0 ** -24
         h2 = dict(host, records=r.astype(np.float32).ravel())
-        f, rad, _ = oracle_world(h2, W, H, branch, L, cfg, nodes, roots)
+        f, rad, _ = oracle_world(h2, W, H, branch, L, cfg, nodes, roots, bsdfs=bsdfs)
         for d in range(L):
             s_rad[d] = np.maximum(s_rad[d], np.nan_to_num(pixel_rel_err(rad[d], orad[d])))
         s_film = np.maximum(s_film, pixel_rel_err(f.reshape(-1, 3), ofilm.reshape(-1, 3)))
@@ -327,3 +329,94 @@ def toon_edge_world(pre, w=0.25, tol=1e-5):
     enc = np.where(lum > 1, lum / w, np.where(lum <= 0.0031308, lum * 12.92,
                                                1.055 * np.power(np.maximum(lum, 0), 1 / 2.4) - 0.055) / w)
     return np.abs(enc - np.round(enc)) < tol * np.maximum(1, np.abs(enc))
+
+
+def glossy_table():
+    """BSDFs of the box scene's materials (floor, ceiling, back, left, right, sphere0, sphere1,
+    light): the principled subset the filter evaluates exactly, from a near mirror (roughness 0,
+    alpha 1e-4, as the reference scene's Gold) to diffuse / metal mixtures"""
+    return N.bsdf_table([
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 0.5, "roughness": 0.4},
+        {},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.3, "specular_tint": (1.0, 0.8, 0.6)},
+        {},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.0},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.2, "specular_tint": (0.9, 0.9, 1.0)},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 0.3, "roughness": 0.6},
+        {"kind": N.BSDF_EMISSION},
+    ])
+
+
+@pytest.mark.parametrize("kernel", ["thread", "wave"])
+@pytest.mark.parametrize("smis", [True, False])
+def test_world_glossy_parity(smis, kernel):
+    """principled (metal and diffuse / metal) materials at the centre and, for SMIS, at every
+    reused neighbour, against the oracle's restatement of the reference's BSDFs"""
+    W, H, branch, L = 48, 40, 4, 4
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "smis": smis, "cluster": False,
+           "radius_scale": [2.0, 3.0], "reuse": [1 << 16], "world_kernel": kernel}
+    nodes, roots = styles()
+    bs = glossy_table()
+    film, rad, host, st = run_world(cfg, W, H, branch, L, nodes, roots, bsdfs=bs)
+    ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots, bsdfs=bs)
+    assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    sens = sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad, bsdfs=bs)
+    check(film, rad, ofilm, orad, host, L, sens=sens)
+    # the table changes the result materially (so the test exercises the glossy lobes)
+    dfilm = oracle_world(host, W, H, branch, L, cfg, nodes, roots)[0]
+    assert np.abs(dfilm - ofilm).max() > 1e-2 * np.abs(ofilm).max()
+    mat = host["material"].reshape(L, -1)
+    assert (mat[host["path_depth"][None, :] >= np.arange(L)[:, None]] == 4).any()  # the roughness-0 wall
+
+
+@pytest.mark.parametrize("kernel", ["thread", "wave"])
+def test_world_glossy_clusters(kernel):
+    """the reference's default settings (clustering, reuse [128, 64], SMIS) with glossy
+    materials: on the wave kernel the SMIS denominators are shared within each cluster's list"""
+    W, H, branch, L = 40, 32, 4, 4
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "radius_scale": [3.0, 4.0], "world_kernel": kernel}
+    nodes, roots = styles()
+    bs = glossy_table()
+    film, rad, host, st = run_world(cfg, W, H, branch, L, nodes, roots, bsdfs=bs)
+    ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots, bsdfs=bs)
+    assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+    sens = sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad, bsdfs=bs)
+    check(film, rad, ofilm, orad, host, L, sens=sens)
+
+
+def test_world_diffuse_tables_are_the_default():
+    """a table of diffuse entries runs the plain kernels (bit for bit the run without a table);
+    principled with metallic 0 and no specular layer is that same diffuse BSDF"""
+    W, H, branch, L = 32, 24, 2, 3
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "cluster": False, "radius_scale": [2.5],
+           "reuse": [1 << 16]}
+    film0, rad0, host, _ = run_world(cfg, W, H, branch, L)
+    film1, rad1, _, _ = run_world(cfg, W, H, branch, L, bsdfs=N.bsdf_table([{}] * 8))
+    assert np.array_equal(film0, film1) and np.array_equal(rad0, rad1)
+    film2, rad2, _, _ = run_world(cfg, W, H, branch, L,
+                                  bsdfs=N.bsdf_table([{"kind": N.BSDF_PRINCIPLED, "metallic": 0.0}] * 7))
+    for d in range(L):
+        m = host["path_depth"] >= d
+        assert pixel_rel_err(rad2[d][m], rad0[d][m]).max() < 1e-6
+    assert pixel_rel_err(film2.reshape(-1, 3), film0.reshape(-1, 3)).max() < 1e-6
+
+
+def test_set_bsdfs_rejects_what_needs_the_albedo_table():
+    bad = [
+        ({"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "coat_weight": 0.2}, 5, "coat"),
+        ({"kind": N.BSDF_PRINCIPLED, "metallic": 0.5, "specular_ior_level": 0.5}, 5, "specular_ior_level"),
+        ({"kind": N.BSDF_PRINCIPLED, "metallic": 0.0, "transmission": 1.0}, 5, "transmission"),
+        ({"kind": N.BSDF_PRINCIPLED, "metallic": 1.5}, 2, "metallic"),
+        ({"kind": N.BSDF_PRINCIPLED, "roughness": float("nan")}, 2, "non-finite"),
+        ({"kind": 3}, 2, "kind"),
+        ({"reserved": 1}, 2, "reserved"),
+    ]
+    with N.Handle({"type": "pf"}) as h:
+        for entry, code, msg in bad:
+            with pytest.raises(N.NlpfError) as e:
+                h.set_bsdfs(N.bsdf_table([{}, entry]))
+            assert e.value.code == code and msg in str(e.value) and "material 1" in str(e.value), entry
+        # metallic 1 ignores the specular layer and the transmission side
+        h.set_bsdfs(N.bsdf_table([{"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "specular_ior_level": 0.5,
+                                   "transmission": 1.0}]))
+        h.set_bsdfs(N.bsdf_table([]))

@@ -95,9 +95,11 @@ def make_records(rng, W, H, branch, L, same_normal=False, tight=False):
 
 
 def numpy_world(rec, depth, W, H, branch, L, radius_scale, smis, nodes=None, roots=(), flags=None, new_flags=None,
-                material=None, no_first_hit_stylization=True, clusters=None):
+                material=None, no_first_hit_stylization=True, clusters=None, bsdfs=None):
     """Independent restatement for cnt <= K (no reservoir) and no bucket collisions: the
-    neighbour set of a vertex is every depth-d vertex within the radius with equal flags."""
+    neighbour set of a vertex is every depth-d vertex within the radius with equal flags.
+    With bsdfs, each vertex's BSDF is its material's entry, evaluated by the oracle's
+    or_bsdf_eval (pinned on its own in test_oracle_bsdf.py)."""
     Np = W * H * branch
     ext = (AMAX - AMIN).astype(np.float32)
     avg = np.float32((ext[0] + ext[1] + ext[2]) / np.float32(3))
@@ -120,28 +122,34 @@ def numpy_world(rec, depth, W, H, branch, L, radius_scale, smis, nodes=None, roo
                 nb = pts[(dist[sa] <= radius) & (clusters[d, pts] == seed) & (nf[d, pts] == nf[d, c])]
             ns, ng, wo, R = r[3:6, c], r[6:9, c], r[9:12, c], r[12:15, c]
 
-            def ev(ns_, ng_, wo_, wi_):
+            def ev(ns_, ng_, wo_, wi_, R_, m_):
+                """(pdf, f |cos_i| per channel)"""
+                if bsdfs is not None:
+                    import oracle as O
+                    return O.bsdf_eval(bsdfs[m_] if m_ < len(bsdfs) else None, R_, ns_, ng_, wo_, wi_)
                 sg = lambda x: 1.0 if x > 0 else -1.0
                 fl = sg(ng_ @ ns_)
                 ok = sg(fl * (wo_ @ ns_)) * sg(wo_ @ ng_) > 0 and sg(fl * (wi_ @ ns_)) * sg(wi_ @ ng_) > 0
                 if not ok or not ((wo_ @ ns_) * (wi_ @ ns_) >= 0):
-                    return 0.0
-                return abs(wi_ @ ns_) / np.pi
+                    return 0.0, np.zeros(3)
+                p = abs(wi_ @ ns_) / np.pi
+                return p, R_ * p
 
+            mc = mat[d, c]
             dsum = np.zeros(3)
             isum = np.zeros(3)
             for j in nb:
                 pd = r[25, j]
                 if pd > 0:
-                    b = ev(ns, ng, wo, r[19:22, j])
-                    dsum += R * b * r[22:25, j] * r[26, j] * (pd / (pd + b)) / pd / len(nb)
+                    b, fc = ev(ns, ng, wo, r[19:22, j], R, mc)
+                    dsum += fc * r[22:25, j] * r[26, j] * (pd / (pd + b)) / pd / len(nb)
                 wi = ind[0:3, j]
                 if np.linalg.norm(wi) == 0:
                     continue
-                b = ev(ns, ng, wo, wi)
-                reused = R * b * ind[3:6, j] * ind[6, j]
+                b, fc = ev(ns, ng, wo, wi, R, mc)
+                reused = fc * ind[3:6, j] * ind[6, j]
                 if smis:
-                    s = sum(ev(r[3:6, k], r[6:9, k], r[9:12, k], wi) for k in nb)
+                    s = sum(ev(r[3:6, k], r[6:9, k], r[9:12, k], wi, r[12:15, k], mat[d, k])[0] for k in nb)
                     isum += reused / s
                 elif b > 0:
                     isum += reused / b / len(nb)
@@ -159,6 +167,30 @@ def numpy_world(rec, depth, W, H, branch, L, radius_scale, smis, nodes=None, roo
                 ind[6, c] = r[18, c]
                 ind[7, c] = rec[d - 1, 27, c]
     return rad
+
+
+@pytest.mark.parametrize("smis", [True, False])
+def test_glossy_materials_match_numpy_restatement(oracle, smis):
+    """each vertex's own material BSDF at the centre, each reused neighbour's own BSDF in the SMIS
+    denominators, RGB f for metals (conductor Fresnel per channel)"""
+    rng = np.random.default_rng(17 + smis)
+    W, H, branch, L = 6, 5, 2, 3
+    rec, depth = make_records(rng, W, H, branch, L)
+    Np = W * H * branch
+    z8 = np.zeros((L, Np), np.uint8)
+    mat = rng.integers(0, 4, (L, Np)).astype(np.uint16)
+    table = np.concatenate([oracle.bsdf(kind=oracle.BSDF_DIFFUSE),
+                            oracle.bsdf(metallic=1.0, roughness=0.3, specular_tint=(1.0, 0.8, 0.5)),
+                            oracle.bsdf(metallic=0.4, roughness=0.6)])  # material 3: no entry (diffuse)
+    cfg = {"cluster": False, "radius_scale": [8.0], "reuse": [1 << 20], "smis": smis, "no_first_hit_stylization": True}
+    film, rad, nbr = oracle.world_run(rec, z8, z8, mat, depth, W, H, branch, L, AMIN, AMAX, cfg,
+                                      want_radiance=True, bsdfs=table)
+    ref = numpy_world(rec, depth, W, H, branch, L, 8.0, smis, material=mat, bsdfs=table)
+    for d in range(L):
+        m = depth >= d
+        np.testing.assert_allclose(rad[d][m], ref[d][m], rtol=1e-12, atol=1e-14)
+    plain = oracle.world_run(rec, z8, z8, mat, depth, W, H, branch, L, AMIN, AMAX, cfg, want_radiance=True)[1]
+    assert np.abs(plain[0][depth >= 0] - rad[0][depth >= 0]).max() > 1e-3  # the table matters
 
 
 @pytest.mark.parametrize("smis", [True, False])
