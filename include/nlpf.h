@@ -414,8 +414,11 @@ int nlpf_write_image_host(const float* rgba, uint32_t width, uint32_t height, co
  *    normals.
  *  - Materials are named "mat<m>_<name>", so that they sort in material-id order. Each holds a
  *    diffuse BSDF of the generator's reflectance for `gen` (the light instead holds an emission
- *    node of colour 1 and strength 15), wrapped in the sty_* nodes lifted from the style table
- *    (nodes, roots); nlpf_style_from_scene reads the same table back. Hue shift, ramp and
+ *    node of colour 1 and strength 15), or, where the BSDF table (bsdfs, n_bsdfs; may be empty)
+ *    gives one, a principled node with that entry's parameters and the reflectance as base
+ *    colour, or an emission node of strength 0 for NLPF_BSDF_EMISSION; it is wrapped in the
+ *    sty_* nodes lifted from the style table (nodes, roots).  nlpf_style_from_scene and
+ *    nlpf_bsdfs_from_scene read the same tables back. Hue shift, ramp and
  *    depth-gated nodes have no reference counterpart: NLPF_E_UNSUPPORTED.
  *  - The camera is the generator's pinhole, in the reference's PerspectiveCamera conventions
  *    (camera/mod.rs:130-152). Rendering the file with a box pixel filter of radius 1 matches
@@ -424,7 +427,8 @@ int nlpf_write_image_host(const float* rgba, uint32_t width, uint32_t height, co
  * come from its Blender exporter). */
 int nlpf_write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
                            const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* material_roots,
-                           uint32_t n_materials, uint32_t sphere_segments);
+                           uint32_t n_materials, uint32_t sphere_segments, const nlpf_bsdf* bsdfs,
+                           uint32_t n_bsdfs);
 
 /* ---- random streams (host side, the same code the kernels run) ----
  * nlpf_pcg32_fill: n draws of Pcg32 new_seq_offset(seq, offset).gen_u32() (sampler/mod.rs:91-117).

@@ -1,6 +1,7 @@
 // Synthetic screen-mode records, FP16 packing and the standalone stylization kernel.
 #include "common.hpp"
 #include "scene.hpp"
+#include "world_bsdf.hpp"
 
 namespace nlpf {
 
@@ -198,6 +199,8 @@ struct WGenArgs {
     const nlpf_style_node* nodes;
     const int32_t* roots;
     uint32_t n_materials;
+    const WBsdf* bsdf;  // BSDF by material (nlpf_set_bsdfs); diffuse beyond n_bsdf
+    uint32_t n_bsdf;
 };
 
 __device__ __forceinline__ int mat_root(const WGenArgs& g, int m) {
@@ -206,6 +209,32 @@ __device__ __forceinline__ int mat_root(const WGenArgs& g, int m) {
 
 #define LIGHT_EMISSION NLPF_LIGHT_EMISSION
 #define LIGHT_AREA (4.0f * SceneConst::lh * SceneConst::lh)
+
+// TrowbridgeReitzDistribution::sample_wh with visible normals (microfacet.rs:118-141) in the
+// frame (t, b, n): the sampled half vector in world space
+__device__ __forceinline__ v3<float> ggx_sample_vndf(v3<float> t, v3<float> b, v3<float> n, v3<float> wo, float alpha,
+                                                     float u1, float u2) {
+    v3<float> w{alpha * dot(wo, t), alpha * dot(wo, b), dot(wo, n)};
+    w = w / m_sqrt(dot(w, w));
+    if (w.z < 0.0f) w = w * -1.0f;
+    v3<float> t1{1.0f, 0.0f, 0.0f};
+    if (w.z < 0.99999f) {
+        t1 = v3<float>{-w.y, w.x, 0.0f};  // (0, 0, 1) x w
+        t1 = t1 / m_sqrt(dot(t1, t1));
+    }
+    v3<float> t2{w.y * t1.z - w.z * t1.y, w.z * t1.x - w.x * t1.z, w.x * t1.y - w.y * t1.x};
+    t2 = t2 / m_sqrt(dot(t2, t2));
+    const float r = m_sqrt(u1), phi = 6.2831853f * u2;  // uniform_sample_disk (sampling.rs:5-9)
+    const float px = r * cosf(phi);
+    float py = r * sinf(phi);
+    const float hh = m_sqrt(1.0f - px * px), s = 0.5f * (1.0f + w.z);
+    py = hh + s * (py - hh);
+    const float pz = m_sqrt(m_max(0.0f, 1.0f - px * px - py * py));
+    const v3<float> nh = t1 * px + t2 * py + w * pz;
+    v3<float> hl{alpha * nh.x, alpha * nh.y, m_max(nh.z, 1e-6f)};
+    hl = hl / m_sqrt(dot(hl, hl));
+    return t * hl.x + b * hl.y + n * hl.z;
+}
 
 __global__ void gen_world_kernel(WGenArgs g) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -229,6 +258,9 @@ __global__ void gen_world_kernel(WGenArgs g) {
         const v3<float> wo{-dir.x, -dir.y, -dir.z};
         const int mat = hit.mat;
         const v3<float> R = material_albedo(mat, key, g.alo, g.ahi);
+        WBsdf bs{};
+        bs.kind = NLPF_BSDF_DIFFUSE;
+        if ((uint32_t)mat < g.n_bsdf) bs = g.bsdf[mat];
         // AreaLight::le, one-sided (light/area.rs:36-49); MIS against light sampling for d > 0
         float le = 0.0f, mis = 1.0f;
         if (mat == NLPF_MAT_LIGHT) {
@@ -250,12 +282,12 @@ __global__ void gen_world_kernel(WGenArgs g) {
         flags = new_flags;
         float direct[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // wi, irradiance, pdf, mis
         float bsdf_pdf = 0.0f;
+        // u0 selects the principled mixture's lobe (BsdfMixture::sample_wi, svm/surface/mod.rs:1595-1611)
         const float u0 = pcg_uniform(rng), u1 = pcg_uniform(rng), u2 = pcg_uniform(rng);
-        (void)u0;
         const float b1 = pcg_uniform(rng), b2 = pcg_uniform(rng);
         bool cont = false;
-        // the light is an emission-only surface: no lobe to sample, so the path ends there
-        if (d + 1 < g.L && mat != NLPF_MAT_LIGHT) {
+        // the light and emission-only materials have no lobe to sample: the path ends there
+        if (d + 1 < g.L && mat != NLPF_MAT_LIGHT && bs.kind != NLPF_BSDF_EMISSION) {
             // next-event estimation to the quad light (sample_light :1260-1316, area.rs:51-110)
             const v3<float> lp{-SceneConst::lh + 2.0f * SceneConst::lh * u1, SceneConst::ly,
                                -SceneConst::lh + 2.0f * SceneConst::lh * u2};
@@ -286,19 +318,36 @@ __global__ void gen_world_kernel(WGenArgs g) {
                     }
                 }
             }
-            // cosine BSDF sample on wo's side (diffuse.rs:42-55) and its pdf (SurfaceClosure::sample)
-            const float sr = m_sqrt(b1), phi = 6.2831853f * b2;
-            const float lx = sr * cosf(phi), ly = sr * sinf(phi);
-            float lz = m_sqrt(m_max(0.0f, 1.0f - lx * lx - ly * ly));
             v3<float> tt, bb;
             onb(ns, tt, bb);
-            v3<float> wi2 = tt * lx + bb * ly + ns * lz;
-            if (dot(wo, ns) * lz < 0.0f) wi2 = wi2 * -1.0f;
-            const float co = dot(wo, ns), ci = dot(wi2, ns);
-            auto sgn = [](float v) { return v > 0.0f ? 1.0f : -1.0f; };
-            const float fl = sgn(dot(ng, ns));
-            const bool valid = (sgn(fl * co) * sgn(dot(wo, ng)) > 0.0f) && (sgn(fl * ci) * sgn(dot(wi2, ng)) > 0.0f);
-            if (valid && co * ci >= 0.0f) bsdf_pdf = fabsf(ci) * 0.318309886f;
+            v3<float> wi2;
+            if (bs.kind == NLPF_BSDF_PRINCIPLED && u0 < bs.metallic) {
+                // the metal lobe: a visible normal, and wo reflected about it (mod.rs:1912-1924)
+                const v3<float> wh = ggx_sample_vndf(tt, bb, ns, wo, bs.alpha, b1, b2);
+                wi2 = wh * (2.0f * dot(wo, wh)) - wo;
+            } else {
+                // cosine BSDF sample on wo's side (diffuse.rs:42-55)
+                const float sr = m_sqrt(b1), phi = 6.2831853f * b2;
+                const float lx = sr * cosf(phi), ly = sr * sinf(phi);
+                float lz = m_sqrt(m_max(0.0f, 1.0f - lx * lx - ly * ly));
+                wi2 = tt * lx + bb * ly + ns * lz;
+                if (dot(wo, ns) * lz < 0.0f) wi2 = wi2 * -1.0f;
+            }
+            // the pdf of the sampled direction is the closure's evaluated pdf (SurfaceClosure::sample)
+            if (bs.kind == NLPF_BSDF_PRINCIPLED) {
+                const GlossCentre gc = gloss_centre(bs, make_float3(R.x, R.y, R.z));
+                float3 fc;
+                bsdf_pdf = principled_eval(bs, gc, make_float3(R.x, R.y, R.z), make_float3(ns.x, ns.y, ns.z),
+                                           make_float3(ng.x, ng.y, ng.z), make_float3(wo.x, wo.y, wo.z),
+                                           make_float3(wi2.x, wi2.y, wi2.z), fc);
+            } else {
+                const float co = dot(wo, ns), ci = dot(wi2, ns);
+                auto sgn = [](float v) { return v > 0.0f ? 1.0f : -1.0f; };
+                const float fl = sgn(dot(ng, ns));
+                const bool valid =
+                    (sgn(fl * co) * sgn(dot(wo, ng)) > 0.0f) && (sgn(fl * ci) * sgn(dot(wi2, ng)) > 0.0f);
+                if (valid && co * ci >= 0.0f) bsdf_pdf = fabsf(ci) * 0.318309886f;
+            }
             if (bsdf_pdf > 0.0f) {
                 const v3<float> ff = dot(ng, wi2) >= 0.0f ? ng : ng * -1.0f;
                 o = p + ff * 1e-4f;
@@ -348,6 +397,8 @@ extern "C" int nlpf_generate_world_records(nlpf_handle* h, const nlpf_world_desc
     g.nodes = h->d_nodes;
     g.roots = h->d_roots;
     g.n_materials = (uint32_t)h->h_roots.size();
+    g.bsdf = (const WBsdf*)h->d_bsdf;
+    g.n_bsdf = (uint32_t)h->h_bsdf.size();
     NLPF_HIP(h, hipMemsetAsync(records, 0, sizeof(float) * NLPF_WORLD_PLANES * g.N * g.L, h->stream));
     NLPF_HIP(h, hipMemsetAsync(flags, 0, g.N * g.L, h->stream));
     NLPF_HIP(h, hipMemsetAsync(new_flags, 0, g.N * g.L, h->stream));

@@ -286,13 +286,20 @@ using namespace nlpf;
 
 static int write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
                              const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* roots,
-                             uint32_t n_materials, uint32_t sphere_segments) {
+                             uint32_t n_materials, uint32_t sphere_segments, const nlpf_bsdf* bsdfs,
+                             uint32_t n_bsdfs) {
     auto fail = [](int code, const std::string& m) {
         set_global_error("nlpf_write_world_scene: " + m);
         return code;
     };
-    if (!path || !*path || !gen || (n_nodes && !nodes) || (n_materials && !roots))
+    if (!path || !*path || !gen || (n_nodes && !nodes) || (n_materials && !roots) || (n_bsdfs && !bsdfs))
         return fail(NLPF_E_INVALID_ARG, "NULL argument");
+    for (uint32_t m = 0; m < n_bsdfs && m < (uint32_t)NLPF_SCENE_MATERIALS; ++m) {
+        const nlpf_bsdf& b = bsdfs[m];
+        if (b.kind > NLPF_BSDF_EMISSION) return fail(NLPF_E_INVALID_ARG, "unknown BSDF kind");
+        if (m == NLPF_MAT_LIGHT && b.kind != NLPF_BSDF_EMISSION && b.kind != NLPF_BSDF_DIFFUSE)
+            return fail(NLPF_E_INVALID_ARG, "the light material stays an emission surface");
+    }
     if (width == 0 || height == 0) return fail(NLPF_E_INVALID_ARG, "width and height must be >= 1");
     if (sphere_segments < 4 || sphere_segments > 4096 || sphere_segments % 2)
         return fail(NLPF_E_INVALID_ARG, "sphere_segments must be even, 4..4096");
@@ -359,7 +366,62 @@ static int write_world_scene(const char* path, uint32_t width, uint32_t height, 
             const float a[3] = {R.x, R.y, R.z};
             lf.out.emplace_back("albedo", "{\"type\": \"rgb\", \"value\": " + vec3s(a) + ", \"colorspace\": \"srgb\"}");
             lf.out.emplace_back("albedo_s", "{\"type\": \"spectral_uplift\", \"rgb\": " + jref("albedo") + "}");
-            lf.out.emplace_back("bsdf", "{\"type\": \"diffuse\", \"color\": " + jref("albedo_s") + "}");
+            const nlpf_bsdf* b = (uint32_t)m < n_bsdfs ? &bsdfs[m] : nullptr;
+            if (b && b->kind == NLPF_BSDF_PRINCIPLED) {
+                // every input of the reference's principled node (shader.rs), the ones the
+                // closure does not read at Blender's defaults
+                auto flt = [&](const char* id, float v) {
+                    lf.out.emplace_back(id, "{\"type\": \"float\", \"value\": " + fnum(v) + "}");
+                };
+                auto col = [&](const char* id, const float c[3]) {
+                    const std::string raw = std::string(id) + "_rgb";
+                    lf.out.emplace_back(raw, "{\"type\": \"rgb\", \"value\": " + vec3s(c) + ", \"colorspace\": \"srgb\"}");
+                    lf.out.emplace_back(id, "{\"type\": \"spectral_uplift\", \"rgb\": " + jref(raw) + "}");
+                };
+                auto f3 = [&](const char* id, const float c[3]) {
+                    lf.out.emplace_back(id, "{\"type\": \"float3\", \"value\": " + vec3s(c) + "}");
+                };
+                const float one3[3] = {1, 1, 1}, zero3[3] = {0, 0, 0}, ssr[3] = {1.0f, 0.2f, 0.1f};
+                flt("p_metallic", b->metallic);
+                flt("p_roughness", b->roughness);
+                col("p_spec_tint", b->specular_tint);
+                flt("p_spec_level", b->specular_ior_level);
+                flt("p_transmission", b->transmission);
+                flt("p_coat_weight", b->coat_weight);
+                f3("p_zero3", zero3);
+                flt("p_zero", 0.0f);
+                flt("p_one", 1.0f);
+                flt("p_ior", 1.45f);
+                flt("p_coat_ior", 1.5f);
+                flt("p_coat_rough", 0.03f);
+                flt("p_ss_scale", 0.05f);
+                col("p_white", one3);
+                f3("p_ss_radius", ssr);
+                const auto r = [&](const char* id) { return jref(id); };
+                lf.out.emplace_back(
+                    "bsdf", std::string("{\"type\": \"principled\", \"preference\": \"mix\", \"base_color\": ") +
+                                r("albedo_s") + ", \"metallic\": " + r("p_metallic") + ", \"roughness\": " +
+                                r("p_roughness") + ", \"specular_tint\": " + r("p_spec_tint") +
+                                ", \"specular_ior_level\": " + r("p_spec_level") + ", \"transmission_weight\": " +
+                                r("p_transmission") + ", \"coat_weight\": " + r("p_coat_weight") +
+                                ", \"normal\": " + r("p_zero3") + ", \"coat_normal\": " + r("p_zero3") +
+                                ", \"tangent\": " + r("p_zero3") + ", \"alpha\": " + r("p_one") + ", \"ior\": " +
+                                r("p_ior") + ", \"coat_ior\": " + r("p_coat_ior") + ", \"coat_roughness\": " +
+                                r("p_coat_rough") + ", \"coat_tint\": " + r("p_white") + ", \"emission_color\": " +
+                                r("p_white") + ", \"emission_strength\": " + r("p_zero") + ", \"sheen_weight\": " +
+                                r("p_zero") + ", \"sheen_tint\": " + r("p_white") + ", \"subsurface_weight\": " +
+                                r("p_zero") + ", \"subsurface_radius\": " + r("p_ss_radius") +
+                                ", \"subsurface_scale\": " + r("p_ss_scale") + ", \"subsurface_anisotropy\": " +
+                                r("p_zero") + ", \"anisotropic\": " + r("p_zero") + ", \"anisotropic_rotation\": " +
+                                r("p_zero") + "}");
+            } else if (b && b->kind == NLPF_BSDF_EMISSION) {
+                // an emission-only surface of strength 0: it reflects nothing and emits nothing
+                lf.out.emplace_back("estr", "{\"type\": \"float\", \"value\": 0.0}");
+                lf.out.emplace_back("bsdf", "{\"type\": \"emission\", \"color\": " + jref("albedo_s") +
+                                                ", \"strength\": " + jref("estr") + "}");
+            } else {
+                lf.out.emplace_back("bsdf", "{\"type\": \"diffuse\", \"color\": " + jref("albedo_s") + "}");
+            }
         }
         lf.out.emplace_back("null", "{\"type\": \"null\"}");
         const int root = (uint32_t)m < n_materials ? roots[m] : -1;
@@ -407,9 +469,11 @@ static int write_world_scene(const char* path, uint32_t width, uint32_t height, 
 
 extern "C" int nlpf_write_world_scene(const char* path, uint32_t width, uint32_t height, const nlpf_gen_desc* gen,
                                       const nlpf_style_node* nodes, uint32_t n_nodes, const int32_t* roots,
-                                      uint32_t n_materials, uint32_t sphere_segments) {
+                                      uint32_t n_materials, uint32_t sphere_segments, const nlpf_bsdf* bsdfs,
+                                      uint32_t n_bsdfs) {
     try {
-        return write_world_scene(path, width, height, gen, nodes, n_nodes, roots, n_materials, sphere_segments);
+        return write_world_scene(path, width, height, gen, nodes, n_nodes, roots, n_materials, sphere_segments, bsdfs,
+                                 n_bsdfs);
     } catch (const std::bad_alloc&) {
         set_global_error("nlpf_write_world_scene: out of host memory");
         return NLPF_E_OOM;

@@ -184,7 +184,7 @@ _SIGS = [
     ("nlpf_style_from_scene", C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_void_p,
                                         C.POINTER(C.c_uint32), C.c_void_p]),
     ("nlpf_write_world_scene", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_GenDesc), C.c_void_p,
-                                         C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]),
+                                         C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("nlpf_pcg32_fill", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p]),
     ("nlpf_set_bsdfs", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("nlpf_bsdfs_from_scene", C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_void_p]),
@@ -291,17 +291,19 @@ SCENE_MATERIALS = ["floor", "ceiling", "back", "left", "right", "sphere0", "sphe
 
 
 def write_world_scene(path, W, H, seed=1, albedo_lo=0.2, albedo_hi=0.8, nodes=None, roots=(),
-                      sphere_segments=128):
-    """The procedural scene of generate_world_records as a reference scene file (JSON + .bin).
-    Needs no GPU."""
+                      sphere_segments=128, bsdfs=None):
+    """The procedural scene of generate_world_records as a reference scene file (JSON + .bin),
+    with the BSDF table's principled / emission materials where it has them. Needs no GPU."""
     lib = load()
     g = _GenDesc(seed, 0.0, 0.0, 0.0, 0.0, albedo_lo, albedo_hi)
     nodes = np.zeros(0, STYLE_NODE_DTYPE) if nodes is None else np.ascontiguousarray(nodes, STYLE_NODE_DTYPE)
     roots = np.ascontiguousarray(roots, np.int32)
+    bs = np.zeros(0, BSDF_DTYPE) if bsdfs is None else np.ascontiguousarray(bsdfs, BSDF_DTYPE)
     rc = lib.nlpf_write_world_scene(os.fsencode(path), W, H, C.byref(g),
                                     nodes.ctypes.data_as(C.c_void_p) if len(nodes) else None, len(nodes),
                                     roots.ctypes.data_as(C.c_void_p) if len(roots) else None, len(roots),
-                                    sphere_segments)
+                                    sphere_segments,
+                                    bs.ctypes.data_as(C.c_void_p) if len(bs) else None, len(bs))
     if rc != NLPF_OK:
         raise NlpfError(rc, lib.nlpf_last_error(None).decode())
 

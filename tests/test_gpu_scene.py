@@ -129,7 +129,9 @@ def _ray_origin(rec, d, sel, target):
     return p + ng * s[:, None] * 1e-4
 
 
-def check_light_transport(host, nodes, roots, W, H, branch, L):
+def check_light_transport(host, nodes, roots, W, H, branch, L, bsdfs=None):
+    """bsdfs: the table the records were traced with; its principled materials' continuation
+    pdf is the mixture pdf of the oracle (oracle_bsdf.c) at the traced direction."""
     import oracle as O
     n = W * H * branch
     rec = host["records"].reshape(L, 28, n).astype(np.float64)
@@ -161,7 +163,8 @@ def check_light_transport(host, nodes, roots, W, H, branch, L):
         pdf = t * t / (AREA * np.abs(wi[:, 1]))
         steep = np.abs(wi[:, 1]) > 0.05  # the plane intersection amplifies FP32 rounding at grazing angles
         assert np.allclose(dirp[stored, 6][steep], pdf[steep], rtol=1e-4), np.abs(dirp[stored, 6] / pdf - 1).max()
-        assert not _first_sphere_hit(ps + 1e-4 * wi, wi, 0.99 * t).any()
+        # the shadow ray leaves from the tracer's offset origin (1e-4 along the normal turned to wi)
+        assert not _first_sphere_hit(_ray_origin(rec, d, stored, q), wi, 0.99 * t).any()
         e = np.where(wi[:, 1] > 0, 15.0, 0.0)
         want = O.stylize(nodes, roots, np.stack([e, e, e], 1), depth=d + 1, flags=nfl[d][stored],
                          material=np.full(int(stored.sum()), 7), pos=q)
@@ -190,8 +193,97 @@ def check_light_transport(host, nodes, roots, W, H, branch, L):
             fwd = tgt - _ray_origin(rec, d, nxt, tgt)
             fwd /= np.linalg.norm(fwd, axis=1)[:, None]
             cosi = np.abs((fwd * rec[d, 3:6].T[nxt]).sum(1))
-            ok = cosi > 0.02
+            glossy = np.zeros(int(nxt.sum()), bool)
+            if bsdfs is not None:
+                mn = mat[d][nxt]
+                glossy = (mn < len(bsdfs)) & (bsdfs["kind"][np.minimum(mn, len(bsdfs) - 1)] == O.BSDF_PRINCIPLED)
+                idx = np.flatnonzero(nxt)
+                for k in np.flatnonzero(glossy):
+                    t = idx[k]
+                    want, _ = O.bsdf_eval(bsdfs[mn[k]], rec[d, 12:15, t], rec[d, 3:6, t], rec[d, 6:9, t],
+                                          rec[d, 9:12, t], -rec[d + 1, 9:12, t])
+                    assert abs(rec[d, 27, t] - want) <= 1e-5 * want, (d, t, rec[d, 27, t], want)
+            ok = (cosi > 0.02) & ~glossy
             assert np.allclose(rec[d, 27][nxt][ok], cosi[ok] / np.pi, rtol=1e-3)
             assert np.abs(rec[d + 1, 9:12].T[nxt] + fwd).max() < 1e-3  # wo at d+1
     # both kinds occur, and the style table acts on the light (toon under style index 2)
     assert n_direct > 1000 and n_occl > 100
+
+
+def _principled_scene_table():
+    # floor: diffuse / metal mixture; ceiling: metal; back: tinted metal; the rest diffuse
+    return N.bsdf_table([
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 0.7, "roughness": 0.35},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.5},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.3, "specular_tint": (1.0, 0.8, 0.6)},
+    ])
+
+
+def test_generator_light_transport_with_principled_materials():
+    """the same closed forms with principled materials: every continuation pdf is the mixture
+    pdf of the sampled direction (MicrofacetReflection / DiffuseBsdf through BsdfMixture)"""
+    W, H, branch, L, seed = 64, 48, 2, 4, 13
+    nodes = N.style_table([N.saturation(0.5, inner=-1)])
+    roots = [0] * 8
+    bs = _principled_scene_table()
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch}
+    with N.Handle(cfg) as h:
+        h.set_style(nodes, roots)
+        h.set_bsdfs(bs)
+        desc = h.world_desc(W, H, branch, L)
+        bufs = h.alloc_world(desc)
+        h.generate_world_records(desc, bufs, seed=seed)
+        host = h.download_world(desc, bufs)
+    check_light_transport(host, nodes, roots, W, H, branch, L, bsdfs=bs)
+
+
+def _cap_inside(r, beta, u, c):
+    """the cap {w : angle(w, r) < beta} lies inside {w : u . w < c}"""
+    ang = np.arccos(np.clip(r @ u, -1, 1))
+    return np.where(ang > beta, np.cos(ang - beta), 1.0) < c
+
+
+def test_generator_samples_the_principled_lobes():
+    """Directions drawn from a density p satisfy E[1{wi in A} / p(wi)] = |A| for any region A
+    (fixed per vertex) inside p's support. A is chosen so that no continuation in it can leave
+    the box through its open front (z = 0.5), so every sample in A is recorded at depth 1:
+      floor (metallic 0.7): the directions above 0.1 of the normal heading to -z, |A| = 0.9 pi;
+      floor and ceiling (metal): the cone of 10 degrees around wo's mirror direction (about the
+      lobes' width), kept for the vertices whose cone lies above the surface and heads to -z,
+      |A| = 2 pi (1 - cos 10)."""
+    W, H, branch, L, seed = 320, 240, 4, 2, 21
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch}
+    with N.Handle(cfg) as h:
+        h.set_bsdfs(_principled_scene_table())
+        desc = h.world_desc(W, H, branch, L)
+        bufs = h.alloc_world(desc)
+        h.generate_world_records(desc, bufs, seed=seed)
+        host = h.download_world(desc, bufs)
+    n = W * H * branch
+    rec = host["records"].reshape(L, 28, n).astype(np.float64)
+    mat = host["material"].reshape(L, n)
+    pd = host["path_depth"]
+    nxt = pd >= 1
+    wi = np.where(nxt[:, None], -rec[1, 9:12].T, 0.0)
+    pdf = rec[0, 27]
+
+    def expect(sel, inA, area, what):
+        x = np.where(inA[sel], 1.0 / np.where(pdf[sel] > 0, pdf[sel], 1.0), 0.0)
+        m, se = x.mean(), x.std() / np.sqrt(len(x))
+        print(f"{what}: {len(x)} vertices, E[1_A / p] = {m:.4f} +- {se:.4f}, |A| = {area:.4f}")
+        assert se < 0.03 * area and abs(m - area) < 5 * se, (what, m, se, area)
+
+    floor = (pd >= 0) & (mat[0] == 0)
+    inA = nxt & (wi[:, 1] > 0.1) & (wi[:, 2] < 0)
+    expect(floor, inA, 0.9 * np.pi, "floor, upper back quarter")
+    beta = np.pi / 18
+    cap = 2 * np.pi * (1 - np.cos(beta))
+    for m_id, up, what in ((0, 1.0, "floor"), (1, -1.0, "ceiling")):
+        on = (pd >= 0) & (mat[0] == m_id)
+        wo = rec[0, 9:12].T
+        nrm = np.array([0.0, up, 0.0])
+        r = 2 * (wo @ nrm)[:, None] * nrm - wo
+        keep = on & _cap_inside(r, beta, -nrm, 0.0) & _cap_inside(r, beta, np.array([0.0, 0.0, 1.0]), 0.0)
+        inC = nxt & ((wi * r).sum(1) > np.cos(beta))
+        assert keep.sum() > 2000, (what, int(keep.sum()))
+        expect(keep, inC, cap, f"{what}, 10 degree cone around the mirror direction")

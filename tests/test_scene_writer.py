@@ -167,3 +167,36 @@ def test_unsupported_tables(tmp_path):
                             roots=[0])
     with pytest.raises(N.NlpfError, match="sphere_segments"):
         N.write_world_scene(p, 8, 8, sphere_segments=7)
+
+
+def test_bsdf_table_round_trip(tmp_path):
+    """principled and emission-only entries written as the reference's nodes read back as the
+    same table (diffuse where the table has no entry); the principled node carries every input
+    of the reference's node"""
+    table = N.bsdf_table([
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 0.7, "roughness": 0.35},
+        {},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.3, "specular_tint": (1.0, 0.8, 0.6),
+         "specular_ior_level": 0.5, "transmission": 1.0},
+        {"kind": N.BSDF_EMISSION},
+    ])
+    path = str(tmp_path / "box.json")
+    N.write_world_scene(path, 16, 12, bsdfs=table, sphere_segments=8)
+    names = [f"mat{m}_{n}" for m, n in enumerate(N.SCENE_MATERIALS)]
+    back = N.bsdfs_from_scene(path, names)
+    assert back[0] == table[0] and back[2] == table[2]
+    assert list(back["kind"]) == [N.BSDF_PRINCIPLED, N.BSDF_DIFFUSE, N.BSDF_PRINCIPLED, N.BSDF_EMISSION] + \
+        [N.BSDF_DIFFUSE] * 3 + [N.BSDF_EMISSION]
+    with open(path) as f:
+        sc = json.load(f)
+    node = next(v for v in sc["materials"][names[0]]["shader"]["nodes"].values() if v["type"] == "principled")
+    want = {"alpha", "anisotropic", "anisotropic_rotation", "base_color", "coat_ior", "coat_normal", "coat_roughness",
+            "coat_tint", "coat_weight", "emission_color", "emission_strength", "ior", "metallic", "normal",
+            "preference", "roughness", "sheen_tint", "sheen_weight", "specular_ior_level", "specular_tint",
+            "subsurface_anisotropy", "subsurface_radius", "subsurface_scale", "subsurface_weight", "tangent",
+            "transmission_weight", "type"}
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "veach_ajar_sty_materials.json")))
+    gnode = next(v for v in golden["materials"]["Gold"]["shader"]["nodes"].values() if v["type"] == "principled")
+    assert set(node) == set(gnode) == want
+    with pytest.raises(N.NlpfError):
+        N.write_world_scene(path, 8, 8, bsdfs=N.bsdf_table([{}] * 7 + [{"kind": N.BSDF_PRINCIPLED}]))
