@@ -319,7 +319,7 @@ def large_leg(h, N):
     return out
 
 
-def world_cpu(host, W, H, B, L, cfg, nodes, roots, film, pts):
+def world_cpu(host, W, H, B, L, cfg, nodes, roots, film, pts, bsdfs=None):
     """The FP64 world-mode oracle (OpenMP over reservoir slots; grid build, sorts and
     propagation single-threaded) on the whole frame's records, beside the GPU pass, and the
     GPU film's relative L2 error against it."""
@@ -328,7 +328,8 @@ def world_cpu(host, W, H, B, L, cfg, nodes, roots, film, pts):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     ofilm = O.world_run(host["records"], host["flags"], host["new_flags"], host["material"], host["path_depth"],
-                        W, H, B, L, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), cfg, nodes=nodes, roots=roots)
+                        W, H, B, L, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), cfg, nodes=nodes, roots=roots,
+                        bsdfs=bsdfs)
     dt = time.perf_counter() - t0
     err = float(np.linalg.norm(film.astype(np.float64) - ofilm) / max(np.linalg.norm(ofilm), 1e-30))
     return {"value": pts / dt, "unit": "vertices/s", "cores": threads, "kind": "port",
@@ -380,6 +381,42 @@ def world_leg(h_unused, N, cpu=True):
         out["reference_defaults"] = {"config": "cluster, reuse [128, 64], radius_scale [0.1, 0.05], SMIS",
                                      "ms_per_frame": ms_d, "value": pts_d / (ms_d / 1e3), "unit": "vertices/s",
                                      "neighbours_per_vertex": sum(st["neighbors"][:L]) / max(pts_d, 1)}
+    # the same scene with principled materials: metal walls and spheres (roughness 0 to 0.5,
+    # as the reference scene's Gold, Hinge and Pot materials), a diffuse / metal floor; the
+    # records are traced with the metal lobes and filtered by the glossy kernels
+    table = N.bsdf_table([
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 0.5, "roughness": 0.4},
+        {},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.3, "specular_tint": (1.0, 0.8, 0.6)},
+        {},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.12},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.0},
+        {"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "roughness": 0.5},
+    ])
+    gl = {}
+    for name, c in (("default_radius", cfg), ("reference_defaults", {"type": "pf", "max_depth": L - 1, "branch": B})):
+        with N.Handle(c) as hw:
+            hw.set_style(nodes, roots)
+            hw.set_bsdfs(table)
+            desc = hw.world_desc(W, H, B, L)
+            bufs = hw.alloc_world(desc)
+            hw.generate_world_records(desc, bufs, seed=1)
+            hw.world_run(desc, bufs)
+            times = []
+            for _ in range(3):
+                gfilm, _ = hw.world_run(desc, bufs)
+                times.append(hw.stats()["total_ms"])
+            st = hw.stats()
+            gms = float(np.median(times))
+            gpts = sum(st["points"][:L])
+            gl[name] = {"ms_per_frame": gms, "value": gpts / (gms / 1e3), "unit": "vertices/s",
+                        "neighbours_per_vertex": sum(st["neighbors"][:L]) / max(gpts, 1)}
+            if cpu and name == "default_radius":
+                log("world mode, principled materials: FP64 oracle on the same frame (CPU)")
+                ghost = hw.download_world(desc, bufs)
+                gl[name]["cpu_oracle"] = world_cpu(ghost, W, H, B, L, c, nodes, roots, gfilm, gpts, bsdfs=table)
+    gl["materials"] = "floor metallic 0.5 / roughness 0.4, back, right, spheres metal (roughness 0.3, 0.12, 0, 0.5)"
+    out["principled"] = gl
     # clustering on and off at three radii under the reference's default reuse [128, 64]
     sweep = []
     for rs in (None, 2.0, 4.0):
@@ -634,6 +671,12 @@ def main():
                                    "radius_scale_2_ms": wl["radius_scale_2"]["ms_per_frame"]}
         if "cpu_oracle" in wl:
             result["world_config1"]["film_l2_vs_fp64_oracle"] = wl["cpu_oracle"]["film_rel_l2_fp32_gpu_vs_fp64"]
+        pr = wl["principled"]
+        result["world_config1"]["principled"] = {"ms": pr["default_radius"]["ms_per_frame"],
+                                                 "reference_defaults_ms": pr["reference_defaults"]["ms_per_frame"]}
+        if "cpu_oracle" in pr["default_radius"]:
+            result["world_config1"]["principled"]["film_l2_vs_fp64_oracle"] = \
+                pr["default_radius"]["cpu_oracle"]["film_rel_l2_fp32_gpu_vs_fp64"]
         h.set_style(table, [root])
         rec32 = h.alloc(verts * 60)
         h.generate_screen_records(d32, rec32, seed=1)
