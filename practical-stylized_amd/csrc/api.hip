@@ -12,16 +12,21 @@ hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant
 hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, int variant, hipStream_t st);
 int screen_variants_fp32();
 
-// tile variant of the screen gather: -1 takes each radius's default; NLPF_SCREEN_VARIANT
-// forces one variant for every radius (for tuning)
-static int screen_variant() {
-    static int v = -2;
-    if (v == -2) {
+// level-kernel variant of a handle's screen passes: the config's screen_kernel ("auto": each
+// radius's fastest, "tiles", "strips"); NLPF_SCREEN_VARIANT (0..3) forces one variant for
+// every radius (for tuning)
+int screen_variant_of(const Config& cfg) {
+    static int forced = -2;
+    if (forced == -2) {
         const char* e = getenv("NLPF_SCREEN_VARIANT");
-        v = e ? atoi(e) : -1;
+        int v = e ? atoi(e) : -1;
         if (v < 0 || v >= screen_variants_fp32()) v = -1;
+        forced = v;
     }
-    return v;
+    if (forced >= 0) return forced;
+    if (cfg.screen_kernel == "tiles") return kScreenTiles;
+    if (cfg.screen_kernel == "strips") return kScreenStrips;
+    return kScreenAuto;
 }
 hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,
                             float* film, hipStream_t st, uint32_t s_total = 0, int accumulate = 0, int finalize = 1);
@@ -390,8 +395,8 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
         a.kn = (float)kn;
         a.kl = (float)kl;
         a.rscale = desc->precision == NLPF_FP16 ? NLPF_FP16_RADIANCE_SCALE : 1.0f;
-        e = desc->precision == NLPF_FP16 ? screen_level_fp16(a, desc->radius, screen_variant(), h->stream)
-                                         : screen_level_fp32(a, desc->radius, screen_variant(), h->stream);
+        e = desc->precision == NLPF_FP16 ? screen_level_fp16(a, desc->radius, screen_variant_of(h->cfg), h->stream)
+                                         : screen_level_fp32(a, desc->radius, screen_variant_of(h->cfg), h->stream);
     }
     if (e != hipSuccess) NLPF_FAIL(h, NLPF_E_HIP, std::string("screen level launch: ") + hipGetErrorString(e));
     return NLPF_OK;

@@ -35,6 +35,11 @@ struct Config {
     // world-mode filter kernel: "auto" (by the expected neighbour count), "thread" (one thread
     // per vertex) or "wave" (one wavefront per vertex); the results are the same
     std::string world_kernel = "auto";
+    // screen-mode level kernel: "auto" (the fastest per window radius), "tiles" (64 x 8..16
+    // output tiles, each staged with its halo) or "strips" (column-strip segments with an LDS
+    // ring of rows; the tiles where a launch does not qualify); the results agree within FP32
+    // summation order
+    std::string screen_kernel = "auto";
 
     // per-depth vectors repeat their last element (lm_path_filtering.rs:1337-1347, 1584-1586)
     float radius_scale_at(uint32_t d) const {

@@ -264,7 +264,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
         a.child_slot = d + 1;
         a.has_child = d + 1 < L;
         // a tile mask covers 64x8 tiles: masked launches take the 8-row tile variant 0
-        const int v = a.tile_mask ? 0 : -1;
+        const int v = a.tile_mask ? 0 : screen_variant_of(h->cfg);
         return fp16 ? screen_level_fp16(a, desc->radius, v, st) : screen_level_fp32(a, desc->radius, v, st);
     };
     auto slice_rec = [&](uint32_t s) {
@@ -408,8 +408,8 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
             b.out_slot = (uint32_t)d;
             b.child_slot = (uint32_t)d + 1;
             b.has_child = 1;  // slot k holds the estimate
-            NLPF_HIP(h, fp16 ? screen_level_fp16(b, desc->radius, -1, st)
-                             : screen_level_fp32(b, desc->radius, -1, st));
+            NLPF_HIP(h, fp16 ? screen_level_fp16(b, desc->radius, screen_variant_of(h->cfg), st)
+                             : screen_level_fp32(b, desc->radius, screen_variant_of(h->cfg), st));
         }
     }
     hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pj, slot(sj, 0), W, H, tiles_x, nd_img);

@@ -1,29 +1,36 @@
 // Screen-mode level kernels, fp16 record storage (float accumulation), radius 1..7,
 // tile variants (waves per workgroup, rows per lane) selected at run time; variant -1
 // takes the radius's default.
-#include "screen_impl.hpp"
+#include "screen_strip.hpp"
 
 namespace nlpf {
 
 // Variants: 4 waves per workgroup with 2 (variant 0, 64x8 tiles), 4 (1) or 3 (2) rows per
-// lane. Default per radius: the fastest at 1920x1080x16 spp x 5 levels on MI355X, FP32 and
-// FP16 records alike (also timed and never fastest: 8 or 2 waves with 2 rows, 2 waves with 4).
+// lane, or column-strip segments with 2 rows per lane (3, screen_strip.hpp). Default per radius:
+// the fastest at 1920x1080x16 spp x 5 levels on MI355X (also timed and never fastest: 8 or 2
+// waves with 2 rows, 2 waves with 4). The strips win from radius 2 to 4; from radius 5 up their
+// gather holds too many registers for two waves per SIMD, and the tiles stay. With FP16 records the strips are the fastest at radius 1 as well.
 // The 2-row tiles run out of VGPRs at radius 5 and 6 (one wave per SIMD); those radii take 4
 // or 3 rows per lane.
-static constexpr int kDefaultVariant[8] = {0, 2, 0, 0, 0, 1, 2, 0};
+static constexpr int kDefaultVariant[8] = {0, 3, 3, 3, 3, 1, 2, 0};
+static constexpr int kTileVariant[8] = {0, 2, 0, 0, 0, 1, 2, 0};  // the fastest tile shape per radius
 
 template <int RAD>
 static hipError_t dispatch_fp16(const ScreenArgs<float>& a, int variant, hipStream_t st) {
     constexpr int K = 2 * RAD + 1;
+    if (variant == kScreenTiles) variant = kTileVariant[RAD];
     if (variant < 0) variant = kDefaultVariant[RAD];
     switch (variant) {
         case 1: return launch_screen_level<__half, float, RAD, 4, (4 < K ? 4 : K)>(a, st);
         case 2: return launch_screen_level<__half, float, RAD, 4, 3>(a, st);
+        case kScreenStrips:  // column strips, 2 rows per lane; the 64x8 tiles where not eligible
+            if (screen_strip_eligible<__half>(a)) return launch_screen_strip<__half, float, RAD, 4, 2>(a, st);
+            return launch_screen_level<__half, float, RAD, 4, 2>(a, st);
         default: return launch_screen_level<__half, float, RAD, 4, 2>(a, st);
     }
 }
 
-int screen_variants_fp16() { return 3; }
+int screen_variants_fp16() { return 4; }
 
 hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st) {
     switch (radius) {

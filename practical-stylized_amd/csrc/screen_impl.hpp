@@ -16,6 +16,11 @@
 
 namespace nlpf {
 
+// level-kernel variants: kScreenAuto takes each radius's fastest, kScreenTiles each radius's
+// tile shape; 0..2 are tile shapes, kScreenStrips the column-strip kernel
+constexpr int kScreenAuto = -1, kScreenTiles = -2, kScreenStrips = 3;
+int screen_variant_of(const Config& cfg);
+
 template <typename A>
 struct ScreenArgs {
     const void* rec;
@@ -29,6 +34,7 @@ struct ScreenArgs {
     int has_child;
     uint64_t plane;
     uint32_t tiles_x, tiles_y, nblocks;
+    uint32_t steps_y, seg_steps, bands;  // strip kernel: steps per strip, steps per segment, segments per strip
     uint32_t abx, aby;  // anchor blocks per row / column (fp16)
     A kp, kn, kl;
     A rscale;  // radiance decode factor (FP16 records store radiance / NLPF_FP16_RADIANCE_SCALE)

@@ -138,7 +138,7 @@ template <typename T> NLPF_HD v3<T> toon(v3<T> li, T w) {
 // branch-free binary search in power-of-two steps (the same k as the linear search, NaN
 // included: every comparison is false and k = 0).
 #define NLPF_RAMP_STOPS 16
-template <typename T> NLPF_HD uint32_t ramp_segment(const float* pos, uint32_t n, T x) {
+template <typename T, typename P> NLPF_HD uint32_t ramp_segment(P pos, uint32_t n, T x) {
     uint32_t k = 0;
 #pragma unroll
     for (uint32_t step = NLPF_RAMP_STOPS; step > 0; step >>= 1) {
@@ -147,7 +147,7 @@ template <typename T> NLPF_HD uint32_t ramp_segment(const float* pos, uint32_t n
     }
     return k;
 }
-template <typename T> NLPF_HD T ramp_eval(const float* pos, const float* val, uint32_t n, T x) {
+template <typename T, typename P> NLPF_HD T ramp_eval(P pos, P val, uint32_t n, T x) {
     if (n == 0) return x;
     if (n > NLPF_RAMP_STOPS) n = NLPF_RAMP_STOPS;
     const uint32_t k = ramp_segment(pos, n, x);
@@ -157,7 +157,7 @@ template <typename T> NLPF_HD T ramp_eval(const float* pos, const float* val, ui
     const T t = m_div(x - x0, x1 - x0);
     return T(val[k - 1]) + t * (T(val[k]) - T(val[k - 1]));
 }
-template <typename T> NLPF_HD v3<T> ramp(v3<T> li, const float* pos, const float* val, uint32_t n) {
+template <typename T, typename P> NLPF_HD v3<T> ramp(v3<T> li, P pos, P val, uint32_t n) {
     li = clamp3(li, T(0), T(100000));
     T l = lum709(li);
     if (l <= T(0)) return li;
@@ -288,8 +288,15 @@ NLPF_HD v3<T> apply_node(const nlpf_style_node& nd, uint32_t depth, v3<T> li, v3
         case NLPF_STY_HALFTONE: return halftone(li, depth, nd, p);
         case NLPF_STY_HUE_SHIFT: return hue_shift(li, T(nd.p[0]));
         case NLPF_STY_RAMP: {
-            const float* q = ramp_tab ? ramp_tab : nd.p;
-            return ramp(li, q, q + 16, nd.count);
+#if defined(__HIP_DEVICE_COMPILE__)
+            // the table lives in LDS: reading it through an LDS pointer keeps the lookups
+            // ds_read (a pointer that may also point at the parameters would be a flat access)
+            if (ramp_tab) {
+                const auto q = (const __attribute__((address_space(3))) float*)ramp_tab;
+                return ramp(li, q, q + 16, nd.count);
+            }
+#endif
+            return ramp(li, nd.p, nd.p + 16, nd.count);
         }
         default: return li;
     }

@@ -86,6 +86,38 @@ def test_every_radius_fp32_fp16(h, radius):
     anc.free()
 
 
+@pytest.mark.parametrize("kernel", ["tiles", "strips"])
+@pytest.mark.parametrize("radius,W,H", [(1, 150, 150), (2, 96, 137), (3, 150, 150), (3, 200, 9), (4, 70, 131),
+                                        (3, 97, 70), (3, 40, 6)])
+def test_tiles_and_strips_each_match_the_oracle(kernel, radius, W, H):
+    """Both level kernels, forced by screen_kernel, on shapes that end strips, 64-row segments
+    and 8-row steps part-way (150 rows: three segments, the last of 3 steps; 137: a last step of
+    one row), a strip narrower than the window (W=40), and an odd width, which the strips do
+    not take (they run the tiles there); FP32 and FP16 records, three levels."""
+    S, L = 2, 3
+    cfg = {"type": "pf", "mode": "screen", "screen_kernel": kernel}
+    with N.Handle(cfg) as hk:
+        table, root = ramp_hue_table(radius + W)
+        hk.set_style(table, [root])
+        hk.set_sigmas(*SIG)
+        assert hk.config()["screen_kernel"] == kernel
+        desc, rec, _ = gen_screen(hk, W, H, S, L, N.FP32, seed=radius + H, radius=radius)
+        film, lv = run_screen(hk, desc, rec)
+        host = host_records_fp32(rec, W, H, S, L)
+        of, olv = O.screen_run(host, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+        check_levels(lv, olv, 1e-5)
+        assert pixel_rel_err(film, of).max() <= 1e-5
+        rec.free()
+        desc, rec, anc = gen_screen(hk, W, H, S, L, N.FP16, seed=radius + H, radius=radius)
+        film, lv = run_screen(hk, desc, rec, anc)
+        dec = decode_fp16(rec, anc, W, H, S, L)
+        of, olv = O.screen_run(dec, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+        check_levels(lv, olv, 1e-5)
+        assert pixel_rel_err(film, of).max() <= 1e-5
+        rec.free()
+        anc.free()
+
+
 def test_fp32_parity_unstylized_and_determinism(h):
     h.set_style(N.style_table([N.toon(0.25)]), [-1])
     W, H, S, L = 130, 50, 2, 3
