@@ -13,7 +13,7 @@ hipError_t screen_level_fp64(const ScreenArgs<double>& a, int radius, int varian
 int screen_variants_fp32();
 
 // level-kernel variant of a handle's screen passes: the config's screen_kernel ("auto": each
-// radius's fastest, "tiles", "strips"); NLPF_SCREEN_VARIANT (0..3) forces one variant for
+// radius's fastest, "tiles", "strips"); NLPF_SCREEN_VARIANT (0..4) forces one variant for
 // every radius (for tuning)
 int screen_variant_of(const Config& cfg) {
     static int forced = -2;
@@ -25,7 +25,7 @@ int screen_variant_of(const Config& cfg) {
     }
     if (forced >= 0) return forced;
     if (cfg.screen_kernel == "tiles") return kScreenTiles;
-    if (cfg.screen_kernel == "strips") return kScreenStrips;
+    if (cfg.screen_kernel == "strips") return kScreenStripsAuto;
     return kScreenAuto;
 }
 hipError_t screen_film_fp32(const float* out, uint32_t S, uint32_t out_levels, uint32_t slot0, uint64_t plane,

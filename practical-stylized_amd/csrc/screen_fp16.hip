@@ -6,19 +6,22 @@
 namespace nlpf {
 
 // Variants: 4 waves per workgroup with 2 (variant 0, 64x8 tiles), 4 (1) or 3 (2) rows per
-// lane, or column-strip segments with 2 rows per lane (3, screen_strip.hpp). Default per radius:
-// the fastest at 1920x1080x16 spp x 5 levels on MI355X (also timed and never fastest: 8 or 2
-// waves with 2 rows, 2 waves with 4). The strips win from radius 2 to 4; from radius 5 up their
-// gather holds too many registers for two waves per SIMD, and the tiles stay. With FP16 records the strips are the fastest at radius 1 as well.
-// The 2-row tiles run out of VGPRs at radius 5 and 6 (one wave per SIMD); those radii take 4
-// or 3 rows per lane.
-static constexpr int kDefaultVariant[8] = {0, 3, 3, 3, 3, 1, 2, 0};
-static constexpr int kTileVariant[8] = {0, 2, 0, 0, 0, 1, 2, 0};  // the fastest tile shape per radius
+// lane, or column-strip segments (screen_strip.hpp) with 2 (3) or 3 (4) rows per lane. Default
+// per radius: the fastest at 1920x1080x16 spp x 5 levels on MI355X (also timed and never
+// fastest: 8 or 2 waves with 2 rows, 2 waves with 4). The strips win at every radius but two:
+// FP32 records at radius 1 (3-row tiles, 1 % ahead; the strips win with FP16 records there) and
+// at radius 5 (4-row tiles, 2 % ahead of the 3-row strips). The 2-row tiles run out of VGPRs at
+// radius 5 and 6 (one wave per SIMD); the 3-row strips need 83 KB of LDS at radius 7 (one
+// workgroup per CU).
+static constexpr int kDefaultVariant[8] = {0, 3, 3, 3, 3, 4, 4, 3};
+static constexpr int kTileVariant[8] = {0, 2, 0, 0, 0, 1, 2, 0};   // the fastest tile shape per radius
+static constexpr int kStripVariant[8] = {3, 3, 3, 3, 3, 4, 4, 3};  // the fastest strip shape per radius
 
 template <int RAD>
 static hipError_t dispatch_fp16(const ScreenArgs<float>& a, int variant, hipStream_t st) {
     constexpr int K = 2 * RAD + 1;
     if (variant == kScreenTiles) variant = kTileVariant[RAD];
+    if (variant == kScreenStripsAuto) variant = kStripVariant[RAD];
     if (variant < 0) variant = kDefaultVariant[RAD];
     switch (variant) {
         case 1: return launch_screen_level<__half, float, RAD, 4, (4 < K ? 4 : K)>(a, st);
@@ -26,11 +29,14 @@ static hipError_t dispatch_fp16(const ScreenArgs<float>& a, int variant, hipStre
         case kScreenStrips:  // column strips, 2 rows per lane; the 64x8 tiles where not eligible
             if (screen_strip_eligible<__half>(a)) return launch_screen_strip<__half, float, RAD, 4, 2>(a, st);
             return launch_screen_level<__half, float, RAD, 4, 2>(a, st);
+        case kScreenStrips + 1:  // column strips, 3 rows per lane
+            if (screen_strip_eligible<__half>(a)) return launch_screen_strip<__half, float, RAD, 4, 3>(a, st);
+            return launch_screen_level<__half, float, RAD, 4, 3>(a, st);
         default: return launch_screen_level<__half, float, RAD, 4, 2>(a, st);
     }
 }
 
-int screen_variants_fp16() { return 4; }
+int screen_variants_fp16() { return 5; }
 
 hipError_t screen_level_fp16(const ScreenArgs<float>& a, int radius, int variant, hipStream_t st) {
     switch (radius) {

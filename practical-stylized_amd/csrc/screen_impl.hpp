@@ -17,8 +17,9 @@
 namespace nlpf {
 
 // level-kernel variants: kScreenAuto takes each radius's fastest, kScreenTiles each radius's
-// tile shape; 0..2 are tile shapes, kScreenStrips the column-strip kernel
-constexpr int kScreenAuto = -1, kScreenTiles = -2, kScreenStrips = 3;
+// fastest tile shape, kScreenStripsAuto its fastest strip shape; 0..2 are tile shapes,
+// kScreenStrips and kScreenStrips + 1 the column-strip kernel with 2 and 3 rows per lane
+constexpr int kScreenAuto = -1, kScreenTiles = -2, kScreenStripsAuto = -3, kScreenStrips = 3;
 int screen_variant_of(const Config& cfg);
 
 template <typename A>

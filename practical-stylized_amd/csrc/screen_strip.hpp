@@ -275,19 +275,16 @@ __global__ void __launch_bounds__(64 * NW) screen_strip_kernel(ScreenArgs<A> a) 
                     if ((r + 1) % G == 0 || r == NR - 1) flush();
                 }
             } else {
-                // edge rows unrolled; the middle rows, which feed every output, two at a time
+                // edge rows unrolled; the middle rows, which feed every output, one per loop
+                // iteration with a Kahan fold after every second (a two-row body, whose rows
+                // may wrap around the ring, was scheduled into register spills)
 #pragma unroll
                 for (int r = 0; r < R - 1; ++r) row(r, 0, r);
                 if (R > 1) flush();
 #pragma unroll 1
-                for (int r = R - 1; r + 1 <= 2 * RAD; r += 2) {
+                for (int r = R - 1; r <= 2 * RAD; ++r) {
                     row(r, 0, R - 1);
-                    row(r + 1, 0, R - 1);
-                    flush();
-                }
-                if ((2 * RAD + 2 - R) % 2) {
-                    row(2 * RAD, 0, R - 1);
-                    flush();
+                    if (((r - (R - 1)) & 1) || r == 2 * RAD) flush();
                 }
 #pragma unroll
                 for (int r = 0; r < R - 1; ++r) row(2 * RAD + 1 + r, r + 1, R - 1);
