@@ -520,6 +520,8 @@ def main():
     ap.add_argument("--detail", default=None, help="write every measurement to this JSON file")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one FP32 pass and one 1 GiB device copy, for rocprofv3 counter runs")
+    ap.add_argument("--screen-cfg", default="{}",
+                    help="JSON object merged into the screen-mode handle's config (e.g. screen_kernel)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -538,7 +540,8 @@ def main():
     plane = W * H
     sig = (0.05, 0.9, 0.5)
     table, root = style(N)
-    h = N.Handle({"type": "pf", "mode": "screen", "max_depth": L - 1, "pixel_radius": R})
+    h = N.Handle({"type": "pf", "mode": "screen", "max_depth": L - 1, "pixel_radius": R,
+                  **json.loads(args.screen_cfg)})
     h.set_style(table, [root])
     h.set_sigmas(*sig)
     verts = plane * S * L
