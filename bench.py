@@ -29,6 +29,12 @@ FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_PAIR = 27.0
 FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per cycle (the build emits no packed FP32)
 VALU_ISSUE_PEAK = 256 * 4 * 16 * 2.4e9  # lane-instructions/s: 256 CUs x 4 SIMD16 at 2.4 GHz
+# the FP32 level kernel each window radius runs by default (the library's per-radius choice:
+# column strips with 2 or 3 rows per lane, 64x12 tiles at radius 1, 64x16 tiles at radius 5)
+LEVEL_KERNEL_FP32 = {1: "screen_level_kernel<float,float,1,4,3>", 2: "screen_strip_kernel<float,float,2,4,2>",
+                     3: "screen_strip_kernel<float,float,3,4,2>", 4: "screen_strip_kernel<float,float,4,4,2>",
+                     5: "screen_level_kernel<float,float,5,4,4>", 6: "screen_strip_kernel<float,float,6,4,3>",
+                     7: "screen_strip_kernel<float,float,7,4,2>"}
 
 
 def log(msg):
@@ -593,7 +599,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / FP32_PEAK_TFLOPS,
                      "traffic": measured_traffic() if (W, H, S, L, R) == (1920, 1080, 16, 5, 7) else None,
-                     "kernel": f"screen_level_kernel<float,float,{R},...>",
+                     "kernel": LEVEL_KERNEL_FP32[R],
                      "avg_launch_ms": float(lv32.mean()),
                      "algorithmic_flops_per_launch": pairs / L * FLOPS_PER_PAIR,
                      "algorithmic_bytes_per_launch": float(np.mean(lb32)),
