@@ -37,6 +37,7 @@ struct BwdArgs {
     float* g_out;           // dL/dout_{d+1} [S][3][H][W] (pass B)
     float* u;               // h / W        [S][3][H][W]
     float* partial;         // [nblocks][ADJ_NP] accumulated over levels
+    int32_t np;             // live parameters: 3 + 16 * (ramp nodes); the rest stay 0
     uint32_t W, H, S, L, d;
     uint64_t plane;
     uint32_t tiles_x, tiles_y, nblocks;
@@ -51,20 +52,15 @@ struct BwdArgs {
 __device__ __forceinline__ void style_vjp(const BwdArgs& a, v3<float> F, v3<float> g, float* pg, v3<float>& h,
                                           const float* tabs) {
     const int n = a.chain.n;
-    v3<float> xs[ADJ_MAX_CHAIN];
-    v3<float> x = F;
     const v3<float> p0{0.0f, 0.0f, 0.0f};
-    for (int k = 0; k < ADJ_MAX_CHAIN; ++k) {
-        if (k < n) {
-            xs[k] = x;
-            x = apply_node<float>(a.chain.node[k], a.d, x, p0, tabs + 32 * k);
-        }
-    }
     for (int k = ADJ_MAX_CHAIN - 1; k >= 0; --k) {
         if (k < n) {
             const nlpf_style_node& nd = a.chain.node[k];
             const float* tab = tabs + 32 * k;
-            const v3<float> xi = xs[k];
+            // node k's input, recomputed from F through nodes 0..k-1 (a per-node array of the
+            // inputs, indexed by the loop counter, would live in scratch memory)
+            v3<float> xi = F;
+            for (int j = 0; j < k; ++j) xi = apply_node<float>(a.chain.node[j], a.d, xi, p0, tabs + 32 * j);
             if (nd.type == NLPF_STY_RAMP && depth_active(nd.depth_mask, a.d)) {
                 const int slot = a.chain.slot[k];
                 const v3<float> c = clamp3(xi, 0.0f, 100000.0f);
@@ -319,13 +315,37 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
     for (int q = 0; q < ADJ_NP; ++q) pg[q] = 0.0f;
     const float* gin = a.g_in + (size_t)s * 3 * plane;
     float* upw = a.u + (size_t)s * 3 * plane;
-#pragma unroll
+    // The loop over the R outputs stays rolled (the chain's VJP is large) and takes output k's
+    // sums by selects: indexed by the loop counter, the per-output arrays would live in scratch.
+#pragma unroll 1
     for (int k = 0; k < R; ++k) {
+        float wt = Wt[0], sx = Sx[0], sy = Sy[0], sz = Sz[0], am[3], bm[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            am[q] = Am[q][0];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bm[q][c] = Bm[q][c][0];
+        }
+#pragma unroll
+        for (int j = 1; j < R; ++j) {
+            if (k == j) {
+                wt = Wt[j];
+                sx = Sx[j];
+                sy = Sy[j];
+                sz = Sz[j];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    am[q] = Am[q][j];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bm[q][c] = Bm[q][c][j];
+                }
+            }
+        }
         const int ox = tx0 + lc, oy = ty0 + r0 + k;
         if (ox < W && oy < H) {
             const size_t o = (size_t)oy * W + ox;
-            const float inv = 1.0f / Wt[k];
-            const v3<float> F{Sx[k] * inv, Sy[k] * inv, Sz[k] * inv};
+            const float inv = 1.0f / wt;
+            const v3<float> F{sx * inv, sy * inv, sz * inv};
             const v3<float> g{gin[o], gin[plane + o], gin[2 * plane + o]};
             v3<float> hh;
             style_vjp(a, F, g, pg, hh, tabs);
@@ -335,7 +355,7 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
             for (int q = 0; q < 3; ++q) {
                 float acc = 0.0f;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) acc = fmaf(hv[c], Bm[q][c][k] - Fv[c] * Am[q][k], acc);
+                for (int c = 0; c < 3; ++c) acc = fmaf(hv[c], bm[q][c] - Fv[c] * am[q], acc);
                 pg[q] += acc * inv * (2.0f * 0.69314718056f / a.sig[q]);
             }
             upw[o] = hh.x * inv;
@@ -347,12 +367,14 @@ __global__ void __launch_bounds__(64 * NW) screen_bwd_a_kernel(BwdArgs a) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < ADJ_NP; ++q) {
-        float v = pg[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) red[wv][q] = v;
+        if (q < a.np) {  // uniform: the ramp slots the chain does not use are skipped
+            float v = pg[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) red[wv][q] = v;
+        }
     }
     __syncthreads();
-    if (threadIdx.x < ADJ_NP) {
+    if ((int)threadIdx.x < a.np) {
         float v = 0.0f;
         for (int w2 = 0; w2 < NW; ++w2) v += red[w2][threadIdx.x];
         a.partial[(size_t)blockIdx.x * ADJ_NP + threadIdx.x] += v;
@@ -568,6 +590,7 @@ extern "C" int nlpf_screen_backward(nlpf_handle* h, const nlpf_screen_desc* desc
     a.rec = (const float*)records;
     a.level_out = (const float*)level_out;
     a.partial = partial;
+    a.np = 3 + 16 * nr;
     a.u = u;
     a.W = desc->width;
     a.H = desc->height;
