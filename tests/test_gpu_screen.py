@@ -118,6 +118,31 @@ def test_tiles_and_strips_each_match_the_oracle(kernel, radius, W, H):
         anc.free()
 
 
+def test_random_shapes_default_kernels():
+    """Twelve frames of random shape (width 2..200, height 1..140, 1-3 slices, 1-4 levels, radius
+    1..7, fixed seed), FP32 and FP16 records, through the default level kernel of each radius,
+    against the oracle at 1e-5 per pixel."""
+    rng = np.random.default_rng(2026)
+    with N.Handle({"type": "pf", "mode": "screen"}) as hk:
+        hk.set_sigmas(*SIG)
+        for case in range(12):
+            W, H = int(rng.integers(2, 201)), int(rng.integers(1, 141))
+            S, L, radius = int(rng.integers(1, 4)), int(rng.integers(1, 5)), int(rng.integers(1, 8))
+            table, root = ramp_hue_table(case)
+            hk.set_style(table, [root])
+            for prec in (N.FP32, N.FP16):
+                desc, rec, anc = gen_screen(hk, W, H, S, L, prec, seed=case, radius=radius)
+                film, lv = run_screen(hk, desc, rec, anc)
+                host = host_records_fp32(rec, W, H, S, L) if prec == N.FP32 else decode_fp16(rec, anc, W, H, S, L)
+                of, olv = O.screen_run(host, W, H, S, L, radius, SIG, nodes=table, root=root, want_levels=True)
+                err = pixel_rel_err(np.moveaxis(lv, 2, -1), np.moveaxis(olv, 2, -1)).max()
+                assert err <= 1e-5, (case, W, H, S, L, radius, prec, err)
+                assert pixel_rel_err(film, of).max() <= 1e-5, (case, W, H, S, L, radius, prec)
+                rec.free()
+                if anc is not None:
+                    anc.free()
+
+
 def test_fp32_parity_unstylized_and_determinism(h):
     h.set_style(N.style_table([N.toon(0.25)]), [-1])
     W, H, S, L = 130, 50, 2, 3
