@@ -35,6 +35,9 @@ struct Config {
     // world-mode filter kernel: "auto" (by the expected neighbour count), "thread" (one thread
     // per vertex) or "wave" (one wavefront per vertex); the results are the same
     std::string world_kernel = "auto";
+    // world-mode grid build: "auto" (counting build on the bucket table, radix sort when a bucket
+    // exceeds 1024 entries) or "sort" (always the radix sort); the results are identical
+    std::string world_grid = "auto";
     // screen-mode level kernel: "auto" (the fastest per window radius), "tiles" (64 x 8..16
     // output tiles, each staged with its halo) or "strips" (column-strip segments with an LDS
     // ring of rows; the tiles where a launch does not qualify); the results agree within FP32

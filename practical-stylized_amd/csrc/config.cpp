@@ -201,6 +201,12 @@ bool parse_config(const char* json, Config& cfg, std::string& err) {
                 return false;
             }
             c.world_kernel = v.str;
+        } else if (k == "world_grid") {
+            if (v.kind != JVal::Str || (v.str != "auto" && v.str != "sort")) {
+                err = "world_grid must be \"auto\" or \"sort\"";
+                return false;
+            }
+            c.world_grid = v.str;
         } else if (k == "screen_kernel") {
             if (v.kind != JVal::Str || (v.str != "auto" && v.str != "tiles" && v.str != "strips")) {
                 err = "screen_kernel must be \"auto\", \"tiles\" or \"strips\"";
@@ -236,7 +242,8 @@ std::string config_to_json(const Config& c) {
       << ",\"sigma_normal\":" << c.sigma_normal << ",\"sigma_lum\":" << c.sigma_lum << ",\"precision\":\""
       << (c.precision == NLPF_FP16 ? "fp16" : c.precision == NLPF_FP64 ? "fp64" : "fp32")
       << "\",\"seed\":" << c.seed << ",\"hash_table_size\":" << c.hash_table_size << ",\"world_kernel\":\""
-      << c.world_kernel << "\",\"screen_kernel\":\"" << c.screen_kernel << "\"}";
+      << c.world_kernel << "\",\"world_grid\":\"" << c.world_grid << "\",\"screen_kernel\":\"" << c.screen_kernel
+      << "\"}";
     return o.str();
 }
 

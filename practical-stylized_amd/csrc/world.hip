@@ -128,8 +128,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // or maxes the slots in the readback it makes anyway.
 constexpr uint32_t NBR_SLOTS = 256;
 
-// paths per last depth: hist[k] = #{t : path_depth[t] == k}, so a depth's point count is a
-// suffix sum (one readback for the whole pass)
+// paths per last depth: hist[k] = #{t : path_depth[t] == k} (k >= 31 counted at 31: depths stop
+// at 30), so a depth's point count is a suffix sum (one readback for the whole pass)
 __global__ void __launch_bounds__(256) w_depth_hist_kernel(const int32_t* path_depth, uint64_t N, uint32_t* hist) {
     __shared__ uint32_t lh[32];
     if (threadIdx.x < 32) lh[threadIdx.x] = 0;
@@ -137,71 +137,171 @@ __global__ void __launch_bounds__(256) w_depth_hist_kernel(const int32_t* path_d
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < N) {
         const int32_t k = path_depth[t];
-        if (k >= 0 && k < 32) atomicAdd(lh + k, 1u);
+        if (k >= 0) atomicAdd(lh + (k < 31 ? k : 31), 1u);
     }
     __syncthreads();
     if (threadIdx.x < 32 && lh[threadIdx.x]) atomicAdd(hist + threadIdx.x, lh[threadIdx.x]);
 }
 
-__global__ void w_flag_kernel(const int32_t* path_depth, uint64_t N, int d, uint8_t* flag) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) flag[i] = path_depth[i] >= d ? 1 : 0;
+// selection predicate of depth d's points: the paths whose last depth reaches d
+struct ReachesDepth {
+    const int32_t* path_depth;
+    int32_t d;
+    __host__ __device__ bool operator()(uint32_t i) const { return path_depth[i] >= d; }
+};
+
+// ---- grid build: a counting sort on the compact table's slots ----
+// Every point finds or claims its bucket's slot and counts itself there, keeping the count it
+// saw (its arrival rank); w_slot_ranges_kernel turns the counts into [start, end) ranges;
+// w_scatter_kernel puts each point at start + arrival rank; w_emit_kernel then moves each entry to
+// start + (entries of its bucket with a smaller point index), the order of a stable sort by
+// bucket (points are in path-id order). Buckets take their ranges in whichever order the slot
+// pass claims them: only the order inside a bucket reaches a result, since every walk goes
+// through a bucket's range. The emit pass costs each entry a pass over its bucket, so buckets
+// above kEmitMaxBucket entries take the radix-sort path instead (w_sorted_emit_kernel).
+constexpr uint32_t kEmitMaxBucket = 1024;
+
+// empty compact table (no bucket, no entries, no seed), zeroed grid counters and range cursor
+__global__ void w_grid_clear_kernel(uint4* ctab, uint32_t slots, uint32_t* counters, uint32_t ncounters,
+                                    uint32_t* cursor) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < slots) ctab[s] = make_uint4(kEmptyKey, 0u, 0u, kEmptyKey);
+    if (s < ncounters) counters[s] = 0u;
+    if (s == 0) *cursor = 0u;
 }
 
-__global__ void w_keys_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g, uint32_t* keys) {
+// slot_of[t] = slot of point t's bucket, arrival[t] = points of that bucket counted before it;
+// the slot's third word counts the bucket's points. Claimed slots count the occupied buckets and
+// the largest arrival rank bounds the largest bucket (both over NBR_SLOTS slots).
+__global__ void __launch_bounds__(256) w_insert_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n,
+                                                       Grid g, uint32_t* slot_of, uint32_t* arrival,
+                                                       uint32_t* occupied, uint32_t* max_bucket) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t i = pts[t];
-    uint32_t c[3];
-    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
-    keys[t] = cell_bucket(c[0], c[1], c[2], g.table);
+    uint32_t claimed = 0, size = 0;
+    if (t < n) {
+        const uint32_t i = pts[t];
+        uint32_t c[3];
+        get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
+        const uint32_t k = cell_bucket(c[0], c[1], c[2], g.table);
+        uint32_t s = ctab_home(g, k);
+        for (;;) {  // ends: the table is at most a quarter full
+            const uint32_t v = __hip_atomic_load(&g.ctab[s].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v == k) break;
+            if (v == kEmptyKey) {
+                const uint32_t old = atomicCAS(&g.ctab[s].x, kEmptyKey, k);
+                if (old == kEmptyKey) {
+                    claimed = 1;
+                    break;
+                }
+                if (old == k) break;
+            }
+            s = (s + 1) & g.cmask;
+        }
+        const uint32_t r = atomicAdd(&g.ctab[s].z, 1u);
+        slot_of[t] = s;
+        arrival[t] = r;
+        size = r + 1;
+    }
+    claimed = wave_sum(claimed);
+    size = wave_max(size);
+    if ((threadIdx.x & 63) == 0) {
+        const uint32_t slot = blockIdx.x & (NBR_SLOTS - 1);
+        if (claimed) atomicAdd(occupied + slot, claimed);
+        if (size) atomicMax(max_bucket + slot, size);
+    }
 }
 
-// Bucket ranges of the sorted keys into the compact table (cleared beforehand): the last entry
-// of each bucket finds the bucket's start by a galloping search back over the sorted keys
-// (O(log size) loads, so large buckets cost no serial walk), claims a slot and writes [start,
-// end); the same entries bound the largest bucket and the first entries count the occupied
-// buckets (both over NBR_SLOTS slots).
-__global__ void w_ranges_kernel(const uint32_t* keys, const uint32_t* ids, uint32_t n, const float* rec, uint64_t N,
-                                const uint8_t* new_flags, Grid g, float4* pos_sorted, uint8_t* flags_sorted,
-                                uint32_t* occupied, uint32_t* max_bucket) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t head = 0, size = 0;
-    if (e < n) {
-        const uint32_t k = keys[e];
-        head = (e == 0 || keys[e - 1] != k) ? 1u : 0u;
-        if (e + 1 == n || keys[e + 1] != k) {
-            uint32_t s = e;
-            if (!head) {
-                // keys[hi] == k throughout; the loop ends with keys[hi - step] != k or hi < step
-                uint32_t hi = e, step = 1;
-                while (hi >= step && keys[hi - step] == k) {
-                    hi -= step;
-                    step <<= 1;
-                }
-                uint32_t lo = hi >= step ? hi - step + 1 : 0;
-                while (lo < hi) {  // first index in [lo, hi] with key k
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (keys[mid] == k) hi = mid; else lo = mid + 1;
-                }
-                s = lo;
-            }
-            size = e + 1 - s;
-            uint32_t slot = ctab_home(g, k);  // one last entry per bucket: k is not in the table yet
-            while (atomicCAS(&g.ctab[slot].x, kEmptyKey, k) != kEmptyKey) slot = (slot + 1) & g.cmask;
-            g.ctab[slot].y = s;
-            g.ctab[slot].z = e + 1;
-        }
-        const uint32_t i = ids[e];
-        pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
-        flags_sorted[e] = new_flags[i];
+// bucket ranges from the slot counts: a workgroup takes 4096 slots (lane-interleaved, 16 per
+// thread), scans their counts, claims room for its total from *cursor (one atomic per
+// workgroup) and writes each occupied slot's [start, end) over its count
+constexpr uint32_t kRangeSlotsPerThread = 16;
+__global__ void __launch_bounds__(256) w_slot_ranges_kernel(uint4* ctab, uint64_t slots, uint32_t* cursor) {
+    __shared__ uint32_t wave_total[4], block_base;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t s0 = (uint64_t)blockIdx.x * 256 * kRangeSlotsPerThread + threadIdx.x;
+    uint32_t cnt[kRangeSlotsPerThread];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRangeSlotsPerThread; ++k) {
+        const uint64_t s = s0 + (uint64_t)k * 256;
+        cnt[k] = s < slots ? ctab[s].z : 0u;
+        sum += cnt[k];
     }
-    head = wave_sum(head);
-    size = wave_max(size);
-    if ((threadIdx.x & 63) == 0) {  // a wavefront may hold a bucket's last entry but no first one
-        const uint32_t slot = blockIdx.x & (NBR_SLOTS - 1);
-        if (head) atomicAdd(occupied + slot, head);
-        if (size) atomicMax(max_bucket + slot, size);
+    uint32_t incl = sum;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= (uint32_t)off) incl += v;
+    }
+    if (lane == 63) wave_total[wv] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
+        block_base = total ? atomicAdd(cursor, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t start = block_base + incl - sum;
+    for (uint32_t k = 0; k < wv; ++k) start += wave_total[k];
+#pragma unroll
+    for (uint32_t k = 0; k < kRangeSlotsPerThread; ++k) {
+        if (cnt[k]) {
+            const uint64_t s = s0 + (uint64_t)k * 256;
+            ctab[s].y = start;
+            ctab[s].z = start + cnt[k];
+            start += cnt[k];
+        }
+    }
+}
+
+// order[start of t's bucket + arrival rank of t] = t
+__global__ void w_scatter_kernel(const uint32_t* slot_of, const uint32_t* arrival, uint32_t n, const uint4* ctab,
+                                 uint32_t* order) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) order[ctab[slot_of[t]].y + arrival[t]] = t;
+}
+
+// per-entry arrays at the entry's place in bucket order: bucket id, path id, position with the
+// path id, stylization flags
+__device__ __forceinline__ void emit_entry(uint32_t f, uint32_t bucket, uint32_t i, const float* rec, uint64_t N,
+                                           const uint8_t* new_flags, uint32_t* keys_sorted, uint32_t* ids_sorted,
+                                           float4* pos_sorted, uint8_t* flags_sorted) {
+    keys_sorted[f] = bucket;
+    ids_sorted[f] = i;
+    pos_sorted[f] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
+    flags_sorted[f] = new_flags[i];
+}
+
+// entry e of the scatter holds point t = order[e]; its place is its bucket's start plus the
+// bucket's entries with a smaller point index (buckets of at most kEmitMaxBucket entries)
+__global__ void w_emit_kernel(const uint32_t* order, const uint32_t* slot_of, const uint32_t* pts, uint32_t n,
+                              const uint4* ctab, const float* rec, uint64_t N, const uint8_t* new_flags,
+                              uint32_t* keys_sorted, uint32_t* ids_sorted, float4* pos_sorted, uint8_t* flags_sorted) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t t = order[e];
+    const uint4 v = ctab[slot_of[t]];
+    uint32_t f = v.y;
+    for (uint32_t q = v.y; q < v.z; ++q) f += order[q] < t ? 1u : 0u;
+    emit_entry(f, v.x, pts[t], rec, N, new_flags, keys_sorted, ids_sorted, pos_sorted, flags_sorted);
+}
+
+// the radix-sort path: entries already sorted by slot (stable, so in path-id order inside a
+// bucket); the last entry of each bucket writes its [start, end) over the bucket's count
+__global__ void w_sorted_emit_kernel(const uint32_t* slot_sorted, const uint32_t* ids_sorted_in, uint32_t n,
+                                     uint4* ctab, const float* rec, uint64_t N, const uint8_t* new_flags,
+                                     uint32_t* keys_sorted, float4* pos_sorted, uint8_t* flags_sorted) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t s = slot_sorted[e];
+    const uint32_t i = ids_sorted_in[e];
+    const uint32_t bucket = ctab[s].x;
+    keys_sorted[e] = bucket;
+    pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
+    flags_sorted[e] = new_flags[i];
+    if (e + 1 == n || slot_sorted[e + 1] != s) {
+        const uint32_t count = ctab[s].z;
+        ctab[s].y = e + 1 - count;
+        ctab[s].z = e + 1;
     }
 }
 
@@ -221,23 +321,24 @@ __global__ void w_cluster_max_kernel(const uint32_t* pts, uint32_t n, const uint
     if ((threadIdx.x & 63) == 0 && members) atomicMax(out_max + (blockIdx.x & (NBR_SLOTS - 1)), members);
 }
 
-// empty compact table: no bucket, no entries, no seed
-__global__ void w_ctab_clear_kernel(uint4* ctab, uint32_t slots) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < slots) ctab[s] = make_uint4(kEmptyKey, 0u, 0u, kEmptyKey);
-}
-
-// per-path payload of level d (indirect record from the previous propagation)
+// per-path payload of level d (indirect record from the previous propagation): paths whose last
+// depth is above ind_above took one at the previous level, the others have none (zeros; the
+// indirect planes hold nothing of theirs from this pass)
 __global__ void w_pack_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, const float* ind,
-                              const uint8_t* new_flags, const uint16_t* material, NbrRec* pk) {
+                              const int32_t* path_depth, int32_t ind_above, const uint8_t* new_flags,
+                              const uint16_t* material, NbrRec* pk) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = pts[t];
     NbrRec r;
     r.dwi_pdf = make_float4(pl(rec, N, 19, i), pl(rec, N, 20, i), pl(rec, N, 21, i), pl(rec, N, 25, i));
     r.dirr_mis = make_float4(pl(rec, N, 22, i), pl(rec, N, 23, i), pl(rec, N, 24, i), pl(rec, N, 26, i));
-    r.iwi_mis = make_float4(ind[i], ind[N + i], ind[2 * N + i], ind[6 * N + i]);
-    r.iirr = make_float4(ind[3 * N + i], ind[4 * N + i], ind[5 * N + i], __uint_as_float((uint32_t)new_flags[i]));
+    r.iwi_mis = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r.iirr = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float((uint32_t)new_flags[i]));
+    if (path_depth[i] > ind_above) {
+        r.iwi_mis = make_float4(ind[i], ind[N + i], ind[2 * N + i], ind[6 * N + i]);
+        r.iirr = make_float4(ind[3 * N + i], ind[4 * N + i], ind[5 * N + i], r.iirr.w);
+    }
     r.ns_wox = make_float4(pl(rec, N, 3, i), pl(rec, N, 4, i), pl(rec, N, 5, i), pl(rec, N, 9, i));
     r.ng_woy = make_float4(pl(rec, N, 6, i), pl(rec, N, 7, i), pl(rec, N, 8, i), pl(rec, N, 10, i));
     r.woz = make_float4(pl(rec, N, 11, i), __uint_as_float((uint32_t)material[i]), 0.0f, 0.0f);
@@ -1031,9 +1132,11 @@ __global__ void w_cluster_size_kernel(const unsigned long long* prop_sorted, uin
 // ---------------------------------------------------------------------------------------
 namespace nlpf {
 
-// Slot arrays after the 8 scalar counters, NBR_SLOTS each: open (unclustered) vertices and
+// Scalar counters: 0 selected points, 1 cluster proposals, SLOT_CURSOR the grid build's range
+// cursor. Slot arrays after the 8 scalars, NBR_SLOTS each: open (unclustered) vertices and
 // occupied buckets (summed), candidate bound and largest bucket or cluster (maxed).
 enum GridSlots : uint32_t {
+    SLOT_CURSOR = 2,
     SLOT_OPEN = 8,
     SLOT_OCCUPIED = 8 + NBR_SLOTS,
     SLOT_CAND = 8 + 2 * NBR_SLOTS,
@@ -1044,8 +1147,11 @@ enum GridSlots : uint32_t {
 
 struct WorldWork {
     uint64_t N = 0, nbuf_elems = 0, rng_n = 0, cub_bytes = 0;
-    uint8_t* flag = nullptr;
-    uint32_t *pts = nullptr, *keys = nullptr, *keys_sorted = nullptr, *ids_sorted = nullptr;
+    // grid build: pts (a depth's paths), keys (slot of each point), arrival (its arrival rank),
+    // order (scattered points, or slots sorted on the radix-sort path), then per entry in bucket
+    // order: keys_sorted (bucket id), ids_sorted (path id), pos_sorted, flags_sorted
+    uint32_t *pts = nullptr, *keys = nullptr, *arrival = nullptr, *order = nullptr, *keys_sorted = nullptr,
+             *ids_sorted = nullptr;
     uint4* ctab = nullptr;  // compact bucket table, ctab_slots slots
     uint64_t ctab_slots = 0;
     uint32_t *cluster_id = nullptr, *cluster_size = nullptr, *counters = nullptr, *nbuf = nullptr;
@@ -1062,10 +1168,11 @@ struct WorldWork {
     // copies into it need no staging blit
     uint32_t* host = nullptr;
     unsigned long long* host_nbr = nullptr;
+    hipEvent_t readback = nullptr;  // end of the grid counters' copy to host
 };
 
 static void world_free(WorldWork& w) {
-    void* ps[] = {w.flag, w.pts, w.keys, w.keys_sorted, w.ids_sorted, w.ctab,
+    void* ps[] = {w.pts, w.keys, w.arrival, w.order, w.keys_sorted, w.ids_sorted, w.ctab,
                   w.cluster_id, w.cluster_size, w.counters, w.nbuf, w.prop, w.prop_sorted, w.nbr,
                   w.pos_sorted, w.flags_sorted, w.pk, w.ind, w.rad, w.rng, w.seed_flags, w.seed_rank, w.seed_entry,
                   w.smis_sum, w.cub};
@@ -1073,6 +1180,7 @@ static void world_free(WorldWork& w) {
         if (p) (void)hipFree(p);
     if (w.host) (void)hipHostFree(w.host);
     if (w.host_nbr) (void)hipHostFree(w.host_nbr);
+    if (w.readback) (void)hipEventDestroy(w.readback);
     w = WorldWork{};
 }
 
@@ -1108,9 +1216,10 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t nbuf
     NLPF_HIP(h, hipStreamSynchronize(h->stream));
     world_free(w);
     int rc = 0;
-    rc |= dalloc(h, &w.flag, N);
     rc |= dalloc(h, &w.pts, N);
     rc |= dalloc(h, &w.keys, N);
+    rc |= dalloc(h, &w.arrival, N);
+    rc |= dalloc(h, &w.order, N);
     rc |= dalloc(h, &w.keys_sorted, N);
     rc |= dalloc(h, &w.ids_sorted, N);
     // compact bucket table: at most N occupied buckets in 4 N or more slots (a power of two)
@@ -1138,10 +1247,12 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t nbuf
     NLPF_HIP(h, hipHostMalloc((void**)&w.host, sizeof(uint32_t) * SLOT_END, hipHostMallocDefault));
     NLPF_HIP(h, hipHostMalloc((void**)&w.host_nbr, sizeof(unsigned long long) * 64 * NBR_SLOTS,
                               hipHostMallocDefault));
+    NLPF_HIP(h, hipEventCreateWithFlags(&w.readback, hipEventDisableTiming));
     // temp storage for the hipcub primitives at the largest size
     size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
     hipcub::CountingInputIterator<uint32_t> it(0);
-    NLPF_HIP(h, hipcub::DeviceSelect::Flagged(nullptr, b1, it, w.flag, w.pts, w.counters, (int)N, h->stream));
+    NLPF_HIP(h, hipcub::DeviceSelect::If(nullptr, b1, it, w.pts, w.counters, (int)N, ReachesDepth{nullptr, 0},
+                                         h->stream));
     NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(nullptr, b2, w.keys, w.keys_sorted, w.pts, w.ids_sorted, (int)N, 0,
                                                    32, h->stream));
     NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(nullptr, b3, w.prop, w.prop_sorted, (int)N, 0, 64, h->stream));
@@ -1262,13 +1373,11 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     WorldWork& w = ws->w;
     int rc = world_reserve(h, w, N, std::max<uint64_t>(nbuf_elems, kmax), batch_max);
     if (rc) return rc;
-    const uint32_t table_bits = std::max(1, (int)std::ceil(std::log2((double)cfg.hash_table_size)));
     hipStream_t st = h->stream;
 
     NLPF_HIP(h, hipMemsetAsync(w.nbr, 0, sizeof(unsigned long long) * 64 * NBR_SLOTS, st));
     hipLaunchKernelGGL(w_rng_init_kernel, grid1(batch_max), dim3(256), 0, st, w.rng, (uint32_t)batch_max,
                        std_seed_from_u64(cfg.seed));
-    hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
     if (radiance_out)
         hipLaunchKernelGGL(w_fill_kernel, grid1(3 * N * L), dim3(256), 0, st, radiance_out, 3 * N * L, 0.0f);
     if ((rc = check_launch(h, "world setup"))) return rc;
@@ -1290,10 +1399,10 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     for (int d = (int)L - 1; d >= 0; --d) {
         const float* recd = rec + (uint64_t)d * WP * N;
         float* rad = radiance_out ? radiance_out + (uint64_t)d * 3 * N : w.rad;
-        hipLaunchKernelGGL(w_flag_kernel, grid1(N), dim3(256), 0, st, path_depth, N, d, w.flag);
         hipcub::CountingInputIterator<uint32_t> it(0);
         size_t tb = w.cub_bytes;
-        NLPF_HIP(h, hipcub::DeviceSelect::Flagged(w.cub, tb, it, w.flag, w.pts, w.counters, (int)N, st));
+        NLPF_HIP(h, hipcub::DeviceSelect::If(w.cub, tb, it, w.pts, w.counters, (int)N, ReachesDepth{path_depth, d},
+                                             st));
         const uint32_t n = points_at[d];  // the count the selection writes to w.counters[0]
         stt.points[d] = n;
         if (n > 0) {
@@ -1315,21 +1424,42 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             g.ctab = w.ctab;
             g.cmask = (uint32_t)((1ull << cbits) - 1);
             g.cshift = 32u - cbits;
-            hipLaunchKernelGGL(w_ctab_clear_kernel, grid1(1ull << cbits), dim3(256), 0, st, w.ctab,
-                               (uint32_t)(1ull << cbits));
-            hipLaunchKernelGGL(w_keys_kernel, grid1(n), dim3(256), 0, st, rec + (uint64_t)d * WP * N, N, w.pts, n, g,
-                               w.keys);
-            tb = w.cub_bytes;
-            NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.keys_sorted, w.pts, w.ids_sorted,
-                                                           (int)n, 0, (int)table_bits, st));
+            const uint64_t slots = 1ull << cbits;
             // slot arrays: occupied buckets, candidate bound, largest bucket (or cluster)
-            NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_OCCUPIED, 0, 3 * NBR_SLOTS * sizeof(uint32_t), st));
-            hipLaunchKernelGGL(w_ranges_kernel, grid1(n), dim3(256), 0, st, w.keys_sorted, w.ids_sorted, n, recd, N,
-                               new_flags + (uint64_t)d * N, g, w.pos_sorted, w.flags_sorted,
+            hipLaunchKernelGGL(w_grid_clear_kernel, grid1(std::max<uint64_t>(slots, 3 * NBR_SLOTS)), dim3(256), 0, st,
+                               w.ctab, (uint32_t)slots, w.counters + SLOT_OCCUPIED, 3 * NBR_SLOTS,
+                               w.counters + SLOT_CURSOR);
+            hipLaunchKernelGGL(w_insert_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g, w.keys, w.arrival,
                                w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST);
-            hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind,
-                               new_flags + (uint64_t)d * N, material + (uint64_t)d * N, w.pk);
+            // occupied buckets and the largest bucket to the host while the payload pack runs
+            uint32_t* hc = w.host + SLOT_OCCUPIED;  // occupied buckets, (candidate bound), largest bucket or cluster
+            NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(uint32_t) * 3 * NBR_SLOTS,
+                                       hipMemcpyDeviceToHost, st));
+            NLPF_HIP(h, hipEventRecord(w.readback, st));
+            hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind, path_depth,
+                               d + 1 < (int)L ? (int32_t)d : INT32_MAX, new_flags + (uint64_t)d * N,
+                               material + (uint64_t)d * N, w.pk);
             if ((rc = check_launch(h, "world grid"))) return rc;
+            NLPF_HIP(h, hipEventSynchronize(w.readback));
+            const uint32_t occupied = slot_sum(hc);
+            uint32_t largest = slot_max(hc + 2 * NBR_SLOTS);
+            if (largest <= kEmitMaxBucket && cfg.world_grid != "sort") {
+                hipLaunchKernelGGL(w_slot_ranges_kernel, grid1(slots, 256 * kRangeSlotsPerThread), dim3(256), 0, st,
+                                   w.ctab, slots, w.counters + SLOT_CURSOR);
+                hipLaunchKernelGGL(w_scatter_kernel, grid1(n), dim3(256), 0, st, w.keys, w.arrival, n, w.ctab,
+                                   w.order);
+                hipLaunchKernelGGL(w_emit_kernel, grid1(n), dim3(256), 0, st, w.order, w.keys, w.pts, n, w.ctab, recd,
+                                   N, new_flags + (uint64_t)d * N, w.keys_sorted, w.ids_sorted, w.pos_sorted,
+                                   w.flags_sorted);
+            } else {
+                tb = w.cub_bytes;
+                NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.order, w.pts, w.ids_sorted,
+                                                               (int)n, 0, (int)cbits, st));
+                hipLaunchKernelGGL(w_sorted_emit_kernel, grid1(n), dim3(256), 0, st, w.order, w.ids_sorted, n, w.ctab,
+                                   recd, N, new_flags + (uint64_t)d * N, w.keys_sorted, w.pos_sorted,
+                                   w.flags_sorted);
+            }
+            if ((rc = check_launch(h, "world grid ranges"))) return rc;
             const uint32_t K = cfg.reuse_at((uint32_t)d);
             if (cfg.cluster) {
                 NLPF_HIP(h, hipMemsetAsync(w.cluster_id, 0xFF, sizeof(uint32_t) * N, st));
@@ -1386,13 +1516,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, w.pts, n, w.cluster_id,
                                    w.cluster_size, w.counters + SLOT_LARGEST);
                 if ((rc = check_launch(h, "world cluster size"))) return rc;
+                NLPF_HIP(h, hipMemcpyAsync(hc + 2 * NBR_SLOTS, w.counters + SLOT_LARGEST, sizeof(uint32_t) * NBR_SLOTS,
+                                           hipMemcpyDeviceToHost, st));
+                NLPF_HIP(h, hipStreamSynchronize(st));
+                largest = slot_max(hc + 2 * NBR_SLOTS);
             }
-            uint32_t* hc = w.host + SLOT_OCCUPIED;  // occupied buckets, (candidate bound), largest bucket or cluster
-            NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(uint32_t) * 3 * NBR_SLOTS,
-                                       hipMemcpyDeviceToHost, st));
-            NLPF_HIP(h, hipStreamSynchronize(st));
-            const uint32_t occupied = slot_sum(hc);
-            const uint64_t bound = 27ull * slot_max(hc + 2 * NBR_SLOTS);
+            const uint64_t bound = 27ull * largest;
             // kernel choice: records lie on surfaces, so about 9 of the 27 visited cells are
             // occupied and a vertex meets ~9 n / (occupied buckets) candidates; a wavefront per
             // vertex pays off from ~16 of them (with clusters, the cluster cap bounds the set)
@@ -1466,12 +1595,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             }
             if ((rc = check_launch(h, "world filter"))) return rc;
         }
-        if (d > 0) {
-            hipLaunchKernelGGL(w_fill_kernel, grid1(8 * N), dim3(256), 0, st, w.ind, 8 * N, 0.0f);
-            if (n > 0)
-                hipLaunchKernelGGL(w_propagate_kernel, grid1(n), dim3(256), 0, st, recd,
-                                   rec + (uint64_t)(d - 1) * WP * N, N, w.pts, n, rad, w.ind);
-        }
+        if (d > 0 && n > 0)  // indirect records of depth d - 1 (only the pack of that depth reads them)
+            hipLaunchKernelGGL(w_propagate_kernel, grid1(n), dim3(256), 0, st, recd, rec + (uint64_t)(d - 1) * WP * N,
+                               N, w.pts, n, rad, w.ind);
         if ((rc = check_launch(h, "world propagate"))) return rc;
         rc = record_event(h, 1 + (L - 1 - d));
         if (rc) return rc;

@@ -64,7 +64,7 @@ DEFAULT_CONFIG = {
     "disable_flag_check": False, "cluster": True, "scene_bounds": None,
     "mode": "world", "pixel_radius": 7, "sigma_pos": 0.05, "sigma_normal": 0.9,
     "sigma_lum": 0.5, "precision": "fp32", "seed": 0, "hash_table_size": 64 * 1024 * 1024,
-    "world_kernel": "auto", "screen_kernel": "auto",
+    "world_kernel": "auto", "world_grid": "auto", "screen_kernel": "auto",
 }
 
 

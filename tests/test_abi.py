@@ -62,6 +62,7 @@ def test_struct_layouts_match_header(tmp_path):
     ('{"sigma_pos":0}', "sigma_pos"),
     ('{"precision":"bf16"}', "precision"),
     ('{"world_kernel":"warp"}', "world_kernel"),
+    ('{"world_grid":"hash"}', "world_grid"),
     ('{"screen_kernel":"rows"}', "screen_kernel"),
     ('{"hash_table_size":0}', "hash_table_size"),
 ])

@@ -128,6 +128,31 @@ def test_world_bucket_collisions_and_flag_gate():
         assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
 
 
+@pytest.mark.parametrize("extra", [
+    {"cluster": False, "radius_scale": [2.0, 3.0]},
+    {"cluster": True, "radius_scale": [2.5], "reuse": [24], "hash_table_size": 13},
+    {"cluster": False, "radius_scale": [2.5], "hash_table_size": 1},
+])
+def test_world_grid_builds_agree(extra):
+    """The counting grid build and the radix-sort build ("world_grid": "sort") give bit-identical
+    films, radiance and neighbour counts: they differ only in where whole buckets sit in the entry
+    arrays.  With one bucket for the whole grid (1536 entries at depth 0, above the counting
+    build's 1024) the default build sorts as well, checked against the oracle."""
+    W, H, branch, L = 32, 24, 2, 3
+    cfg = {"type": "pf", "max_depth": L - 1, "branch": branch, "reuse": [1 << 16]}
+    cfg.update(extra)
+    nodes, roots = styles()
+    film0, rad0, host, st0 = run_world(dict(cfg, world_grid="auto"), W, H, branch, L, nodes, roots, seed=9)
+    film1, rad1, _, st1 = run_world(dict(cfg, world_grid="sort"), W, H, branch, L, nodes, roots, seed=9)
+    assert np.array_equal(film0, film1) and np.array_equal(rad0, rad1)
+    assert st0["neighbors"][:L] == st1["neighbors"][:L] and sum(st0["neighbors"][:L]) > 0
+    if extra.get("hash_table_size") == 1:
+        assert st0["points"][0] > 1024
+        ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots)
+        check(film0, rad0, ofilm, orad, host, L)
+        assert [st0["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]]
+
+
 def test_world_toon_depth1_mostly_exact():
     """Config 1: 4-level toon (w = 0.25) on every diffuse material, active at depth 1 only.
     Band-edge flips (1 FP32 ulp) may change a handful of vertices; everything else is exact."""
