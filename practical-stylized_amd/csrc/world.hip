@@ -128,27 +128,101 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // or maxes the slots in the readback it makes anyway.
 constexpr uint32_t NBR_SLOTS = 256;
 
-// paths per last depth: hist[k] = #{t : path_depth[t] == k} (k >= 31 counted at 31: depths stop
-// at 30), so a depth's point count is a suffix sum (one readback for the whole pass)
-__global__ void __launch_bounds__(256) w_depth_hist_kernel(const int32_t* path_depth, uint64_t N, uint32_t* hist) {
-    __shared__ uint32_t lh[32];
-    if (threadIdx.x < 32) lh[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < N) {
-        const int32_t k = path_depth[t];
-        if (k >= 0) atomicAdd(lh + (k < 31 ? k : 31), 1u);
+// ---- the points of every depth, selected once per pass ----
+// Depth d's points are the paths whose last depth reaches d (a last depth beyond the pass's
+// levels counts as the deepest level), in path order. Paths are taken in blocks of kSelBlock:
+// w_sel_count_kernel counts each block's points per depth, w_sel_offsets_kernel scans the counts
+// of each depth over the blocks (one workgroup) and totals them, and w_sel_scatter_kernel writes
+// each path into the list of every depth it reaches, at its block's offset plus its rank in the
+// block. Depth d's list starts at base[d] = the points of the depths below d.
+constexpr uint32_t kSelBlock = 1024;  // paths per block: 256 threads x 4, path b * 1024 + 256 j + thread
+struct SelBases {
+    uint32_t base[32];
+};
+__device__ __forceinline__ int32_t sel_depth(const int32_t* path_depth, uint64_t N, uint64_t t, int32_t last) {
+    if (t >= N) return -1;
+    const int32_t k = path_depth[t];
+    return k < last ? k : last;  // k < 0: a path without vertices
+}
+// cnt[d * nblk + b] = points of block b at depth d
+__global__ void __launch_bounds__(256) w_sel_count_kernel(const int32_t* path_depth, uint64_t N, uint32_t L,
+                                                          uint32_t* cnt, uint32_t nblk) {
+    __shared__ uint32_t wc[32][4];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int32_t k[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        k[j] = sel_depth(path_depth, N, (uint64_t)blockIdx.x * kSelBlock + 256u * j + threadIdx.x, (int32_t)L - 1);
+    for (uint32_t d = 0; d < L; ++d) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c += (uint32_t)__popcll(__ballot(k[j] >= (int32_t)d));
+        if (lane == 0) wc[d][wv] = c;
     }
     __syncthreads();
-    if (threadIdx.x < 32 && lh[threadIdx.x]) atomicAdd(hist + threadIdx.x, lh[threadIdx.x]);
+    if (threadIdx.x < L) cnt[(uint64_t)threadIdx.x * nblk + blockIdx.x] = wc[threadIdx.x][0] + wc[threadIdx.x][1] +
+                                                                          wc[threadIdx.x][2] + wc[threadIdx.x][3];
 }
-
-// selection predicate of depth d's points: the paths whose last depth reaches d
-struct ReachesDepth {
-    const int32_t* path_depth;
-    int32_t d;
-    __host__ __device__ bool operator()(uint32_t i) const { return path_depth[i] >= d; }
-};
+// cnt[d * nblk + b] -> exclusive prefix over the blocks; total[d] = points at depth d
+__global__ void __launch_bounds__(1024) w_sel_offsets_kernel(uint32_t* cnt, uint32_t nblk, uint32_t L,
+                                                             uint32_t* total) {
+    __shared__ uint32_t wave_total[16];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (uint32_t d = 0; d < L; ++d) {
+        uint32_t running = 0;  // uniform
+        for (uint32_t b0 = 0; b0 < nblk; b0 += 1024) {
+            const uint32_t b = b0 + threadIdx.x;
+            const uint32_t v = b < nblk ? cnt[(uint64_t)d * nblk + b] : 0u;
+            uint32_t incl = v;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)incl, off);
+                if (lane >= (uint32_t)off) incl += u;
+            }
+            if (lane == 63) wave_total[wv] = incl;
+            __syncthreads();
+            uint32_t before = running, chunk = 0;
+            for (uint32_t k = 0; k < 16; ++k) {
+                if (k < wv) before += wave_total[k];
+                chunk += wave_total[k];
+            }
+            if (b < nblk) cnt[(uint64_t)d * nblk + b] = before + incl - v;
+            running += chunk;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) total[d] = running;
+    }
+}
+__global__ void __launch_bounds__(256) w_sel_scatter_kernel(const int32_t* path_depth, uint64_t N, uint32_t L,
+                                                            const uint32_t* off, uint32_t nblk, SelBases sb,
+                                                            uint32_t* pts) {
+    __shared__ uint32_t wc[4][4];  // [item j][wave]: this depth's points
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kSelBlock + threadIdx.x;
+    int32_t k[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) k[j] = sel_depth(path_depth, N, t0 + 256u * j, (int32_t)L - 1);
+    for (uint32_t d = 0; d < L; ++d) {
+        uint64_t m[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            m[j] = __ballot(k[j] >= (int32_t)d);
+            if (lane == 0) wc[j][wv] = (uint32_t)__popcll(m[j]);
+        }
+        __syncthreads();
+        // rank of item (j, wave, lane) = the block's points before it in path order
+        uint32_t r = sb.base[d] + off[(uint64_t)d * nblk + blockIdx.x];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t at = r;
+            for (uint32_t w = 0; w < wv; ++w) at += wc[j][w];
+            if (k[j] >= (int32_t)d) pts[at + (uint32_t)__popcll(m[j] & below)] = (uint32_t)(t0 + 256u * j);
+            r += wc[j][0] + wc[j][1] + wc[j][2] + wc[j][3];
+        }
+        __syncthreads();
+    }
+}
 
 // ---- grid build: a counting sort on the compact table's slots ----
 // Every point finds or claims its bucket's slot and counts itself there, keeping the count it
@@ -170,9 +244,12 @@ __global__ void w_grid_clear_kernel(uint4* ctab, uint32_t slots, uint32_t* count
     if (s == 0) *cursor = 0u;
 }
 
-// slot_of[t] = slot of point t's bucket, arrival[t] = points of that bucket counted before it;
-// the slot's third word counts the bucket's points. Claimed slots count the occupied buckets and
-// the largest arrival rank bounds the largest bucket (both over NBR_SLOTS slots).
+// slot_of[t] = slot of point t's bucket, arrival[t] = points of that bucket counted before it.
+// While the table is built a slot's first two words are one 64-bit word {bucket, count}: a
+// point claims an empty slot with count 1 in one compare-and-swap, and a point whose swap finds
+// its bucket there adds 1 to the count (two atomics; no plain probe loads). Claimed slots count
+// the occupied buckets and the largest arrival rank bounds the largest bucket (both over
+// NBR_SLOTS slots).
 __global__ void __launch_bounds__(256) w_insert_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n,
                                                        Grid g, uint32_t* slot_of, uint32_t* arrival,
                                                        uint32_t* occupied, uint32_t* max_bucket) {
@@ -183,21 +260,21 @@ __global__ void __launch_bounds__(256) w_insert_kernel(const float* rec, uint64_
         uint32_t c[3];
         get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
         const uint32_t k = cell_bucket(c[0], c[1], c[2], g.table);
-        uint32_t s = ctab_home(g, k);
+        const unsigned long long empty = (unsigned long long)kEmptyKey, mine = (1ull << 32) | k;
+        uint32_t s = ctab_home(g, k), r = 0;
         for (;;) {  // ends: the table is at most a quarter full
-            const uint32_t v = __hip_atomic_load(&g.ctab[s].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (v == k) break;
-            if (v == kEmptyKey) {
-                const uint32_t old = atomicCAS(&g.ctab[s].x, kEmptyKey, k);
-                if (old == kEmptyKey) {
-                    claimed = 1;
-                    break;
-                }
-                if (old == k) break;
+            unsigned long long* w = reinterpret_cast<unsigned long long*>(&g.ctab[s]);
+            const unsigned long long old = atomicCAS(w, empty, mine);
+            if ((uint32_t)old == kEmptyKey) {
+                claimed = 1;
+                break;
+            }
+            if ((uint32_t)old == k) {
+                r = (uint32_t)(atomicAdd(w, 1ull << 32) >> 32);
+                break;
             }
             s = (s + 1) & g.cmask;
         }
-        const uint32_t r = atomicAdd(&g.ctab[s].z, 1u);
         slot_of[t] = s;
         arrival[t] = r;
         size = r + 1;
@@ -213,7 +290,7 @@ __global__ void __launch_bounds__(256) w_insert_kernel(const float* rec, uint64_
 
 // bucket ranges from the slot counts: a workgroup takes 4096 slots (lane-interleaved, 16 per
 // thread), scans their counts, claims room for its total from *cursor (one atomic per
-// workgroup) and writes each occupied slot's [start, end) over its count
+// workgroup) and writes each occupied slot's [start, end) over {count, 0}
 constexpr uint32_t kRangeSlotsPerThread = 16;
 __global__ void __launch_bounds__(256) w_slot_ranges_kernel(uint4* ctab, uint64_t slots, uint32_t* cursor) {
     __shared__ uint32_t wave_total[4], block_base;
@@ -224,7 +301,7 @@ __global__ void __launch_bounds__(256) w_slot_ranges_kernel(uint4* ctab, uint64_
 #pragma unroll
     for (uint32_t k = 0; k < kRangeSlotsPerThread; ++k) {
         const uint64_t s = s0 + (uint64_t)k * 256;
-        cnt[k] = s < slots ? ctab[s].z : 0u;
+        cnt[k] = s < slots ? ctab[s].y : 0u;
         sum += cnt[k];
     }
     uint32_t incl = sum;  // inclusive scan over the wavefront
@@ -299,7 +376,7 @@ __global__ void w_sorted_emit_kernel(const uint32_t* slot_sorted, const uint32_t
     pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
     flags_sorted[e] = new_flags[i];
     if (e + 1 == n || slot_sorted[e + 1] != s) {
-        const uint32_t count = ctab[s].z;
+        const uint32_t count = ctab[s].y;
         ctab[s].y = e + 1 - count;
         ctab[s].z = e + 1;
     }
@@ -1141,17 +1218,19 @@ enum GridSlots : uint32_t {
     SLOT_OCCUPIED = 8 + NBR_SLOTS,
     SLOT_CAND = 8 + 2 * NBR_SLOTS,
     SLOT_LARGEST = 8 + 3 * NBR_SLOTS,
-    SLOT_DEPTHS = 8 + 4 * NBR_SLOTS,  // paths per last depth 0..31
+    SLOT_DEPTHS = 8 + 4 * NBR_SLOTS,  // points per depth 0..31
     SLOT_END = 8 + 4 * NBR_SLOTS + 32
 };
 
 struct WorldWork {
     uint64_t N = 0, nbuf_elems = 0, rng_n = 0, cub_bytes = 0;
+    // point lists of every depth (pts_cap entries) and the selection's per-block counts;
     // grid build: pts (a depth's paths), keys (slot of each point), arrival (its arrival rank),
     // order (scattered points, or slots sorted on the radix-sort path), then per entry in bucket
     // order: keys_sorted (bucket id), ids_sorted (path id), pos_sorted, flags_sorted
     uint32_t *pts = nullptr, *keys = nullptr, *arrival = nullptr, *order = nullptr, *keys_sorted = nullptr,
-             *ids_sorted = nullptr;
+             *ids_sorted = nullptr, *sel_cnt = nullptr;
+    uint64_t pts_cap = 0;
     uint4* ctab = nullptr;  // compact bucket table, ctab_slots slots
     uint64_t ctab_slots = 0;
     uint32_t *cluster_id = nullptr, *cluster_size = nullptr, *counters = nullptr, *nbuf = nullptr;
@@ -1172,7 +1251,7 @@ struct WorldWork {
 };
 
 static void world_free(WorldWork& w) {
-    void* ps[] = {w.pts, w.keys, w.arrival, w.order, w.keys_sorted, w.ids_sorted, w.ctab,
+    void* ps[] = {w.pts, w.sel_cnt, w.keys, w.arrival, w.order, w.keys_sorted, w.ids_sorted, w.ctab,
                   w.cluster_id, w.cluster_size, w.counters, w.nbuf, w.prop, w.prop_sorted, w.nbr,
                   w.pos_sorted, w.flags_sorted, w.pk, w.ind, w.rad, w.rng, w.seed_flags, w.seed_rank, w.seed_entry,
                   w.smis_sum, w.cub};
@@ -1216,7 +1295,9 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t nbuf
     NLPF_HIP(h, hipStreamSynchronize(h->stream));
     world_free(w);
     int rc = 0;
-    rc |= dalloc(h, &w.pts, N);
+    rc |= dalloc(h, &w.pts, N);  // grown by nlpf_world_run when the depths hold more points
+    w.pts_cap = N;
+    rc |= dalloc(h, &w.sel_cnt, 32 * ((N + kSelBlock - 1) / kSelBlock));
     rc |= dalloc(h, &w.keys, N);
     rc |= dalloc(h, &w.arrival, N);
     rc |= dalloc(h, &w.order, N);
@@ -1249,15 +1330,12 @@ static int world_reserve(nlpf_handle* h, WorldWork& w, uint64_t N, uint64_t nbuf
                               hipHostMallocDefault));
     NLPF_HIP(h, hipEventCreateWithFlags(&w.readback, hipEventDisableTiming));
     // temp storage for the hipcub primitives at the largest size
-    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-    hipcub::CountingInputIterator<uint32_t> it(0);
-    NLPF_HIP(h, hipcub::DeviceSelect::If(nullptr, b1, it, w.pts, w.counters, (int)N, ReachesDepth{nullptr, 0},
-                                         h->stream));
+    size_t b2 = 0, b3 = 0, b4 = 0;
     NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(nullptr, b2, w.keys, w.keys_sorted, w.pts, w.ids_sorted, (int)N, 0,
                                                    32, h->stream));
     NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(nullptr, b3, w.prop, w.prop_sorted, (int)N, 0, 64, h->stream));
     NLPF_HIP(h, hipcub::DeviceScan::ExclusiveSum(nullptr, b4, w.seed_flags, w.seed_rank, (int)(N + 1), h->stream));
-    w.cub_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
+    w.cub_bytes = std::max(b2, std::max(b3, b4));
     NLPF_HIP(h, hipMalloc(&w.cub, w.cub_bytes));
     w.N = N;
     w.nbuf_elems = nbuf_elems;
@@ -1383,15 +1461,35 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     if ((rc = check_launch(h, "world setup"))) return rc;
     rc = record_event(h, 0);
     if (rc) return rc;
-    // points of every depth in one readback: #{t : path_depth[t] >= d}
-    NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_DEPTHS, 0, 32 * sizeof(uint32_t), st));
-    hipLaunchKernelGGL(w_depth_hist_kernel, grid1(N), dim3(256), 0, st, path_depth, N, w.counters + SLOT_DEPTHS);
+    // the point lists of every depth: counts per block, their offsets and the totals (one
+    // readback), then the lists
+    const uint32_t nblk = (uint32_t)((N + kSelBlock - 1) / kSelBlock);
+    hipLaunchKernelGGL(w_sel_count_kernel, dim3(nblk), dim3(256), 0, st, path_depth, N, L, w.sel_cnt, nblk);
+    hipLaunchKernelGGL(w_sel_offsets_kernel, dim3(1), dim3(1024), 0, st, w.sel_cnt, nblk, L,
+                       w.counters + SLOT_DEPTHS);
     if ((rc = check_launch(h, "world depth counts"))) return rc;
-    NLPF_HIP(h, hipMemcpyAsync(w.host + SLOT_DEPTHS, w.counters + SLOT_DEPTHS, 32 * sizeof(uint32_t),
+    NLPF_HIP(h, hipMemcpyAsync(w.host + SLOT_DEPTHS, w.counters + SLOT_DEPTHS, L * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, st));
     NLPF_HIP(h, hipStreamSynchronize(st));
-    uint32_t points_at[33] = {};
-    for (int k = 31; k >= 0; --k) points_at[k] = points_at[k + 1] + w.host[SLOT_DEPTHS + k];
+    uint32_t points_at[32] = {};
+    SelBases sb{};
+    uint64_t listed = 0;
+    for (uint32_t d = 0; d < L; ++d) {
+        points_at[d] = w.host[SLOT_DEPTHS + d];
+        sb.base[d] = (uint32_t)listed;
+        listed += points_at[d];
+    }
+    if (listed > 0xFFFFFFFFull) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "world desc: more than 2^32 vertices over the levels");
+    if (listed > w.pts_cap) {
+        (void)hipFree(w.pts);
+        w.pts = nullptr;
+        w.pts_cap = 0;
+        if ((rc = dalloc(h, &w.pts, listed))) return rc;
+        w.pts_cap = listed;
+    }
+    hipLaunchKernelGGL(w_sel_scatter_kernel, dim3(nblk), dim3(256), 0, st, path_depth, N, L, w.sel_cnt, nblk, sb,
+                       w.pts);
+    if ((rc = check_launch(h, "world point lists"))) return rc;
 
     nlpf_stats stt{};
     const float ext[3] = {bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]};
@@ -1399,11 +1497,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
     for (int d = (int)L - 1; d >= 0; --d) {
         const float* recd = rec + (uint64_t)d * WP * N;
         float* rad = radiance_out ? radiance_out + (uint64_t)d * 3 * N : w.rad;
-        hipcub::CountingInputIterator<uint32_t> it(0);
+        const uint32_t* pts = w.pts + sb.base[d];  // depth d's points, in path order
         size_t tb = w.cub_bytes;
-        NLPF_HIP(h, hipcub::DeviceSelect::If(w.cub, tb, it, w.pts, w.counters, (int)N, ReachesDepth{path_depth, d},
-                                             st));
-        const uint32_t n = points_at[d];  // the count the selection writes to w.counters[0]
+        const uint32_t n = points_at[d];
         stt.points[d] = n;
         if (n > 0) {
             Grid g;
@@ -1429,14 +1525,14 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             hipLaunchKernelGGL(w_grid_clear_kernel, grid1(std::max<uint64_t>(slots, 3 * NBR_SLOTS)), dim3(256), 0, st,
                                w.ctab, (uint32_t)slots, w.counters + SLOT_OCCUPIED, 3 * NBR_SLOTS,
                                w.counters + SLOT_CURSOR);
-            hipLaunchKernelGGL(w_insert_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g, w.keys, w.arrival,
+            hipLaunchKernelGGL(w_insert_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g, w.keys, w.arrival,
                                w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST);
             // occupied buckets and the largest bucket to the host while the payload pack runs
             uint32_t* hc = w.host + SLOT_OCCUPIED;  // occupied buckets, (candidate bound), largest bucket or cluster
             NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(uint32_t) * 3 * NBR_SLOTS,
                                        hipMemcpyDeviceToHost, st));
             NLPF_HIP(h, hipEventRecord(w.readback, st));
-            hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, w.ind, path_depth,
+            hipLaunchKernelGGL(w_pack_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, w.ind, path_depth,
                                d + 1 < (int)L ? (int32_t)d : INT32_MAX, new_flags + (uint64_t)d * N,
                                material + (uint64_t)d * N, w.pk);
             if ((rc = check_launch(h, "world grid"))) return rc;
@@ -1448,12 +1544,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                    w.ctab, slots, w.counters + SLOT_CURSOR);
                 hipLaunchKernelGGL(w_scatter_kernel, grid1(n), dim3(256), 0, st, w.keys, w.arrival, n, w.ctab,
                                    w.order);
-                hipLaunchKernelGGL(w_emit_kernel, grid1(n), dim3(256), 0, st, w.order, w.keys, w.pts, n, w.ctab, recd,
+                hipLaunchKernelGGL(w_emit_kernel, grid1(n), dim3(256), 0, st, w.order, w.keys, pts, n, w.ctab, recd,
                                    N, new_flags + (uint64_t)d * N, w.keys_sorted, w.ids_sorted, w.pos_sorted,
                                    w.flags_sorted);
             } else {
                 tb = w.cub_bytes;
-                NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.order, w.pts, w.ids_sorted,
+                NLPF_HIP(h, hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.keys, w.order, pts, w.ids_sorted,
                                                                (int)n, 0, (int)cbits, st));
                 hipLaunchKernelGGL(w_sorted_emit_kernel, grid1(n), dim3(256), 0, st, w.order, w.ids_sorted, n, w.ctab,
                                    recd, N, new_flags + (uint64_t)d * N, w.keys_sorted, w.pos_sorted,
@@ -1467,9 +1563,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 for (int outer = 0;; ++outer) {
                     if (outer > 4096) NLPF_FAIL(h, NLPF_E_NUMERIC, "clustering did not converge");
                     // per-bucket seeds live in the compact table's fourth word
-                    hipLaunchKernelGGL(w_cluster_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                    hipLaunchKernelGGL(w_cluster_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
                                        w.cluster_id);
-                    hipLaunchKernelGGL(w_cluster_mark_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                    hipLaunchKernelGGL(w_cluster_mark_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
                                        w.cluster_id);
                     // seed entries of every bucket, for the propose walks of this round
                     hipLaunchKernelGGL(w_seed_flags_kernel, grid1((uint64_t)n + 1), dim3(256), 0, st, w.keys_sorted,
@@ -1482,7 +1578,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     uint32_t n_uncl = 0;
                     for (int inner = 0;; ++inner) {
                         NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * (SLOT_OPEN + NBR_SLOTS), st));
-                        hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                        hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
                                            w.pos_sorted, w.seed_rank, w.seed_entry, w.cluster_size,
                                            K, w.cluster_id, w.prop, w.counters + 1, w.counters + SLOT_OPEN);
                         if ((rc = check_launch(h, "world cluster"))) return rc;
@@ -1513,7 +1609,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             // bucket, and so its members, more than once), else counted exactly.
             if (cfg.cluster) {
                 NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_LARGEST, 0, NBR_SLOTS * sizeof(uint32_t), st));
-                hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, w.pts, n, w.cluster_id,
+                hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, pts, n, w.cluster_id,
                                    w.cluster_size, w.counters + SLOT_LARGEST);
                 if ((rc = check_launch(h, "world cluster size"))) return rc;
                 NLPF_HIP(h, hipMemcpyAsync(hc + 2 * NBR_SLOTS, w.counters + SLOT_LARGEST, sizeof(uint32_t) * NBR_SLOTS,
@@ -1531,7 +1627,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                       std::min<double>(cand, cfg.cluster ? (double)K + 1.0 : cand) >= 16.0);
             uint64_t max_cand = bound;
             if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
-                hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, w.pts, n, g,
+                hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
                                    cfg.cluster ? w.cluster_id : nullptr, w.counters + SLOT_CAND);
                 if ((rc = check_launch(h, "world candidates"))) return rc;
                 NLPF_HIP(h, hipMemcpyAsync(hc + NBR_SLOTS, w.counters + SLOT_CAND, NBR_SLOTS * sizeof(uint32_t),
@@ -1551,7 +1647,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     FilterArgs fa;
                     fa.rec = recd;
                     fa.N = N;
-                    fa.pts = w.pts;
+                    fa.pts = pts;
                     fa.off = (uint32_t)off;
                     fa.count = cnt;
                     fa.new_flags = new_flags + (uint64_t)d * N;
@@ -1597,7 +1693,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
         }
         if (d > 0 && n > 0)  // indirect records of depth d - 1 (only the pack of that depth reads them)
             hipLaunchKernelGGL(w_propagate_kernel, grid1(n), dim3(256), 0, st, recd, rec + (uint64_t)(d - 1) * WP * N,
-                               N, w.pts, n, rad, w.ind);
+                               N, pts, n, rad, w.ind);
         if ((rc = check_launch(h, "world propagate"))) return rc;
         rc = record_event(h, 1 + (L - 1 - d));
         if (rc) return rc;
