@@ -119,9 +119,11 @@ typedef struct nlpf_handle nlpf_handle;
  * disable_flag_check false, cluster true, scene_bounds null) plus the screen-mode fields
  * mode ("world"|"screen", default "world"), pixel_radius (7), sigma_pos (0.05),
  * sigma_normal (0.9), sigma_lum (0.5), precision ("fp32"|"fp16"|"fp64"), seed (0),
- * hash_table_size (64 Mi buckets) and world_kernel ("auto"|"thread"|"wave": world-mode filter
+ * hash_table_size (64 Mi buckets), world_kernel ("auto"|"thread"|"wave": world-mode filter
  * with one thread or one wavefront per vertex, same results; "auto" picks by the expected
- * neighbour count).
+ * neighbour count), world_grid ("auto"|"sort": the world-mode grid's counting build, or the
+ * radix sort at every depth; bit-identical results) and screen_kernel ("auto"|"tiles"|"strips":
+ * the screen-mode level kernel; results agree within FP32 summation order).
  * Unknown keys are ignored, as serde does for the reference (no deny_unknown_fields). */
 /* Replaces PathFiltering::new / Config deserialisation (lm_path_filtering.rs:264-298, 1317)
  * and the device setup of lm_path_filtering::render (:1802-1813). */
