@@ -445,3 +445,33 @@ def test_set_bsdfs_rejects_what_needs_the_albedo_table():
         h.set_bsdfs(N.bsdf_table([{"kind": N.BSDF_PRINCIPLED, "metallic": 1.0, "specular_ior_level": 0.5,
                                    "transmission": 1.0}]))
         h.set_bsdfs(N.bsdf_table([]))
+
+
+def test_world_random_configs():
+    """Eight random world-mode cases against the oracle: frame shape (width 3..36, height
+    2..30, 1..3 paths per pixel), 1..4 levels, radius_scale 1.5..3, clustering and SMIS on or
+    off, a colliding, a sparse or the default bucket table, either filter kernel, reuse caps
+    from 6 (reservoir draws over batches of 1 MB) to 2^16.  Neighbour counts exactly, values by
+    the world-mode rule."""
+    rng = np.random.default_rng(2027)
+    nodes, roots = styles()
+    for case in range(8):
+        W, H = int(rng.integers(3, 37)), int(rng.integers(2, 31))
+        branch, L = int(rng.integers(1, 4)), int(rng.integers(1, 5))
+        cfg = {"type": "pf", "max_depth": L - 1, "branch": branch,
+               "radius_scale": [float(rng.uniform(1.5, 3.0))],
+               "cluster": bool(rng.integers(0, 2)), "smis": bool(rng.integers(0, 2)),
+               "world_kernel": ["auto", "thread", "wave"][int(rng.integers(0, 3))],
+               "reuse": [[1 << 16], [64], [6]][int(rng.integers(0, 3))]}
+        table = [13, 1000003, None][int(rng.integers(0, 3))]
+        if table is not None:
+            cfg["hash_table_size"] = table
+        if cfg["reuse"] == [6]:
+            cfg["vram"] = 1
+        seed = int(rng.integers(1, 1000))
+        print("case", case, W, H, branch, L, cfg, "seed", seed, flush=True)
+        film, rad, host, st = run_world(cfg, W, H, branch, L, nodes, roots, seed=seed)
+        ofilm, orad, onbr = oracle_world(host, W, H, branch, L, cfg, nodes, roots)
+        assert [st["neighbors"][d] for d in range(L)] == [int(v) for v in onbr[:L]], case
+        sens = sensitivity(host, W, H, branch, L, cfg, nodes, roots, ofilm, orad, trials=2)
+        check(film, rad, ofilm, orad, host, L, sens=sens)
