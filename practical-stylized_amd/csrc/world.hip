@@ -1606,7 +1606,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             // point's candidate count can exceed K there are no draws, and the whole depth runs
             // as one launch with list slots sized by that count: first bounded by 27 x the
             // largest bucket (27 x the largest cluster when clustering: colliding cells walk one
-            // bucket, and so its members, more than once), else counted exactly.
+            // bucket, and so its members, more than once), else counted exactly when the depth
+            // spans more than one batch.
             if (cfg.cluster) {
                 NLPF_HIP(h, hipMemsetAsync(w.counters + SLOT_LARGEST, 0, NBR_SLOTS * sizeof(uint32_t), st));
                 hipLaunchKernelGGL(w_cluster_max_kernel, grid1(n), dim3(256), 0, st, pts, n, w.cluster_id,
@@ -1625,8 +1626,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             const bool wave_kernel = cfg.world_kernel == "wave" ||
                                      (cfg.world_kernel == "auto" &&
                                       std::min<double>(cand, cfg.cluster ? (double)K + 1.0 : cand) >= 16.0);
+            // the reference's batch (vram * 2^20 / (4 K) points); the exact candidate count can only
+            // merge several of them into one launch: with one batch the batched path is a single
+            // launch already, and its reservoir draws happen exactly when a count exceeds K
+            const uint64_t ref_batch = std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
             uint64_t max_cand = bound;
-            if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems)) {
+            if (!(bound <= K && (uint64_t)n * std::max<uint64_t>(bound, 1) <= w.nbuf_elems) && n > ref_batch) {
                 hipLaunchKernelGGL(w_cand_max_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
                                    cfg.cluster ? w.cluster_id : nullptr, w.counters + SLOT_CAND);
                 if ((rc = check_launch(h, "world candidates"))) return rc;
@@ -1636,8 +1641,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 max_cand = slot_max(hc + NBR_SLOTS);
             }
             const bool single = max_cand <= K && (uint64_t)n * std::max<uint64_t>(max_cand, 1) <= w.nbuf_elems;
-            const uint64_t batch =
-                single ? n : std::max<uint64_t>(1, cfg.vram * 1024 * 1024 / ((uint64_t)K * 4));
+            const uint64_t batch = single ? n : ref_batch;
             // clustering + SMIS on the wave kernel: a build pass over every batch fills the
             // shared SMIS denominators before the filter pass reads them
             const bool shared_smis = cfg.cluster && cfg.smis && wave_kernel;
