@@ -211,3 +211,29 @@ def test_mirror_constants_match_header():
     m = re.search(r"#define NLPF_NOALLOC_SCRATCH_BYTES \((\d+)ull << (\d+)\)", txt)
     assert m and int(m.group(1)) << int(m.group(2)) == N.NOALLOC_SCRATCH_BYTES
     assert re.search(r"#define NLPF_SCREEN_PLANES (\d+)", txt).group(1) == str(N.SCREEN_PLANES)
+
+
+def test_documented_rust_config_covers_every_field():
+    """The Rust Config facade of the integration notes names every field the library parses, with
+    the Python mirror's defaults."""
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = re.search(r"pub struct Config \{(.*?)\n\}", txt, flags=re.S).group(1)
+    fields = set(re.findall(r"pub (\w+)\s*:", body))
+    assert fields == set(N.DEFAULT_CONFIG), fields ^ set(N.DEFAULT_CONFIG)
+    dflt = re.search(r"impl Default for Config \{.*?Self \{(.*?)\}\s*\}\s*\}", txt, flags=re.S).group(1)
+    for k, v in N.DEFAULT_CONFIG.items():
+        m = re.search(rf"\b{k}: (vec!\[[^\]]*\]|\"[^\"]*\"\.into\(\)|[^,}}]+)", dflt)
+        assert m, k
+        got = m.group(1).strip()
+        if isinstance(v, bool):
+            assert got == str(v).lower(), (k, got)
+        elif isinstance(v, str):
+            assert got == f'"{v}".into()', (k, got)
+        elif v is None:
+            assert got == "None", (k, got)
+        elif isinstance(v, list):
+            assert [float(x) for x in re.findall(r"[\d.]+", got)] == [float(x) for x in v], (k, got)
+        elif k == "hash_table_size":
+            assert eval(got) == v, (k, got)
+        else:
+            assert float(got) == float(v), (k, got)
