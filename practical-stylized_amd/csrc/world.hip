@@ -249,14 +249,19 @@ __global__ void w_grid_clear_kernel(uint4* ctab, uint32_t slots, uint32_t* count
 // point claims an empty slot with count 1 in one compare-and-swap, and a point whose swap finds
 // its bucket there adds 1 to the count (two atomics; no plain probe loads). Claimed slots count
 // the occupied buckets and the largest arrival rank bounds the largest bucket (both over
-// NBR_SLOTS slots).
+// NBR_SLOTS slots). With clustering, the points also start unclustered (no cluster, no joiners).
 __global__ void __launch_bounds__(256) w_insert_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n,
                                                        Grid g, uint32_t* slot_of, uint32_t* arrival,
-                                                       uint32_t* occupied, uint32_t* max_bucket) {
+                                                       uint32_t* occupied, uint32_t* max_bucket, uint32_t* cluster_id,
+                                                       uint32_t* cluster_size) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t claimed = 0, size = 0;
     if (t < n) {
         const uint32_t i = pts[t];
+        if (cluster_id) {
+            cluster_id[i] = 0xFFFFFFFFu;
+            cluster_size[i] = 0u;
+        }
         uint32_t c[3];
         get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
         const uint32_t k = cell_bucket(c[0], c[1], c[2], g.table);
@@ -1091,9 +1096,12 @@ __global__ void w_seed_flags_kernel(const uint32_t* keys_sorted, const uint32_t*
     if (e < n && (e == 0 || keys_sorted[e - 1] != keys_sorted[e]))
         g.ctab[bucket_slot(g, keys_sorted[e])].w = kEmptyKey;
 }
+// (also zeroes the counters of the round's first propose pass: ncounters words)
 __global__ void w_seed_scatter_kernel(const uint32_t* flags, const uint32_t* seed_rank, uint32_t n,
-                                      uint32_t* seed_entry) {
+                                      uint32_t* seed_entry, uint32_t* counters, uint32_t ncounters) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < ncounters; k += blockDim.x) counters[k] = 0u;
     if (e < n && flags[e]) seed_entry[seed_rank[e]] = e;
 }
 // every unclustered vertex proposes to the first seed (visit order of :60-123) within the
@@ -1178,10 +1186,14 @@ __device__ __forceinline__ uint32_t run_start(const unsigned long long* prop_sor
     }
     return lo;
 }
-// accept the first (cap - size) proposals of each seed, in ascending vertex order
+// accept the first (cap - size) proposals of each seed, in ascending vertex order (also zeroes
+// the counters of the next propose pass, which the host has read: ncounters words)
 __global__ void w_cluster_accept_kernel(const unsigned long long* prop_sorted, uint32_t n, uint32_t* cluster_id,
-                                        uint32_t* cluster_size, uint32_t cap) {
+                                        uint32_t* cluster_size, uint32_t cap, uint32_t* counters,
+                                        uint32_t ncounters) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < ncounters; k += blockDim.x) counters[k] = 0u;
     if (e >= n) return;
     const uint32_t seed = (uint32_t)(prop_sorted[e] >> 32);
     const uint32_t rank = e - run_start(prop_sorted, e, seed);
@@ -1526,7 +1538,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                                w.ctab, (uint32_t)slots, w.counters + SLOT_OCCUPIED, 3 * NBR_SLOTS,
                                w.counters + SLOT_CURSOR);
             hipLaunchKernelGGL(w_insert_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g, w.keys, w.arrival,
-                               w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST);
+                               w.counters + SLOT_OCCUPIED, w.counters + SLOT_LARGEST,
+                               cfg.cluster ? w.cluster_id : nullptr, w.cluster_size);
             // occupied buckets and the largest bucket to the host while the payload pack runs
             uint32_t* hc = w.host + SLOT_OCCUPIED;  // occupied buckets, (candidate bound), largest bucket or cluster
             NLPF_HIP(h, hipMemcpyAsync(hc, w.counters + SLOT_OCCUPIED, sizeof(uint32_t) * 3 * NBR_SLOTS,
@@ -1557,9 +1570,7 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
             }
             if ((rc = check_launch(h, "world grid ranges"))) return rc;
             const uint32_t K = cfg.reuse_at((uint32_t)d);
-            if (cfg.cluster) {
-                NLPF_HIP(h, hipMemsetAsync(w.cluster_id, 0xFF, sizeof(uint32_t) * N, st));
-                NLPF_HIP(h, hipMemsetAsync(w.cluster_size, 0, sizeof(uint32_t) * N, st));
+            if (cfg.cluster) {  // the insert pass has marked every point unclustered
                 for (int outer = 0;; ++outer) {
                     if (outer > 4096) NLPF_FAIL(h, NLPF_E_NUMERIC, "clustering did not converge");
                     // per-bucket seeds live in the compact table's fourth word
@@ -1573,11 +1584,12 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                     tb = w.cub_bytes;
                     NLPF_HIP(h, hipcub::DeviceScan::ExclusiveSum(w.cub, tb, w.seed_flags, w.seed_rank, (int)(n + 1),
                                                                  st));
+                    // the propose counters (proposals at 1, open-vertex slots) are zeroed by the seed
+                    // scatter for a round's first pass and by the accept pass for the next one
                     hipLaunchKernelGGL(w_seed_scatter_kernel, grid1(n), dim3(256), 0, st, w.seed_flags, w.seed_rank, n,
-                                       w.seed_entry);
+                                       w.seed_entry, w.counters, SLOT_OPEN + NBR_SLOTS);
                     uint32_t n_uncl = 0;
                     for (int inner = 0;; ++inner) {
-                        NLPF_HIP(h, hipMemsetAsync(w.counters, 0, sizeof(uint32_t) * (SLOT_OPEN + NBR_SLOTS), st));
                         hipLaunchKernelGGL(w_cluster_propose_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
                                            w.pos_sorted, w.seed_rank, w.seed_entry, w.cluster_size,
                                            K, w.cluster_id, w.prop, w.counters + 1, w.counters + SLOT_OPEN);
@@ -1593,7 +1605,8 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                         NLPF_HIP(h, hipcub::DeviceRadixSort::SortKeys(w.cub, tb, w.prop, w.prop_sorted, (int)n_prop, 0,
                                                                       64, st));
                         hipLaunchKernelGGL(w_cluster_accept_kernel, grid1(n_prop), dim3(256), 0, st, w.prop_sorted,
-                                           n_prop, w.cluster_id, w.cluster_size, K);
+                                           n_prop, w.cluster_id, w.cluster_size, K, w.counters,
+                                           SLOT_OPEN + NBR_SLOTS);
                         hipLaunchKernelGGL(w_cluster_size_kernel, grid1(n_prop), dim3(256), 0, st, w.prop_sorted, n_prop,
                                            w.cluster_size, K);
                         if (inner > 1 << 20) NLPF_FAIL(h, NLPF_E_NUMERIC, "cluster joining did not converge");
