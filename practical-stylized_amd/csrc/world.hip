@@ -56,12 +56,6 @@ __device__ __forceinline__ uint2 bucket_range_from(const Grid& g, uint32_t b, ui
     }
 }
 __device__ __forceinline__ uint2 bucket_range(const Grid& g, uint32_t b) { return bucket_range_from(g, b, ctab_home(g, b)); }
-// slot of bucket b, which holds at least one entry
-__device__ __forceinline__ uint32_t bucket_slot(const Grid& g, uint32_t b) {
-    uint32_t s = ctab_home(g, b);
-    while (g.ctab[s].x != b) s = (s + 1) & g.cmask;
-    return s;
-}
 
 // HashGridVar::get_cell (:46-53)
 __device__ __forceinline__ void get_cell(const Grid& g, float px, float py, float pz, uint32_t c[3]) {
@@ -342,12 +336,12 @@ __global__ void w_scatter_kernel(const uint32_t* slot_of, const uint32_t* arriva
     if (t < n) order[ctab[slot_of[t]].y + arrival[t]] = t;
 }
 
-// per-entry arrays at the entry's place in bucket order: bucket id, path id, position with the
-// path id, stylization flags
-__device__ __forceinline__ void emit_entry(uint32_t f, uint32_t bucket, uint32_t i, const float* rec, uint64_t N,
+// per-entry arrays at the entry's place in bucket order: its bucket's table slot, path id,
+// position with the path id, stylization flags
+__device__ __forceinline__ void emit_entry(uint32_t f, uint32_t slot, uint32_t i, const float* rec, uint64_t N,
                                            const uint8_t* new_flags, uint32_t* keys_sorted, uint32_t* ids_sorted,
                                            float4* pos_sorted, uint8_t* flags_sorted) {
-    keys_sorted[f] = bucket;
+    keys_sorted[f] = slot;
     ids_sorted[f] = i;
     pos_sorted[f] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
     flags_sorted[f] = new_flags[i];
@@ -361,10 +355,11 @@ __global__ void w_emit_kernel(const uint32_t* order, const uint32_t* slot_of, co
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const uint32_t t = order[e];
-    const uint4 v = ctab[slot_of[t]];
+    const uint32_t s = slot_of[t];
+    const uint4 v = ctab[s];
     uint32_t f = v.y;
     for (uint32_t q = v.y; q < v.z; ++q) f += order[q] < t ? 1u : 0u;
-    emit_entry(f, v.x, pts[t], rec, N, new_flags, keys_sorted, ids_sorted, pos_sorted, flags_sorted);
+    emit_entry(f, s, pts[t], rec, N, new_flags, keys_sorted, ids_sorted, pos_sorted, flags_sorted);
 }
 
 // the radix-sort path: entries already sorted by slot (stable, so in path-id order inside a
@@ -376,8 +371,7 @@ __global__ void w_sorted_emit_kernel(const uint32_t* slot_sorted, const uint32_t
     if (e >= n) return;
     const uint32_t s = slot_sorted[e];
     const uint32_t i = ids_sorted_in[e];
-    const uint32_t bucket = ctab[s].x;
-    keys_sorted[e] = bucket;
+    keys_sorted[e] = s;
     pos_sorted[e] = make_float4(pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), __uint_as_float(i));
     flags_sorted[e] = new_flags[i];
     if (e + 1 == n || slot_sorted[e + 1] != s) {
@@ -1062,26 +1056,23 @@ __global__ void w_rng_init_kernel(Pcg32* rng, uint32_t n, StdKey key) {
 }
 
 // ---- deterministic clustering (replaces the CAS seeding / joining of :532-623) ----
-// seed per bucket = smallest unclustered path id in it
-__global__ void w_cluster_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+// seed per bucket = smallest unclustered path id in it (slot_of[t]: the table slot of point t's
+// bucket, from the grid build)
+__global__ void w_cluster_seed_kernel(const uint32_t* pts, const uint32_t* slot_of, uint32_t n, Grid g,
                                       const uint32_t* cluster_id) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = pts[t];
     if (cluster_id[i] != 0xFFFFFFFFu) return;
-    uint32_t c[3];
-    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
-    atomicMin(&g.ctab[bucket_slot(g, cell_bucket(c[0], c[1], c[2], g.table))].w, i);
+    atomicMin(&g.ctab[slot_of[t]].w, i);
 }
-__global__ void w_cluster_mark_seed_kernel(const float* rec, uint64_t N, const uint32_t* pts, uint32_t n, Grid g,
+__global__ void w_cluster_mark_seed_kernel(const uint32_t* pts, const uint32_t* slot_of, uint32_t n, Grid g,
                                            uint32_t* cluster_id) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = pts[t];
     if (cluster_id[i] != 0xFFFFFFFFu) return;
-    uint32_t c[3];
-    get_cell(g, pl(rec, N, 0, i), pl(rec, N, 1, i), pl(rec, N, 2, i), c);
-    if (g.ctab[bucket_slot(g, cell_bucket(c[0], c[1], c[2], g.table))].w == i) cluster_id[i] = i;
+    if (g.ctab[slot_of[t]].w == i) cluster_id[i] = i;
 }
 // seed index of one seeding round over the bucket-sorted entries: seed_rank[e] = number of
 // seed entries before e (exclusive scan of these flags), seed_entry[seed_rank[e]] = e
@@ -1093,8 +1084,7 @@ __global__ void w_seed_flags_kernel(const uint32_t* keys_sorted, const uint32_t*
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e > n) return;
     flags[e] = e < n && cluster_id[ids_sorted[e]] == ids_sorted[e] ? 1u : 0u;
-    if (e < n && (e == 0 || keys_sorted[e - 1] != keys_sorted[e]))
-        g.ctab[bucket_slot(g, keys_sorted[e])].w = kEmptyKey;
+    if (e < n && (e == 0 || keys_sorted[e - 1] != keys_sorted[e])) g.ctab[keys_sorted[e]].w = kEmptyKey;
 }
 // (also zeroes the counters of the round's first propose pass: ncounters words)
 __global__ void w_seed_scatter_kernel(const uint32_t* flags, const uint32_t* seed_rank, uint32_t n,
@@ -1239,7 +1229,8 @@ struct WorldWork {
     // point lists of every depth (pts_cap entries) and the selection's per-block counts;
     // grid build: pts (a depth's paths), keys (slot of each point), arrival (its arrival rank),
     // order (scattered points, or slots sorted on the radix-sort path), then per entry in bucket
-    // order: keys_sorted (bucket id), ids_sorted (path id), pos_sorted, flags_sorted
+    // order: keys_sorted (the table slot of the entry's bucket), ids_sorted (path id), pos_sorted,
+    // flags_sorted
     uint32_t *pts = nullptr, *keys = nullptr, *arrival = nullptr, *order = nullptr, *keys_sorted = nullptr,
              *ids_sorted = nullptr, *sel_cnt = nullptr;
     uint64_t pts_cap = 0;
@@ -1574,9 +1565,9 @@ extern "C" int nlpf_world_run(nlpf_handle* h, const nlpf_world_desc* desc, const
                 for (int outer = 0;; ++outer) {
                     if (outer > 4096) NLPF_FAIL(h, NLPF_E_NUMERIC, "clustering did not converge");
                     // per-bucket seeds live in the compact table's fourth word
-                    hipLaunchKernelGGL(w_cluster_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
+                    hipLaunchKernelGGL(w_cluster_seed_kernel, grid1(n), dim3(256), 0, st, pts, w.keys, n, g,
                                        w.cluster_id);
-                    hipLaunchKernelGGL(w_cluster_mark_seed_kernel, grid1(n), dim3(256), 0, st, recd, N, pts, n, g,
+                    hipLaunchKernelGGL(w_cluster_mark_seed_kernel, grid1(n), dim3(256), 0, st, pts, w.keys, n, g,
                                        w.cluster_id);
                     // seed entries of every bucket, for the propose walks of this round
                     hipLaunchKernelGGL(w_seed_flags_kernel, grid1((uint64_t)n + 1), dim3(256), 0, st, w.keys_sorted,
