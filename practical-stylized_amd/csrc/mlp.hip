@@ -188,6 +188,9 @@ __device__ __forceinline__ void mlp_forward_wave(const WF& wf, const BF& bf, bf1
                                                  f32x4 (&y)[MLP_QT]) {
     f32x4 acc[MLP_QT][4];
     bf16x8 xb[MLP_QT][2];
+    // each weight fragment is read from LDS one step ahead of the MFMAs that use it, so the read
+    // is in flight while the previous fragment's MFMAs run
+    bf16x8 w = wf(0, 0, 0);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const f32x4 bias = bf(0, t);
@@ -195,9 +198,11 @@ __device__ __forceinline__ void mlp_forward_wave(const WF& wf, const BF& bf, bf1
         for (int q = 0; q < MLP_QT; ++q) acc[q][t] = bias;
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            const bf16x8 w = wf(0, t, s);
+            const bf16x8 wn = s < 2 ? wf(0, t, s + 1) : (t < 3 ? wf(0, t + 1, 0) : wf(1, 0, 0));
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < MLP_QT; ++q) acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xin[q][s], acc[q][t], 0, 0, 0);
+            w = wn;
         }
     }
 #pragma unroll
@@ -214,10 +219,12 @@ __device__ __forceinline__ void mlp_forward_wave(const WF& wf, const BF& bf, bf1
             for (int q = 0; q < MLP_QT; ++q) acc[q][t] = bias;
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                const bf16x8 w = wf(l, t, s);
+                const bf16x8 wn = s < 1 ? wf(l, t, 1) : (t < 3 ? wf(l, t + 1, 0) : wf(l + 1, 0, 0));
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int q = 0; q < MLP_QT; ++q)
                     acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[q][s], acc[q][t], 0, 0, 0);
+                w = wn;
             }
         }
     }
@@ -231,9 +238,11 @@ __device__ __forceinline__ void mlp_forward_wave(const WF& wf, const BF& bf, bf1
     for (int q = 0; q < MLP_QT; ++q) y[q] = bias;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const bf16x8 w = wf(MLP_LAYERS - 1, 0, s);
+        const bf16x8 wn = s < 1 ? wf(MLP_LAYERS - 1, 0, 1) : w;
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int q = 0; q < MLP_QT; ++q) y[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[q][s], y[q], 0, 0, 0);
+        w = wn;
     }
 }
 
