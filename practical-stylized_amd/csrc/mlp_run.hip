@@ -295,20 +295,12 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
                                need_pos);
         return check_launch(h, "style depth k");
     };
-    // one level over the slices other than the audit slice (two contiguous ranges)
+    // one level over the slices other than the audit slice: one launch that passes over it
     auto level_others = [&](ScreenArgs<float> a, uint32_t d) -> hipError_t {
-        const uint32_t r0[2] = {0, sj + 1}, rn[2] = {sj, S - 1 - sj};
-        for (int r = 0; r < 2; ++r) {
-            if (!rn[r]) continue;
-            ScreenArgs<float> b = a;
-            b.S = rn[r];
-            b.rec = slice_rec(r0[r]);
-            b.anchors = slice_anc(r0[r]);
-            b.out = slot(r0[r], 0);
-            const hipError_t e = level(b, d);
-            if (e != hipSuccess) return e;
-        }
-        return hipSuccess;
+        if (S < 2) return hipSuccess;
+        a.S = S - 1;
+        a.hole1 = sj + 1;
+        return level(a, d);
     };
 
     nlpf_mlp_run_stats rs{};

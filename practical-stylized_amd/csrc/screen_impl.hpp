@@ -31,6 +31,7 @@ struct ScreenArgs {
     int root;
     int need_pos;
     uint32_t W, H, S, L, d;
+    uint32_t hole1;  // 1 + a slice the launch passes over (later slices move up by one); 0: none
     uint32_t out_levels, out_slot, child_slot;
     int has_child;
     uint64_t plane;
@@ -110,7 +111,9 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
-    const uint32_t s = t / tiles, tile = t - s * tiles;
+    uint32_t s = t / tiles;
+    const uint32_t tile = t - s * tiles;
+    s += a.hole1 && s + 1 >= a.hole1 ? 1u : 0u;
     if (a.tile_mask && !a.tile_mask[tile]) return;  // uniform per workgroup, before any barrier
     const int tx0 = (int)(tile % a.tiles_x) * TW, ty0 = (int)(tile / a.tiles_x) * TH;
     const int W = (int)a.W, H = (int)a.H;

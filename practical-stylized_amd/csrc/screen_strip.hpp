@@ -93,7 +93,9 @@ __global__ void __launch_bounds__(64 * NW) screen_strip_kernel(ScreenArgs<A> a) 
     // once are neighbours in the image and share their halo columns in L2
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t per_slice = a.tiles_x * a.bands;
-    const uint32_t s = t / per_slice, rem = t - s * per_slice;
+    uint32_t s = t / per_slice;
+    const uint32_t rem = t - s * per_slice;
+    s += a.hole1 && s + 1 >= a.hole1 ? 1u : 0u;
     const uint32_t band = rem / a.tiles_x, tx = rem - band * a.tiles_x;
     const uint32_t sy0 = band * a.seg_steps;
     const uint32_t n = min(a.seg_steps, a.steps_y - sy0);
