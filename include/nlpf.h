@@ -492,6 +492,8 @@ int nlpf_mlp_reference(nlpf_handle* h, const nlpf_screen_desc* desc, const void*
  *                are filtered exactly at depths levels-1..depth in every slice (with the
  *                dependency halo: the level-0 tile mask dilated by one tile per level);
  *   levels depth-1..0 and the film as nlpf_screen_run.
+ * The audit slice's exact levels depth-1..0 run on a second stream of the handle, beside the
+ * training and inference; the call returns after both streams are done.
  * The network must be initialised (nlpf_mlp_init); training continues across calls.
  * film: float [H][W][3]; level_out (optional): float [spp][levels][3][H][W], where the levels
  * deeper than `depth` of the non-audit slices are defined only inside fallback tiles. */

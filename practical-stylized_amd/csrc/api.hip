@@ -184,6 +184,12 @@ int nlpf_destroy(nlpf_handle* h) {
     if (h->d_bsdf) (void)hipFree(h->d_bsdf);
     if (h->scratch) (void)hipFree(h->scratch);
     for (hipEvent_t ev : h->events) (void)hipEventDestroy(ev);
+    if (h->aux) {
+        (void)hipStreamSynchronize(h->aux);
+        (void)hipStreamDestroy(h->aux);
+    }
+    if (h->fork) (void)hipEventDestroy(h->fork);
+    if (h->join) (void)hipEventDestroy(h->join);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return NLPF_OK;

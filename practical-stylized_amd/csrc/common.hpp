@@ -67,6 +67,10 @@ struct nlpf_handle {
     nlpf::Config cfg;
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the cache pass (work that overlaps the handle's stream) and the events
+    // that order the two; created on first use, default priority
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
     // stylization table
     nlpf_style_node* d_nodes = nullptr;

@@ -258,15 +258,16 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     a0.kl = (float)sqrt(log2e / (2.0 * (double)h->cfg.sigma_lum * h->cfg.sigma_lum));
     a0.rscale = fp16 ? NLPF_FP16_RADIANCE_SCALE : 1.0f;
     a0.tile_mask = nullptr;
-    auto level = [&](ScreenArgs<float> a, uint32_t d) -> hipError_t {
+    auto level_on = [&](ScreenArgs<float> a, uint32_t d, hipStream_t s) -> hipError_t {
         a.d = d;
         a.out_slot = d;
         a.child_slot = d + 1;
         a.has_child = d + 1 < L;
         // a tile mask covers 64x8 tiles: masked launches take the 8-row tile variant 0
         const int v = a.tile_mask ? 0 : screen_variant_of(h->cfg);
-        return fp16 ? screen_level_fp16(a, desc->radius, v, st) : screen_level_fp32(a, desc->radius, v, st);
+        return fp16 ? screen_level_fp16(a, desc->radius, v, s) : screen_level_fp32(a, desc->radius, v, s);
     };
+    auto level = [&](ScreenArgs<float> a, uint32_t d) -> hipError_t { return level_on(a, d, st); };
     auto slice_rec = [&](uint32_t s) {
         return (const void*)((const char*)records + (size_t)s * L * NLPF_SCREEN_PLANES * plane * esz);
     };
@@ -316,7 +317,18 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     }
     NLPF_HIP(h, hipMemcpyAsync(Fj, slot(sj, k), f_bytes, hipMemcpyDeviceToDevice, st));
     if ((rc = style_k(slot(sj, 0), L, sj, 1))) return rc;
-    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level(aj, (uint32_t)d));
+    // levels k-1..0 of the audit slice (its final images, which the tile audit compares against)
+    // run on the second stream, overlapping the training and inference below; the handle's
+    // stream waits for them before the audit
+    if (!h->aux) {
+        NLPF_HIP(h, hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+        NLPF_HIP(h, hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
+        NLPF_HIP(h, hipEventCreateWithFlags(&h->join, hipEventDisableTiming));
+    }
+    NLPF_HIP(h, hipEventRecord(h->fork, st));
+    NLPF_HIP(h, hipStreamWaitEvent(h->aux, h->fork, 0));
+    for (int d = (int)k - 1; d >= 0; --d) NLPF_HIP(h, level_on(aj, (uint32_t)d, h->aux));
+    NLPF_HIP(h, hipEventRecord(h->join, h->aux));
     rc = record_event(h, 1);
     if (rc) return rc;
     // 2. online training on `train_records` audit vertices (a PCG stream of `seed` picks them);
@@ -404,6 +416,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
                              : screen_level_fp32(b, desc->radius, screen_variant_of(h->cfg), st));
         }
     }
+    NLPF_HIP(h, hipStreamWaitEvent(st, h->join, 0));  // the audit slice's exact level 0
     hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pj, slot(sj, 0), W, H, tiles_x, nd_img);
     if ((rc = check_launch(h, "tile audit"))) return rc;
     std::vector<double> hi(2 * (size_t)ntiles), hf(2 * (size_t)ntiles);
