@@ -16,6 +16,9 @@
 //                   mask dilated d times;
 //   6. levels k-1..0 plain NL-PF on the mixed depth-k output of the other slices, then the
 //      film over all slices.
+// A second stream of the handle takes the audit slice's exact levels k-1..0 (beside training
+// and inference) and then its rerun (beside the other slices' inference); the handle's stream
+// waits for both before the per-tile errors.
 #include "common.hpp"
 #include "screen_impl.hpp"
 #include "scene.hpp"
@@ -361,31 +364,22 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     }
     rc = record_event(h, 2);
     if (rc) return rc;
-    // 3. inference at depth k: the other slices into their level slots, the audit slice into
-    //    slot k of its separate image; then the depth-k stylization
+    // 3. the estimate at depth k: first the audit slice's (slot k of its separate image), its fit
+    //    against the targets and its stylization; its audit rerun then runs on the second stream
+    //    (4.) while the other slices' estimates are formed and stylized here
     float* Pk = Pj + (size_t)k * 3 * plane;
+    const float* c3 = nullptr;  // the controls' constant (zero when NULL)
     if (est == NLPF_CACHE_NETWORK) {
-        rc = mlp_infer_planar(h, desc, records, anchors, k, 0, (uint64_t)sj * plane, lv, L, k, 0);
-        if (rc) return rc;
-        rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)(sj + 1) * plane,
-                              (uint64_t)(S - 1 - sj) * plane, lv, L, k, 0);
-        if (rc) return rc;
         rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)sj * plane, plane, Pj, k + 1, k, sj);
         if (rc) return rc;
     } else {
         // controls: the audit slice's mean unstylized depth-k value everywhere, or zero (the
         // paths cut at depth k)
-        const float* c3 = nullptr;
         if (est == NLPF_CACHE_MEAN) {
             hipLaunchKernelGGL(plane_partial_kernel, dim3(256, 3), dim3(256), 0, st, Fj, plane, part);
             hipLaunchKernelGGL(plane_mean_kernel, dim3(1), dim3(64), 0, st, part, plane, mean3);
             c3 = mean3;
         }
-        const uint32_t r0[2] = {0, sj + 1}, rn[2] = {sj, S - 1 - sj};
-        for (int r = 0; r < 2; ++r)
-            if (rn[r])
-                hipLaunchKernelGGL(fill_slot_kernel, dim3((uint32_t)((plane * rn[r] + 255) / 256)), dim3(256), 0, st,
-                                   slot(r0[r], 0), rn[r], plane, L, k, c3);
         hipLaunchKernelGGL(fill_slot_kernel, dim3((uint32_t)((plane + 255) / 256)), dim3(256), 0, st, Pj, 1u, plane,
                            k + 1, k, c3);
         if ((rc = check_launch(h, "cache control estimate"))) return rc;
@@ -393,15 +387,11 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pk, Fj, W, H, tiles_x, nd_fit);
     if ((rc = check_launch(h, "cache fit"))) return rc;
     // a filtered estimate is stylized at depth k; a path cut at depth k carries nothing up
-    if (est != NLPF_CACHE_ZERO) {
-        if ((rc = style_k(lv, L, 0, sj))) return rc;
-        if ((rc = style_k(slot(sj + 1, 0), L, sj + 1, S - 1 - sj))) return rc;
-        if ((rc = style_k(Pj, k + 1, sj, 1))) return rc;
-    }
-    rc = record_event(h, 3);
-    if (rc) return rc;
-    // 4. audit: levels k-1..0 of the audit slice on its estimate, per-tile error against its
-    //    exact level-0 image
+    if (est != NLPF_CACHE_ZERO && (rc = style_k(Pj, k + 1, sj, 1))) return rc;
+    // 4. audit, on the second stream after the exact levels: levels k-1..0 of the audit slice on
+    //    its estimate
+    NLPF_HIP(h, hipEventRecord(h->fork, st));
+    NLPF_HIP(h, hipStreamWaitEvent(h->aux, h->fork, 0));
     {
         ScreenArgs<float> ap = aj;
         ap.out = Pj;
@@ -412,11 +402,31 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
             b.out_slot = (uint32_t)d;
             b.child_slot = (uint32_t)d + 1;
             b.has_child = 1;  // slot k holds the estimate
-            NLPF_HIP(h, fp16 ? screen_level_fp16(b, desc->radius, screen_variant_of(h->cfg), st)
-                             : screen_level_fp32(b, desc->radius, screen_variant_of(h->cfg), st));
+            NLPF_HIP(h, fp16 ? screen_level_fp16(b, desc->radius, screen_variant_of(h->cfg), h->aux)
+                             : screen_level_fp32(b, desc->radius, screen_variant_of(h->cfg), h->aux));
         }
     }
-    NLPF_HIP(h, hipStreamWaitEvent(st, h->join, 0));  // the audit slice's exact level 0
+    NLPF_HIP(h, hipEventRecord(h->join, h->aux));
+    // the other slices' estimates (two ranges around the audit slice) and their stylization
+    const uint32_t r0[2] = {0, sj + 1}, rn[2] = {sj, S - 1 - sj};
+    for (int r = 0; r < 2; ++r) {
+        if (!rn[r]) continue;
+        if (est == NLPF_CACHE_NETWORK) {
+            rc = mlp_infer_planar(h, desc, records, anchors, k, (uint64_t)r0[r] * plane, (uint64_t)rn[r] * plane, lv,
+                                  L, k, 0);
+            if (rc) return rc;
+        } else {
+            hipLaunchKernelGGL(fill_slot_kernel, dim3((uint32_t)((plane * rn[r] + 255) / 256)), dim3(256), 0, st,
+                               slot(r0[r], 0), rn[r], plane, L, k, c3);
+            if ((rc = check_launch(h, "cache control estimate"))) return rc;
+        }
+        if (est != NLPF_CACHE_ZERO && (rc = style_k(slot(r0[r], 0), L, r0[r], rn[r]))) return rc;
+    }
+    rc = record_event(h, 3);
+    if (rc) return rc;
+    // the per-tile error of the rerun against the audit slice's exact level-0 image (both from
+    // the second stream)
+    NLPF_HIP(h, hipStreamWaitEvent(st, h->join, 0));  // the rerun and the exact level 0
     hipLaunchKernelGGL(tile_error_kernel, dim3(ntiles), dim3(256), 0, st, Pj, slot(sj, 0), W, H, tiles_x, nd_img);
     if ((rc = check_launch(h, "tile audit"))) return rc;
     std::vector<double> hi(2 * (size_t)ntiles), hf(2 * (size_t)ntiles);
