@@ -71,6 +71,9 @@ struct nlpf_handle {
     // that order the two; created on first use, default priority
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // identity permutation of an image's pixels, for the cache pass's training picks (a partial
+    // shuffle that is undone after each pick)
+    std::vector<uint32_t> pick_perm;
     std::string err;
     // stylization table
     nlpf_style_node* d_nodes = nullptr;

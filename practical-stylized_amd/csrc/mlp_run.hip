@@ -335,19 +335,31 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     rc = record_event(h, 1);
     if (rc) return rc;
     // 2. online training on `train_records` audit vertices (a PCG stream of `seed` picks them);
-    //    the control estimators train nothing
+    //    the control estimators train nothing. The picks and the loss live to the end of the call:
+    //    their copies complete before the synchronisation of the tile audit below.
+    std::vector<uint32_t> pick_l, pick_g;
+    float loss = 0.0f;
     if (est == NLPF_CACHE_NETWORK) {
-        std::vector<uint32_t> perm(plane);
-        for (uint64_t i = 0; i < plane; ++i) perm[i] = (uint32_t)i;
+        // a partial Fisher-Yates shuffle of the pixels on the handle's identity permutation, undone
+        // once the picks are read (the host work is O(train_records), not O(pixels))
+        std::vector<uint32_t>& perm = h->pick_perm;
+        if (perm.size() != plane) {
+            perm.resize(plane);
+            for (uint64_t i = 0; i < plane; ++i) perm[i] = (uint32_t)i;
+        }
         Pcg32 g = pcg_seq_offset(0x7EA1ull, splitmix64(run->seed));
+        std::vector<uint32_t> js(run->train_records);
         for (uint32_t i = 0; i < run->train_records; ++i) {
             const uint32_t j = i + (uint32_t)(((uint64_t)pcg_next(g) * (plane - i)) >> 32);
+            js[i] = j;
             std::swap(perm[i], perm[j]);
         }
-        std::vector<uint32_t> ids(run->train_records);
-        for (uint32_t i = 0; i < run->train_records; ++i) ids[i] = sj * (uint32_t)plane + perm[i];
-        NLPF_HIP(h, hipMemcpyAsync(lidx, perm.data(), idx_bytes, hipMemcpyHostToDevice, st));
-        NLPF_HIP(h, hipMemcpyAsync(gidx, ids.data(), idx_bytes, hipMemcpyHostToDevice, st));
+        pick_l.assign(perm.begin(), perm.begin() + run->train_records);
+        pick_g.resize(run->train_records);
+        for (uint32_t i = 0; i < run->train_records; ++i) pick_g[i] = sj * (uint32_t)plane + pick_l[i];
+        for (uint32_t i = run->train_records; i-- > 0;) std::swap(perm[i], perm[js[i]]);  // back to the identity
+        NLPF_HIP(h, hipMemcpyAsync(lidx, pick_l.data(), idx_bytes, hipMemcpyHostToDevice, st));
+        NLPF_HIP(h, hipMemcpyAsync(gidx, pick_g.data(), idx_bytes, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(gather_targets_kernel, dim3((run->train_records + 255) / 256), dim3(256), 0, st, Fj, plane,
                            lidx, run->train_records, tgt);
         if ((rc = check_launch(h, "gather targets"))) return rc;
@@ -356,11 +368,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
                                    tgt + (size_t)b * run->batch * 3, run->batch, run->lr, 1, nullptr);
             if (rc) return rc;
         }
-        float loss = 0.0f;
         NLPF_HIP(h, hipMemcpyAsync(&loss, mlp_loss_device(h), sizeof(float), hipMemcpyDeviceToHost, st));
-        // the host vectors (and `loss`) must outlive the asynchronous copies
-        NLPF_HIP(h, hipStreamSynchronize(st));
-        rs.train_loss = loss;
     }
     rc = record_event(h, 2);
     if (rc) return rc;
@@ -433,6 +441,7 @@ extern "C" int nlpf_screen_run_mlp(nlpf_handle* h, const nlpf_screen_desc* desc,
     NLPF_HIP(h, hipMemcpyAsync(hi.data(), nd_img, nd_bytes, hipMemcpyDeviceToHost, st));
     NLPF_HIP(h, hipMemcpyAsync(hf.data(), nd_fit, nd_bytes, hipMemcpyDeviceToHost, st));
     NLPF_HIP(h, hipStreamSynchronize(st));
+    rs.train_loss = loss;
     std::vector<uint8_t> m(ntiles, 0);
     double in_ = 0.0, id_ = 0.0, fn_ = 0.0, fd_ = 0.0;
     uint64_t nfall = 0;
