@@ -191,7 +191,8 @@ int nlpf_stylize(nlpf_handle* h, const float* rgb_in, float* rgb_out, uint64_t n
  *   in_i  = radiance_i + throughput_i * out_{d+1,i}        (out_{D+1} = 0)
  *   w_ij  = exp(-|p_i-p_j|^2/2sp^2 - (1-n_i.n_j)^2/2sn^2 - (l_i-l_j)^2/2sl^2), l = lum(albedo)
  *   out_i = S_d( sum_j w_ij in_j / sum_j w_ij ),  j over the (2r+1)^2 window clipped to the image
- * S_d = material 0's stylization tree at depth d (flags 0). film[y][x][c] = mean_s out_{0}.
+ * S_d = material 0's stylization tree at depth d (flags 0; nlpf_screen_run_materials takes each
+ * vertex's material and flags). film[y][x][c] = mean_s out_{0}.
  * Outputs are double for NLPF_FP64 and float otherwise:
  *   film_rgb  [H][W][3];
  *   level_out (optional, NULL to skip) every level's output [spp][levels][3][H][W]. */
@@ -208,6 +209,18 @@ typedef struct nlpf_screen_desc {
  * kernel_update_film :428-506) with an image-space window instead of the hash grid. */
 int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
                     const float* anchors, void* film_rgb, void* level_out);
+
+/* nlpf_screen_run with each vertex's own material and stylization flags in place of material 0
+ * and flags 0: vertex i of level d is stylized by S = material[i]'s tree (identity when
+ * material[i] >= the style table's material count or its root is -1) with flags[i], as
+ * kernel_filter stylizes a vertex through its own surface closure with its own flags
+ * (lm_path_filtering.rs:947-969, apply_stylization svm/surface/mod.rs:1824-1842).  material
+ * (uint16) and flags (uint8, NULL: all 0) are device planes laid out like one record plane,
+ *   material[(s*levels + d)*H*W + y*W + x].
+ * Halftone positions are read whenever any material's tree holds a halftone node. */
+int nlpf_screen_run_materials(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                              const float* anchors, const uint16_t* material, const uint8_t* flags,
+                              void* film_rgb, void* level_out);
 
 /* One chunk of a frame too large for HBM at once: desc->spp slices that are spp slices
  * [slice0, slice0 + desc->spp) of a total_spp-slice frame (records, anchors and level_out

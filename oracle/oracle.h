@@ -47,6 +47,17 @@ int or_screen_run_f64(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius
 int or_screen_run_f32(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, double sigma_pos,
                       double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
                       float* film, float* level_out, int threads);
+/* The same with per-vertex material (uint16) and flags (uint8, NULL: 0) planes [S][L][H][W]:
+ * vertex i takes roots[material[i]]'s tree (identity past n_materials or for root -1) with
+ * flags[i]; material NULL: root's tree with flags 0 for every vertex. */
+int or_screen_run_shaded_f64(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, double sigma_pos,
+                             double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
+                             const int32_t* roots, uint32_t n_materials, const uint16_t* material,
+                             const uint8_t* flags, double* film, double* level_out, int threads);
+int or_screen_run_shaded_f32(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, double sigma_pos,
+                             double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
+                             const int32_t* roots, uint32_t n_materials, const uint16_t* material,
+                             const uint8_t* flags, float* film, float* level_out, int threads);
 
 /* World-mode BSDFs per material (oracle_bsdf.c); the same 40-byte layout as the library's
  * nlpf_bsdf. */

@@ -37,6 +37,10 @@ def load():
         fn = getattr(lib, name)
         fn.argtypes = [u32, u32, u32, u32, i32, dbl, dbl, dbl, vp, vp, i32, vp, vp, i32]
         fn.restype = i32
+    for name in ("or_screen_run_shaded_f64", "or_screen_run_shaded_f32"):
+        fn = getattr(lib, name)
+        fn.argtypes = [u32, u32, u32, u32, i32, dbl, dbl, dbl, vp, vp, i32, vp, u32, vp, vp, vp, vp, i32]
+        fn.restype = i32
     lib.or_world_neighbor_counts.argtypes = [vp, u32, vp, vp, C.c_float, u64, vp]
     lib.or_world_neighbor_counts.restype = i32
     lib.or_world_run.argtypes = [u32, u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, u32, u64, i32, i32,
@@ -139,8 +143,9 @@ def hsv_to_rgb(c):
 
 
 def screen_run(records, W, H, S, L, radius, sigmas, nodes=None, root=-1, precision="f64",
-               want_levels=False, threads=0):
-    """records: float32 planes [S][L][15][H][W] with absolute positions."""
+               want_levels=False, threads=0, material=None, flags=None, roots=None):
+    """records: float32 planes [S][L][15][H][W] with absolute positions.  With material
+    (uint16 [S][L][H][W]) each vertex takes roots[material]'s tree with flags (uint8, or 0)."""
     lib = load()
     records = np.ascontiguousarray(records, np.float32)
     assert records.size == S * L * 15 * H * W
@@ -151,9 +156,22 @@ def screen_run(records, W, H, S, L, radius, sigmas, nodes=None, root=-1, precisi
         nodes_arr, root = np.zeros(1, dtype=[("b", "V256")]), -1
     else:
         nodes_arr = np.ascontiguousarray(nodes)
-    fn = lib.or_screen_run_f64 if precision == "f64" else lib.or_screen_run_f32
-    rc = fn(W, H, S, L, radius, sigmas[0], sigmas[1], sigmas[2], _p(records), _p(nodes_arr), int(root),
-            _p(film), _p(lv), int(threads))
+    if material is None:
+        fn = lib.or_screen_run_f64 if precision == "f64" else lib.or_screen_run_f32
+        rc = fn(W, H, S, L, radius, sigmas[0], sigmas[1], sigmas[2], _p(records), _p(nodes_arr), int(root),
+                _p(film), _p(lv), int(threads))
+    else:
+        material = np.ascontiguousarray(material, np.uint16)
+        assert material.size == S * L * H * W
+        if flags is not None:
+            flags = np.ascontiguousarray(flags, np.uint8)
+            assert flags.size == S * L * H * W
+        roots = np.ascontiguousarray(roots if roots is not None else [], np.int32)
+        if nodes is None or len(nodes) == 0:
+            roots = np.full(len(roots), -1, np.int32)
+        fn = lib.or_screen_run_shaded_f64 if precision == "f64" else lib.or_screen_run_shaded_f32
+        rc = fn(W, H, S, L, radius, sigmas[0], sigmas[1], sigmas[2], _p(records), _p(nodes_arr), -1,
+                _p(roots), len(roots), _p(material), _p(flags), _p(film), _p(lv), int(threads))
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return (film, lv) if want_levels else film

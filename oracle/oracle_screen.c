@@ -16,7 +16,7 @@
 #include "oracle.h"
 
 #define REAL double
-#define FN or_screen_run_f64
+#define FN or_screen_run_shaded_f64
 #define EXPF exp
 #include "oracle_screen_core.inc"
 #undef REAL
@@ -24,9 +24,23 @@
 #undef EXPF
 
 #define REAL float
-#define FN or_screen_run_f32
+#define FN or_screen_run_shaded_f32
 #define EXPF expf
 #include "oracle_screen_core.inc"
 #undef REAL
 #undef FN
 #undef EXPF
+
+/* one tree (root) with flags 0 for every vertex */
+int or_screen_run_f64(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, double sigma_pos,
+                      double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
+                      double* film, double* level_out, int threads) {
+    return or_screen_run_shaded_f64(W, H, S, L, radius, sigma_pos, sigma_normal, sigma_lum, rec, nodes, root, NULL,
+                                    0u, NULL, NULL, film, level_out, threads);
+}
+int or_screen_run_f32(uint32_t W, uint32_t H, uint32_t S, uint32_t L, int radius, double sigma_pos,
+                      double sigma_normal, double sigma_lum, const float* rec, const or_node* nodes, int root,
+                      float* film, float* level_out, int threads) {
+    return or_screen_run_shaded_f32(W, H, S, L, radius, sigma_pos, sigma_normal, sigma_lum, rec, nodes, root, NULL,
+                                    0u, NULL, NULL, film, level_out, threads);
+}

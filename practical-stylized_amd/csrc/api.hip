@@ -337,13 +337,20 @@ static int validate_screen_desc(nlpf_handle* h, const nlpf_screen_desc* d) {
 // slot `slot` of out ([S][out_levels][3][H][W]) and reading the child level from slot `cslot`
 static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
                                const float* anchors, void* out, uint32_t out_levels, uint32_t s0, uint32_t S,
-                               int d) {
+                               int d, const uint16_t* material = nullptr, const uint8_t* flags = nullptr) {
     const uint64_t plane = (uint64_t)desc->width * desc->height;
     const uint32_t L = desc->levels;
     const uint32_t slot = out_levels == L ? (uint32_t)d : (uint32_t)(d & 1);
     const uint32_t cslot = out_levels == L ? (uint32_t)(d + 1) : (uint32_t)((d + 1) & 1);
     const int root = h->h_roots.empty() ? -1 : h->h_roots[0];
-    const int need_pos = root >= 0 && tree_has_type(h->h_nodes, root, NLPF_STY_HALFTONE);
+    int need_pos = root >= 0 && tree_has_type(h->h_nodes, root, NLPF_STY_HALFTONE);
+    if (material)  // any material's tree may be the one a vertex takes
+        for (const int32_t r : h->h_roots) need_pos |= r >= 0 && tree_has_type(h->h_nodes, r, NLPF_STY_HALFTONE);
+    // per-vertex material and flags planes [s][L][H][W], from this launch's first slice
+    const size_t shade_off = (size_t)s0 * L * plane;
+    const uint16_t* mat = material ? material + shade_off : nullptr;
+    const uint8_t* flg = material && flags ? flags + shade_off : nullptr;
+    const uint32_t n_mat = (uint32_t)h->h_roots.size();
     const double log2e = 1.4426950408889634;
     const double kp = sqrt(log2e / (2.0 * (double)h->cfg.sigma_pos * h->cfg.sigma_pos));
     const double kn = sqrt(log2e / (2.0 * (double)h->cfg.sigma_normal * h->cfg.sigma_normal));
@@ -361,6 +368,10 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
         a.root = root;
         a.chain = make_chain(h->h_nodes.data(), root, 0u);
         a.need_pos = need_pos;
+        a.material = mat;
+        a.flags = flg;
+        a.roots = h->d_roots;
+        a.n_materials = n_mat;
         a.W = desc->width;
         a.H = desc->height;
         a.S = S;
@@ -385,6 +396,10 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
         a.root = root;
         a.chain = make_chain(h->h_nodes.data(), root, 0u);
         a.need_pos = need_pos;
+        a.material = mat;
+        a.flags = flg;
+        a.roots = h->d_roots;
+        a.n_materials = n_mat;
         a.W = desc->width;
         a.H = desc->height;
         a.S = S;
@@ -430,7 +445,7 @@ static nlpf_stats screen_pass_stats(const nlpf_screen_desc* desc) {
 // sums between calls (accumulate: continue them; finalize: divide by s_total)
 static int screen_run_impl(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,
                            void* film, void* level_out, uint32_t s_total, int accumulate, int finalize,
-                           const char* name) {
+                           const char* name, const uint16_t* material = nullptr, const uint8_t* flags = nullptr) {
     int rc = validate_screen_desc(h, desc);
     if (rc) return rc;
     if (!records || !film) NLPF_FAIL(h, NLPF_E_INVALID_ARG, std::string(name) + ": NULL records or film");
@@ -451,7 +466,7 @@ static int screen_run_impl(nlpf_handle* h, const nlpf_screen_desc* desc, const v
     rc = record_event(h, 0);
     if (rc) return rc;
     for (int d = (int)L - 1; d >= 0; --d) {
-        rc = screen_level_launch(h, desc, records, anchors, outbuf, out_levels, 0, S, d);
+        rc = screen_level_launch(h, desc, records, anchors, outbuf, out_levels, 0, S, d, material, flags);
         if (rc) return rc;
         rc = record_event(h, 1 + (L - 1 - d));
         if (rc) return rc;
@@ -481,6 +496,15 @@ int nlpf_screen_run(nlpf_handle* h, const nlpf_screen_desc* desc, const void* re
     if (!h) return NLPF_E_INVALID_ARG;
     return screen_run_impl(h, desc, records, anchors, film, level_out, desc ? desc->spp : 0, 0, 1,
                            "nlpf_screen_run");
+}
+
+int nlpf_screen_run_materials(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records,
+                              const float* anchors, const uint16_t* material, const uint8_t* flags, void* film,
+                              void* level_out) {
+    if (!h) return NLPF_E_INVALID_ARG;
+    if (!material) NLPF_FAIL(h, NLPF_E_INVALID_ARG, "nlpf_screen_run_materials: NULL material plane");
+    return screen_run_impl(h, desc, records, anchors, film, level_out, desc ? desc->spp : 0, 0, 1,
+                           "nlpf_screen_run_materials", material, flags);
 }
 
 int nlpf_screen_run_chunk(nlpf_handle* h, const nlpf_screen_desc* desc, const void* records, const float* anchors,

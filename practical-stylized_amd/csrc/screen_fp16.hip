@@ -20,6 +20,11 @@ static constexpr int kStripVariant[8] = {3, 3, 3, 3, 3, 4, 4, 3};  // the fastes
 template <int RAD>
 static hipError_t dispatch_fp16(const ScreenArgs<float>& a, int variant, hipStream_t st) {
     constexpr int K = 2 * RAD + 1;
+    if (a.material) {  // per-vertex shading: 2-row strips, or 2-row 64x8 tiles for a tile variant
+        const bool tiles = variant == kScreenTiles || (variant >= 0 && variant < kScreenStrips);
+        if (!tiles && screen_strip_eligible<__half>(a)) return launch_screen_strip<__half, float, RAD, 4, 2, true>(a, st);
+        return launch_screen_level<__half, float, RAD, 4, 2, true>(a, st);
+    }
     if (variant == kScreenTiles) variant = kTileVariant[RAD];
     if (variant == kScreenStripsAuto) variant = kStripVariant[RAD];
     if (variant < 0) variant = kDefaultVariant[RAD];

@@ -7,6 +7,7 @@ namespace nlpf {
 template <int RAD>
 static hipError_t dispatch_fp64(const ScreenArgs<double>& a, int variant, hipStream_t st) {
     constexpr int K = 2 * RAD + 1;
+    if (a.material) return launch_screen_level<double, double, RAD, 4, (2 < K ? 2 : K), true>(a, st);
     switch (variant) {
         case 0: return launch_screen_level<double, double, RAD, 4, (2 < K ? 2 : K)>(a, st);
         default: return launch_screen_level<double, double, RAD, 4, (2 < K ? 2 : K)>(a, st);

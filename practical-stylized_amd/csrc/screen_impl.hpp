@@ -42,7 +42,34 @@ struct ScreenArgs {
     A rscale;  // radiance decode factor (FP16 records store radiance / NLPF_FP16_RADIANCE_SCALE)
     const uint8_t* tile_mask;  // optional [ceil(H/8)][ceil(W/64)]: 0 = skip the 64x8 tile (all slices)
     StyleChain chain;          // root's chain for flags 0 by value (not valid: walk nodes from root)
+    // optional per-vertex shading, [S][L][H][W] like one record plane: vertex i is stylized by
+    // material[i]'s tree (roots[material[i]], identity for materials >= n_materials or root -1)
+    // with flags[i] (0 when NULL); NULL material: `root` / `chain` with flags 0 for every vertex
+    const uint16_t* material;
+    const uint8_t* flags;
+    const int32_t* roots;
+    uint32_t n_materials;
 };
+
+// the epilogue's stylization of vertex o of slice s: per-vertex material and flags in the
+// SHADE instances (launched only when the launch has a material plane), else the launch's one
+// tree (resolved chain or node walk) with flags 0.  A compile-time switch: a run-time branch on
+// a.material costs the 15x15 kernels 2 % even when it is not taken.
+template <typename A, bool SHADE>
+__device__ __forceinline__ v3<A> screen_stylize(const ScreenArgs<A>& a, uint32_t s, size_t o, v3<A> F, v3<A> p,
+                                                const float* ramp_tabs) {
+    if constexpr (SHADE) {
+        const size_t q = ((size_t)s * a.L + a.d) * a.plane + o;
+        const uint32_t m = a.material[q];
+        const int root = m < a.n_materials ? a.roots[m] : -1;
+        if (root >= 0) F = apply_style<A>(a.nodes, root, a.d, a.flags ? (uint32_t)a.flags[q] : 0u, F, p);
+    } else if (a.chain.valid) {
+        F = apply_chain<A>(a.chain, a.d, F, p, ramp_tabs);
+    } else if (a.root >= 0) {
+        F = apply_style<A>(a.nodes, a.root, a.d, 0u, F, p);
+    }
+    return F;
+}
 
 __device__ __forceinline__ float ld_rec(const float* p, size_t i) { return p[i]; }
 __device__ __forceinline__ float ld_rec(const __half* p, size_t i) { return __half2float(p[i]); }
@@ -92,7 +119,7 @@ struct alignas(2 * sizeof(A)) V2 {
     A x, y;
 };
 
-template <typename S, typename A, int RAD, int NW, int R>
+template <typename S, typename A, int RAD, int NW, int R, bool SHADE = false>
 __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) {
     static_assert(R <= 2 * RAD + 1, "rows per lane must not exceed the window height");
     static_assert(64 * NW >= NLPF_CHAIN_MAX * 32, "one lane per ramp-table entry");
@@ -345,10 +372,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
                     p.z += (A)q[2];
                 }
             }
-            if (a.chain.valid)
-                F = apply_chain<A>(a.chain, a.d, F, p, ramp_tabs);
-            else if (a.root >= 0)
-                F = apply_style<A>(a.nodes, a.root, a.d, 0u, F, p);
+            F = screen_stylize<A, SHADE>(a, s, o, F, p, ramp_tabs);
             outp[o] = F.x;
             outp[plane + o] = F.y;
             outp[2 * plane + o] = F.z;
@@ -383,7 +407,7 @@ __global__ void screen_film_kernel(const A* out, uint32_t S, uint32_t out_levels
     film[3 * i + 2] = acc[2] * inv;
 }
 
-template <typename S, typename A, int RAD, int NW, int R>
+template <typename S, typename A, int RAD, int NW, int R, bool SHADE = false>
 hipError_t launch_screen_level(const ScreenArgs<A>& a0, hipStream_t st) {
     constexpr int TW = 64, TH = NW * R;
     if (a0.tile_mask && TH != 8) return hipErrorInvalidValue;  // masks cover 64x8 tiles
@@ -392,7 +416,7 @@ hipError_t launch_screen_level(const ScreenArgs<A>& a0, hipStream_t st) {
     a.tiles_x = (a.W + TW - 1) / TW;
     a.tiles_y = (a.H + TH - 1) / TH;
     a.nblocks = a.tiles_x * a.tiles_y * a.S;
-    hipLaunchKernelGGL((screen_level_kernel<S, A, RAD, NW, R>), dim3(a.nblocks), dim3(64 * NW), 0, st, a);
+    hipLaunchKernelGGL((screen_level_kernel<S, A, RAD, NW, R, SHADE>), dim3(a.nblocks), dim3(64 * NW), 0, st, a);
     return hipGetLastError();
 }
 

@@ -68,7 +68,7 @@ struct StripLoads {
     float an[3];
 };
 
-template <typename S, typename A, int RAD, int NW, int R>
+template <typename S, typename A, int RAD, int NW, int R, bool SHADE = false>
 __global__ void __launch_bounds__(64 * NW) screen_strip_kernel(ScreenArgs<A> a) {
     static_assert(std::is_same<A, float>::value, "strip kernel: FP32 accumulation");
     static_assert(R <= 2 * RAD + 1, "rows per lane must not exceed the window height");
@@ -325,10 +325,7 @@ __global__ void __launch_bounds__(64 * NW) screen_strip_kernel(ScreenArgs<A> a) 
                             p = v3<A>{strip_ld1(rr, e, 0u), strip_ld1(rr, e, pbytes), strip_ld1(rr, e, 2 * pbytes)};
                         }
                     }
-                    if (a.chain.valid)
-                        F = apply_chain<A>(a.chain, a.d, F, p, ramp_tabs);
-                    else if (a.root >= 0)
-                        F = apply_style<A>(a.nodes, a.root, a.d, 0u, F, p);
+                    F = screen_stylize<A, SHADE>(a, s, o, F, p, ramp_tabs);
                     strip_st1(ro, o * 4u, 0u, F.x);
                     strip_st1(ro, o * 4u, obytes, F.y);
                     strip_st1(ro, o * 4u, 2 * obytes, F.z);
@@ -394,7 +391,7 @@ inline uint32_t strip_segment_steps() {
     return (uint32_t)v;
 }
 
-template <typename S, typename A, int RAD, int NW, int R>
+template <typename S, typename A, int RAD, int NW, int R, bool SHADE = false>
 hipError_t launch_screen_strip(const ScreenArgs<A>& a0, hipStream_t st) {
     constexpr int TW = 64, TH = NW * R;
     if (!screen_strip_eligible<S>(a0)) return hipErrorInvalidValue;
@@ -407,7 +404,7 @@ hipError_t launch_screen_strip(const ScreenArgs<A>& a0, hipStream_t st) {
     const uint64_t nb = (uint64_t)a.tiles_x * a.bands * a.S;
     if (nb >= (1ull << 31)) return hipErrorInvalidValue;
     a.nblocks = (uint32_t)nb;
-    hipLaunchKernelGGL((screen_strip_kernel<S, A, RAD, NW, R>), dim3(a.nblocks), dim3(64 * NW), 0, st, a);
+    hipLaunchKernelGGL((screen_strip_kernel<S, A, RAD, NW, R, SHADE>), dim3(a.nblocks), dim3(64 * NW), 0, st, a);
     return hipGetLastError();
 }
 

@@ -141,6 +141,8 @@ _SIGS = [
                                C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_screen_run", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
+    ("nlpf_screen_run_materials", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nlpf_screen_run_chunk", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("nlpf_screen_run_noalloc", C.c_int, [C.c_void_p, C.POINTER(_ScreenDesc), C.c_void_p, C.c_void_p,
@@ -451,6 +453,12 @@ class Handle:
     def screen_run(self, desc, records, anchors=None, film=None, level_out=None):
         self._check(self.lib.nlpf_screen_run(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
                                              _ptr(film), _ptr(level_out)))
+
+    def screen_run_materials(self, desc, records, anchors, material, flags, film, level_out=None):
+        """screen_run with per-vertex material (uint16) and flags (uint8 or None) planes laid out
+        [spp][levels][H][W]; vertex i takes material[i]'s stylization tree with flags[i]."""
+        self._check(self.lib.nlpf_screen_run_materials(self.h, C.byref(desc), _ptr(records), _ptr(anchors),
+                                                       _ptr(material), _ptr(flags), _ptr(film), _ptr(level_out)))
 
     def screen_run_chunk(self, desc, records, anchors, film, slice0, total_spp, level_out=None):
         """Slices [slice0, slice0 + desc.spp) of a total_spp-slice frame; film carries the running

@@ -390,3 +390,67 @@ def test_config2_unspaced_ramp_within_sensitivity(h):
     print(f"config 2 unspaced ramp: max per-pixel error {err.max():.2e}, fraction above 1e-5 per level "
           f"{['%.2e' % f for f in frac]}, film max {ef.max():.2e}, film fraction above 1e-5 {(ef > 1e-5).mean():.2e}")
     rec.free()
+
+
+@pytest.mark.parametrize("prec,kernel,radius,W,H", [(N.FP32, "tiles", 7, 96, 70), (N.FP32, "strips", 3, 150, 70),
+                                                    (N.FP16, "tiles", 2, 53, 41), (N.FP16, "strips", 3, 128, 37),
+                                                    (N.FP64, "tiles", 4, 40, 29)])
+def test_per_vertex_materials_and_flags(prec, kernel, radius, W, H):
+    """nlpf_screen_run_materials: every vertex stylized by its own material's tree (EnableIf
+    branches picked by its flags, SetStyleIdx, a material past the table and a root of -1 giving
+    the identity) at its own slice and depth, three levels, against the oracle fed the same
+    planes: 1e-5 per pixel (1e-10 for FP64 records).  Without a flags plane every vertex takes
+    flags 0; with every vertex on material 0 the film equals nlpf_screen_run's."""
+    S, L = 2, 3
+    rng = np.random.default_rng(radius * 1000 + W)
+    pos, val = ramp_stops(rng)
+    table = N.style_table([N.rex(0.6), N.enable_if(2, inner=0), N.saturation(0.5), N.enable_if(0, inner=2, else_=1),
+                           N.hue_shift(0.25), N.set_idx(2, inner=4), N.ramp(pos, val)])
+    roots = [3, 5, -1, 6]
+    material = rng.integers(0, 5, (S, L, H, W)).astype(np.uint16)
+    flags = rng.choice([0, N.FLAG_STYLE2, N.FLAG_STYLE3, N.FLAG_STYLE4, N.FLAG_DISABLED],
+                       (S, L, H, W)).astype(np.uint8)
+    tol = 1e-10 if prec == N.FP64 else 1e-5
+    dt = np.float64 if prec == N.FP64 else np.float32
+    with N.Handle({"type": "pf", "mode": "screen", "screen_kernel": kernel}) as hk:
+        hk.set_style(table, roots)
+        hk.set_sigmas(*SIG)
+        desc, rec, anc = gen_screen(hk, W, H, S, L, N.FP16 if prec == N.FP16 else N.FP32, seed=W + H, radius=radius)
+        if prec == N.FP16:
+            host = decode_fp16(rec, anc, W, H, S, L)
+        else:
+            host = host_records_fp32(rec, W, H, S, L)
+        if prec == N.FP64:
+            rec.free()
+            rec = hk.to_device(host.astype(np.float64))
+            desc = hk.screen_desc(W, H, S, L, N.FP64, radius)
+        sig = tuple(float(np.float32(v)) for v in SIG) if prec == N.FP64 else SIG
+        dm, df = hk.to_device(material), hk.to_device(flags)
+        film = hk.alloc(H * W * 3 * dt().itemsize)
+        lvb = hk.alloc(S * L * 3 * H * W * dt().itemsize)
+
+        def run(m, f, levels):
+            hk.screen_run_materials(desc, rec, anc, m, f, film, lvb if levels else None)
+            out = film.download(dt, H * W * 3).reshape(H, W, 3)
+            return (out, lvb.download(dt, S * L * 3 * H * W).reshape(S, L, 3, H, W)) if levels else out
+
+        g, glv = run(dm, df, True)
+        of, olv = O.screen_run(host, W, H, S, L, radius, sig, nodes=table, want_levels=True, material=material,
+                               flags=flags, roots=roots)
+        check_levels(glv, olv, tol)
+        assert pixel_rel_err(g, of).max() <= tol
+        # the planes reach the kernel: material 0 everywhere gives another film
+        plain = O.screen_run(host, W, H, S, L, radius, sig, nodes=table, root=roots[0])
+        assert pixel_rel_err(of, plain).max() > 1e-3
+        g0 = run(dm, None, False)
+        of0 = O.screen_run(host, W, H, S, L, radius, sig, nodes=table, material=material, roots=roots)
+        assert pixel_rel_err(g0, of0).max() <= tol
+        zero = hk.to_device(np.zeros_like(material))
+        gz = run(zero, None, False)
+        hk.screen_run(desc, rec, anc, film, None)
+        gp = film.download(dt, H * W * 3).reshape(H, W, 3)
+        assert pixel_rel_err(gz, gp).max() <= tol * 0.1
+        for b in (dm, df, zero, film, lvb, rec):
+            b.free()
+        if anc is not None:
+            anc.free()

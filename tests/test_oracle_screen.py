@@ -26,7 +26,7 @@ def rand_records(rng, W, H, S, L, pos_scale=0.05):
     return rec
 
 
-def brute(rec, W, H, S, L, radius, sig, style=None):
+def brute(rec, W, H, S, L, radius, sig, style=None, vstyle=None):
     """Independent numpy restatement: explicit per-pixel window loops."""
     sp, sn, sl = sig
     rec = rec.astype(np.float64)
@@ -49,7 +49,11 @@ def brute(rec, W, H, S, L, radius, sig, style=None):
                     w = np.exp(-(dp ** 2).sum(0) / (2 * sp * sp) - (1 - nn) ** 2 / (2 * sn * sn)
                                - dl ** 2 / (2 * sl * sl))
                     f = (w * inp[:, ys, xs]).sum((1, 2)) / w.sum()
-                    levels[s, d, :, y, x] = style(f, d) if style else f
+                    if vstyle:
+                        f = vstyle(f, s, d, y, x)
+                    elif style:
+                        f = style(f, d)
+                    levels[s, d, :, y, x] = f
     film = levels[:, 0].mean(0).transpose(1, 2, 0)
     return film, levels
 
@@ -107,6 +111,46 @@ def test_style_epilogue_and_recursion(oracle):
     bf, blv = brute(rec, W, H, S, L, 2, sig, style=style)
     assert np.allclose(lv, blv, rtol=1e-11, atol=1e-13)
     assert np.allclose(film, bf, rtol=1e-11, atol=1e-13)
+
+
+def test_per_vertex_materials_and_flags(oracle):
+    """Material and flags planes: each vertex is stylized by its own material's tree with its own
+    flags, at its own slice and depth (materials past the table and root -1 give the identity).
+    The brute force applies the oracle's stylize per vertex; all-zero planes reproduce the
+    one-tree run bit for bit."""
+    rng = np.random.default_rng(11)
+    W, H, S, L = 9, 7, 2, 3
+    sig = (0.08, 0.4, 0.25)
+    rec = rand_records(rng, W, H, S, L)
+    table = N.style_table([N.rex(0.6), N.enable_if(2, inner=0), N.saturation(0.5),
+                           N.enable_if(0, inner=2, else_=1), N.hue_shift(0.25), N.set_idx(2, inner=4),
+                           N.toon(0.3)])
+    roots = [3, 5, -1, 6]
+    material = rng.integers(0, 5, (S, L, H, W)).astype(np.uint16)  # 4 is past the table
+    flags = rng.choice([0, N.FLAG_STYLE2, N.FLAG_STYLE3, N.FLAG_STYLE4, N.FLAG_DISABLED],
+                       (S, L, H, W)).astype(np.uint8)
+    film, lv = oracle.screen_run(rec, W, H, S, L, 2, sig, nodes=table, want_levels=True, material=material,
+                                 flags=flags, roots=roots)
+
+    def vstyle(f, s, d, y, x):
+        return oracle.stylize(table, roots, f[None], depth=d, flags=[flags[s, d, y, x]],
+                              material=[material[s, d, y, x]])[0]
+
+    bf, blv = brute(rec, W, H, S, L, 2, sig, vstyle=vstyle)
+    assert np.abs(lv - blv).max() <= 1e-12 * max(1.0, np.abs(blv).max())
+    assert np.abs(film - bf).max() <= 1e-12 * max(1.0, np.abs(bf).max())
+    # the planes matter: the same run with every vertex on material 0 and flags 0 differs
+    zero_m = np.zeros_like(material)
+    f0, lv0 = oracle.screen_run(rec, W, H, S, L, 2, sig, nodes=table, want_levels=True, material=zero_m,
+                                flags=np.zeros_like(flags), roots=roots)
+    assert np.abs(f0 - film).max() > 1e-3
+    one, lv1 = oracle.screen_run(rec, W, H, S, L, 2, sig, nodes=table, root=roots[0], want_levels=True)
+    assert np.array_equal(f0, one) and np.array_equal(lv0, lv1)
+    for prec in ("f64", "f32"):
+        a = oracle.screen_run(rec, W, H, S, L, 2, sig, nodes=table, root=roots[0], precision=prec)
+        b = oracle.screen_run(rec, W, H, S, L, 2, sig, nodes=table, precision=prec, material=zero_m,
+                              flags=None, roots=roots)
+        assert np.array_equal(a, b)
 
 
 def test_box_limit_and_constant_invariant(oracle):
