@@ -72,6 +72,39 @@ def timed_runs(h, desc, rec, anc, film, steps, warmup):
     return wall / steps * 1e3, lv_ms / steps
 
 
+def materials_leg(h, N, desc, rec, film, steps, warmup):
+    """The FP32 pass through nlpf_screen_run_materials: every vertex stylized by its own material
+    (uniform over four: the headline ramp16+hue tree, rex, a flags-selected EnableIf between
+    saturation and toon, and none) with its own style flags (uniform over 0 and styles 2-4).
+    Level kernels and bytes as the one-tree pass, plus the 3-byte material and flags planes per
+    vertex. Restores nothing: the caller sets its style table again."""
+    W, H, S, L = desc.width, desc.height, desc.spp, desc.levels
+    n = W * H * S * L
+    rng = np.random.default_rng(2024)
+    pos = np.sort(rng.random(16)).astype(np.float32)
+    val = np.sort(rng.random(16)).astype(np.float32)
+    table = N.style_table([N.ramp(pos, val), N.hue_shift(0.1, inner=0), N.rex(0.6), N.saturation(0.5),
+                           N.toon(0.25), N.enable_if(2, inner=3, else_=4)])
+    h.set_style(table, [1, 2, 5, -1])
+    rng = np.random.default_rng(5)
+    material = h.to_device(rng.integers(0, 4, n, dtype=np.uint16))
+    flags = h.to_device(rng.choice(np.array([0, N.FLAG_STYLE2, N.FLAG_STYLE3, N.FLAG_STYLE4], np.uint8), n))
+    for _ in range(warmup):
+        h.screen_run_materials(desc, rec, None, material, flags, film)
+    h.sync()
+    lv_ms = np.zeros(L)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.screen_run_materials(desc, rec, None, material, flags, film)
+        lv_ms += np.array(h.stats()["level_ms"][:L])
+    h.sync()
+    wall = time.perf_counter() - t0
+    material.free()
+    flags.free()
+    return {"ms_per_step": wall / steps * 1e3, "level_ms": (lv_ms / steps).round(3).tolist(),
+            "materials": 4, "flags": [0, 2, 3, 4]}
+
+
 def window_sweep(h, N, W, H, S, L, recs, steps=3, warmup=1):
     """Level-kernel rate for every window radius 1..7 on the same records: per-pair work falls
     with the square of the window while the bytes per vertex stay fixed, so the small windows
@@ -571,6 +604,10 @@ def main():
     log("timing FP32")
     ms32, lv32 = timed_runs(h, d32, rec32, None, film32, args.steps, args.warmup)
     log(f"FP32: {ms32:.2f} ms/step, levels {np.round(lv32, 2).tolist()} ms")
+    log("timing FP32 with per-vertex materials and flags")
+    mats = materials_leg(h, N, d32, rec32, film32, args.steps, args.warmup)
+    h.set_style(table, [root])
+    log(f"FP32 per-vertex materials: {mats['ms_per_step']:.2f} ms/step")
     lb32 = level_bytes(S, plane, L, 60)
     achieved_gbs = sum(lb32) / 1e9 / (lv32.sum() / 1e3)
     pairs = verts * (2 * R + 1) ** 2
@@ -611,7 +648,9 @@ def main():
         result["roofline"]["valu_issue"] = {
             "valu_per_pair": counters.get("valu_instructions_per_pair"),
             "frac": pair_rate * counters.get("valu_instructions_per_pair", 0) / VALU_ISSUE_PEAK}
-    detail = {"roofline": dict(result["roofline"]), "level_ms_fp32": lv32.tolist()}
+    result["screen_materials"] = {"ms_per_step": round(mats["ms_per_step"], 3),
+                                  "vs_one_tree": round(mats["ms_per_step"] / ms32, 4)}
+    detail = {"roofline": dict(result["roofline"]), "level_ms_fp32": lv32.tolist(), "screen_materials": mats}
 
     h2d = pcie_upload_gbs(h)
     rec_bytes = verts * 60
