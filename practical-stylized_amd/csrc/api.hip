@@ -351,6 +351,7 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
     const uint16_t* mat = material ? material + shade_off : nullptr;
     const uint8_t* flg = material && flags ? flags + shade_off : nullptr;
     const uint32_t n_mat = (uint32_t)h->h_roots.size();
+    const uint32_t lds_nodes = h->h_nodes.size() <= kShadeLdsNodes ? (uint32_t)h->h_nodes.size() : 0u;
     const double log2e = 1.4426950408889634;
     const double kp = sqrt(log2e / (2.0 * (double)h->cfg.sigma_pos * h->cfg.sigma_pos));
     const double kn = sqrt(log2e / (2.0 * (double)h->cfg.sigma_normal * h->cfg.sigma_normal));
@@ -372,6 +373,7 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
         a.flags = flg;
         a.roots = h->d_roots;
         a.n_materials = n_mat;
+        a.lds_nodes = lds_nodes;
         a.W = desc->width;
         a.H = desc->height;
         a.S = S;
@@ -400,6 +402,7 @@ static int screen_level_launch(nlpf_handle* h, const nlpf_screen_desc* desc, con
         a.flags = flg;
         a.roots = h->d_roots;
         a.n_materials = n_mat;
+        a.lds_nodes = lds_nodes;
         a.W = desc->width;
         a.H = desc->height;
         a.S = S;

@@ -49,7 +49,11 @@ struct ScreenArgs {
     const uint8_t* flags;
     const int32_t* roots;
     uint32_t n_materials;
+    uint32_t lds_nodes;  // SHADE instances: style-table nodes staged in LDS (0: walk `nodes` in memory)
 };
+
+// style-table nodes the SHADE instances stage in LDS (4 KB); larger tables are walked in memory
+constexpr uint32_t kShadeLdsNodes = 16;
 
 // the epilogue's stylization of vertex o of slice s: per-vertex material and flags in the
 // SHADE instances (launched only when the launch has a material plane), else the launch's one
@@ -57,12 +61,19 @@ struct ScreenArgs {
 // a.material costs the 15x15 kernels 2 % even when it is not taken.
 template <typename A, bool SHADE>
 __device__ __forceinline__ v3<A> screen_stylize(const ScreenArgs<A>& a, uint32_t s, size_t o, v3<A> F, v3<A> p,
-                                                const float* ramp_tabs) {
+                                                const float* ramp_tabs, const nlpf_style_node* lds_nodes) {
     if constexpr (SHADE) {
         const size_t q = ((size_t)s * a.L + a.d) * a.plane + o;
         const uint32_t m = a.material[q];
         const int root = m < a.n_materials ? a.roots[m] : -1;
-        if (root >= 0) F = apply_style<A>(a.nodes, root, a.d, a.flags ? (uint32_t)a.flags[q] : 0u, F, p);
+        const uint32_t fl = a.flags ? (uint32_t)a.flags[q] : 0u;
+        if (root >= 0) {
+            // neighbouring vertices walk different trees: per-lane reads, from LDS when staged
+            if (a.lds_nodes)
+                F = apply_style<A>(lds_nodes, root, a.d, fl, F, p);
+            else
+                F = apply_style<A>(a.nodes, root, a.d, fl, F, p);
+        }
     } else if (a.chain.valid) {
         F = apply_chain<A>(a.chain, a.d, F, p, ramp_tabs);
     } else if (a.root >= 0) {
@@ -135,6 +146,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
     __shared__ V4<A> shB[HT * WS + 2];
     __shared__ V2<A> shC[HT * WS + 2];
     __shared__ float ramp_tabs[NLPF_CHAIN_MAX * 32];  // the chain's ramp stops, for per-lane lookups
+    __shared__ nlpf_style_node shade_nodes[SHADE ? kShadeLdsNodes : 1];  // referenced by SHADE only
 
     const uint32_t t = xcd_remap(blockIdx.x, a.nblocks);
     const uint32_t tiles = a.tiles_x * a.tiles_y;
@@ -153,6 +165,11 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
         const int i = threadIdx.x >> 5;
         if (a.chain.valid && i < a.chain.n && a.chain.node[i].type == NLPF_STY_RAMP)
             ramp_tabs[threadIdx.x] = a.chain.node[i].p[threadIdx.x & 31];
+    }
+    if constexpr (SHADE) {  // the style table for the epilogue's walks, 64 words per node
+        const uint32_t* src = (const uint32_t*)a.nodes;
+        uint32_t* dst = (uint32_t*)shade_nodes;
+        for (uint32_t w = threadIdx.x; w < a.lds_nodes * 64u; w += 64 * NW) dst[w] = src[w];
     }
 
     // ---- stage tile + halo in LDS, two horizontally adjacent records per thread ----
@@ -372,7 +389,7 @@ __global__ void __launch_bounds__(64 * NW) screen_level_kernel(ScreenArgs<A> a) 
                     p.z += (A)q[2];
                 }
             }
-            F = screen_stylize<A, SHADE>(a, s, o, F, p, ramp_tabs);
+            F = screen_stylize<A, SHADE>(a, s, o, F, p, ramp_tabs, shade_nodes);
             outp[o] = F.x;
             outp[plane + o] = F.y;
             outp[2 * plane + o] = F.z;

@@ -88,6 +88,7 @@ __global__ void __launch_bounds__(64 * NW) screen_strip_kernel(ScreenArgs<A> a) 
     __shared__ V4<A> shB[HT * WS];
     __shared__ V2<A> shC[HT * WS];
     __shared__ float ramp_tabs[NLPF_CHAIN_MAX * 32];
+    __shared__ nlpf_style_node shade_nodes[SHADE ? kShadeLdsNodes : 1];  // referenced by SHADE only
 
     // block -> segment: strips fastest, then bands, then slices, so the segments resident at
     // once are neighbours in the image and share their halo columns in L2
@@ -103,6 +104,11 @@ __global__ void __launch_bounds__(64 * NW) screen_strip_kernel(ScreenArgs<A> a) 
         const int i = threadIdx.x >> 5;
         if (a.chain.valid && i < a.chain.n && a.chain.node[i].type == NLPF_STY_RAMP)
             ramp_tabs[threadIdx.x] = a.chain.node[i].p[threadIdx.x & 31];
+    }
+    if constexpr (SHADE) {  // the style table for the epilogue's walks, 64 words per node
+        const uint32_t* src = (const uint32_t*)a.nodes;
+        uint32_t* dst = (uint32_t*)shade_nodes;
+        for (uint32_t w = threadIdx.x; w < a.lds_nodes * 64u; w += 64 * NW) dst[w] = src[w];
     }
     const int W = (int)a.W, H = (int)a.H;
     const uint32_t plane = (uint32_t)a.plane;
@@ -325,7 +331,7 @@ __global__ void __launch_bounds__(64 * NW) screen_strip_kernel(ScreenArgs<A> a) 
                             p = v3<A>{strip_ld1(rr, e, 0u), strip_ld1(rr, e, pbytes), strip_ld1(rr, e, 2 * pbytes)};
                         }
                     }
-                    F = screen_stylize<A, SHADE>(a, s, o, F, p, ramp_tabs);
+                    F = screen_stylize<A, SHADE>(a, s, o, F, p, ramp_tabs, shade_nodes);
                     strip_st1(ro, o * 4u, 0u, F.x);
                     strip_st1(ro, o * 4u, obytes, F.y);
                     strip_st1(ro, o * 4u, 2 * obytes, F.z);
