@@ -454,3 +454,45 @@ def test_per_vertex_materials_and_flags(prec, kernel, radius, W, H):
             b.free()
         if anc is not None:
             anc.free()
+
+
+@pytest.mark.parametrize("prec", [N.FP32, N.FP16])
+def test_per_vertex_materials_halftone_positions(prec):
+    """Positions are read when any material's tree holds a halftone node, here material 1's
+    only (material 0 is a rex): one level, so a tone that a dot-radius tie flips stays on its
+    own pixel; as in the stylize test, 99.9 % of pixels within 1e-5 of the oracle, which takes
+    the positions from the same (decoded) records."""
+    W, H, S, L, R = 96, 70, 2, 1, 3
+    rng = np.random.default_rng(17)
+    aff = np.array([[1, 0.2, 0, 0.1], [0, 1, 0.3, 0], [0.1, 0, 1, -0.2]], np.float32)
+    table = N.style_table([N.rex(0.6), N.halftone(1.0, 0.8, 0.5, 0.3, 0.7, [0.1, 0.1, 0.2], [1.0, 0.95, 0.9], aff,
+                                                  max_depth=3)])
+    roots = [0, 1]
+    material = rng.integers(0, 2, (S, L, H, W)).astype(np.uint16)
+    with N.Handle({"type": "pf", "mode": "screen"}) as hk:
+        hk.set_style(table, roots)
+        hk.set_sigmas(*SIG)
+        desc, rec, anc = gen_screen(hk, W, H, S, L, prec, seed=5, radius=R)
+        host = decode_fp16(rec, anc, W, H, S, L) if prec == N.FP16 else host_records_fp32(rec, W, H, S, L)
+        dm = hk.to_device(material)
+        film = hk.alloc(H * W * 12)
+        lvb = hk.alloc(S * L * 3 * H * W * 4)
+        hk.screen_run_materials(desc, rec, anc, dm, None, film, lvb)
+        glv = lvb.download(np.float32, S * L * 3 * H * W).reshape(S, L, 3, H, W)
+        of, olv = O.screen_run(host, W, H, S, L, R, SIG, nodes=table, want_levels=True, material=material,
+                               roots=roots)
+        err = pixel_rel_err(np.moveaxis(glv, 2, -1), np.moveaxis(olv, 2, -1))
+        assert (err <= 1e-5).mean() > 0.999, (err > 1e-5).mean()
+        # the positions matter: the halftone of the unstylized filter output at position 0
+        # differs from the level output on the halftone pixels
+        _, raw = O.screen_run(host, W, H, S, L, R, SIG, nodes=table, want_levels=True, material=material,
+                              roots=[-1, -1])
+        ht = material[:, 0] == 1
+        f_ht = np.moveaxis(raw[:, 0], 1, -1)[ht]
+        at0 = O.stylize(table, roots, f_ht, depth=0, material=np.ones(len(f_ht), np.uint16),
+                        pos=np.zeros((len(f_ht), 3)))
+        assert np.abs(at0 - np.moveaxis(olv[:, 0], 1, -1)[ht]).max() > 1e-2
+        for b in (dm, film, lvb, rec):
+            b.free()
+        if anc is not None:
+            anc.free()
