@@ -80,6 +80,7 @@ def test_null_arguments_rejected():
     assert lib.nlpf_create(b"{}", None) == 2
     assert lib.nlpf_destroy(None) == 2
     assert lib.nlpf_sync(None) == 2
+    assert lib.nlpf_screen_run_materials(None, None, None, None, None, None, None, None) == 2
 
 
 def rust_block():

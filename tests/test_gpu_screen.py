@@ -248,6 +248,12 @@ def test_invalid_screen_arguments(h):
     desc = h.screen_desc(8, 8, 1, 1, N.FP16, 3)
     with pytest.raises(N.NlpfError):  # FP16 without anchors
         h.screen_run(desc, 1234, None, 5678, None)
+    desc = h.screen_desc(8, 8, 1, 1, N.FP32, 3)
+    with pytest.raises(N.NlpfError, match="material"):  # no material plane
+        h.screen_run_materials(desc, 1234, None, None, None, 5678)
+    desc = h.screen_desc(0, 8, 1, 1, N.FP32, 3)
+    with pytest.raises(N.NlpfError):  # the descriptor is checked as for nlpf_screen_run
+        h.screen_run_materials(desc, 1234, None, 4321, None, 5678)
 
 
 @pytest.mark.parametrize("radius,W,H,S,L", [(7, 5, 3, 1, 2), (7, 1, 1, 2, 2), (2, 33, 9, 2, 3), (7, 66, 9, 1, 2),
